@@ -1,0 +1,190 @@
+"""Throughput benchmark: Kalman track fit on one MI355X.
+
+Workload (the plain-fit-at-scale configuration): 1,000,000 seeded toy-MC tracks
+through 25 barrel layers r = 3..110 cm (|z| <= 120 cm), uniform Bz = 3.8 T,
+x/X0 = 0.02 per layer with multiple scattering and energy loss; FP32 state.
+A "step" is one complete fit of the batch (every track through all of its
+hits), inputs resident in HBM, output written out of place.
+
+Two forms of the same fit are timed on the same inputs:
+  plain    -- mkf_fit_plain, one kernel launch per layer (state round-trips HBM)
+  batched  -- mkf_fit_batched, one persistent fused kernel from a cached hipGraph
+`value` is the batched throughput; the plain one and their ratio ride along.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tracks fitted/sec on 1 MI355X, batched vs plain kernel, at ≤1e-5 rel param err"
+# Algorithmic FP32 operations of one barrel track-layer step (propagate with
+# material + chi2 + update), counted on the oracle restatement by
+# oracle/flopcount.cc (+,-,*,/ = 1; sqrt, sin, cos, log = 1).
+FLOP_PER_TRACK_LAYER = 1394
+# Algorithmic HBM bytes per track-layer of the batched kernel: the hit (3 + 6
+# floats) and its flag byte; per track: seed in (27 floats + charge), result out
+# (27 floats + charge + chi2) and the hit count.
+HIT_BYTES = 9 * 4 + 1
+TRACK_BYTES = (27 + 1) * 4 + (27 + 1 + 1) * 4 + 4
+# Plain kernel, per track-layer: state in and out (27 floats + charge + chi2 each
+# way), the hit, its flag and the hit count.
+PLAIN_BYTES = (27 + 1 + 1) * 4 * 2 + 9 * 4 + 1 + 4
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-tracks", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import mkfit_amd as mk
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    geo = mk.geometry(2)
+    rl, xi = mk.material_grid(geo)
+    params = mk.default_toy_params(seed=args.seed, bfield=3.8)
+    n = args.n_tracks
+    t0 = time.time()
+    ev = mk.generate(geo, params, n, geo.n_barrel)
+    log("generated %d tracks in %.2f s; mean hits %.2f" % (n, time.time() - t0, ev["n_hits"].mean()))
+    pflags = mk.PF_APPLY_MATERIAL
+    fflags = mk.FIT_CHI2
+    ctx = mk.Context(bfield=params.bfield, rl_grid=rl, xi_grid=xi, device=0)
+    batch = mk.FitBatch.from_numpy(ev, dev, pflags=pflags, fit_flags=fflags, out_of_place=True)
+    plain_out = mk.FitBatch.from_numpy(ev, dev, pflags=pflags, fit_flags=fflags, out_of_place=True)
+    n_track_layers = int(ev["n_hits"].sum())
+    stream = torch.cuda.current_stream()
+
+    shape, tuned_ms = None, None
+    if not args.no_autotune:
+        shape, tuned_ms = ctx.autotune_batched(batch, reps=2)
+        log("autotuned batched shape %s: %.3f ms" % (shape, tuned_ms))
+    shape = ctx.get_batched_shape()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        t_start = time.perf_counter()
+        for e0, e1 in evs:
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        per = [e0.elapsed_time(e1) for e0, e1 in evs]
+        return wall, per
+
+    wall_p, per_p = timed(lambda: ctx.fit_plain(plain_out), args.steps, args.warmup)
+    wall_b, per_b = timed(lambda: ctx.fit_batched(batch), args.steps, args.warmup)
+    ms_b = wall_b * 1e3 / args.steps
+    ms_p = wall_p * 1e3 / args.steps
+    tps_b = n / (ms_b * 1e-3)
+    tps_p = n / (ms_p * 1e-3)
+    kern_b = float(np.mean(per_b))  # one fused kernel per step, timed with events on its stream
+    log("plain %.3f ms/step (%.3g tracks/s); batched %.3f ms/step (%.3g tracks/s), kernel %.3f ms" %
+        (ms_p, tps_p, ms_b, tps_b, kern_b))
+
+    # Parity gate on the measured output: batched == plain, and a sample vs the CPU oracle.
+    out_b = [batch.par.cpu().numpy(), batch.err.cpu().numpy(), batch.chi2.cpu().numpy()]
+    out_p = [plain_out.par.cpu().numpy(), plain_out.err.cpu().numpy(), plain_out.chi2.cpu().numpy()]
+    identical = all(np.array_equal(a, b) for a, b in zip(out_b, out_p))
+    import oracle
+    rng = np.random.default_rng(0)
+    idx = np.sort(rng.choice(n, min(n, 20000), replace=False))
+    sub = {k: np.ascontiguousarray(v[..., idx]) for k, v in ev.items()}
+    orc = oracle.Oracle(bfield=params.bfield, rl_grid=rl, xi_grid=xi)
+    ncpu = min(16, os.cpu_count() or 1)
+    opar, _, _, _ = orc.fit(sub, pflags=pflags, fit_flags=fflags, n_threads=ncpu)
+    fl = np.array([1.0, 1.0, 1.0, 0.1, 1.0, 1.0])[:, None]
+    d = np.abs(out_b[0][:, idx].astype(np.float64) - opar)
+    d[4] = np.minimum(d[4], np.abs(d[4] - 2 * np.pi))
+    rel = d / np.maximum(np.abs(opar), fl)
+    frac_ok = float(np.mean(rel.max(axis=0) <= 1e-5))
+    log("parity: batched==plain %s; sample max rel par err %.3g, fraction of tracks <= 1e-5: %.5f" %
+        (identical, rel.max(), frac_ok))
+
+    cpu = None
+    if not args.no_cpu:
+        best = 1e30
+        for _ in range(args.cpu_reps):
+            t = time.perf_counter()
+            orc.fit(ev, pflags=pflags, fit_flags=fflags, n_threads=ncpu)
+            best = min(best, time.perf_counter() - t)
+        cpu = {"value": n / best, "unit": "tracks/s", "cores": ncpu, "kind": "port",
+               "sample": "the full %d-track batch, best of %d passes, oracle FP32 Matriplex restatement "
+                         "(NN=%d, %s), %d threads" % (n, args.cpu_reps, oracle.simd_width(), oracle.isa_name(),
+                                                       ncpu)}
+        log("cpu baseline %.3g tracks/s (%.2f s per pass)" % (n / best, best))
+
+    flops = FLOP_PER_TRACK_LAYER * n_track_layers
+    achieved_tflops = flops / (kern_b * 1e-3) / 1e12
+    alg_bytes_b = n_track_layers * HIT_BYTES + n * TRACK_BYTES
+    res = {
+        "metric": METRIC,
+        "value": tps_b,
+        "unit": "tracks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_b,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded toy MC (helix + multiple scattering + energy loss), seed %d" % args.seed,
+        "config": {
+            "workload": "plain-fit-at-scale: %d tracks, 25 barrel layers r=3-110 cm, |z|<=120 cm, Bz=3.8 T, "
+                        "x/X0=0.02 per layer, material on, chi2 accumulated" % n,
+            "n_tracks": n, "layers": 25, "track_layers": n_track_layers,
+            "batched_shape": {"block": shape[0], "blocks_per_cu": shape[1], "variant": shape[2]},
+        },
+        "plain": {"value": tps_p, "ms_per_step": ms_p, "unit": "tracks/s",
+                  "achieved_GBps_algorithmic": n_track_layers * PLAIN_BYTES / (ms_p * 1e-3) / 1e9},
+        "batched_over_plain": tps_b / tps_p,
+        "parity": {"batched_equals_plain": identical, "sample_tracks": int(len(idx)),
+                   "max_rel_par_err_vs_cpu": float(rel.max()), "frac_tracks_within_1e-5": frac_ok},
+        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_fit_fused", "kernel_ms": kern_b,
+                     "flop_per_track_layer": FLOP_PER_TRACK_LAYER,
+                     "algorithmic_GBps": alg_bytes_b / (kern_b * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,325 @@
+"""mkfit_amd -- MI355X (gfx950) Kalman track fit behind mkFit's Matriplex interface.
+
+Thin Python harness over the C ABI in include/mkfit_amd.h (libmkfit_amd.so) and
+the toy Monte Carlo in include/mkfit_toymc.h (libmkfit_toymc.so).  Torch is
+used only to own device memory and streams; every computation runs in the HIP
+library.  Importing this package never falls back to CPU code: if the native
+library is missing the import of `lib()` raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_TOY = None
+
+PF_USE_PARAM_B_FIELD = 0x1
+PF_APPLY_MATERIAL = 0x2
+KFO_CALCULATE_CHI2 = 1
+KFO_UPDATE_PARAMS = 2
+FIT_CHI2 = 1
+FIT_CHARGE_FLIP = 2
+HIT_ENDCAP = 1
+HIT_MISSING = 2
+NBINS_Z, NBINS_R = 300, 120
+MAX_LAYERS = 64
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_void_p = ctypes.c_void_p
+
+
+class MkfConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("bfield", ctypes.c_float),
+                ("rl_grid", _c_void_p), ("xi_grid", _c_void_p)]
+
+
+class MkfFitBatch(ctypes.Structure):
+    _fields_ = [("n_tracks", ctypes.c_int), ("n_slots", ctypes.c_int),
+                ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p), ("chi2", _c_void_p),
+                ("hit_pos", _c_void_p), ("hit_err", _c_void_p), ("n_hits", _c_void_p),
+                ("hit_flags", _c_void_p), ("pflags", ctypes.c_int), ("fit_flags", ctypes.c_int),
+                ("seed_par", _c_void_p), ("seed_err", _c_void_p), ("seed_chg", _c_void_p)]
+
+
+class MkfLaunchShape(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int), ("variant", ctypes.c_int)]
+
+
+class MkfGeometry(ctypes.Structure):
+    _fields_ = [("n_barrel", ctypes.c_int),
+                ("barrel_r", ctypes.c_float * MAX_LAYERS), ("barrel_zmax", ctypes.c_float * MAX_LAYERS),
+                ("n_disk", ctypes.c_int),
+                ("disk_z", ctypes.c_float * MAX_LAYERS), ("disk_rmin", ctypes.c_float * MAX_LAYERS),
+                ("disk_rmax", ctypes.c_float * MAX_LAYERS),
+                ("x_over_x0", ctypes.c_float * MAX_LAYERS), ("xi", ctypes.c_float * MAX_LAYERS)]
+
+
+class MkfToyParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("bfield", ctypes.c_float),
+                ("pt_min", ctypes.c_float), ("pt_max", ctypes.c_float), ("eta_max", ctypes.c_float),
+                ("vtx_sxy", ctypes.c_float), ("vtx_sz", ctypes.c_float),
+                ("sig_rphi", ctypes.c_float), ("sig_z", ctypes.c_float), ("sig_r_endcap", ctypes.c_float),
+                ("simulate_material", ctypes.c_int), ("hits_min", ctypes.c_int), ("hits_max", ctypes.c_int),
+                ("missing_frac", ctypes.c_float),
+                ("seed_sig_pos", ctypes.c_float), ("seed_sig_z", ctypes.c_float),
+                ("seed_rel_ipt", ctypes.c_float), ("seed_sig_ang", ctypes.c_float)]
+
+
+class MkfToyOutput(ctypes.Structure):
+    _fields_ = [("n_tracks", ctypes.c_int), ("n_slots", ctypes.c_int),
+                ("seed_par", _c_void_p), ("seed_err", _c_void_p), ("chg", _c_void_p),
+                ("hit_pos", _c_void_p), ("hit_err", _c_void_p), ("n_hits", _c_void_p),
+                ("hit_flags", _c_void_p), ("hit_layer", _c_void_p), ("truth_par", _c_void_p)]
+
+
+ABI_SYMBOLS = [
+    "mkf_context_create", "mkf_context_destroy", "mkf_last_error",
+    "mkf_propagate_helix_to_r", "mkf_propagate_helix_to_z",
+    "mkf_kalman_operation", "mkf_kalman_operation_endcap",
+    "mkf_fit_plain", "mkf_fit_batched",
+    "mkf_set_batched_shape", "mkf_get_batched_shape", "mkf_autotune_batched", "mkf_num_batched_variants",
+]
+
+
+def lib_path():
+    return os.path.join(_HERE, "libmkfit_amd.so")
+
+
+def lib():
+    """The HIP library; raises if it has not been built (no CPU fallback exists)."""
+    global _LIB
+    if _LIB is None:
+        p = lib_path()
+        if not os.path.exists(p):
+            raise RuntimeError("libmkfit_amd.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(p)
+        L.mkf_last_error.restype = ctypes.c_char_p
+        L.mkf_context_create.argtypes = [ctypes.POINTER(MkfConfig), ctypes.POINTER(_c_void_p)]
+        L.mkf_context_destroy.argtypes = [_c_void_p]
+        L.mkf_propagate_helix_to_r.argtypes = [_c_void_p, ctypes.c_int] + [_c_void_p] * 6 + [ctypes.c_int] + [_c_void_p] * 3
+        L.mkf_propagate_helix_to_z.argtypes = [_c_void_p, ctypes.c_int] + [_c_void_p] * 6 + [ctypes.c_int] + [_c_void_p] * 2
+        L.mkf_kalman_operation.argtypes = [_c_void_p, ctypes.c_int, ctypes.c_int] + [_c_void_p] * 8
+        L.mkf_kalman_operation_endcap.argtypes = [_c_void_p, ctypes.c_int, ctypes.c_int] + [_c_void_p] * 8
+        L.mkf_fit_plain.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p]
+        L.mkf_fit_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p]
+        L.mkf_set_batched_shape.argtypes = [_c_void_p, ctypes.POINTER(MkfLaunchShape)]
+        L.mkf_get_batched_shape.argtypes = [_c_void_p, ctypes.POINTER(MkfLaunchShape)]
+        L.mkf_autotune_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p, ctypes.c_int,
+                                           ctypes.POINTER(MkfLaunchShape), _c_float_p]
+        _LIB = L
+    return _LIB
+
+
+def toy():
+    global _TOY
+    if _TOY is None:
+        p = os.path.join(_HERE, "libmkfit_toymc.so")
+        if not os.path.exists(p):
+            raise RuntimeError("libmkfit_toymc.so is not built (run __graft_entry__.build())")
+        T = ctypes.CDLL(p)
+        T.mkf_toymc_geometry_preset.argtypes = [ctypes.c_int, ctypes.POINTER(MkfGeometry)]
+        T.mkf_toymc_default_params.argtypes = [ctypes.POINTER(MkfToyParams)]
+        T.mkf_toymc_generate.argtypes = [ctypes.POINTER(MkfGeometry), ctypes.POINTER(MkfToyParams),
+                                         ctypes.POINTER(MkfToyOutput), ctypes.c_int]
+        T.mkf_toymc_material_grid.argtypes = [ctypes.POINTER(MkfGeometry), _c_void_p, _c_void_p]
+        _TOY = T
+    return _TOY
+
+
+class MkfError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise MkfError("mkfit_amd error %d: %s" % (rc, lib().mkf_last_error().decode()))
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+# ----------------------------------------------------------------------------
+# Toy Monte Carlo
+# ----------------------------------------------------------------------------
+
+def geometry(preset):
+    g = MkfGeometry()
+    if toy().mkf_toymc_geometry_preset(int(preset), ctypes.byref(g)) != 0:
+        raise ValueError("unknown geometry preset %r" % (preset,))
+    return g
+
+
+def default_toy_params(**kw):
+    p = MkfToyParams()
+    toy().mkf_toymc_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def material_grid(geo):
+    rl = np.zeros((NBINS_Z, NBINS_R), np.float32)
+    xi = np.zeros((NBINS_Z, NBINS_R), np.float32)
+    toy().mkf_toymc_material_grid(ctypes.byref(geo), rl.ctypes.data_as(_c_void_p), xi.ctypes.data_as(_c_void_p))
+    return rl, xi
+
+
+def generate(geo, params, n_tracks, n_slots, n_threads=None, truth=False):
+    """Runs the toy MC; returns a dict of numpy arrays in the fit's layout."""
+    n, S = int(n_tracks), int(n_slots)
+    out = {
+        "seed_par": np.zeros((6, n), np.float32), "seed_err": np.zeros((21, n), np.float32),
+        "chg": np.zeros(n, np.int32), "hit_pos": np.zeros((S, 3, n), np.float32),
+        "hit_err": np.zeros((S, 6, n), np.float32), "n_hits": np.zeros(n, np.int32),
+        "hit_flags": np.zeros((S, n), np.uint8), "hit_layer": np.zeros((S, n), np.int32),
+    }
+    if truth:
+        out["truth_par"] = np.zeros((S, 6, n), np.float32)
+    o = MkfToyOutput()
+    o.n_tracks, o.n_slots = n, S
+    for k in ("seed_par", "seed_err", "chg", "hit_pos", "hit_err", "n_hits", "hit_flags", "hit_layer"):
+        setattr(o, k, out[k].ctypes.data_as(_c_void_p))
+    o.truth_par = out["truth_par"].ctypes.data_as(_c_void_p) if truth else None
+    if n_threads is None:
+        n_threads = min(16, os.cpu_count() or 1)
+    rc = toy().mkf_toymc_generate(ctypes.byref(geo), ctypes.byref(params), ctypes.byref(o), int(n_threads))
+    if rc != 0:
+        raise RuntimeError("toy MC generation failed (%d)" % rc)
+    return out
+
+
+# ----------------------------------------------------------------------------
+# Device-side harness
+# ----------------------------------------------------------------------------
+
+class Context:
+    """Owns an mkf_context (B field + material grid on one device)."""
+
+    def __init__(self, bfield=3.8112, rl_grid=None, xi_grid=None, device=0):
+        cfg = MkfConfig()
+        cfg.device = int(device)
+        cfg.bfield = float(bfield)
+        self._rl = None if rl_grid is None else np.ascontiguousarray(rl_grid, np.float32)
+        self._xi = None if xi_grid is None else np.ascontiguousarray(xi_grid, np.float32)
+        cfg.rl_grid = None if self._rl is None else self._rl.ctypes.data_as(_c_void_p)
+        cfg.xi_grid = None if self._xi is None else self._xi.ctypes.data_as(_c_void_p)
+        h = _c_void_p()
+        _check(lib().mkf_context_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.device = int(device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mkf_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self, stream):
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(self.device)
+        return _c_void_p(stream.cuda_stream)
+
+    def propagate_helix_to_r(self, in_err, in_par, in_chg, ms_rad, out_err, out_par, pflags=0, no_mat=None,
+                             out_fail=None, stream=None):
+        n = in_par.shape[-1]
+        _check(lib().mkf_propagate_helix_to_r(self.h, n, _ptr(in_err), _ptr(in_par), _ptr(in_chg), _ptr(ms_rad),
+                                              _ptr(out_err), _ptr(out_par), int(pflags), _ptr(no_mat),
+                                              _ptr(out_fail), self._stream(stream)))
+
+    def propagate_helix_to_z(self, in_err, in_par, in_chg, ms_z, out_err, out_par, pflags=0, no_mat=None,
+                             stream=None):
+        n = in_par.shape[-1]
+        _check(lib().mkf_propagate_helix_to_z(self.h, n, _ptr(in_err), _ptr(in_par), _ptr(in_chg), _ptr(ms_z),
+                                              _ptr(out_err), _ptr(out_par), int(pflags), _ptr(no_mat),
+                                              self._stream(stream)))
+
+    def kalman_operation(self, kf_op, ps_err, ps_par, ms_err, ms_par, out_err=None, out_par=None, out_chi2=None,
+                         endcap=False, stream=None):
+        n = ps_par.shape[-1]
+        f = lib().mkf_kalman_operation_endcap if endcap else lib().mkf_kalman_operation
+        _check(f(self.h, int(kf_op), n, _ptr(ps_err), _ptr(ps_par), _ptr(ms_err), _ptr(ms_par), _ptr(out_err),
+                 _ptr(out_par), _ptr(out_chi2), self._stream(stream)))
+
+    def fit_plain(self, batch, stream=None):
+        _check(lib().mkf_fit_plain(self.h, ctypes.byref(batch.c_struct()), self._stream(stream)))
+
+    def fit_batched(self, batch, stream=None):
+        _check(lib().mkf_fit_batched(self.h, ctypes.byref(batch.c_struct()), self._stream(stream)))
+
+    def set_batched_shape(self, block, blocks_per_cu, variant):
+        s = MkfLaunchShape(int(block), int(blocks_per_cu), int(variant))
+        _check(lib().mkf_set_batched_shape(self.h, ctypes.byref(s)))
+
+    def get_batched_shape(self):
+        s = MkfLaunchShape()
+        _check(lib().mkf_get_batched_shape(self.h, ctypes.byref(s)))
+        return (s.block, s.blocks_per_cu, s.variant)
+
+    def autotune_batched(self, batch, reps=3):
+        import torch
+        scratch = torch.empty(29 * batch.n_tracks, dtype=torch.float32, device=batch.par.device)
+        s = MkfLaunchShape()
+        ms = ctypes.c_float()
+        torch.cuda.synchronize(self.device)
+        _check(lib().mkf_autotune_batched(self.h, ctypes.byref(batch.c_struct()), _ptr(scratch), int(reps),
+                                          ctypes.byref(s), ctypes.byref(ms)))
+        return (s.block, s.blocks_per_cu, s.variant), ms.value
+
+
+class FitBatch:
+    """Device-resident batch (torch tensors) in the layout of mkf_fit_batch."""
+
+    def __init__(self, par, err, chg, chi2, hit_pos, hit_err, n_hits, hit_flags=None, pflags=0,
+                 fit_flags=FIT_CHI2, seed=None):
+        self.par, self.err, self.chg, self.chi2 = par, err, chg, chi2
+        self.seed = seed  # optional (seed_par, seed_err, seed_chg) for an out-of-place fit
+        self.hit_pos, self.hit_err, self.n_hits, self.hit_flags = hit_pos, hit_err, n_hits, hit_flags
+        self.pflags, self.fit_flags = int(pflags), int(fit_flags)
+        self.n_tracks = int(par.shape[-1])
+        self.n_slots = int(hit_pos.shape[0])
+
+    @classmethod
+    def from_numpy(cls, ev, device, pflags=0, fit_flags=FIT_CHI2, with_flags=True, out_of_place=False):
+        import torch
+        d = torch.device(device)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(d)
+        n = ev["seed_par"].shape[1]
+        sp, se, sc = t(ev["seed_par"]), t(ev["seed_err"]), t(ev["chg"])
+        if out_of_place:
+            par, err, chg = torch.zeros_like(sp), torch.zeros_like(se), torch.zeros_like(sc)
+            seed = (sp, se, sc)
+        else:
+            par, err, chg, seed = sp, se, sc, None
+        return cls(par, err, chg, torch.zeros(n, dtype=torch.float32, device=d),
+                   t(ev["hit_pos"]), t(ev["hit_err"]), t(ev["n_hits"]),
+                   t(ev["hit_flags"]) if with_flags else None, pflags=pflags, fit_flags=fit_flags, seed=seed)
+
+    def clone_state(self):
+        return [x.clone() for x in (self.par, self.err, self.chg, self.chi2)]
+
+    def restore_state(self, st):
+        for dst, src in zip((self.par, self.err, self.chg, self.chi2), st):
+            dst.copy_(src)
+
+    def c_struct(self):
+        b = MkfFitBatch()
+        b.n_tracks, b.n_slots = self.n_tracks, self.n_slots
+        b.par, b.err, b.chg, b.chi2 = _ptr(self.par), _ptr(self.err), _ptr(self.chg), _ptr(self.chi2)
+        b.hit_pos, b.hit_err, b.n_hits = _ptr(self.hit_pos), _ptr(self.hit_err), _ptr(self.n_hits)
+        b.hit_flags = _ptr(self.hit_flags)
+        b.pflags, b.fit_flags = self.pflags, self.fit_flags
+        if self.seed is not None:
+            b.seed_par, b.seed_err, b.seed_chg = (_ptr(x) for x in self.seed)
+        return b

@@ -1,0 +1,59 @@
+"""Build recipe for the native libraries (no cmake; direct hipcc / g++ lines).
+
+  mkfit_amd/libmkfit_amd.so    HIP kernels + C ABI (include/mkfit_amd.h), gfx950
+  mkfit_amd/libmkfit_toymc.so  toy Monte Carlo generator (include/mkfit_toymc.h), host C++
+  oracle/liborc_avx512.so      CPU oracle, NN=16 (test infrastructure)
+  oracle/liborc_avx2.so        CPU oracle, NN=8  (test infrastructure)
+"""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "mkfit_amd")
+CSRC = os.path.join(PKG, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # Packed FP32 (v_pk_*_f32) is kept out of generated code.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), r.stdout))
+    return r.stdout
+
+
+def build_hip():
+    out = os.path.join(PKG, "libmkfit_amd.so")
+    srcs = [os.path.join(CSRC, "mkf_fit.hip")]
+    _run([HIPCC] + HIP_FLAGS + ["-o", out + ".tmp"] + srcs)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_toymc():
+    out = os.path.join(PKG, "libmkfit_toymc.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", out + ".tmp",
+          os.path.join(CSRC, "toymc.cc")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_oracle():
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir, "-B", "all"])
+
+
+def build_all():
+    build_oracle()
+    build_toymc()
+    build_hip()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built")

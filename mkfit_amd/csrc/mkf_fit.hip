@@ -1,0 +1,642 @@
+// mkf_fit.hip -- propagation / Kalman / fit kernels and the C ABI (include/mkfit_amd.h).
+//
+// Layout in HBM: every per-track quantity is element-major, [element][track],
+// so lane t of a wavefront reads a[e*n + t] and each element load is one
+// fully coalesced 256-byte wave transaction.  Per-hit arrays are
+// [slot][element][track].
+//
+// Two fit forms share the same per-lane step (mkf_track_math.h):
+//   plain   -- k_fit_slot, one launch per hit slot; the 27-float state makes a
+//              round trip through HBM per slot (~270 B per track-slot).
+//   batched -- k_fit_fused, a persistent grid-stride kernel that loads the
+//              state once, walks all of the track's slots with the state in
+//              VGPRs and streams only the hits (~37 B per track-slot).  It is
+//              launched from a cached hipGraph; launch shape and register
+//              budget come from mkf_autotune_batched.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mkfit_amd.h"
+#include "mkf_track_math.h"
+
+using namespace mkf;
+
+// ============================================================================
+// Context and error reporting
+// ============================================================================
+
+struct GraphKey {
+  const void* ptrs[12];
+  int n, slots, pflags, fflags, block, bpc, variant;
+  bool operator<(const GraphKey& o) const {
+    int c = std::memcmp(ptrs, o.ptrs, sizeof(ptrs));
+    if (c) return c < 0;
+    return std::tie(n, slots, pflags, fflags, block, bpc, variant) <
+           std::tie(o.n, o.slots, o.pflags, o.fflags, o.block, o.bpc, o.variant);
+  }
+};
+
+struct mkf_context {
+  int device = 0;
+  int num_cus = 256;
+  float bfield = 3.8112f;
+  float* d_rl = nullptr;
+  float* d_xi = nullptr;
+  mkf_launch_shape shape{256, 8, 0};
+  hipStream_t capture_stream = nullptr;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::mutex mu;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail(MKF_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+// A device pointer must be non-null, device (or managed) memory, and the
+// allocation it lies in must extend `bytes` past it.
+int check_dev(const void* p, size_t bytes, const char* name) {
+  if (!p) return fail(MKF_ERR_INVALID_ARG, "%s is NULL", name);
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(MKF_ERR_INVALID_ARG, "%s is not a HIP allocation", name);
+  }
+  if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)
+    return fail(MKF_ERR_INVALID_ARG, "%s is not device memory", name);
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(MKF_ERR_INVALID_ARG, "%s: cannot query its allocation", name);
+  }
+  const char* b = (const char*)base;
+  const char* q = (const char*)p;
+  if (q < b || (size_t)(q - b) + bytes > size)
+    return fail(MKF_ERR_BOUNDS, "%s: %zu bytes needed past the pointer, allocation has %zu", name, bytes,
+                size - (size_t)(q - b));
+  return MKF_OK;
+}
+
+#define CHECK(expr)            \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_) return rc_;       \
+  } while (0)
+
+int set_device(mkf_context* c) {
+  if (!c) return fail(MKF_ERR_INVALID_ARG, "context is NULL");
+  HIP_TRY(hipSetDevice(c->device));
+  return MKF_OK;
+}
+
+Material mat_of(const mkf_context* c) { return Material{c->d_rl, c->d_xi}; }
+
+}  // namespace
+
+// ============================================================================
+// Kernels: single operations (the MPlex free functions)
+// ============================================================================
+
+__global__ void __launch_bounds__(256)
+k_propagate_r(int n, const float* __restrict__ in_err, const float* __restrict__ in_par,
+              const int* __restrict__ in_chg, const float* __restrict__ ms_rad, float* __restrict__ out_err,
+              float* __restrict__ out_par, int pflags, const int* __restrict__ no_mat, int* __restrict__ out_fail,
+              float bfield, Material mat) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float par[6], err[21];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) par[e] = in_par[e * n + t];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) err[e] = in_err[e * n + t];
+  const bool nm = no_mat ? no_mat[t] != 0 : false;
+  const bool failed = propagate_to_r(par, err, in_chg[t], ms_rad[t], pflags, nm, bfield, mat);
+#pragma unroll
+  for (int e = 0; e < 6; ++e) out_par[e * n + t] = par[e];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) out_err[e * n + t] = err[e];
+  if (out_fail) out_fail[t] = failed ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256)
+k_propagate_z(int n, const float* __restrict__ in_err, const float* __restrict__ in_par,
+              const int* __restrict__ in_chg, const float* __restrict__ ms_z, float* __restrict__ out_err,
+              float* __restrict__ out_par, int pflags, const int* __restrict__ no_mat, float bfield, Material mat) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float par[6], err[21];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) par[e] = in_par[e * n + t];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) err[e] = in_err[e * n + t];
+  const bool nm = no_mat ? no_mat[t] != 0 : false;
+  propagate_to_z(par, err, in_chg[t], ms_z[t], pflags, nm, bfield, mat);
+#pragma unroll
+  for (int e = 0; e < 6; ++e) out_par[e * n + t] = par[e];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) out_err[e * n + t] = err[e];
+}
+
+template <bool ENDCAP>
+__global__ void __launch_bounds__(256)
+k_kalman(int op, int n, const float* __restrict__ ps_err, const float* __restrict__ ps_par,
+         const float* __restrict__ ms_err, const float* __restrict__ ms_par, float* __restrict__ out_err,
+         float* __restrict__ out_par, float* __restrict__ out_chi2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float par[6], err[21], mp[3], me[6];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) par[e] = ps_par[e * n + t];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) err[e] = ps_err[e * n + t];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) mp[e] = ms_par[e * n + t];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) me[e] = ms_err[e * n + t];
+  float chi2;
+  if (op & MKF_KFO_UPDATE_PARAMS)
+    chi2 = ENDCAP ? kalman_endcap<true, true>(par, err, mp, me) : kalman_barrel<true, true>(par, err, mp, me);
+  else
+    chi2 = ENDCAP ? kalman_endcap<true, false>(par, err, mp, me) : kalman_barrel<true, false>(par, err, mp, me);
+  if ((op & MKF_KFO_CALCULATE_CHI2) && out_chi2) out_chi2[t] = chi2;
+  if (op & MKF_KFO_UPDATE_PARAMS) {
+#pragma unroll
+    for (int e = 0; e < 6; ++e) out_par[e * n + t] = par[e];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) out_err[e * n + t] = err[e];
+  }
+}
+
+// ============================================================================
+// Fit kernels
+// ============================================================================
+
+struct FitArgs {
+  int n, n_slots;
+  float* __restrict__ par;
+  float* __restrict__ err;
+  int* __restrict__ chg;
+  float* __restrict__ chi2;
+  const float* __restrict__ hit_pos;
+  const float* __restrict__ hit_err;
+  const int* __restrict__ n_hits;
+  const unsigned char* __restrict__ hit_flags;
+  const float* __restrict__ seed_par;
+  const float* __restrict__ seed_err;
+  const int* __restrict__ seed_chg;
+  int pflags, fit_flags;
+  float bfield;
+  Material mat;
+};
+
+// One hit slot of one track: the body of the reference's per-layer loop
+// (MkFinder::BkFitFitTracks, MkFinder.cc:1612-1740; FitTracksSteered,
+// MkFitter.cc:477-500).
+__device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int& chg, float& chi2, const FitArgs& a,
+                                         int s, int t) {
+  const unsigned flags = a.hit_flags ? a.hit_flags[(size_t)s * a.n + t] : 0u;
+  const bool missing = flags & MKF_HIT_MISSING;
+  float mp[3], me[6];
+  if (missing) {
+    mp[0] = par[0]; mp[1] = par[1]; mp[2] = par[2];
+    me[0] = 666.f; me[1] = 0.f; me[2] = 666.f; me[3] = 0.f; me[4] = 0.f; me[5] = 666.f;
+  } else {
+    const float* hp = a.hit_pos + (size_t)s * 3 * a.n;
+    const float* he = a.hit_err + (size_t)s * 6 * a.n;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) mp[e] = hp[e * a.n + t];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) me[e] = he[e * a.n + t];
+  }
+  float c2;
+  if (!(flags & MKF_HIT_ENDCAP)) {
+    const float r = hypotf(mp[0], mp[1]);                                  // MkBase.h:55
+    propagate_to_r(par, err, chg, r, a.pflags, missing, a.bfield, a.mat);
+    c2 = kalman_barrel<true, true>(par, err, mp, me);
+  } else {
+    propagate_to_z(par, err, chg, mp[2], a.pflags, missing, a.bfield, a.mat);
+    c2 = kalman_endcap<true, true>(par, err, mp, me);
+  }
+  if (a.fit_flags & MKF_FIT_CHI2) chi2 += c2;
+  if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) && par[3] < 0.f) {
+    chg = -chg;
+    par[3] = -par[3];
+  }
+}
+
+// from_seed: read the seed arrays (first touch of an out-of-place fit).
+__device__ __forceinline__ void load_state(const FitArgs& a, int t, bool from_seed, float (&par)[6], float (&err)[21],
+                                           int& chg, float& chi2) {
+  const float* P = from_seed ? a.seed_par : a.par;
+  const float* E = from_seed ? a.seed_err : a.err;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) par[e] = P[e * a.n + t];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) err[e] = E[e * a.n + t];
+  chg = from_seed ? a.seed_chg[t] : a.chg[t];
+  chi2 = ((a.fit_flags & MKF_FIT_CHI2) && !from_seed) ? a.chi2[t] : 0.f;
+}
+
+__device__ __forceinline__ void store_state(const FitArgs& a, int t, const float (&par)[6], const float (&err)[21],
+                                            int chg, float chi2) {
+#pragma unroll
+  for (int e = 0; e < 6; ++e) a.par[e * a.n + t] = par[e];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) a.err[e * a.n + t] = err[e];
+  if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) || a.seed_par) a.chg[t] = chg;
+  if (a.fit_flags & MKF_FIT_CHI2) a.chi2[t] = chi2;
+}
+
+// Plain form: one launch per slot s.
+// Plain form: one launch per slot s.  With a seed input, the slot-0 launch
+// also carries tracks without hits over to the output.
+__global__ void __launch_bounds__(256) k_fit_slot(FitArgs a, int s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  const bool from_seed = (s == 0) && a.seed_par != nullptr;
+  const bool active = s < min(a.n_hits[t], a.n_slots);
+  if (!active && !from_seed) return;
+  float par[6], err[21], chi2;
+  int chg;
+  load_state(a, t, from_seed, par, err, chg, chi2);
+  if (active) fit_step(par, err, chg, chi2, a, s, t);
+  store_state(a, t, par, err, chg, chi2);
+}
+
+// Batched form: persistent grid-stride loop, all slots of a track in registers.
+// MINW = minimum waves per SIMD requested from the register allocator.
+template <int MINW>
+__global__ void __attribute__((amdgpu_flat_work_group_size(64, 512), amdgpu_waves_per_eu(MINW, 8)))
+k_fit_fused(FitArgs a) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += stride) {
+    float par[6], err[21], chi2;
+    int chg;
+    load_state(a, t, a.seed_par != nullptr, par, err, chg, chi2);
+    const int nh = min(a.n_hits[t], a.n_slots);
+    for (int s = 0; s < nh; ++s) fit_step(par, err, chg, chi2, a, s, t);
+    store_state(a, t, par, err, chg, chi2);
+  }
+}
+
+using FusedKernel = void (*)(FitArgs);
+static const FusedKernel kFusedVariants[] = {k_fit_fused<1>, k_fit_fused<2>, k_fit_fused<4>, k_fit_fused<6>};
+static constexpr int kNumFusedVariants = sizeof(kFusedVariants) / sizeof(kFusedVariants[0]);
+
+// ============================================================================
+// C ABI
+// ============================================================================
+
+namespace {
+
+int validate_batch(const mkf_fit_batch* b, FitArgs& a, const mkf_context* c) {
+  if (!b) return fail(MKF_ERR_INVALID_ARG, "batch is NULL");
+  if (b->n_tracks < 0 || b->n_slots < 0) return fail(MKF_ERR_INVALID_ARG, "negative batch size");
+  const size_t n = (size_t)b->n_tracks, S = (size_t)b->n_slots;
+  if (n * (S > 0 ? S : 1) * 6 > (size_t)0x7fffffff)
+    return fail(MKF_ERR_INVALID_ARG, "batch too large for 32-bit element offsets");
+  a.n = b->n_tracks;
+  a.n_slots = b->n_slots;
+  a.par = b->par;
+  a.err = b->err;
+  a.chg = b->chg;
+  a.chi2 = b->chi2;
+  a.hit_pos = b->hit_pos;
+  a.hit_err = b->hit_err;
+  a.n_hits = b->n_hits;
+  a.hit_flags = b->hit_flags;
+  a.seed_par = b->seed_par;
+  a.seed_err = b->seed_err;
+  a.seed_chg = b->seed_chg;
+  a.pflags = b->pflags;
+  a.fit_flags = b->fit_flags;
+  a.bfield = c->bfield;
+  a.mat = mat_of(c);
+  if (n == 0) return MKF_OK;
+  CHECK(check_dev(b->par, 6 * n * 4, "par"));
+  CHECK(check_dev(b->err, 21 * n * 4, "err"));
+  CHECK(check_dev(b->chg, n * 4, "chg"));
+  if (b->fit_flags & MKF_FIT_CHI2) CHECK(check_dev(b->chi2, n * 4, "chi2"));
+  if (b->seed_par) {
+    CHECK(check_dev(b->seed_par, 6 * n * 4, "seed_par"));
+    CHECK(check_dev(b->seed_err, 21 * n * 4, "seed_err"));
+    CHECK(check_dev(b->seed_chg, n * 4, "seed_chg"));
+  } else if (b->seed_err || b->seed_chg) {
+    return fail(MKF_ERR_INVALID_ARG, "seed_err/seed_chg set without seed_par");
+  }
+  CHECK(check_dev(b->n_hits, n * 4, "n_hits"));
+  if (S > 0) {
+    CHECK(check_dev(b->hit_pos, 3 * n * S * 4, "hit_pos"));
+    CHECK(check_dev(b->hit_err, 6 * n * S * 4, "hit_err"));
+    if (b->hit_flags) CHECK(check_dev(b->hit_flags, n * S, "hit_flags"));
+  }
+  if ((b->pflags & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
+    return fail(MKF_ERR_INVALID_ARG, "material requested but the context has no material grid");
+  return MKF_OK;
+}
+
+int grid_for(int n, int block) { return (n + block - 1) / block; }
+
+int launch_fused(mkf_context* c, const FitArgs& a, const mkf_launch_shape& sh, hipStream_t st) {
+  if (a.n == 0) return MKF_OK;
+  const int block = sh.block;
+  int grid = sh.blocks_per_cu > 0 ? sh.blocks_per_cu * c->num_cus : grid_for(a.n, block);
+  grid = std::max(1, std::min(grid, grid_for(a.n, block)));
+  hipLaunchKernelGGL(kFusedVariants[sh.variant], dim3(grid), dim3(block), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}
+
+bool shape_ok(const mkf_launch_shape& s) {
+  return (s.block == 64 || s.block == 128 || s.block == 256 || s.block == 512) && s.blocks_per_cu >= 0 &&
+         s.blocks_per_cu <= 64 && s.variant >= 0 && s.variant < kNumFusedVariants;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mkf_last_error(void) { return g_last_error.c_str(); }
+
+int mkf_num_batched_variants(void) { return kNumFusedVariants; }
+
+int mkf_context_create(const mkf_config* cfg, mkf_context** out) {
+  if (!cfg || !out) return fail(MKF_ERR_INVALID_ARG, "cfg/out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return fail(MKF_ERR_NO_DEVICE, "no HIP device");
+  }
+  if (cfg->device < 0 || cfg->device >= ndev) return fail(MKF_ERR_INVALID_ARG, "device %d out of range", cfg->device);
+  if ((cfg->rl_grid == nullptr) != (cfg->xi_grid == nullptr))
+    return fail(MKF_ERR_INVALID_ARG, "rl_grid and xi_grid must both be set or both be NULL");
+  auto* c = new mkf_context();
+  c->device = cfg->device;
+  c->bfield = cfg->bfield;
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->device);
+  if (e == hipSuccess && cfg->rl_grid) {
+    const size_t bytes = sizeof(float) * kNBinsZ * kNBinsR;
+    e = hipMalloc(&c->d_rl, bytes);
+    if (e == hipSuccess) e = hipMalloc(&c->d_xi, bytes);
+    if (e == hipSuccess) e = hipMemcpy(c->d_rl, cfg->rl_grid, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->d_xi, cfg->xi_grid, bytes, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    int rc = fail(MKF_ERR_HIP, "context setup: %s", hipGetErrorString(e));
+    mkf_context_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MKF_OK;
+}
+
+int mkf_context_destroy(mkf_context* c) {
+  if (!c) return MKF_OK;
+  (void)hipSetDevice(c->device);
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
+  if (c->d_rl) (void)hipFree(c->d_rl);
+  if (c->d_xi) (void)hipFree(c->d_xi);
+  delete c;
+  return MKF_OK;
+}
+
+int mkf_propagate_helix_to_r(mkf_context* c, int n, const float* in_err, const float* in_par, const int* in_chg,
+                             const float* ms_rad, float* out_err, float* out_par, int pflags, const int* no_mat,
+                             int* out_fail, void* stream) {
+  CHECK(set_device(c));
+  if (n < 0) return fail(MKF_ERR_INVALID_ARG, "n < 0");
+  if (n == 0) return MKF_OK;
+  const size_t N = n;
+  CHECK(check_dev(in_err, 21 * N * 4, "in_err"));
+  CHECK(check_dev(in_par, 6 * N * 4, "in_par"));
+  CHECK(check_dev(in_chg, N * 4, "in_chg"));
+  CHECK(check_dev(ms_rad, N * 4, "ms_rad"));
+  CHECK(check_dev(out_err, 21 * N * 4, "out_err"));
+  CHECK(check_dev(out_par, 6 * N * 4, "out_par"));
+  if (no_mat) CHECK(check_dev(no_mat, N * 4, "no_mat"));
+  if (out_fail) CHECK(check_dev(out_fail, N * 4, "out_fail"));
+  if ((pflags & MKF_PF_APPLY_MATERIAL) && !c->d_rl) return fail(MKF_ERR_INVALID_ARG, "no material grid");
+  hipLaunchKernelGGL(k_propagate_r, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, in_err, in_par,
+                     in_chg, ms_rad, out_err, out_par, pflags, no_mat, out_fail, c->bfield, mat_of(c));
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}
+
+int mkf_propagate_helix_to_z(mkf_context* c, int n, const float* in_err, const float* in_par, const int* in_chg,
+                             const float* ms_z, float* out_err, float* out_par, int pflags, const int* no_mat,
+                             void* stream) {
+  CHECK(set_device(c));
+  if (n < 0) return fail(MKF_ERR_INVALID_ARG, "n < 0");
+  if (n == 0) return MKF_OK;
+  const size_t N = n;
+  CHECK(check_dev(in_err, 21 * N * 4, "in_err"));
+  CHECK(check_dev(in_par, 6 * N * 4, "in_par"));
+  CHECK(check_dev(in_chg, N * 4, "in_chg"));
+  CHECK(check_dev(ms_z, N * 4, "ms_z"));
+  CHECK(check_dev(out_err, 21 * N * 4, "out_err"));
+  CHECK(check_dev(out_par, 6 * N * 4, "out_par"));
+  if (no_mat) CHECK(check_dev(no_mat, N * 4, "no_mat"));
+  if ((pflags & MKF_PF_APPLY_MATERIAL) && !c->d_rl) return fail(MKF_ERR_INVALID_ARG, "no material grid");
+  hipLaunchKernelGGL(k_propagate_z, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, n, in_err, in_par,
+                     in_chg, ms_z, out_err, out_par, pflags, no_mat, c->bfield, mat_of(c));
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}
+
+static int kalman_common(mkf_context* c, int op, int n, const float* ps_err, const float* ps_par,
+                         const float* ms_err, const float* ms_par, float* out_err, float* out_par, float* out_chi2,
+                         void* stream, bool endcap) {
+  CHECK(set_device(c));
+  if (n < 0) return fail(MKF_ERR_INVALID_ARG, "n < 0");
+  if (!(op & (MKF_KFO_CALCULATE_CHI2 | MKF_KFO_UPDATE_PARAMS)) || (op & ~3))
+    return fail(MKF_ERR_INVALID_ARG, "kf_op must be a nonzero combination of CALCULATE_CHI2 and UPDATE_PARAMS");
+  if (n == 0) return MKF_OK;
+  const size_t N = n;
+  CHECK(check_dev(ps_err, 21 * N * 4, "ps_err"));
+  CHECK(check_dev(ps_par, 6 * N * 4, "ps_par"));
+  CHECK(check_dev(ms_err, 6 * N * 4, "ms_err"));
+  CHECK(check_dev(ms_par, 3 * N * 4, "ms_par"));
+  if (op & MKF_KFO_UPDATE_PARAMS) {
+    CHECK(check_dev(out_err, 21 * N * 4, "out_err"));
+    CHECK(check_dev(out_par, 6 * N * 4, "out_par"));
+  }
+  if ((op & MKF_KFO_CALCULATE_CHI2) && out_chi2) CHECK(check_dev(out_chi2, N * 4, "out_chi2"));
+  if (endcap)
+    hipLaunchKernelGGL(k_kalman<true>, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, op, n, ps_err,
+                       ps_par, ms_err, ms_par, out_err, out_par, out_chi2);
+  else
+    hipLaunchKernelGGL(k_kalman<false>, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, op, n, ps_err,
+                       ps_par, ms_err, ms_par, out_err, out_par, out_chi2);
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}
+
+int mkf_kalman_operation(mkf_context* c, int op, int n, const float* ps_err, const float* ps_par,
+                         const float* ms_err, const float* ms_par, float* out_err, float* out_par, float* out_chi2,
+                         void* stream) {
+  return kalman_common(c, op, n, ps_err, ps_par, ms_err, ms_par, out_err, out_par, out_chi2, stream, false);
+}
+
+int mkf_kalman_operation_endcap(mkf_context* c, int op, int n, const float* ps_err, const float* ps_par,
+                                const float* ms_err, const float* ms_par, float* out_err, float* out_par,
+                                float* out_chi2, void* stream) {
+  return kalman_common(c, op, n, ps_err, ps_par, ms_err, ms_par, out_err, out_par, out_chi2, stream, true);
+}
+
+int mkf_fit_plain(mkf_context* c, const mkf_fit_batch* b, void* stream) {
+  CHECK(set_device(c));
+  FitArgs a;
+  CHECK(validate_batch(b, a, c));
+  if (a.n == 0) return MKF_OK;
+  const int launches = std::max(a.n_slots, a.seed_par ? 1 : 0);
+  for (int s = 0; s < launches; ++s) {
+    hipLaunchKernelGGL(k_fit_slot, dim3(grid_for(a.n, 256)), dim3(256), 0, (hipStream_t)stream, a, s);
+    HIP_TRY(hipGetLastError());
+  }
+  return MKF_OK;
+}
+
+int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
+  CHECK(set_device(c));
+  FitArgs a;
+  CHECK(validate_batch(b, a, c));
+  if (a.n == 0) return MKF_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  GraphKey key;
+  const void* ptrs[12] = {b->par,    b->err,       b->chg,      b->chi2,     b->hit_pos,  b->hit_err,
+                          b->n_hits, b->hit_flags, b->seed_par, b->seed_err, b->seed_chg, nullptr};
+  std::memcpy(key.ptrs, ptrs, sizeof(ptrs));
+  key.n = a.n; key.slots = a.n_slots; key.pflags = a.pflags; key.fflags = a.fit_flags;
+  key.block = c->shape.block; key.bpc = c->shape.blocks_per_cu; key.variant = c->shape.variant;
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal));
+    int rc = launch_fused(c, a, c->shape, c->capture_stream);
+    hipError_t e = hipStreamEndCapture(c->capture_stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(MKF_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(MKF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    it = c->graphs.emplace(key, ge).first;
+  }
+  HIP_TRY(hipGraphLaunch(it->second, (hipStream_t)stream));
+  return MKF_OK;
+}
+
+int mkf_set_batched_shape(mkf_context* c, const mkf_launch_shape* s) {
+  if (!c || !s) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  if (!shape_ok(*s)) return fail(MKF_ERR_INVALID_ARG, "invalid launch shape");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->shape = *s;
+  return MKF_OK;
+}
+
+int mkf_get_batched_shape(mkf_context* c, mkf_launch_shape* s) {
+  if (!c || !s) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  *s = c->shape;
+  return MKF_OK;
+}
+
+int mkf_autotune_batched(mkf_context* c, const mkf_fit_batch* b, float* scratch, int reps, mkf_launch_shape* best,
+                         float* best_ms) {
+  CHECK(set_device(c));
+  FitArgs a;
+  CHECK(validate_batch(b, a, c));
+  if (a.n == 0) return fail(MKF_ERR_INVALID_ARG, "empty batch");
+  if (reps < 1) reps = 1;
+  const size_t N = a.n;
+  CHECK(check_dev(scratch, 29 * N * 4, "scratch_state"));
+  hipStream_t st = c->capture_stream;
+  // Save the state the fit mutates: par, err, chi2, chg.
+  auto save = [&](bool restore) -> hipError_t {
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    float* sp = scratch;
+    hipError_t e;
+    if (restore) {
+      if ((e = hipMemcpyAsync(a.par, sp, 6 * N * 4, k, st))) return e;
+      if ((e = hipMemcpyAsync(a.err, sp + 6 * N, 21 * N * 4, k, st))) return e;
+      if ((e = hipMemcpyAsync(a.chg, sp + 27 * N, N * 4, k, st))) return e;
+      if (a.fit_flags & MKF_FIT_CHI2) if ((e = hipMemcpyAsync(a.chi2, sp + 28 * N, N * 4, k, st))) return e;
+    } else {
+      if ((e = hipMemcpyAsync(sp, a.par, 6 * N * 4, k, st))) return e;
+      if ((e = hipMemcpyAsync(sp + 6 * N, a.err, 21 * N * 4, k, st))) return e;
+      if ((e = hipMemcpyAsync(sp + 27 * N, a.chg, N * 4, k, st))) return e;
+      if (a.fit_flags & MKF_FIT_CHI2) if ((e = hipMemcpyAsync(sp + 28 * N, a.chi2, N * 4, k, st))) return e;
+    }
+    return hipSuccess;
+  };
+  HIP_TRY(save(false));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int blocks[] = {64, 128, 256, 512};
+  const int bpcs[] = {0, 1, 2, 4, 8, 16};
+  mkf_launch_shape bs = c->shape;
+  float bt = 1e30f;
+  int rc = MKF_OK;
+  for (int v = 0; v < kNumFusedVariants && !rc; ++v)
+    for (int bl : blocks)
+      for (int bp : bpcs) {
+        if (rc) break;
+        mkf_launch_shape sh{bl, bp, v};
+        float tot = 0.f;
+        for (int r = 0; r < reps + 1 && !rc; ++r) {  // first run is a warm-up
+          if (save(true) != hipSuccess) { rc = fail(MKF_ERR_HIP, "autotune restore failed"); break; }
+          (void)hipEventRecord(e0, st);
+          rc = launch_fused(c, a, sh, st);
+          (void)hipEventRecord(e1, st);
+          if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(MKF_ERR_HIP, "autotune sync failed"); break; }
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (r > 0) tot += ms;
+        }
+        if (!rc && tot / reps < bt) { bt = tot / reps; bs = sh; }
+      }
+  if (!rc && save(true) != hipSuccess) rc = fail(MKF_ERR_HIP, "autotune final restore failed");
+  (void)hipStreamSynchronize(st);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->shape = bs;
+  }
+  if (best) *best = bs;
+  if (best_ms) *best_ms = bt;
+  return MKF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,310 @@
+// oracle.cc -- C entry points over the oracle_core.h restatement.
+//
+// TEST INFRASTRUCTURE ONLY (checker and CPU baseline); the product library
+// never links it.
+//
+// The fit driver restates the reference's harness and fit loops:
+//   runFittingTestPlex            mkFit/fittestMPlex.cc:45-110  (blocks of NN,
+//                                 grain max(1, 32/NN) blocks per task)
+//   MkFitter::InputTracksForFit   mkFit/MkFitter.cc:286-321     (packing)
+//   MkFitter::FitTracksSteered    mkFit/MkFitter.cc:473-503     (barrel/endcap)
+//   MkFinder::BkFitFitTracks      mkFit/MkFinder.cc:1573-1742   (chi2 accumulation,
+//                                 missing-hit fake hit {666,0,666,0,0,666} at the
+//                                 current track position with no material, charge flip)
+//   MkBase::PropagateTracksToHit{R,Z} mkFit/MkBase.h:48-89      (target = hypot / z of hit)
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "oracle_core.h"
+
+using namespace orc;
+
+namespace {
+
+Ctx make_ctx(const orc_ctx* c) {
+  Ctx x;
+  x.bfield = c ? c->bfield : 3.8112f;
+  x.rl_grid = c ? c->rl_grid : nullptr;
+  x.xi_grid = c ? c->xi_grid : nullptr;
+  return x;
+}
+
+// Gather lanes [t0, t0+cnt) of an element-major batch array into a block;
+// padding lanes replicate lane t0 so every lane computes on valid numbers.
+template <class Td, class Ts>
+inline void pack(Td* blk, const Ts* src, int ne, int n, int t0, int cnt) {
+  for (int e = 0; e < ne; ++e)
+    for (int l = 0; l < NN; ++l) blk[e * NN + l] = Td(src[(size_t)e * n + t0 + (l < cnt ? l : 0)]);
+}
+template <class Td, class Ts>
+inline void unpack(Td* dst, const Ts* blk, int ne, int n, int t0, int cnt) {
+  for (int e = 0; e < ne; ++e)
+    for (int l = 0; l < cnt; ++l) dst[(size_t)e * n + t0 + l] = Td(blk[e * NN + l]);
+}
+
+template <class F>
+void for_each_block(int n, int n_threads, F&& body) {
+  const int nblk = (n + NN - 1) / NN;
+  const int grain = std::max(1, 32 / NN);                 // fittestMPlex.cc:67, Config::numSeedsPerTask
+  if (n_threads <= 1) {
+    for (int b = 0; b < nblk; ++b) body(b);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < n_threads; ++t)
+    pool.emplace_back([&]() {
+      for (;;) {
+        const int b0 = next.fetch_add(grain);
+        if (b0 >= nblk) break;
+        for (int b = b0; b < std::min(nblk, b0 + grain); ++b) body(b);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
+template <class T>
+int propagate_r_batch(const orc_ctx* c, int n, const T* in_err, const T* in_par, const int* in_chg, const T* ms_rad,
+                      T* out_err, T* out_par, int pflags, const int* no_mat, int* fail_out) {
+  const Ctx ctx = make_ctx(c);
+  for_each_block(n, 1, [&](int b) {
+    const int t0 = b * NN, cnt = std::min(NN, n - t0);
+    alignas(64) T ie[21 * NN], ip[6 * NN], oe[21 * NN], op[6 * NN], r[NN];
+    alignas(64) int ch[NN], nm[NN], fl[NN];
+    pack(ie, in_err, 21, n, t0, cnt);
+    pack(ip, in_par, 6, n, t0, cnt);
+    pack(r, ms_rad, 1, n, t0, cnt);
+    pack(ch, in_chg, 1, n, t0, cnt);
+    if (no_mat) pack(nm, no_mat, 1, n, t0, cnt);
+    propagate_helix_to_r(ie, ip, ch, r, oe, op, cnt, pflags, no_mat ? nm : nullptr, fl, ctx);
+    unpack(out_err, oe, 21, n, t0, cnt);
+    unpack(out_par, op, 6, n, t0, cnt);
+    if (fail_out) unpack(fail_out, fl, 1, n, t0, cnt);
+  });
+  return 0;
+}
+
+template <class T>
+int propagate_z_batch(const orc_ctx* c, int n, const T* in_err, const T* in_par, const int* in_chg, const T* ms_z,
+                      T* out_err, T* out_par, int pflags, const int* no_mat) {
+  const Ctx ctx = make_ctx(c);
+  for_each_block(n, 1, [&](int b) {
+    const int t0 = b * NN, cnt = std::min(NN, n - t0);
+    alignas(64) T ie[21 * NN], ip[6 * NN], oe[21 * NN], op[6 * NN], z[NN];
+    alignas(64) int ch[NN], nm[NN];
+    pack(ie, in_err, 21, n, t0, cnt);
+    pack(ip, in_par, 6, n, t0, cnt);
+    pack(z, ms_z, 1, n, t0, cnt);
+    pack(ch, in_chg, 1, n, t0, cnt);
+    if (no_mat) pack(nm, no_mat, 1, n, t0, cnt);
+    propagate_helix_to_z(ie, ip, ch, z, oe, op, cnt, pflags, no_mat ? nm : nullptr, ctx);
+    unpack(out_err, oe, 21, n, t0, cnt);
+    unpack(out_par, op, 6, n, t0, cnt);
+  });
+  return 0;
+}
+
+template <class T>
+int helix_r_batch(const orc_ctx* c, int n, const T* in_par, const int* in_chg, const T* ms_rad, T* out_par, T* out_jac,
+                  int* fail_out, int pflags) {
+  const Ctx ctx = make_ctx(c);
+  for_each_block(n, 1, [&](int b) {
+    const int t0 = b * NN, cnt = std::min(NN, n - t0);
+    alignas(64) T ip[6 * NN], op[6 * NN], r[NN], J[36 * NN];
+    alignas(64) int ch[NN], fl[NN];
+    pack(ip, in_par, 6, n, t0, cnt);
+    pack(r, ms_rad, 1, n, t0, cnt);
+    pack(ch, in_chg, 1, n, t0, cnt);
+    std::memcpy(op, ip, sizeof(op));
+    helix_at_r(ip, ch, r, op, J, fl, pflags, ctx);
+    unpack(out_par, op, 6, n, t0, cnt);
+    if (out_jac) unpack(out_jac, J, 36, n, t0, cnt);
+    if (fail_out) unpack(fail_out, fl, 1, n, t0, cnt);
+  });
+  return 0;
+}
+
+template <class T>
+int helix_z_batch(const orc_ctx* c, int n, const T* in_par, const int* in_chg, const T* ms_z, T* out_par, T* out_jac,
+                  int pflags) {
+  const Ctx ctx = make_ctx(c);
+  for_each_block(n, 1, [&](int b) {
+    const int t0 = b * NN, cnt = std::min(NN, n - t0);
+    alignas(64) T ip[6 * NN], op[6 * NN], z[NN], J[36 * NN];
+    alignas(64) int ch[NN];
+    pack(ip, in_par, 6, n, t0, cnt);
+    pack(z, ms_z, 1, n, t0, cnt);
+    pack(ch, in_chg, 1, n, t0, cnt);
+    std::memcpy(op, ip, sizeof(op));
+    helix_at_z(ip, ch, z, op, J, pflags, ctx);
+    unpack(out_par, op, 6, n, t0, cnt);
+    if (out_jac) unpack(out_jac, J, 36, n, t0, cnt);
+  });
+  return 0;
+}
+
+template <class T>
+int kalman_batch(int op, int endcap, int n, const T* ps_err, const T* ps_par, const T* ms_err, const T* ms_par,
+                 T* out_err, T* out_par, T* out_chi2) {
+  for_each_block(n, 1, [&](int b) {
+    const int t0 = b * NN, cnt = std::min(NN, n - t0);
+    alignas(64) T pe[21 * NN], pp[6 * NN], me[6 * NN], mp[3 * NN], oe[21 * NN], opar[6 * NN], c2[NN];
+    pack(pe, ps_err, 21, n, t0, cnt);
+    pack(pp, ps_par, 6, n, t0, cnt);
+    pack(me, ms_err, 6, n, t0, cnt);
+    pack(mp, ms_par, 3, n, t0, cnt);
+    if (endcap)
+      kalman_operation_endcap(op, pe, pp, me, mp, oe, opar, c2);
+    else
+      kalman_operation(op, pe, pp, me, mp, oe, opar, c2);
+    if (op & KFO_Update_Params) {
+      if (out_err) unpack(out_err, oe, 21, n, t0, cnt);
+      if (out_par) unpack(out_par, opar, 6, n, t0, cnt);
+    }
+    if ((op & KFO_Calculate_Chi2) && out_chi2) unpack(out_chi2, c2, 1, n, t0, cnt);
+  });
+  return 0;
+}
+
+// One NN block of the fit: BkFitFitTracks-style per-layer loop with per-lane
+// hit counts, per-lane barrel/endcap choice (FitTracksSteered generalised to a
+// per-lane pattern: both operations run where a block mixes them, each lane
+// keeps its own) and the fake-hit rule for missing layers.
+template <class T, class B>
+void fit_block(const Ctx& ctx, const B& bt, int t0, int cnt) {
+  const int n = bt.n_tracks;
+  alignas(64) T err[21 * NN], par[6 * NN], chi2[NN];
+  alignas(64) T errP[21 * NN], parP[6 * NN], errU[21 * NN], parU[6 * NN], c2[NN];
+  alignas(64) T errE[21 * NN], parE[6 * NN], c2E[NN];
+  alignas(64) T msPar[3 * NN], msErr[6 * NN], tgt[NN];
+  alignas(64) int chg[NN], nh[NN], nomat[NN], fail[NN];
+  unsigned char fl[NN];
+  pack(err, bt.err, 21, n, t0, cnt);
+  pack(par, bt.par, 6, n, t0, cnt);
+  pack(chi2, bt.chi2, 1, n, t0, cnt);
+  pack(chg, bt.chg, 1, n, t0, cnt);
+  int maxh = 0;
+  for (int l = 0; l < NN; ++l) {
+    nh[l] = l < cnt ? bt.n_hits[t0 + l] : 0;
+    maxh = std::max(maxh, nh[l]);
+  }
+  maxh = std::min(maxh, bt.n_slots);
+  for (int s = 0; s < maxh; ++s) {
+    const float* hp = bt.hit_pos + (size_t)s * 3 * n;
+    const float* he = bt.hit_err + (size_t)s * 6 * n;
+    bool any_b = false, any_e = false;
+    for (int l = 0; l < NN; ++l) {
+      const bool act = l < cnt && s < nh[l];
+      const int t = t0 + (l < cnt ? l : 0);
+      fl[l] = bt.hit_flags ? bt.hit_flags[(size_t)s * n + t] : 0;
+      if (!act) fl[l] |= 0x80;
+      if ((fl[l] & ORC_HIT_MISSING) || !act) {            // MkFinder.cc:1635-1647
+        nomat[l] = 1;
+        for (int i = 0; i < 3; ++i) msPar[i * NN + l] = par[i * NN + l];
+        const T fake[6] = {666, 0, 666, 0, 0, 666};
+        for (int i = 0; i < 6; ++i) msErr[i * NN + l] = fake[i];
+      } else {
+        nomat[l] = 0;
+        for (int i = 0; i < 3; ++i) msPar[i * NN + l] = T(hp[(size_t)i * n + t]);
+        for (int i = 0; i < 6; ++i) msErr[i * NN + l] = T(he[(size_t)i * n + t]);
+      }
+      if (act) {
+        if (fl[l] & ORC_HIT_ENDCAP) any_e = true; else any_b = true;
+      }
+    }
+    const int kfop = KFO_Calculate_Chi2 | KFO_Update_Params;
+    if (any_b) {
+      for (int l = 0; l < NN; ++l) tgt[l] = T(std::hypot(msPar[0 * NN + l], msPar[1 * NN + l]));   // MkBase.h:55
+      propagate_helix_to_r(err, par, chg, tgt, errP, parP, cnt, bt.pflags, nomat, fail, ctx);
+      kalman_operation(kfop, errP, parP, msErr, msPar, errU, parU, c2);
+    }
+    if (any_e) {
+      for (int l = 0; l < NN; ++l) tgt[l] = msPar[2 * NN + l];                                      // MkBase.h:84
+      propagate_helix_to_z(err, par, chg, tgt, errP, parP, cnt, bt.pflags, nomat, ctx);
+      kalman_operation_endcap(kfop, errP, parP, msErr, msPar, errE, parE, c2E);
+    }
+    for (int l = 0; l < cnt; ++l) {
+      if (fl[l] & 0x80) continue;
+      const bool ec = fl[l] & ORC_HIT_ENDCAP;
+      const T* se = ec ? errE : errU;
+      const T* sp = ec ? parE : parU;
+      for (int i = 0; i < 21; ++i) err[i * NN + l] = se[i * NN + l];
+      for (int i = 0; i < 6; ++i) par[i * NN + l] = sp[i * NN + l];
+      if (bt.fit_flags & ORC_FIT_CHI2) chi2[l] += ec ? c2E[l] : c2[l];                              // MkFinder.cc:1740
+      if ((bt.fit_flags & ORC_FIT_CHARGE_FLIP) && par[3 * NN + l] < 0) {                             // MkFinder.cc:1671-1678
+        chg[l] = -chg[l];
+        par[3 * NN + l] = -par[3 * NN + l];
+      }
+    }
+  }
+  unpack(bt.err, err, 21, n, t0, cnt);
+  unpack(bt.par, par, 6, n, t0, cnt);
+  unpack(bt.chi2, chi2, 1, n, t0, cnt);
+  unpack(bt.chg, chg, 1, n, t0, cnt);
+}
+
+template <class T, class B>
+int fit_batch(const orc_ctx* c, const B* b, int n_threads) {
+  if (!b || b->n_tracks < 0 || b->n_slots < 0) return 1;
+  const Ctx ctx = make_ctx(c);
+  for_each_block(b->n_tracks, n_threads, [&](int blk) {
+    const int t0 = blk * NN, cnt = std::min(NN, b->n_tracks - t0);
+    fit_block<T>(ctx, *b, t0, cnt);
+  });
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_simd_width(void) { return NN; }
+
+int orc_propagate_to_r_f32(const orc_ctx* c, int n, const float* ie, const float* ip, const int* ch, const float* r,
+                           float* oe, float* op, int pf, const int* nm, int* fo) {
+  return propagate_r_batch<float>(c, n, ie, ip, ch, r, oe, op, pf, nm, fo);
+}
+int orc_propagate_to_r_f64(const orc_ctx* c, int n, const double* ie, const double* ip, const int* ch,
+                           const double* r, double* oe, double* op, int pf, const int* nm, int* fo) {
+  return propagate_r_batch<double>(c, n, ie, ip, ch, r, oe, op, pf, nm, fo);
+}
+int orc_propagate_to_z_f32(const orc_ctx* c, int n, const float* ie, const float* ip, const int* ch, const float* z,
+                           float* oe, float* op, int pf, const int* nm) {
+  return propagate_z_batch<float>(c, n, ie, ip, ch, z, oe, op, pf, nm);
+}
+int orc_propagate_to_z_f64(const orc_ctx* c, int n, const double* ie, const double* ip, const int* ch,
+                           const double* z, double* oe, double* op, int pf, const int* nm) {
+  return propagate_z_batch<double>(c, n, ie, ip, ch, z, oe, op, pf, nm);
+}
+int orc_helix_at_r_f32(const orc_ctx* c, int n, const float* ip, const int* ch, const float* r, float* op, float* J,
+                       int* fo, int pf) {
+  return helix_r_batch<float>(c, n, ip, ch, r, op, J, fo, pf);
+}
+int orc_helix_at_r_f64(const orc_ctx* c, int n, const double* ip, const int* ch, const double* r, double* op,
+                       double* J, int* fo, int pf) {
+  return helix_r_batch<double>(c, n, ip, ch, r, op, J, fo, pf);
+}
+int orc_helix_at_z_f32(const orc_ctx* c, int n, const float* ip, const int* ch, const float* z, float* op, float* J,
+                       int pf) {
+  return helix_z_batch<float>(c, n, ip, ch, z, op, J, pf);
+}
+int orc_helix_at_z_f64(const orc_ctx* c, int n, const double* ip, const int* ch, const double* z, double* op,
+                       double* J, int pf) {
+  return helix_z_batch<double>(c, n, ip, ch, z, op, J, pf);
+}
+int orc_kalman_op_f32(int op, int ec, int n, const float* pe, const float* pp, const float* me, const float* mp,
+                      float* oe, float* opar, float* c2) {
+  return kalman_batch<float>(op, ec, n, pe, pp, me, mp, oe, opar, c2);
+}
+int orc_kalman_op_f64(int op, int ec, int n, const double* pe, const double* pp, const double* me, const double* mp,
+                      double* oe, double* opar, double* c2) {
+  return kalman_batch<double>(op, ec, n, pe, pp, me, mp, oe, opar, c2);
+}
+int orc_fit_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int nt) { return fit_batch<float>(c, b, nt); }
+int orc_fit_f64(const orc_ctx* c, const orc_fit_batch_f64* b, int nt) { return fit_batch<double>(c, b, nt); }
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+"""Shared helpers for the parity tests: error metrics, tolerances, cached events.
+
+Error metrics (used by every parity test):
+  parameters  |a-b| / max(|b|, floor), floors x,y,z: 1 cm; 1/pT: 0.1 /GeV;
+              phi, theta: 1 rad -- a relative error against a near-zero
+              coordinate (x of a track at phi=90 deg) is not meaningful.
+  covariance  |a_ij-b_ij| / sqrt(b_ii b_jj), the element's error relative to
+              the correlation scale of its row and column.
+  chi2        |a-b| / max(|b|, 1).
+"""
+import functools
+
+import numpy as np
+
+PAR_FLOOR = np.array([1.0, 1.0, 1.0, 0.1, 1.0, 1.0])
+SYM_DIAG = [0, 2, 5, 9, 14, 20]
+SYM_IJ = [(i, j) for i in range(6) for j in range(i + 1)]
+
+# Tolerances stated by the hot-path parity contract.
+TOL_PAR = 1e-5
+TOL_COV = 1e-4
+TOL_CHI2 = 1e-4
+
+
+def par_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    fl = PAR_FLOOR.reshape((6,) + (1,) * (b.ndim - 1))
+    d = np.abs(a - b)
+    if b.ndim >= 1:
+        d4 = np.abs(a[4] - b[4])
+        d[4] = np.minimum(d4, np.abs(d4 - 2 * np.pi))  # phi wraps at +-pi
+    return d / np.maximum(np.abs(b), fl)
+
+
+def cov_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    out = np.empty_like(b)
+    for k, (i, j) in enumerate(SYM_IJ):
+        s = np.sqrt(np.abs(b[SYM_DIAG[i]] * b[SYM_DIAG[j]]))
+        out[k] = np.abs(a[k] - b[k]) / np.maximum(s, 1e-30)
+    return out
+
+
+def chi2_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+
+
+def sym_full(c):
+    """[21, n] packed -> [n, 6, 6]."""
+    c = np.asarray(c)
+    n = c.shape[1]
+    m = np.zeros((n, 6, 6), c.dtype)
+    for k, (i, j) in enumerate(SYM_IJ):
+        m[:, i, j] = c[k]
+        m[:, j, i] = c[k]
+    return m
+
+
+def sym_pack(m):
+    """[n, 6, 6] -> [21, n]."""
+    return np.stack([m[:, i, j] for (i, j) in SYM_IJ])
+
+
+@functools.lru_cache(maxsize=None)
+def event(preset, n_tracks, seed, simulate_material=1, xi_scale=1.0, hits_min=0, hits_max=0, missing_frac=0.0,
+          eta_max=None):
+    """Cached toy-MC sample: (geometry, params, event dict, rl_grid, xi_grid)."""
+    import mkfit_amd as mk
+    geo = mk.geometry(preset)
+    for lyr in range(geo.n_barrel + 2 * geo.n_disk):
+        geo.xi[lyr] *= xi_scale
+    rl, xi = mk.material_grid(geo)
+    kw = dict(seed=seed, simulate_material=simulate_material, hits_min=hits_min, hits_max=hits_max,
+              missing_frac=missing_frac)
+    if eta_max is not None:
+        kw["eta_max"] = eta_max
+    elif preset == 3:
+        kw["eta_max"] = 2.5
+    p = mk.default_toy_params(**kw)
+    n_slots = 30 if preset == 3 else geo.n_barrel
+    if hits_max:
+        n_slots = max(n_slots, hits_max)
+    ev = mk.generate(geo, p, n_tracks, n_slots, truth=True)
+    return geo, p, ev, rl, xi
+
+
+def summary(name, rel):
+    rel = np.asarray(rel)
+    return "%s: max %.3g p99.9 %.3g p99 %.3g" % (name, rel.max(), np.percentile(rel, 99.9), np.percentile(rel, 99))
