@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--shape", type=str, default=None,
+                    help="block,blocks_per_cu,variant for the batched kernel (skips the autotuner)")
     return ap.parse_args()
 
 
@@ -86,7 +88,10 @@ def main():
     stream = torch.cuda.current_stream()
 
     shape, tuned_ms = None, None
-    if not args.no_autotune:
+    if args.shape:
+        b_, p_, v_ = (int(x) for x in args.shape.split(","))
+        ctx.set_batched_shape(b_, p_, v_)
+    elif not args.no_autotune:
         shape, tuned_ms = ctx.autotune_batched(batch, reps=2)
         log("autotuned batched shape %s: %.3f ms" % (shape, tuned_ms))
     shape = ctx.get_batched_shape()

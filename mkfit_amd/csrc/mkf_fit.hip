@@ -6,17 +6,18 @@
 // [slot][element][track].
 //
 // Two fit forms share the same per-lane step (mkf_track_math.h):
-//   plain   -- k_fit_slot, one launch per hit slot; the 27-float state makes a
+//   plain   -- k_fit launched once per hit slot; the 27-float state makes a
 //              round trip through HBM per slot (~270 B per track-slot).
-//   batched -- k_fit_fused, a persistent grid-stride kernel that loads the
-//              state once, walks all of the track's slots with the state in
-//              VGPRs and streams only the hits (~37 B per track-slot).  It is
+//   batched -- k_fit launched once over all slots on a persistent grid: the
+//              state is loaded once, stays in VGPRs across the track's slots,
+//              and only the hits stream (~37 B per track-slot).  It is
 //              launched from a cached hipGraph; launch shape and register
 //              budget come from mkf_autotune_batched.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -196,6 +197,33 @@ k_kalman(int op, int n, const float* __restrict__ ps_err, const float* __restric
 // Fit kernels
 // ============================================================================
 
+// Buffer addressing for the fit kernel.  Every array gets one buffer resource
+// (base, byte size) held in SGPRs; element e of track t is read with the
+// per-lane byte offset t*4 in a VGPR and the element stride e*n*4 in an SGPR
+// soffset, so no per-lane address arithmetic is spent on the 27 state and 9
+// hit elements.  The byte size doubles as a hardware bounds check: an access
+// past it returns 0 instead of touching other memory.
+constexpr int kBufFlags = 0x00020000;  // gfx9-family raw buffer: 32-bit dword data
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_buf(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, kBufFlags);
+}
+__device__ __forceinline__ float bld_f(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ int bld_i(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ unsigned bld_u8(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  return (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ void bst_f(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ void bst_i(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, int v) {
+  __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)vo, (int)so, 0);
+}
+
 struct FitArgs {
   int n, n_slots;
   float* __restrict__ par;
@@ -217,25 +245,48 @@ struct FitArgs {
 // One hit slot of one track: the body of the reference's per-layer loop
 // (MkFinder::BkFitFitTracks, MkFinder.cc:1612-1740; FitTracksSteered,
 // MkFitter.cc:477-500).
+// Buffer resources of one fit launch (uniform across the grid).
+struct FitBufs {
+  __amdgpu_buffer_rsrc_t par, err, chg, chi2, hit_pos, hit_err, n_hits, hit_flags, seed_par, seed_err, seed_chg;
+  uint32_t nb;  // n * 4: byte stride between elements
+};
+
+__device__ __forceinline__ FitBufs make_bufs(const FitArgs& a) {
+  FitBufs b;
+  const uint32_t n = (uint32_t)a.n, nb = n * 4u, S = (uint32_t)a.n_slots;
+  b.nb = nb;
+  b.par = make_buf(a.par, 6u * nb);
+  b.err = make_buf(a.err, 21u * nb);
+  b.chg = make_buf(a.chg, nb);
+  b.chi2 = make_buf(a.chi2, a.chi2 ? nb : 0u);
+  b.hit_pos = make_buf(a.hit_pos, 3u * nb * S);
+  b.hit_err = make_buf(a.hit_err, 6u * nb * S);
+  b.n_hits = make_buf(a.n_hits, nb);
+  b.hit_flags = make_buf(a.hit_flags, a.hit_flags ? n * S : 0u);
+  b.seed_par = make_buf(a.seed_par, a.seed_par ? 6u * nb : 0u);
+  b.seed_err = make_buf(a.seed_err, a.seed_par ? 21u * nb : 0u);
+  b.seed_chg = make_buf(a.seed_chg, a.seed_par ? nb : 0u);
+  return b;
+}
+
 __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int& chg, float& chi2, const FitArgs& a,
-                                         int s, int t) {
-  const unsigned flags = a.hit_flags ? a.hit_flags[(size_t)s * a.n + t] : 0u;
+                                         const FitBufs& b, int s, uint32_t tb) {
+  const unsigned flags = a.hit_flags ? bld_u8(b.hit_flags, tb >> 2, (uint32_t)s * (uint32_t)a.n) : 0u;
   const bool missing = flags & MKF_HIT_MISSING;
   float mp[3], me[6];
   if (missing) {
     mp[0] = par[0]; mp[1] = par[1]; mp[2] = par[2];
     me[0] = 666.f; me[1] = 0.f; me[2] = 666.f; me[3] = 0.f; me[4] = 0.f; me[5] = 666.f;
   } else {
-    const float* hp = a.hit_pos + (size_t)s * 3 * a.n;
-    const float* he = a.hit_err + (size_t)s * 6 * a.n;
+    const uint32_t sp = (uint32_t)s * 3u * b.nb, se = (uint32_t)s * 6u * b.nb;
 #pragma unroll
-    for (int e = 0; e < 3; ++e) mp[e] = hp[e * a.n + t];
+    for (int e = 0; e < 3; ++e) mp[e] = bld_f(b.hit_pos, tb, sp + (uint32_t)e * b.nb);
 #pragma unroll
-    for (int e = 0; e < 6; ++e) me[e] = he[e * a.n + t];
+    for (int e = 0; e < 6; ++e) me[e] = bld_f(b.hit_err, tb, se + (uint32_t)e * b.nb);
   }
   float c2;
   if (!(flags & MKF_HIT_ENDCAP)) {
-    const float r = hypotf(mp[0], mp[1]);                                  // MkBase.h:55
+    const float r = fhypot(mp[0], mp[1]);                                  // MkBase.h:55
     propagate_to_r(par, err, chg, r, a.pflags, missing, a.bfield, a.mat);
     c2 = kalman_barrel<true, true>(par, err, mp, me);
   } else {
@@ -250,62 +301,63 @@ __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int&
 }
 
 // from_seed: read the seed arrays (first touch of an out-of-place fit).
-__device__ __forceinline__ void load_state(const FitArgs& a, int t, bool from_seed, float (&par)[6], float (&err)[21],
-                                           int& chg, float& chi2) {
-  const float* P = from_seed ? a.seed_par : a.par;
-  const float* E = from_seed ? a.seed_err : a.err;
+__device__ __forceinline__ void load_state(const FitArgs& a, const FitBufs& b, uint32_t tb, bool from_seed,
+                                           float (&par)[6], float (&err)[21], int& chg, float& chi2) {
+  const __amdgpu_buffer_rsrc_t P = from_seed ? b.seed_par : b.par;
+  const __amdgpu_buffer_rsrc_t E = from_seed ? b.seed_err : b.err;
 #pragma unroll
-  for (int e = 0; e < 6; ++e) par[e] = P[e * a.n + t];
+  for (int e = 0; e < 6; ++e) par[e] = bld_f(P, tb, (uint32_t)e * b.nb);
 #pragma unroll
-  for (int e = 0; e < 21; ++e) err[e] = E[e * a.n + t];
-  chg = from_seed ? a.seed_chg[t] : a.chg[t];
-  chi2 = ((a.fit_flags & MKF_FIT_CHI2) && !from_seed) ? a.chi2[t] : 0.f;
+  for (int e = 0; e < 21; ++e) err[e] = bld_f(E, tb, (uint32_t)e * b.nb);
+  chg = bld_i(from_seed ? b.seed_chg : b.chg, tb, 0);
+  chi2 = ((a.fit_flags & MKF_FIT_CHI2) && !from_seed) ? bld_f(b.chi2, tb, 0) : 0.f;
 }
 
-__device__ __forceinline__ void store_state(const FitArgs& a, int t, const float (&par)[6], const float (&err)[21],
-                                            int chg, float chi2) {
+__device__ __forceinline__ void store_state(const FitArgs& a, const FitBufs& b, uint32_t tb, const float (&par)[6],
+                                            const float (&err)[21], int chg, float chi2) {
 #pragma unroll
-  for (int e = 0; e < 6; ++e) a.par[e * a.n + t] = par[e];
+  for (int e = 0; e < 6; ++e) bst_f(b.par, tb, (uint32_t)e * b.nb, par[e]);
 #pragma unroll
-  for (int e = 0; e < 21; ++e) a.err[e * a.n + t] = err[e];
-  if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) || a.seed_par) a.chg[t] = chg;
-  if (a.fit_flags & MKF_FIT_CHI2) a.chi2[t] = chi2;
+  for (int e = 0; e < 21; ++e) bst_f(b.err, tb, (uint32_t)e * b.nb, err[e]);
+  if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) || a.seed_par) bst_i(b.chg, tb, 0, chg);
+  if (a.fit_flags & MKF_FIT_CHI2) bst_f(b.chi2, tb, 0, chi2);
 }
 
-// Plain form: one launch per slot s.
-// Plain form: one launch per slot s.  With a seed input, the slot-0 launch
-// also carries tracks without hits over to the output.
-__global__ void __launch_bounds__(256) k_fit_slot(FitArgs a, int s) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.n) return;
-  const bool from_seed = (s == 0) && a.seed_par != nullptr;
-  const bool active = s < min(a.n_hits[t], a.n_slots);
-  if (!active && !from_seed) return;
-  float par[6], err[21], chi2;
-  int chg;
-  load_state(a, t, from_seed, par, err, chg, chi2);
-  if (active) fit_step(par, err, chg, chi2, a, s, t);
-  store_state(a, t, par, err, chg, chi2);
-}
-
-// Batched form: persistent grid-stride loop, all slots of a track in registers.
-// MINW = minimum waves per SIMD requested from the register allocator.
-template <int MINW>
+// One kernel body serves both fit forms, so they execute identical machine
+// code and agree bit for bit:
+//   plain   -- launched once per slot s with [s_begin, s_end) = [s, s+1) and one
+//              thread per track; the state round-trips through HBM per launch.
+//   batched -- launched once with [0, n_slots) on a persistent grid; each
+//              thread keeps its track in VGPRs across all of its slots.
+// With a seed input, the launch that starts at slot 0 reads the seeds and also
+// carries tracks without hits over to the output.  MINW = minimum waves per
+// SIMD requested from the register allocator (the autotuner's occupancy knob).
+// FORM only names the instantiation (0 = plain, 1 = batched) so profiles keep
+// the two forms apart; the body does not depend on it.
+template <int MINW, int FORM>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 512), amdgpu_waves_per_eu(MINW, 8)))
-k_fit_fused(FitArgs a) {
+k_fit(FitArgs a, int s_begin, int s_end) {
   const int stride = gridDim.x * blockDim.x;
+  const bool from_seed = s_begin == 0 && a.seed_par != nullptr;
+  const FitBufs b = make_bufs(a);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += stride) {
+    const uint32_t tb = (uint32_t)t * 4u;  // lane byte offset in a 4-byte element array
+    const int s1 = min(min(bld_i(b.n_hits, tb, 0), a.n_slots), s_end);
+    if (s_begin >= s1 && !from_seed) continue;
     float par[6], err[21], chi2;
     int chg;
-    load_state(a, t, a.seed_par != nullptr, par, err, chg, chi2);
-    const int nh = min(a.n_hits[t], a.n_slots);
-    for (int s = 0; s < nh; ++s) fit_step(par, err, chg, chi2, a, s, t);
-    store_state(a, t, par, err, chg, chi2);
+    load_state(a, b, tb, from_seed, par, err, chg, chi2);
+    for (int s = s_begin; s < s1; ++s) fit_step(par, err, chg, chi2, a, b, s, tb);
+    store_state(a, b, tb, par, err, chg, chi2);
   }
 }
 
-using FusedKernel = void (*)(FitArgs);
-static const FusedKernel kFusedVariants[] = {k_fit_fused<1>, k_fit_fused<2>, k_fit_fused<4>, k_fit_fused<6>};
+using FitKernel = void (*)(FitArgs, int, int);
+// Register budgets above 128 VGPRs/lane (4 waves) make the compiler demote the
+// private arrays to scratch, which changes FMA contraction and hence the bits;
+// the variants stop there so every choice gives identical results.
+static const FitKernel kFusedVariants[] = {k_fit<1, 1>, k_fit<2, 1>, k_fit<4, 1>};
+static const FitKernel kPlainKernel = k_fit<1, 0>;
 static constexpr int kNumFusedVariants = sizeof(kFusedVariants) / sizeof(kFusedVariants[0]);
 
 // ============================================================================
@@ -318,8 +370,9 @@ int validate_batch(const mkf_fit_batch* b, FitArgs& a, const mkf_context* c) {
   if (!b) return fail(MKF_ERR_INVALID_ARG, "batch is NULL");
   if (b->n_tracks < 0 || b->n_slots < 0) return fail(MKF_ERR_INVALID_ARG, "negative batch size");
   const size_t n = (size_t)b->n_tracks, S = (size_t)b->n_slots;
-  if (n * (S > 0 ? S : 1) * 6 > (size_t)0x7fffffff)
-    return fail(MKF_ERR_INVALID_ARG, "batch too large for 32-bit element offsets");
+  // Buffer addressing: every array of the batch must stay below 4 GiB.
+  if (21 * n * 4 >= ((size_t)1 << 32) || 6 * n * 4 * (S > 0 ? S : 1) >= ((size_t)1 << 32))
+    return fail(MKF_ERR_INVALID_ARG, "batch too large: split it so every array stays below 4 GiB");
   a.n = b->n_tracks;
   a.n_slots = b->n_slots;
   a.par = b->par;
@@ -367,7 +420,7 @@ int launch_fused(mkf_context* c, const FitArgs& a, const mkf_launch_shape& sh, h
   const int block = sh.block;
   int grid = sh.blocks_per_cu > 0 ? sh.blocks_per_cu * c->num_cus : grid_for(a.n, block);
   grid = std::max(1, std::min(grid, grid_for(a.n, block)));
-  hipLaunchKernelGGL(kFusedVariants[sh.variant], dim3(grid), dim3(block), 0, st, a);
+  hipLaunchKernelGGL(kFusedVariants[sh.variant], dim3(grid), dim3(block), 0, st, a, 0, a.n_slots);
   HIP_TRY(hipGetLastError());
   return MKF_OK;
 }
@@ -519,7 +572,7 @@ int mkf_fit_plain(mkf_context* c, const mkf_fit_batch* b, void* stream) {
   if (a.n == 0) return MKF_OK;
   const int launches = std::max(a.n_slots, a.seed_par ? 1 : 0);
   for (int s = 0; s < launches; ++s) {
-    hipLaunchKernelGGL(k_fit_slot, dim3(grid_for(a.n, 256)), dim3(256), 0, (hipStream_t)stream, a, s);
+    hipLaunchKernelGGL(kPlainKernel, dim3(grid_for(a.n, 256)), dim3(256), 0, (hipStream_t)stream, a, s, s + 1);
     HIP_TRY(hipGetLastError());
   }
   return MKF_OK;

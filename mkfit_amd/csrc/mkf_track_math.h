@@ -41,7 +41,29 @@ struct Material {
 
 __host__ __device__ constexpr int S(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
-__device__ __forceinline__ float hipo(float x, float y) { return sqrtf(x * x + y * y); }
+// FP32 divide, reciprocal and square root as hardware approximations refined by
+// one FMA residual step: for operands in the normal range (all the fit's
+// operands) the results are the correctly rounded ones or within one ulp, at
+// 4 instructions instead of the ~10 of the IEEE sequences with full
+// special-case handling.
+__device__ __forceinline__ float frcp(float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  return __builtin_fmaf(__builtin_fmaf(-b, r, 1.f), r, r);
+}
+__device__ __forceinline__ float fdiv(float a, float b) {
+  const float r = __builtin_amdgcn_rcpf(b);
+  const float q = a * r;
+  return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+}
+__device__ __forceinline__ float fsqrt(float x) {
+  const float y = __builtin_amdgcn_sqrtf(x);
+  return __builtin_fmaf(__builtin_fmaf(-y, y, x), 0.5f * __builtin_amdgcn_rcpf(y), y);
+}
+
+__device__ __forceinline__ float hipo(float x, float y) { return fsqrt(x * x + y * y); }
+// std::hypot of the reference (MkBase.h:55, KalmanUtilsMPlex.cc:585): the sum of
+// squares with one rounding, then the refined square root.
+__device__ __forceinline__ float fhypot(float x, float y) { return fsqrt(__builtin_fmaf(x, x, y * y)); }
 
 __device__ __forceinline__ void sincos4(float x, float& s, float& c) {   // Matrix.h:48-56
   const float x2 = x * x;
@@ -80,17 +102,17 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
   const float ipt = par[3], phiin = par[4], theta = par[5];
   float r0 = hipo(xin, yin);
   const float bf = param_b ? bfield_from_zr(zin, r0) : bfield;
-  const float k = float(chg) * 100.f / (-kSol * bf);
-  const float kinv = 1.f / k;
-  const float pt = 1.f / ipt;
+  const float k = fdiv(float(chg) * 100.f, -kSol * bf);
+  const float kinv = frcp(k);
+  const float pt = frcp(ipt);
 
   float D = 0.f, cosa = 0.f, sina = 0.f, cosah = 0.f, sinah = 0.f, id = 0.f;
   float cosPorT = cosf(phiin), sinPorT = sinf(phiin);
   float pxin = cosPorT * pt;
   float pyin = sinPorT * pt;
 
-  float dDdx = r0 > 0.f ? -xin / r0 : 0.f;
-  float dDdy = r0 > 0.f ? -yin / r0 : 0.f;
+  float dDdx = r0 > 0.f ? fdiv(-xin, r0) : 0.f;
+  float dDdy = r0 > 0.f ? fdiv(-yin, r0) : 0.f;
   float dDdipt = 0.f;
   float dDdphi = 0.f;
   bool fail = false;
@@ -99,7 +121,7 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
 #pragma unroll
   for (int i = 0; i < kNiter; ++i) {
     r0 = hipo(x, y);
-    const float oodotp = r0 * pt / (pxin * x + pyin * y);
+    const float oodotp = fdiv(r0 * pt, pxin * x + pyin * y);
     if (oodotp > 5.0f || oodotp < 0) {
       id = 0.0f;
       fail = true;
@@ -113,7 +135,7 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
     sina = 2.f * sinah * cosah;
 
     if (i + 1 != kNiter) {
-      const float oor0 = (r0 > 0.f && fabsf(r - r0) < 0.0001f) ? 1.f / r0 : 0.f;
+      const float oor0 = (r0 > 0.f && fabsf(r - r0) < 0.0001f) ? frcp(r0) : 0.f;
       const float dadipt = id * kinv;
       const float dadx = -x * ipt * kinv * oor0;
       const float dady = -y * ipt * kinv * oor0;
@@ -158,7 +180,7 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
 
   cosPorT = cosf(theta);
   sinPorT = sinf(theta);
-  sinPorT = 1.f / sinPorT;
+  sinPorT = frcp(sinPorT);
 
   par[0] = x;
   par[1] = y;
@@ -180,13 +202,13 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
 __device__ __forceinline__ void helix_at_z(float (&par)[6], int chg, float zout, float bfield, bool param_b, JacZ& J) {
   const float zin = par[2], ipt = par[3], phiin = par[4], theta = par[5];
   const float bf = param_b ? bfield_from_zr(zin, hipo(par[0], par[1])) : bfield;
-  const float k = float(chg) * 100.f / (-kSol * bf);
-  const float kinv = 1.f / k;
-  const float pt = 1.f / ipt;
+  const float k = fdiv(float(chg) * 100.f, -kSol * bf);
+  const float kinv = frcp(k);
+  const float pt = frcp(ipt);
   const float cosP = cosf(phiin), sinP = sinf(phiin);
   const float cosT = cosf(theta), sinT = sinf(theta);
-  const float tanT = sinT / cosT;
-  const float icos2T = 1.f / (cosT * cosT);
+  const float tanT = fdiv(sinT, cosT);
+  const float icos2T = frcp(cosT * cosT);
   const float pxin = cosP * pt;
   const float pyin = sinP * pt;
   const float deltaZ = zout - zin;
@@ -221,37 +243,37 @@ __device__ __forceinline__ void material_effects(float radL, float xi, float pro
                                                  float (&err)[21], bool is_barrel) {
   if (radL < 1e-13f) return;
   const float theta = par[5];
-  const float pt = 1.f / par[3];
-  const float p = pt / sinf(theta);
+  const float pt = frcp(par[3]);
+  const float p = fdiv(pt, sinf(theta));
   const float p2 = p * p;
   constexpr float mpi = 0.140;
   constexpr float mpi2 = mpi * mpi;
-  const float beta2 = p2 / (p2 + mpi2);
-  const float beta = sqrtf(beta2);
-  const float invCos = is_barrel ? p / pt : 1.f / fabsf(cosf(theta));
+  const float beta2 = fdiv(p2, p2 + mpi2);
+  const float beta = fsqrt(beta2);
+  const float invCos = is_barrel ? fdiv(p, pt) : frcp(fabsf(cosf(theta)));
   radL = radL * invCos;
   if (radL < 1e-13f) return;
-  const float thetaMSC = 0.0136f * (1.f + 0.038f * logf(radL)) / (beta * p);
+  const float thetaMSC = fdiv(0.0136f * (1.f + 0.038f * logf(radL)), beta * p);
   const float thetaMSC2 = thetaMSC * thetaMSC * radL;
   err[S(4, 4)] += thetaMSC2;
   err[S(5, 5)] += thetaMSC2;
-  const float gamma2 = (p2 + mpi2) / mpi2;
-  const float gamma = sqrtf(gamma2);
+  const float gamma2 = fdiv(p2 + mpi2, mpi2);
+  const float gamma = fsqrt(gamma2);
   constexpr float me = 0.0005;
-  const float wmax = 2.f * me * beta2 * gamma2 / (1.f + 2.f * gamma * me / mpi + me * me / (mpi * mpi));
+  const float wmax = fdiv(2.f * me * beta2 * gamma2, 1.f + fdiv(2.f * gamma * me, mpi) + me * me / (mpi * mpi));
   constexpr float I = 16.0e-9 * 10.75;
-  const float deltahalf = logf(28.816e-9f * sqrtf(2.33f * 0.498f) / I) + logf(beta * gamma) - 0.5f;
+  const float deltahalf = logf(28.816e-9f * sqrtf(2.33f * 0.498f) / I) + logf(beta * gamma) - 0.5f;  // first term folds to a constant
   const float dEdx = beta < 1.f
-                         ? (2.f * (xi * invCos * (0.5f * logf(2.f * me * beta2 * gamma2 * wmax / (I * I)) - beta2 - deltahalf) / beta2))
+                         ? (2.f * fdiv(xi * invCos * (0.5f * logf(fdiv(2.f * me * beta2 * gamma2 * wmax, I * I)) - beta2 - deltahalf), beta2))
                          : 0.f;
-  const float dP = propSign * dEdx / beta;
-  par[3] = p / (fmaxf(p + dP, 0.001f) * pt);
-  err[S(3, 3)] += dP * dP / (p2 * pt * pt);
+  const float dP = fdiv(propSign * dEdx, beta);
+  par[3] = fdiv(p, fmaxf(p + dP, 0.001f) * pt);
+  err[S(3, 3)] += fdiv(dP * dP, p2 * pt * pt);
 }
 
 __device__ __forceinline__ void lookup_material(const Material& m, float zq, float rq, float& rl, float& xi) {
-  const int zb = (int)((fabsf(zq) * kNBinsZ) / (kRangeZ));   // MaterialEffects.h:8-9
-  const int rb = (int)((rq * kNBinsR) / (kRangeR));
+  const int zb = (int)fdiv(fabsf(zq) * kNBinsZ, kRangeZ);   // MaterialEffects.h:8-9
+  const int rb = (int)fdiv(rq * kNBinsR, kRangeR);
   if (m.rl && zb >= 0 && zb < kNBinsZ && rb >= 0 && rb < kNBinsR) {
     rl = m.rl[zb * kNBinsR + rb];
     xi = m.xi[zb * kNBinsR + rb];
@@ -363,7 +385,7 @@ __device__ __forceinline__ void propagate_to_z(float (&par)[6], float (&err)[21]
   helix_at_z(par, chg, zout, bfield, pflags & PF_PARAM_B, J);
   if (pflags & PF_MATERIAL) {
     float rl = 0.f, xi = 0.f;
-    if (!no_mat) lookup_material(mat, zout, hypotf(par[0], par[1]), rl, xi);
+    if (!no_mat) lookup_material(mat, zout, fhypot(par[0], par[1]), rl, xi);
     const float propSign = fabsf(zout) > fabsf(zin) ? 1.f : -1.f;
     material_effects(rl, xi, propSign, par, err, false);
   }
@@ -374,7 +396,11 @@ __device__ __forceinline__ void propagate_to_z(float (&par)[6], float (&err)[21]
 // ---- 2x2 inverse with a double determinant ----------------------------------
 __device__ __forceinline__ void invert_sym2(float& a0, float& a1, float& a2) {
   const double det = (double)a0 * a2 - (double)a1 * a1;
-  const float s = (float)(1.0 / det);
+  // 1/det to well below float resolution: hardware FP64 reciprocal refined by
+  // one Newton step, then rounded to float as the reference's TT(1)/det is.
+  double rd = __builtin_amdgcn_rcp(det);
+  rd = __builtin_fma(__builtin_fma(-det, rd, 1.0), rd, rd);
+  const float s = (float)rd;
   const float tmp = s * a2;
   a1 *= -s;
   a2 = s * a0;
@@ -386,9 +412,9 @@ __device__ __forceinline__ void invert_sym2(float& a0, float& a1, float& a2) {
 template <bool CHI2, bool UPDATE>
 __device__ __forceinline__ float kalman_barrel(float (&par)[6], float (&err)[21], const float (&mp)[3],
                                                const float (&me)[6]) {
-  const float r = hypotf(mp[0], mp[1]);
-  const float a00 = -(mp[1] + par[1]) / (2 * r);
-  const float a01 = (mp[0] + par[0]) / (2 * r);
+  const float r = fhypot(mp[0], mp[1]);
+  const float a00 = fdiv(-(mp[1] + par[1]), 2 * r);
+  const float a01 = fdiv(mp[0] + par[0], 2 * r);
   const float rg0 = mp[0] - par[0], rg1 = mp[1] - par[1], rg2 = mp[2] - par[2];
   float b[6];
 #pragma unroll

@@ -187,9 +187,12 @@ void gen_track(const mkf_geometry& g, const mkf_toymc_params& p, const mkf_toymc
       if (s.pz == 0) continue;
       const double l = (zd - s.z) * ptc / s.pz;
       if (l <= 1e-9) continue;
+      // only on the outgoing half-turn: no curlers reaching a disk after looping
+      if (std::fabs(kappa * l) >= kPi) continue;
       const State a = advance(s, l, kappa);
       const double ra = std::hypot(a.x, a.y);
       if (ra < g.disk_rmin[i % nd] || ra > g.disk_rmax[i % nd]) continue;
+      if (a.x * a.px + a.y * a.py < kMinCosCrossing * ra * std::hypot(a.px, a.py)) continue;
       if (best < 0 || l < best) { best = l; bl = lid; }
     }
     if (bl < 0) break;

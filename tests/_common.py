@@ -91,3 +91,35 @@ def event(preset, n_tracks, seed, simulate_material=1, xi_scale=1.0, hits_min=0,
 def summary(name, rel):
     rel = np.asarray(rel)
     return "%s: max %.3g p99.9 %.3g p99 %.3g" % (name, rel.max(), np.percentile(rel, 99.9), np.percentile(rel, 99))
+
+
+def assert_parity(name, gpu, cpu32, cpu64, kind, tol, frac=0.999):
+    """Parity of a GPU result with the FP32 oracle, bounded by FP32 itself.
+
+    kind: "par" [6, n], "cov" [21, n] or "chi2" [n].  Requires
+      * at least `frac` of the (element, track) entries within `tol` of the
+        FP32 oracle (the reference's CPU semantics), and
+      * at the 50/90/99/99.9th percentiles and the maximum, the GPU-to-FP32
+        distance no larger than three times the FP32 oracle's own distance to the
+        FP64 oracle (plus tol/10): entries outside `tol` must be FP32
+        conditioning that the reference's CPU path shares, not a GPU defect.
+        (For scale: the CPU path against itself built without FMA contraction
+        keeps 99.8% of fitted-parameter entries within 1e-5 after 17 layers,
+        with a maximum of 1.6e-2.)
+    Returns a one-line summary.
+    """
+    f = {"par": par_rel, "cov": cov_rel, "chi2": chi2_rel}[kind]
+    d = f(gpu, cpu32)
+    g64 = f(gpu, cpu64)
+    c64 = f(cpu32, cpu64)
+    within = float(np.mean(d <= tol))
+    line = "%s %s: %s | within %.0e: %.5f | gpu-vs-f64 max %.3g, cpu32-vs-f64 max %.3g" % (
+        name, kind, summary("vs cpu32", d), tol, within, g64.max(), c64.max())
+    print(line)
+    assert within >= frac, line
+    for q in (50, 90, 99, 99.9, 100):
+        # two independent FP32 roundings of the same quantity may sit on
+        # opposite sides of the exact value: allow 3x the one-sided spread
+        assert np.percentile(d, q) <= 3 * np.percentile(c64, q) + tol / 10, (q, line)
+        assert np.percentile(g64, q) <= 3 * np.percentile(c64, q) + tol / 10, (q, line)
+    return line
