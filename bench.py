@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--shape", type=str, default=None,
                     help="block,blocks_per_cu,variant for the batched kernel (skips the autotuner)")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter-calibration run for rocprofv3 --pmc: one 256 MiB device copy, then "
+                         "--steps batched and plain fits; no timing line")
     return ap.parse_args()
 
 
@@ -95,6 +98,22 @@ def main():
         shape, tuned_ms = ctx.autotune_batched(batch, reps=2)
         log("autotuned batched shape %s: %.3f ms" % (shape, tuned_ms))
     shape = ctx.get_batched_shape()
+
+    if args.probe:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE calibrate the counters.
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            ctx.fit_batched(batch)
+        ctx.fit_plain(plain_out)
+        torch.cuda.synchronize()
+        log("probe: copy %d bytes; %d batched fits and 1 plain fit of %d tracks, %d track-layers, shape %s" %
+            (src.numel() * 4, args.steps, n, n_track_layers, shape))
+        ctx.close()
+        return
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):

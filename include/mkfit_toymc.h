@@ -66,6 +66,32 @@ typedef struct mkf_toymc_output {
   float* truth_par;        /* [n_slots][6][n] or NULL: true CCS state at each crossing */
 } mkf_toymc_output;
 
+/* One event for track finding: every layer's hits (signal hits of the
+ * generated tracks plus noise hits uniform on the layer surface) in the
+ * reference's Hit order -- pos x, y, z then the packed 3x3 covariance
+ * (Hit.h:165-183) -- and one seed per track built from its hits on the first
+ * `seed_layers` layers (state: truth at the last seed hit, smeared as for the
+ * fit seeds).  Tracks that do not cross all seed layers get no seed.
+ * Barrel-only geometries. */
+typedef struct mkf_toymc_find_output {
+  int n_tracks;            /* generated tracks (capacity of the seed arrays) */
+  int layer_capacity;      /* hits per layer the caller allocated */
+  float* hits;             /* [n_layers][layer_capacity][9] */
+  int* n_layer_hits;       /* [n_layers] filled */
+  int* hit_mc;             /* [n_layers][layer_capacity]: generating track, -1 for noise */
+  int n_seeds;             /* out */
+  float* seed_par;         /* [6][n_tracks] (first n_seeds used) */
+  float* seed_err;         /* [21][n_tracks] */
+  int* seed_chg;           /* [n_tracks] */
+  int* seed_hits;          /* [seed_layers][n_tracks]: hit index on each seed layer */
+  int* seed_mc;            /* [n_tracks]: generating track of each seed */
+} mkf_toymc_find_output;
+
+int mkf_toymc_find_event(const mkf_geometry* geo, const mkf_toymc_params* p, float noise_per_cm2, int seed_layers,
+                         mkf_toymc_find_output* out, int n_threads);
+/* Noise hits that mkf_toymc_find_event puts on barrel layer `layer`. */
+int mkf_toymc_noise_count(const mkf_geometry* geo, int layer, float noise_per_cm2);
+
 /* Presets: 1 = 17 barrel layers r = 4 + 6i cm, |z| <= 120;
  *          2 = 25 barrel layers r = 3 + (107/24)i cm, |z| <= 120;
  *          3 = the 25 barrel layers with staged lengths plus 2 x 9 disks at |z| = 30..270 cm.

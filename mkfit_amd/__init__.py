@@ -68,6 +68,13 @@ class MkfToyParams(ctypes.Structure):
                 ("seed_rel_ipt", ctypes.c_float), ("seed_sig_ang", ctypes.c_float)]
 
 
+class MkfToyFindOutput(ctypes.Structure):
+    _fields_ = [("n_tracks", ctypes.c_int), ("layer_capacity", ctypes.c_int), ("hits", _c_void_p),
+                ("n_layer_hits", _c_void_p), ("hit_mc", _c_void_p), ("n_seeds", ctypes.c_int),
+                ("seed_par", _c_void_p), ("seed_err", _c_void_p), ("seed_chg", _c_void_p),
+                ("seed_hits", _c_void_p), ("seed_mc", _c_void_p)]
+
+
 class MkfToyOutput(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("n_slots", ctypes.c_int),
                 ("seed_par", _c_void_p), ("seed_err", _c_void_p), ("chg", _c_void_p),
@@ -85,7 +92,8 @@ ABI_SYMBOLS = [
 
 
 def lib_path():
-    return os.path.join(_HERE, "libmkfit_amd.so")
+    # MKFIT_AMD_LIB selects another build of the same library (A/B timing of kernel variants).
+    return os.environ.get("MKFIT_AMD_LIB") or os.path.join(_HERE, "libmkfit_amd.so")
 
 
 def lib():
@@ -125,6 +133,9 @@ def toy():
         T.mkf_toymc_generate.argtypes = [ctypes.POINTER(MkfGeometry), ctypes.POINTER(MkfToyParams),
                                          ctypes.POINTER(MkfToyOutput), ctypes.c_int]
         T.mkf_toymc_material_grid.argtypes = [ctypes.POINTER(MkfGeometry), _c_void_p, _c_void_p]
+        T.mkf_toymc_find_event.argtypes = [ctypes.POINTER(MkfGeometry), ctypes.POINTER(MkfToyParams), ctypes.c_float,
+                                           ctypes.c_int, ctypes.POINTER(MkfToyFindOutput), ctypes.c_int]
+        T.mkf_toymc_noise_count.argtypes = [ctypes.POINTER(MkfGeometry), ctypes.c_int, ctypes.c_float]
         _TOY = T
     return _TOY
 
@@ -191,6 +202,57 @@ def generate(geo, params, n_tracks, n_slots, n_threads=None, truth=False):
     rc = toy().mkf_toymc_generate(ctypes.byref(geo), ctypes.byref(params), ctypes.byref(o), int(n_threads))
     if rc != 0:
         raise RuntimeError("toy MC generation failed (%d)" % rc)
+    return out
+
+
+def generate_find_event(geo, params, n_tracks, noise_per_cm2=2.0, seed_layers=3, n_threads=None):
+    """A barrel finding event: hits of every layer (signal + noise) and 3-hit seeds.
+
+    Returns a dict: hits_flat [total][9], hit_offset [L+1], hit_mc [total], n_seeds,
+    seed_par [6][n], seed_err [21][n], seed_chg [n], seed_hits [seed_layers][n], seed_mc [n].
+    """
+    L = geo.n_barrel
+    cap = n_tracks + max(toy().mkf_toymc_noise_count(ctypes.byref(geo), l, float(noise_per_cm2)) for l in range(L))
+    hits = np.zeros((L, cap, 9), np.float32)
+    nlh = np.zeros(L, np.int32)
+    mc = np.zeros((L, cap), np.int32)
+    n = int(n_tracks)
+    sp = np.zeros((6, n), np.float32); se = np.zeros((21, n), np.float32); sc = np.zeros(n, np.int32)
+    sh = np.zeros((seed_layers, n), np.int32); smc = np.zeros(n, np.int32)
+    o = MkfToyFindOutput()
+    o.n_tracks, o.layer_capacity = n, cap
+    o.hits, o.n_layer_hits, o.hit_mc = (a.ctypes.data_as(_c_void_p) for a in (hits, nlh, mc))
+    o.seed_par, o.seed_err, o.seed_chg, o.seed_hits, o.seed_mc = (a.ctypes.data_as(_c_void_p)
+                                                                  for a in (sp, se, sc, sh, smc))
+    if n_threads is None:
+        n_threads = min(16, os.cpu_count() or 1)
+    rc = toy().mkf_toymc_find_event(ctypes.byref(geo), ctypes.byref(params), float(noise_per_cm2), int(seed_layers),
+                                    ctypes.byref(o), int(n_threads))
+    if rc:
+        raise RuntimeError("find-event generation failed (%d)" % rc)
+    off = np.concatenate([[0], np.cumsum(nlh)]).astype(np.int32)
+    flat = np.concatenate([hits[l, :nlh[l]] for l in range(L)]).astype(np.float32)
+    mcf = np.concatenate([mc[l, :nlh[l]] for l in range(L)]).astype(np.int32)
+    ns = o.n_seeds
+    return dict(hits_flat=flat, hit_offset=off, hit_mc=mcf, n_seeds=ns, seed_par=np.ascontiguousarray(sp[:, :ns]),
+                seed_err=np.ascontiguousarray(se[:, :ns]), seed_chg=sc[:ns].copy(),
+                seed_hits=np.ascontiguousarray(sh[:, :ns]), seed_mc=smc[:ns].copy())
+
+
+def barrel_layer_descs(geo, q_bin=2.0, min_dphi=0.01, max_dphi=0.05, min_dq=1.0, max_dq=2.0, half_thickness=0.01):
+    """Per-layer LayerInfo + selection limits for a barrel geometry of thin layers at r.
+
+    Defaults are the toy cylinder geometry's (Geoms/CylCowWLids.cc:64-79: q_bin 2 cm, selection
+    limits 0.01/0.05 rad and 1/2 cm, dynamic-window coefficients zero), except that the track is
+    propagated to the layer's centre radius, where the toy hits lie, rather than to r_in.
+    """
+    out = []
+    for l in range(geo.n_barrel):
+        r, zm = geo.barrel_r[l], geo.barrel_zmax[l]
+        out.append(dict(is_barrel=1, rin=r - half_thickness, rout=r + half_thickness, zmin=-zm, zmax=zm,
+                        propagate_to=r, q_bin=q_bin,
+                        min_dphi=min_dphi, max_dphi=max_dphi, min_dq=min_dq, max_dq=max_dq,
+                        c_dp_sf=1.1, c_dq_sf=1.1, c_c2_sf=1.1))
     return out
 
 

@@ -372,6 +372,96 @@ int mkf_toymc_generate(const mkf_geometry* geo, const mkf_toymc_params* p, mkf_t
   return 0;
 }
 
+int mkf_toymc_noise_count(const mkf_geometry* g, int layer, float noise_per_cm2) {
+  if (!g || layer < 0 || layer >= g->n_barrel || noise_per_cm2 <= 0) return 0;
+  const double area = 2 * kPi * g->barrel_r[layer] * 2 * g->barrel_zmax[layer];
+  return (int)std::llround(area * noise_per_cm2);
+}
+
+int mkf_toymc_find_event(const mkf_geometry* g, const mkf_toymc_params* p, float noise_per_cm2, int seed_layers,
+                         mkf_toymc_find_output* o, int n_threads) {
+  if (!g || !p || !o || o->n_tracks < 0 || g->n_disk != 0 || seed_layers < 1 || seed_layers > g->n_barrel) return 1;
+  const int n = o->n_tracks, L = g->n_barrel, cap = o->layer_capacity;
+  for (int l = 0; l < L; ++l)
+    if (n + mkf_toymc_noise_count(g, l, noise_per_cm2) > cap) return 2;
+  // Tracks through the fit generator (one hit per crossed layer, in layer order).
+  std::vector<float> seed_par(6 * (size_t)n), seed_err(21 * (size_t)n), hit_pos((size_t)L * 3 * n),
+      hit_err((size_t)L * 6 * n), truth((size_t)L * 6 * n);
+  std::vector<int> chg(n), nh(n), hit_layer((size_t)L * n);
+  mkf_toymc_output to;
+  to.n_tracks = n;
+  to.n_slots = L;
+  to.seed_par = seed_par.data();
+  to.seed_err = seed_err.data();
+  to.chg = chg.data();
+  to.hit_pos = hit_pos.data();
+  to.hit_err = hit_err.data();
+  to.n_hits = nh.data();
+  to.hit_flags = nullptr;
+  to.hit_layer = hit_layer.data();
+  to.truth_par = truth.data();
+  mkf_toymc_params pp = *p;
+  pp.hits_min = pp.hits_max = 0;
+  pp.missing_frac = 0;
+  int rc = mkf_toymc_generate(g, &pp, &to, n_threads);
+  if (rc) return rc;
+  // Signal hits per layer, in track order; then noise.
+  std::vector<std::vector<int>> sig_index(L, std::vector<int>(n, -1));
+  for (int l = 0; l < L; ++l) o->n_layer_hits[l] = 0;
+  for (int t = 0; t < n; ++t)
+    for (int k = 0; k < nh[t]; ++k) {
+      const int l = hit_layer[(size_t)k * n + t];
+      const int i = o->n_layer_hits[l]++;
+      float* h = o->hits + ((size_t)l * cap + i) * 9;
+      for (int e = 0; e < 3; ++e) h[e] = hit_pos[((size_t)k * 3 + e) * n + t];
+      for (int e = 0; e < 6; ++e) h[3 + e] = hit_err[((size_t)k * 6 + e) * n + t];
+      o->hit_mc[(size_t)l * cap + i] = t;
+      sig_index[l][t] = i;
+    }
+  for (int l = 0; l < L; ++l) {
+    Rng rng(p->seed ^ 0x5bd1e995u, (uint64_t)(1u << 30) + l);
+    const int nn = mkf_toymc_noise_count(g, l, noise_per_cm2);
+    const double r = g->barrel_r[l], zmax = g->barrel_zmax[l];
+    const double vr = double(p->sig_rphi) * p->sig_rphi, vz = double(p->sig_z) * p->sig_z;
+    for (int k = 0; k < nn; ++k) {
+      const double ph = -kPi + 2 * kPi * rng.uniform(), z = -zmax + 2 * zmax * rng.uniform();
+      const int i = o->n_layer_hits[l]++;
+      float* h = o->hits + ((size_t)l * cap + i) * 9;
+      const double sp = std::sin(ph), cp = std::cos(ph);
+      h[0] = float(r * cp); h[1] = float(r * sp); h[2] = float(z);
+      h[3] = float(vr * sp * sp); h[4] = float(-vr * sp * cp); h[5] = float(vr * cp * cp);
+      h[6] = 0.f; h[7] = 0.f; h[8] = float(vz);
+      o->hit_mc[(size_t)l * cap + i] = -1;
+    }
+  }
+  // Seeds: tracks with hits on all seed layers; state = truth at the last seed
+  // hit, smeared with the fit-seed widths.
+  int ns = 0;
+  for (int t = 0; t < n; ++t) {
+    if (nh[t] < seed_layers) continue;
+    Rng rng(p->seed ^ 0x27d4eb2fu, (uint64_t)t);
+    const int k = seed_layers - 1;
+    double sd[6];
+    for (int e = 0; e < 6; ++e) sd[e] = truth[((size_t)k * 6 + e) * n + t];
+    const double sig[6] = {p->seed_sig_pos, p->seed_sig_pos, p->seed_sig_z, p->seed_rel_ipt * sd[3], p->seed_sig_ang,
+                           p->seed_sig_ang};
+    for (int e = 0; e < 6; ++e) {
+      double v = sd[e] + rng.gauss() * sig[e];
+      if (e == 3) v = std::fabs(v);
+      if (e == 4) v = std::atan2(std::sin(v), std::cos(v));
+      o->seed_par[(size_t)e * n + ns] = float(v);
+    }
+    for (int e = 0; e < 21; ++e) o->seed_err[(size_t)e * n + ns] = 0.f;
+    for (int i = 0; i < 6; ++i) o->seed_err[(size_t)(i * (i + 1) / 2 + i) * n + ns] = float(sig[i] * sig[i]);
+    o->seed_chg[ns] = chg[t];
+    for (int l = 0; l < seed_layers; ++l) o->seed_hits[(size_t)l * n + ns] = sig_index[l][t];
+    o->seed_mc[ns] = t;
+    ++ns;
+  }
+  o->n_seeds = ns;
+  return 0;
+}
+
 int mkf_toymc_material_grid(const mkf_geometry* g, float* rl, float* xi) {
   if (!g || !rl || !xi) return 1;
   const int NZ = 300, NR = 120;

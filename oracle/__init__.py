@@ -151,3 +151,83 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle fit failed (%d)" % rc)
         return par, err, chg, chi2
+
+
+# ---------------------------------------------------------------------------
+# Best-hit finding (oracle_find.cc)
+# ---------------------------------------------------------------------------
+
+class LayerDesc(ctypes.Structure):
+    _fields_ = [("is_barrel", ctypes.c_int), ("rin", ctypes.c_float), ("rout", ctypes.c_float),
+                ("zmin", ctypes.c_float), ("zmax", ctypes.c_float), ("propagate_to", ctypes.c_float),
+                ("q_bin", ctypes.c_float), ("min_dphi", ctypes.c_float), ("max_dphi", ctypes.c_float),
+                ("min_dq", ctypes.c_float), ("max_dq", ctypes.c_float)] + [
+        (nm, ctypes.c_float) for nm in ("c_dp_sf", "c_dp_0", "c_dp_1", "c_dp_2", "c_dq_sf", "c_dq_0", "c_dq_1",
+                                        "c_dq_2", "c_c2_sf", "c_c2_0", "c_c2_1", "c_c2_2")]
+
+
+class _FindEvent(ctypes.Structure):
+    _fields_ = [("n_layers", ctypes.c_int), ("layers", ctypes.c_void_p), ("hit_offset", ctypes.c_void_p),
+                ("hits", ctypes.c_void_p), ("first_layer", ctypes.c_int), ("chi2_cut", ctypes.c_float),
+                ("pflags", ctypes.c_int)]
+
+
+class _FindTracks(ctypes.Structure):
+    _fields_ = [("n_tracks", ctypes.c_int), ("par", ctypes.c_void_p), ("err", ctypes.c_void_p),
+                ("chg", ctypes.c_void_p), ("chi2", ctypes.c_void_p), ("hot_idx", ctypes.c_void_p),
+                ("n_found", ctypes.c_void_p)]
+
+
+def layer_array(descs):
+    arr = (LayerDesc * len(descs))()
+    for i, d in enumerate(descs):
+        for k, v in d.items():
+            setattr(arr[i], k, v)
+    return arr
+
+
+def loh_build(desc, hits):
+    """LayerOfHits of one layer: (ranks, bins [nq*256, 2], phis, qs, nq)."""
+    L = lib()
+    d = layer_array([desc])
+    nq = ctypes.c_int()
+    L.orc_loh_geometry(ctypes.byref(d[0]), ctypes.byref(nq), None, None, None)
+    h = np.ascontiguousarray(hits, np.float32)
+    n = h.shape[0]
+    ranks = np.zeros(n, np.int32); bins = np.zeros((nq.value * 256, 2), np.int32)
+    phis = np.zeros(n, np.float32); qs = np.zeros(n, np.float32)
+    L.orc_loh_build(ctypes.byref(d[0]), n, _p(h), _p(ranks), _p(bins), _p(phis), _p(qs))
+    return ranks, bins, phis, qs, nq.value
+
+
+def select_hit_indices(desc, hits, par, err):
+    L = lib()
+    d = layer_array([desc])
+    h = np.ascontiguousarray(hits, np.float32)
+    p = np.ascontiguousarray(par, np.float32); e = np.ascontiguousarray(err, np.float32)
+    n = p.shape[1]
+    xs = np.zeros(n, np.int32); xa = np.zeros((16, n), np.int32); w = np.zeros(n, np.int32)
+    L.orc_select_hit_indices(ctypes.byref(d[0]), h.shape[0], _p(h), n, _p(p), _p(e), _p(xs), _p(xa), _p(w))
+    return xs, xa, w
+
+
+def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n_threads=1):
+    """Best-hit finding over a toy event dict (see mkfit_amd.generate_find_event)."""
+    L = lib()
+    layers = layer_array(descs)
+    n = ev["n_seeds"]
+    par = np.array(ev["seed_par"][:, :n], np.float32); err = np.array(ev["seed_err"][:, :n], np.float32)
+    chg = np.array(ev["seed_chg"][:n], np.int32); chi2 = np.zeros(n, np.float32)
+    nl = len(descs)
+    hot = np.full((nl, n), -9, np.int32)
+    hot[:first_layer] = ev["seed_hits"][:first_layer, :n]
+    nfound = np.zeros(n, np.int32)
+    off = np.ascontiguousarray(ev["hit_offset"], np.int32)
+    hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
+    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags)
+    ft = _FindTracks(n, _p(par), _p(err), _p(chg), _p(chi2), _p(hot), _p(nfound))
+    ctx = OrcCtx(float(bfield), None, None)
+    rc = L.orc_find_best_hit(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(ft), int(n_threads))
+    if rc:
+        raise RuntimeError("orc_find_best_hit failed (%d)" % rc)
+    return dict(par=par, err=err, chg=chg, chi2=chi2, hot_idx=hot, n_found=nfound)

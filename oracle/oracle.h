@@ -88,6 +88,49 @@ int orc_kalman_op_f64(int op, int endcap, int n, const double* ps_err, const dou
 int orc_fit_f32(const orc_ctx* ctx, const orc_fit_batch_f32* b, int n_threads);
 int orc_fit_f64(const orc_ctx* ctx, const orc_fit_batch_f64* b, int n_threads);
 
+/* ---- Best-hit track finding (oracle_find.cc) ------------------------------ */
+
+typedef struct orc_layer_desc {
+  int is_barrel;
+  float rin, rout, zmin, zmax; /* LayerInfo limits */
+  float propagate_to;          /* LayerInfo::m_propagate_to */
+  float q_bin;                 /* > 0 bin width, < 0 number of bins */
+  float min_dphi, max_dphi, min_dq, max_dq;          /* IterationLayerConfig selection limits */
+  float c_dp_sf, c_dp_0, c_dp_1, c_dp_2;             /* dynamic windows (IterationConfig.h:70-86) */
+  float c_dq_sf, c_dq_0, c_dq_1, c_dq_2;
+  float c_c2_sf, c_c2_0, c_c2_1, c_c2_2;
+} orc_layer_desc;
+
+typedef struct orc_find_event {
+  int n_layers;
+  const orc_layer_desc* layers;  /* [n_layers] */
+  const int* hit_offset;         /* [n_layers + 1] */
+  const float* hits;             /* [total][9] x y z xx yx yy zx zy zz */
+  int first_layer;               /* first layer searched (the seed hits fill the ones before) */
+  float chi2_cut;                /* IterationParams::chi2Cut_min */
+  int pflags;                    /* finding_inter_layer_pflags */
+} orc_find_event;
+
+typedef struct orc_find_tracks {
+  int n_tracks;
+  float* par;     /* [6][n] in: seed state, out: state at the end */
+  float* err;     /* [21][n] */
+  int* chg;       /* [n] */
+  float* chi2;    /* [n] */
+  int* hot_idx;   /* [n_layers][n] in: seed hit indices on layers < first_layer; out: hit index per layer,
+                     -1/-2/-3 fake-hit codes (index-desc.txt), -9 layer not reached */
+  int* n_found;   /* [n] out */
+} orc_find_tracks;
+
+int orc_loh_geometry(const orc_layer_desc* d, int* nq, float* qmin, float* qmax, float* fq);
+/* LayerOfHits of one layer: ranks [n] (sorted -> original), bins [nq*256][2], phis/qs [n] sorted. */
+int orc_loh_build(const orc_layer_desc* d, int n_hits, const float* hits, int* ranks, int* bins, float* phis,
+                  float* qs);
+/* SelectHitIndices on states already at the layer: xsize [n], xarr [16][n] (-1 padded), wsr [n]. */
+int orc_select_hit_indices(const orc_layer_desc* d, int n_hits, const float* hits, int n, const float* par,
+                           const float* err, int* xsize, int* xarr, int* wsr);
+int orc_find_best_hit(const orc_ctx* ctx, const orc_find_event* ev, orc_find_tracks* tracks, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,506 @@
+// oracle_find.cc -- CPU restatement of mkFit's best-hit track finding.
+//
+// TEST INFRASTRUCTURE ONLY (checker and CPU baseline).  Restates:
+//   LayerOfHits::setup_bins / SuckInHits     mkFit/HitStructures.cc:11-48, 77-183
+//     (bin geometry, key = phi_fine + (q_bin << 16), stable sort by key, per
+//     (q, phi) bin [first, end) ranges; 32-bit ranges here, the reference's are
+//     uint16 (HitStructures.h:45) and agree wherever a layer holds < 65536 hits)
+//   bin helpers                               mkFit/HitStructures.h:237-255
+//   LayerInfo sensitive-region tests          TrackerInfo.h:118-138
+//   MkFinder::getHitSelDynamicWindows/Chi2Cut mkFit/MkFinder.cc:166-220
+//   MkFinder::SelectHitIndices                mkFit/MkFinder.cc:227-684
+//   MkFinder::AddBestHit                      mkFit/MkFinder.cc:691-832, add_hit MkFinder.h:312-360
+//   MkBuilder::FindTracksBestHit layer loop   mkFit/MkBuilder.cc:1395-1478 (barrel plan)
+// Propagation and Kalman math come from oracle_core.h.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <numeric>
+#include <thread>
+#include <vector>
+
+#include "oracle.h"
+#include "oracle_core.h"
+
+using namespace orc;
+
+namespace {
+
+constexpr int kNPhi = 256;                                    // Config::m_nphi
+constexpr float kFPhiFine = 1024 / kTwoPI;                    // HitStructures.h:153-158
+constexpr int kPhiMask = 0xff;
+constexpr int kPhiMaskFine = 0x3ff;
+constexpr int kPhiBitsShift = 2;
+constexpr int kPhiFineXMask = ~((1 << kPhiBitsShift) - 1);
+constexpr int kHitIdxMax = 16;                                // MPlexHitIdxMax, MkFitter.h:18
+constexpr int kMaxTrkHits = 64;                               // Config::nMaxTrkHits
+enum { WSR_Inside = 0, WSR_Edge = 1, WSR_Outside = 2 };
+
+struct Bins {
+  int nq;
+  float qmin, qmax, fq;
+};
+
+// setup_bins (HitStructures.cc:11-32)
+Bins setup_bins(float qmin, float qmax, float dq) {
+  Bins b;
+  if (dq < 0) {
+    b.nq = (int)-dq;
+    b.qmin = qmin;
+    b.qmax = qmax;
+  } else {
+    const float extent = qmax - qmin;
+    b.nq = (int)std::ceil(extent / dq);
+    const float extra = 0.5f * (b.nq * dq - extent);
+    b.qmin = qmin - extra;
+    b.qmax = qmax + extra;
+  }
+  b.fq = b.nq / (qmax - qmin);
+  return b;
+}
+
+Bins layer_bins(const orc_layer_desc& d) {
+  return d.is_barrel ? setup_bins(d.zmin, d.zmax, d.q_bin) : setup_bins(d.rin, d.rout, d.q_bin);
+}
+
+inline int q_bin(const Bins& b, float q) { return (int)((q - b.qmin) * b.fq); }
+inline int q_bin_checked(const Bins& b, float q) { return std::clamp(q_bin(b, q), 0, b.nq - 1); }
+inline int phi_bin_fine(float phi) { return (int)std::floor(kFPhiFine * (phi + kPI)); }
+inline int phi_bin(float phi) { return phi_bin_fine(phi) >> kPhiBitsShift; }
+inline float cdist(float a) { return a > kPI ? kTwoPI - a : a; }   // Config.h:441
+
+struct LoH {
+  Bins b;
+  std::vector<int> first, end;  // [nq * 256]
+  std::vector<int> ranks;       // sorted position -> original hit index
+  std::vector<float> phis, qs;  // in sorted order
+};
+
+// SuckInHits (HitStructures.cc:77-183); the LSD radix sort's ranks are the
+// stable order of the keys.
+void build_loh(const orc_layer_desc& d, int n, const float* hits, LoH& L) {
+  L.b = layer_bins(d);
+  std::vector<uint32_t> key(n);
+  std::vector<float> phi(n), q(n);
+  for (int i = 0; i < n; ++i) {
+    const float* h = hits + (size_t)i * 9;
+    phi[i] = std::atan2(h[1], h[0]);                              // Hit::phi, Hit.h:64-67
+    q[i] = d.is_barrel ? h[2] : std::sqrt(h[0] * h[0] + h[1] * h[1]);  // Hit::z / Hit::r
+    key[i] = (uint32_t)(phi_bin_fine(phi[i]) + (q_bin_checked(L.b, q[i]) << 16));
+  }
+  L.ranks.resize(n);
+  std::iota(L.ranks.begin(), L.ranks.end(), 0);
+  std::stable_sort(L.ranks.begin(), L.ranks.end(), [&](int a, int c) { return key[a] < key[c]; });
+  L.first.assign((size_t)L.b.nq * kNPhi, 0);
+  L.end.assign((size_t)L.b.nq * kNPhi, 0);
+  L.phis.resize(n);
+  L.qs.resize(n);
+  int curr = -1;
+  for (int i = 0; i < n; ++i) {
+    const int j = L.ranks[i];
+    L.phis[i] = phi[j];
+    L.qs[i] = q[j];
+    const int jq = (int)key[j] & kPhiFineXMask;
+    const int pb = (jq & kPhiMaskFine) >> kPhiBitsShift;
+    const int qb = jq >> 16;
+    const size_t bi = (size_t)qb * kNPhi + pb;
+    if (jq != curr) {
+      L.first[bi] = i;
+      L.end[bi] = i;
+      curr = jq;
+    }
+    L.end[bi]++;
+  }
+}
+
+int wsr_z(const orc_layer_desc& d, float z, float dz) {      // TrackerInfo.h:118-123
+  if (z > d.zmax + dz || z < d.zmin - dz) return WSR_Outside;
+  if (z < d.zmax - dz && z > d.zmin + dz) return WSR_Inside;
+  return WSR_Edge;
+}
+int wsr_r(const orc_layer_desc& d, float r, float dr) {      // TrackerInfo.h:125-138 (no r hole)
+  if (r > d.rout + dr || r < d.rin - dr) return WSR_Outside;
+  if (r < d.rout - dr && r > d.rin + dr) return WSR_Inside;
+  return WSR_Edge;
+}
+
+// getHitSelDynamicWindows (MkFinder.cc:166-195)
+void dyn_windows(const orc_layer_desc& d, float invpt, float theta, float& min_dq, float& max_dq, float& min_dphi,
+                 float& max_dphi) {
+  float max_invpt = invpt;
+  if (invpt > 10.0) max_invpt = 10.0;
+  const float this_dq = d.c_dq_0 * max_invpt + d.c_dq_1 * theta + d.c_dq_2;
+  if (d.c_dq_sf * this_dq > 0.f) {
+    min_dq = d.c_dq_sf * this_dq;
+    max_dq = 2.0f * min_dq;
+  }
+  const float this_dphi = d.c_dp_0 * max_invpt + d.c_dp_1 * theta + d.c_dp_2;
+  if (d.c_dp_sf * this_dphi > min_dphi) {
+    min_dphi = d.c_dp_sf * this_dphi;
+    max_dphi = 2.0f * min_dphi;
+  }
+}
+
+// getHitSelDynamicChi2Cut (MkFinder.cc:202-220)
+float dyn_chi2_cut(const orc_layer_desc& d, float chi2_cut_min, float invpt, float theta_in) {
+  const float theta = std::abs(theta_in - kPIOver2);
+  float max_invpt = invpt;
+  if (invpt > 10.0) max_invpt = 10.0;
+  const float this_c2 = d.c_c2_0 * max_invpt + d.c_c2_1 * theta + d.c_c2_2;
+  if (d.c_c2_sf * this_c2 > chi2_cut_min) return d.c_c2_sf * this_c2;
+  return chi2_cut_min;
+}
+
+// SelectHitIndices for NN lanes of propagated state (MkFinder.cc:227-684).
+void select_hit_indices(const orc_layer_desc& d, const LoH& L, const float* par, const float* err, int n_proc,
+                        int* xsize, int* xarr /*[16][NN]*/, int* wsr) {
+  float dqv[NN], dphiv[NN], qv[NN], phiv[NN];
+  int qb1v[NN], qb2v[NN], pb1v[NN], pb2v[NN];
+  const float nSigmaPhi = 3, nSigmaZ = 3, nSigmaR = 3;
+  for (int it = 0; it < NN; ++it) {
+    xsize[it] = 0;
+    float min_dq = d.min_dq, max_dq = d.max_dq, min_dphi = d.min_dphi, max_dphi = d.max_dphi;
+    const float invpt = par[3 * NN + it];
+    const float theta = std::fabs(par[5 * NN + it] - kPIOver2);
+    dyn_windows(d, invpt, theta, min_dq, max_dq, min_dphi, max_dphi);
+    const float x = par[0 * NN + it], y = par[1 * NN + it];
+    const float r2 = x * x + y * y;
+    const float dphidx = -y / r2, dphidy = x / r2;
+    const float dphi2 = dphidx * dphidx * err[sym(0, 0) * NN + it] + dphidy * dphidy * err[sym(1, 1) * NN + it] +
+                        2 * dphidx * dphidy * err[sym(0, 1) * NN + it];
+    const float phi = std::atan2(y, x);
+    float dphi = std::max(nSigmaPhi * std::sqrt(std::abs(dphi2)), min_dphi);
+    float q, dq;
+    if (d.is_barrel) {
+      const float z = par[2 * NN + it];
+      dq = std::abs(nSigmaZ * std::sqrt(err[sym(2, 2) * NN + it]));
+      const float edgeCorr = std::abs(0.5f * (d.rout - d.rin) / std::tan(par[5 * NN + it]));
+      wsr[it] = wsr_z(d, z, std::sqrt(dq * dq + edgeCorr * edgeCorr));
+      q = z;
+    } else {
+      const float r = std::sqrt(r2);
+      dq = nSigmaR * std::sqrt(std::abs(x * x * err[sym(0, 0) * NN + it] + y * y * err[sym(1, 1) * NN + it] +
+                                        2 * x * y * err[sym(0, 1) * NN + it]) / r2);
+      const float edgeCorr = std::abs(0.5f * (d.zmax - d.zmin) * std::tan(par[5 * NN + it]));
+      wsr[it] = wsr_r(d, r, std::sqrt(dq * dq + edgeCorr * edgeCorr));
+      q = r;
+    }
+    // assignbins
+    dphi = std::clamp(std::abs(dphi), min_dphi, max_dphi);
+    dq = std::clamp(dq, min_dq, max_dq);
+    qv[it] = q;
+    phiv[it] = phi;
+    dphiv[it] = dphi;
+    dqv[it] = dq;
+    qb1v[it] = q_bin_checked(L.b, q - dq);
+    qb2v[it] = q_bin_checked(L.b, q + dq) + 1;
+    pb1v[it] = phi_bin(phi - dphi);
+    pb2v[it] = phi_bin(phi + dphi) + 1;
+  }
+  for (int it = 0; it < n_proc; ++it) {
+    if (wsr[it] == WSR_Outside) {
+      xsize[it] = -1;
+      continue;
+    }
+    const float q = qv[it], phi = phiv[it], dphi = dphiv[it], dq = dqv[it];
+    for (int qi = qb1v[it]; qi < qb2v[it]; ++qi)
+      for (int pi = pb1v[it]; pi < pb2v[it]; ++pi) {
+        const int pb = pi & kPhiMask;
+        const size_t bi = (size_t)qi * kNPhi + pb;
+        for (int hi = L.first[bi]; hi < L.end[bi]; ++hi) {
+          if (xsize[it] >= kHitIdxMax) break;
+          const float ddq = std::abs(q - L.qs[hi]);
+          const float ddphi = cdist(std::abs(phi - L.phis[hi]));
+          if (ddq >= dq) continue;
+          if (ddphi >= dphi) continue;
+          xarr[xsize[it]++ * NN + it] = L.ranks[hi];
+        }
+      }
+  }
+}
+
+struct FindBlock {
+  float par[6 * NN], err[21 * NN], chi2[NN];
+  int chg[NN];
+  int nhits[NN], nfound[NN], ninside[NN], ntail[NN], stopped[NN];
+  int hot_idx[kMaxTrkHits][NN], hot_lyr[kMaxTrkHits][NN];
+};
+
+// add_hit (MkFinder.h:312-354)
+void add_hit(FindBlock& B, int it, int index, int layer) {
+  int& nt = B.nhits[it];
+  int& nf = B.nfound[it];
+  if (nt < kMaxTrkHits) {
+    B.hot_idx[nt][it] = index;
+    B.hot_lyr[nt][it] = layer;
+    ++nt;
+    if (index >= 0) {
+      ++nf;
+      B.ninside[it] += B.ntail[it];
+      B.ntail[it] = 0;
+    } else if (index == -1) {
+      ++B.ntail[it];
+    }
+  } else {
+    const int LH = kMaxTrkHits - 1;
+    if (index >= 0) {
+      if (B.hot_idx[LH][it] < 0) ++nf;
+      B.hot_idx[LH][it] = index;
+      B.hot_lyr[LH][it] = layer;
+    } else if (index == -2) {
+      if (B.hot_idx[LH][it] >= 0) --nf;
+      B.hot_idx[LH][it] = index;
+      B.hot_lyr[LH][it] = layer;
+    }
+  }
+}
+
+struct EventLayers {
+  int n_layers;
+  const orc_layer_desc* desc;
+  std::vector<LoH> loh;
+  const int* hit_off;
+  const float* hits;  // [total][9]
+};
+
+// AddBestHit (MkFinder.cc:691-832) for one block, propagated state in parP/errP.
+void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock& B, const float* parP,
+                  const float* errP, const int* xsize, const int* xarr, const int* wsr, int n_proc) {
+  const orc_layer_desc& d = E.desc[layer];
+  const float* hits = E.hits + (size_t)E.hit_off[layer] * 9;
+  float minChi2[NN];
+  int bestHit[NN];
+  int maxSize = 0;
+  for (int it = 0; it < NN; ++it) {
+    if (it < n_proc && xsize[it] > 0) maxSize = std::max(maxSize, xsize[it]);
+    bestHit[it] = -1;
+    minChi2[it] = dyn_chi2_cut(d, chi2_cut_min, parP[3 * NN + it], parP[5 * NN + it]);
+  }
+  alignas(64) float msPar[3 * NN], msErr[6 * NN], outChi2[NN];
+  alignas(64) float dErr[21 * NN], dPar[6 * NN];
+  for (int l = 0; l < 3 * NN; ++l) msPar[l] = 0.f;
+  for (int l = 0; l < 6 * NN; ++l) msErr[l] = 1.f;
+  for (int hc = 0; hc < maxSize; ++hc) {
+    for (int it = 0; it < n_proc; ++it)
+      if (hc < xsize[it]) {
+        const float* h = hits + (size_t)xarr[hc * NN + it] * 9;
+        for (int e = 0; e < 3; ++e) msPar[e * NN + it] = h[e];
+        for (int e = 0; e < 6; ++e) msErr[e * NN + it] = h[3 + e];
+      }
+    if (d.is_barrel)
+      kalman_operation(KFO_Calculate_Chi2, errP, parP, msErr, msPar, dErr, dPar, outChi2);
+    else
+      kalman_operation_endcap(KFO_Calculate_Chi2, errP, parP, msErr, msPar, dErr, dPar, outChi2);
+    for (int it = 0; it < n_proc; ++it)
+      if (hc < xsize[it]) {
+        const float c2 = std::abs(outChi2[it]);
+        if (c2 < minChi2[it]) {
+          minChi2[it] = c2;
+          bestHit[it] = xarr[hc * NN + it];
+        }
+      }
+  }
+  for (int it = 0; it < n_proc; ++it) {
+    auto fake = [&]() {
+      const float f[6] = {666, 0, 666, 0, 0, 666};
+      for (int e = 0; e < 6; ++e) msErr[e * NN + it] = f[e];
+      for (int e = 0; e < 3; ++e) msPar[e * NN + it] = parP[e * NN + it];
+    };
+    if (wsr[it] == WSR_Outside) {
+      fake();
+      continue;
+    }
+    if (bestHit[it] >= 0) {
+      const float* h = hits + (size_t)bestHit[it] * 9;
+      for (int e = 0; e < 3; ++e) msPar[e * NN + it] = h[e];
+      for (int e = 0; e < 6; ++e) msErr[e * NN + it] = h[3 + e];
+      B.chi2[it] += minChi2[it];
+      add_hit(B, it, bestHit[it], layer);
+    } else {
+      int fake_idx = -1;
+      if (wsr[it] == WSR_Edge)
+        fake_idx = -3;
+      else if (B.ninside[it] + B.ntail[it])
+        fake_idx = -2;
+      fake();
+      add_hit(B, it, fake_idx, layer);
+    }
+  }
+  // kalmanPropagateAndUpdate without re-propagation (finding_requires_propagation_to_hit_pos = false)
+  if (d.is_barrel)
+    kalman_operation(KFO_Update_Params, errP, parP, msErr, msPar, B.err, B.par, outChi2);
+  else
+    kalman_operation_endcap(KFO_Update_Params, errP, parP, msErr, msPar, B.err, B.par, outChi2);
+  for (int it = 0; it < NN; ++it)
+    if (B.par[3 * NN + it] < 0) {
+      B.chg[it] = -B.chg[it];
+      B.par[3 * NN + it] = -B.par[3 * NN + it];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_loh_geometry(const orc_layer_desc* d, int* nq, float* qmin, float* qmax, float* fq) {
+  if (!d) return 1;
+  const Bins b = layer_bins(*d);
+  if (nq) *nq = b.nq;
+  if (qmin) *qmin = b.qmin;
+  if (qmax) *qmax = b.qmax;
+  if (fq) *fq = b.fq;
+  return 0;
+}
+
+int orc_loh_build(const orc_layer_desc* d, int n, const float* hits, int* ranks, int* bins, float* phis, float* qs) {
+  if (!d || n < 0) return 1;
+  LoH L;
+  build_loh(*d, n, hits, L);
+  for (int i = 0; i < n; ++i) {
+    if (ranks) ranks[i] = L.ranks[i];
+    if (phis) phis[i] = L.phis[i];
+    if (qs) qs[i] = L.qs[i];
+  }
+  if (bins)
+    for (size_t b = 0; b < L.first.size(); ++b) {
+      bins[2 * b] = L.first[b];
+      bins[2 * b + 1] = L.end[b];
+    }
+  return 0;
+}
+
+int orc_select_hit_indices(const orc_layer_desc* d, int n_hits, const float* hits, int n, const float* par,
+                           const float* err, int* xsize, int* xarr, int* wsr) {
+  if (!d || n < 0) return 1;
+  LoH L;
+  build_loh(*d, n_hits, hits, L);
+  for (int t0 = 0; t0 < n; t0 += NN) {
+    const int cnt = std::min(NN, n - t0);
+    alignas(64) float p[6 * NN], e[21 * NN];
+    int xs[NN], xa[kHitIdxMax * NN], w[NN];
+    for (int k = 0; k < 6; ++k)
+      for (int l = 0; l < NN; ++l) p[k * NN + l] = par[(size_t)k * n + t0 + (l < cnt ? l : 0)];
+    for (int k = 0; k < 21; ++k)
+      for (int l = 0; l < NN; ++l) e[k * NN + l] = err[(size_t)k * n + t0 + (l < cnt ? l : 0)];
+    select_hit_indices(*d, L, p, e, cnt, xs, xa, w);
+    for (int l = 0; l < cnt; ++l) {
+      xsize[t0 + l] = xs[l];
+      wsr[t0 + l] = w[l];
+      for (int k = 0; k < kHitIdxMax; ++k) xarr[(size_t)k * n + t0 + l] = k < xs[l] ? xa[k * NN + l] : -1;
+    }
+  }
+  return 0;
+}
+
+int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_tracks* tr, int n_threads) {
+  if (!ev || !tr || ev->n_layers <= 0 || ev->first_layer < 0 || ev->first_layer > ev->n_layers) return 1;
+  Ctx ctx;
+  ctx.bfield = c ? c->bfield : 3.8112f;
+  ctx.rl_grid = c ? c->rl_grid : nullptr;
+  ctx.xi_grid = c ? c->xi_grid : nullptr;
+  EventLayers E;
+  E.n_layers = ev->n_layers;
+  E.desc = ev->layers;
+  E.hit_off = ev->hit_offset;
+  E.hits = ev->hits;
+  E.loh.resize(ev->n_layers);
+  for (int l = 0; l < ev->n_layers; ++l)
+    build_loh(ev->layers[l], ev->hit_offset[l + 1] - ev->hit_offset[l], ev->hits + (size_t)ev->hit_offset[l] * 9,
+              E.loh[l]);
+  const int n = tr->n_tracks;
+  const int nblk = (n + NN - 1) / NN;
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const int blk = next.fetch_add(1);
+      if (blk >= nblk) break;
+      const int t0 = blk * NN, cnt = std::min(NN, n - t0);
+      FindBlock B;
+      FindBlock Out;  // snapshot of stopped lanes (OutputTrackAndHitIdx at stop time)
+      auto lane = [&](int l) { return t0 + (l < cnt ? l : 0); };
+      for (int k = 0; k < 6; ++k)
+        for (int l = 0; l < NN; ++l) B.par[k * NN + l] = tr->par[(size_t)k * n + lane(l)];
+      for (int k = 0; k < 21; ++k)
+        for (int l = 0; l < NN; ++l) B.err[k * NN + l] = tr->err[(size_t)k * n + lane(l)];
+      for (int l = 0; l < NN; ++l) {
+        B.chg[l] = tr->chg[lane(l)];
+        B.chi2[l] = tr->chi2[lane(l)];
+        B.nhits[l] = 0;
+        B.nfound[l] = 0;
+        B.ninside[l] = 0;
+        B.ntail[l] = 0;
+        B.stopped[l] = 0;
+        for (int k = 0; k < ev->first_layer && k < kMaxTrkHits; ++k) {  // seed hits
+          B.hot_idx[B.nhits[l]][l] = tr->hot_idx[(size_t)k * n + lane(l)];
+          B.hot_lyr[B.nhits[l]][l] = k;
+          B.nhits[l]++;
+          if (B.hot_idx[B.nhits[l] - 1][l] >= 0) B.nfound[l]++;
+        }
+      }
+      alignas(64) float parP[6 * NN], errP[21 * NN], rad[NN];
+      int xs[NN], xa[kHitIdxMax * NN], wsr[NN], fail[NN];
+      for (int layer = ev->first_layer; layer < ev->n_layers; ++layer) {
+        const orc_layer_desc& d = ev->layers[layer];
+        for (int l = 0; l < NN; ++l) rad[l] = d.propagate_to;
+        if (d.is_barrel)
+          propagate_helix_to_r(B.err, B.par, B.chg, rad, errP, parP, cnt, ev->pflags, nullptr, fail, ctx);
+        else
+          propagate_helix_to_z(B.err, B.par, B.chg, rad, errP, parP, cnt, ev->pflags, nullptr, ctx);
+        select_hit_indices(d, E.loh[layer], parP, errP, cnt, xs, xa, wsr);
+        if (d.is_barrel) {                                        // MkBuilder.cc:1431-1455
+          const float r_min_sqr = d.rin * d.rin;
+          for (int i = 0; i < cnt; ++i) {
+            if (!B.stopped[i]) {
+              const float x = parP[0 * NN + i], y = parP[1 * NN + i];
+              if (x * x + y * y < r_min_sqr) {
+                B.stopped[i] = 1;
+                for (int k = 0; k < 6; ++k) Out.par[k * NN + i] = B.par[k * NN + i];
+                for (int k = 0; k < 21; ++k) Out.err[k * NN + i] = B.err[k * NN + i];
+                Out.chg[i] = B.chg[i]; Out.chi2[i] = B.chi2[i];
+                Out.nhits[i] = B.nhits[i]; Out.nfound[i] = B.nfound[i];
+                for (int k = 0; k < B.nhits[i]; ++k) { Out.hot_idx[k][i] = B.hot_idx[k][i]; Out.hot_lyr[k][i] = B.hot_lyr[k][i]; }
+                wsr[i] = WSR_Outside;
+                xs[i] = 0;
+              }
+            } else {
+              wsr[i] = WSR_Outside;
+              xs[i] = 0;
+            }
+          }
+        }
+        add_best_hit(E, layer, ev->chi2_cut, B, parP, errP, xs, xa, wsr, cnt);
+        for (int i = 0; i < cnt; ++i)                             // MkBuilder.cc:1465-1473
+          if (!B.stopped[i] && B.nhits[i] > 0 && B.hot_idx[std::min(B.nhits[i], kMaxTrkHits) - 1][i] == -2) {
+            B.stopped[i] = 1;
+            for (int k = 0; k < 6; ++k) Out.par[k * NN + i] = B.par[k * NN + i];
+            for (int k = 0; k < 21; ++k) Out.err[k * NN + i] = B.err[k * NN + i];
+            Out.chg[i] = B.chg[i]; Out.chi2[i] = B.chi2[i];
+            Out.nhits[i] = B.nhits[i]; Out.nfound[i] = B.nfound[i];
+            for (int k = 0; k < B.nhits[i]; ++k) { Out.hot_idx[k][i] = B.hot_idx[k][i]; Out.hot_lyr[k][i] = B.hot_lyr[k][i]; }
+          }
+      }
+      for (int i = 0; i < cnt; ++i) {
+        const FindBlock& S = B.stopped[i] ? Out : B;
+        const size_t t = (size_t)t0 + i;
+        for (int k = 0; k < 6; ++k) tr->par[k * (size_t)n + t] = S.par[k * NN + i];
+        for (int k = 0; k < 21; ++k) tr->err[k * (size_t)n + t] = S.err[k * NN + i];
+        tr->chg[t] = S.chg[i];
+        tr->chi2[t] = S.chi2[i];
+        tr->n_found[t] = S.nfound[i];
+        for (int k = 0; k < ev->n_layers; ++k) tr->hot_idx[(size_t)k * n + t] = -9;
+        for (int k = 0; k < S.nhits[i] && k < kMaxTrkHits; ++k) {
+          const int ly = S.hot_lyr[k][i];
+          if (ly >= 0 && ly < ev->n_layers) tr->hot_idx[(size_t)ly * n + t] = S.hot_idx[k][i];
+        }
+      }
+    }
+  };
+  n_threads = std::max(1, n_threads);
+  std::vector<std::thread> pool;
+  for (int i = 1; i < n_threads; ++i) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  return 0;
+}
+
+}  // extern "C"

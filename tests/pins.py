@@ -121,6 +121,28 @@ def pin_jacobian_fd(make):
         assert med[i, j] < 0.06, ((i, j), med[i, j])
     for (i, j) in [(2, 2), (3, 3), (5, 5)]:
         assert np.all(J[i, j] == 1)
+    # Sub-micron steps (|r - r0| < 1e-4 from the first iteration on): here the
+    # accumulated derivative terms (PropagationMPlex.icc:115-144) carry the
+    # whole derivative, so J - I must follow the finite differences closely,
+    # including the tiny phi-phi term.
+    r0 = np.hypot(a[0], a[1])
+    rs = r0 + 5e-5
+    _, J, _ = orc.helix_at_r(a, chg, rs, dtype=np.float64)
+    for j in range(6):
+        ap, am = a.copy(), a.copy()
+        ap[j] += h[j]
+        am[j] -= h[j]
+        pp, _, _ = orc.helix_at_r(ap, chg, rs, dtype=np.float64)
+        pm, _, _ = orc.helix_at_r(am, chg, rs, dtype=np.float64)
+        Jfd[:, j] = (pp - pm) / (2 * h[j])
+    eye = np.eye(6)[:, :, None]
+    rel = np.abs((J - eye) - (Jfd - eye)) / np.maximum(np.abs(Jfd - eye), 1e-12)
+    med = np.median(rel, axis=2)
+    bounds = {(0, 0): 5e-3, (0, 1): 5e-3, (1, 0): 5e-3, (1, 1): 5e-3, (2, 0): 5e-3, (2, 1): 5e-3,
+              (4, 0): 5e-3, (4, 1): 5e-3, (0, 4): 5e-3, (1, 4): 5e-3, (4, 3): 5e-3, (2, 4): 0.05,
+              (2, 5): 1e-3, (4, 4): 0.5}
+    for (i, j), bnd in bounds.items():
+        assert med[i, j] < bnd, ("sub-micron", (i, j), med[i, j])
     # endcap: helixAtZ's Jacobian is analytic -> tight agreement
     geo, p, ev = _ev(3, n=20000, seed=6, eta_max=2.5)
     a, b, chg = _steps(ev, 2, endcap=True)

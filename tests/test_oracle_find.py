@@ -1,0 +1,290 @@
+"""Pins of the oracle's best-hit finding (LayerOfHits, SelectHitIndices, AddBestHit, FindTracksBestHit).
+
+The reference holds no golden vectors for finding, so each piece is checked against an independent
+formulation:
+  * LayerOfHits (HitStructures.cc:77-183): numpy stable argsort of the bin keys, bin ranges by scan;
+  * SelectHitIndices (MkFinder.cc:227-684): brute-force window test over all hits of the layer, in the
+    (q-bin, phi-bin, sorted position) scan order, with the 16-hit cap;
+  * the FindTracksBestHit loop (MkBuilder.cc:1395-1478, MkFinder.cc:691-832): a numpy composition of the
+    oracle's propagation / chi2 / update primitives with the stop rules and fake-hit codes written out
+    per track, plus physics: on noise-free events nearly every picked hit is the generating track's.
+"""
+import numpy as np
+import pytest
+
+import mkfit_amd as mk
+import oracle
+
+TWO_PI = np.float32(2 * np.pi)
+F_PHI_FINE = np.float32(1024 / np.float32(2 * np.pi))
+
+
+def _layer(**kw):
+    d = dict(is_barrel=1, rin=19.99, rout=20.01, zmin=-60.0, zmax=60.0, propagate_to=20.0, q_bin=2.0,
+             min_dphi=0.002, max_dphi=0.1, min_dq=0.3, max_dq=2.0, c_dp_sf=1.1, c_dq_sf=1.1, c_c2_sf=1.1)
+    d.update(kw)
+    return d
+
+
+def _random_hits(rng, n, r=20.0, zmax=62.0, ties=0):
+    phi = rng.uniform(-np.pi, np.pi, n)
+    z = rng.uniform(-zmax, zmax, n)
+    if ties:   # duplicate positions so that equal keys must keep their input order
+        k = rng.integers(0, n, ties)
+        phi[-ties:] = phi[k]
+        z[-ties:] = z[k]
+    h = np.zeros((n, 9), np.float32)
+    h[:, 0] = r * np.cos(phi); h[:, 1] = r * np.sin(phi); h[:, 2] = z
+    h[:, 3] = 1e-6; h[:, 5] = 1e-6; h[:, 8] = 1e-4
+    return h
+
+
+def _keys(desc, phis, qs):
+    nq = np.int32(0)
+    _, _, _, _, nq = oracle.loh_build(desc, np.zeros((0, 9), np.float32))
+    # setup_bins (HitStructures.cc:11-32), bin width q_bin > 0
+    qmin, qmax, dq = np.float32(desc["zmin"]), np.float32(desc["zmax"]), np.float32(desc["q_bin"])
+    extent = qmax - qmin
+    nqb = int(np.ceil(extent / dq))
+    extra = np.float32(0.5) * (np.float32(nqb) * dq - extent)
+    fq = np.float32(nqb) / (qmax - qmin)
+    qb = np.clip(((qs - (qmin - extra)) * fq).astype(np.int64), 0, nqb - 1)
+    pf = np.floor(F_PHI_FINE * (phis + np.float32(np.pi))).astype(np.int64)
+    return pf + (qb << 16), nqb
+
+
+def test_loh_matches_stable_sort():
+    rng = np.random.default_rng(11)
+    desc = _layer()
+    h = _random_hits(rng, 5000, ties=300)
+    ranks, bins, phis, qs, nq = oracle.loh_build(desc, h)
+    assert sorted(ranks.tolist()) == list(range(len(h)))
+    # phi / q of the sorted hits are those of the ranked input hits (Hit::phi, Hit::z)
+    np.testing.assert_allclose(phis, np.arctan2(h[ranks, 1], h[ranks, 0]), atol=3e-7)
+    assert np.array_equal(qs, h[ranks, 2])
+    key, nqb = _keys(desc, phis, qs)
+    assert nqb == nq
+    # stable sort: the oracle's order equals numpy's stable argsort of the same keys
+    key_in = np.empty_like(key)
+    key_in[ranks] = key
+    assert np.array_equal(np.argsort(key_in, kind="stable"), ranks)
+    # bins: each (q, phi) bin's [first, end) holds exactly the sorted hits with that coarse key
+    coarse = (key >> 16) * 256 + ((key & 0x3ff) >> 2)
+    first, end = bins[:, 0], bins[:, 1]
+    for b in np.unique(coarse):
+        idx = np.nonzero(coarse == b)[0]
+        assert first[b] == idx[0] and end[b] == idx[-1] + 1
+    empty = np.setdiff1d(np.arange(nq * 256), coarse)
+    assert np.all(first[empty] == end[empty])
+
+
+def test_loh_q_bins_clamped_and_counted():
+    desc = _layer(q_bin=-7.0)   # q_bin < 0: a fixed number of bins over [zmin, zmax]
+    _, _, _, _, nq = oracle.loh_build(desc, np.zeros((0, 9), np.float32))
+    assert nq == 7
+    desc = _layer(zmin=-59.0, zmax=60.0, q_bin=2.0)   # extent 119 -> 60 bins
+    _, _, _, _, nq = oracle.loh_build(desc, np.zeros((0, 9), np.float32))
+    assert nq == 60
+    rng = np.random.default_rng(3)
+    h = _random_hits(rng, 500, zmax=90.0)   # hits beyond the layer's z range land in the edge bins
+    ranks, bins, phis, qs, nq = oracle.loh_build(desc, h)
+    assert bins[:, 1].max() == len(h)
+
+
+def _cdist(a):
+    return np.where(a > np.float32(np.pi), TWO_PI - a, a)
+
+
+def _brute_select(desc, h, par, err, ranks, phis, qs):
+    """Window test over every hit; returns per-track lists in the reference's scan order."""
+    n = par.shape[1]
+    key, nqb = _keys(desc, phis, qs)
+    pos_of = np.empty(len(h), np.int64)
+    pos_of[ranks] = np.arange(len(h))
+    out, wins = [], []
+    for t in range(n):
+        x, y, z = par[0, t], par[1, t], par[2, t]
+        r2 = x * x + y * y
+        dphidx, dphidy = -y / r2, x / r2
+        dphi2 = dphidx * dphidx * err[0, t] + dphidy * dphidy * err[2, t] + 2 * dphidx * dphidy * err[1, t]
+        phi = np.arctan2(y, x)
+        dphi = np.clip(max(3 * np.sqrt(abs(dphi2)), desc["min_dphi"]), desc["min_dphi"], desc["max_dphi"])
+        dq = np.clip(abs(3 * np.sqrt(err[5, t])), desc["min_dq"], desc["max_dq"])
+        ddq = np.abs(z - h[:, 2])
+        ddphi = _cdist(np.abs(phi - np.arctan2(h[:, 1], h[:, 0])))
+        inside = np.nonzero((ddq < dq) & (ddphi < dphi))[0]
+        # scan order: q bin, then phi bin counted from the window's first phi bin, then sorted position
+        pb1 = int(np.floor(F_PHI_FINE * (phi - dphi + np.float32(np.pi)))) >> 2
+        kk = key[pos_of[inside]]
+        order = np.lexsort((pos_of[inside], (((kk & 0x3ff) >> 2) - pb1) % 256, kk >> 16))
+        out.append(inside[order])
+        wins.append((dq, dphi))
+    return out, wins
+
+
+def test_select_hit_indices_matches_brute_force():
+    rng = np.random.default_rng(5)
+    desc = _layer()
+    h = _random_hits(rng, 80000)
+    n = 3000
+    par = np.zeros((6, n), np.float32); err = np.zeros((21, n), np.float32)
+    phi = rng.uniform(-np.pi, np.pi, n).astype(np.float32)
+    par[0] = 20 * np.cos(phi); par[1] = 20 * np.sin(phi); par[2] = rng.uniform(-61, 61, n)
+    par[3] = rng.uniform(0.1, 2, n); par[4] = phi; par[5] = rng.uniform(0.3, 2.8, n)
+    s = rng.uniform(1e-4, 0.3, n).astype(np.float32) ** 2     # position variances: windows at both clamps
+    err[0] = s; err[2] = s; err[1] = rng.uniform(-0.5, 0.5, n) * s; err[5] = rng.uniform(1e-4, 0.8, n) ** 2
+    xs, xa, wsr = oracle.select_hit_indices(desc, h, par, err)
+    ranks, bins, phis, qs, nq = oracle.loh_build(desc, h)
+    ref, wins = _brute_select(desc, h, par, err, ranks, phis, qs)
+    capped = edge = 0
+    for t in range(n):
+        if wsr[t] == 2:
+            assert xs[t] == -1
+            assert abs(par[2, t]) > 60.0
+            continue
+        got = xa[:xs[t], t]
+        exp = ref[t][:16]
+        if not np.array_equal(got, exp):
+            # only hits on the window edge (float32 rounding of the limits) may differ
+            dq, dphi = wins[t]
+            diff = np.setxor1d(got, exp)
+            ph = np.arctan2(h[diff, 1], h[diff, 0])
+            near = (np.abs(np.abs(par[2, t] - h[diff, 2]) - dq) < 1e-5) | \
+                   (np.abs(_cdist(np.abs(np.arctan2(par[1, t], par[0, t]) - ph)) - dphi) < 1e-5)
+            assert near.all(), (t, got, exp)
+            edge += 1
+        capped += len(ref[t]) > 16
+    assert capped > 50 and edge <= 3
+    assert set(np.unique(wsr)) <= {0, 1, 2}
+
+
+def _find_composed(O, ev, descs, first_layer, chi2_cut):
+    """FindTracksBestHit over barrel layers from the oracle's primitives (MkBuilder.cc:1395-1478)."""
+    n = ev["n_seeds"]
+    par = ev["seed_par"].copy(); err = ev["seed_err"].copy(); chg = ev["seed_chg"].copy()
+    chi2 = np.zeros(n, np.float32)
+    L = len(descs)
+    hot = np.full((L, n), -9, np.int32)
+    hot[:first_layer] = ev["seed_hits"][:first_layer]
+    nfound = np.sum(hot[:first_layer] >= 0, axis=0).astype(np.int32)
+    ninside = np.zeros(n, np.int32); ntail = np.zeros(n, np.int32)
+    stopped = np.zeros(n, bool)
+    snap = {}
+    off = ev["hit_offset"]; hits = ev["hits_flat"]
+    for l in range(first_layer, L):
+        d = descs[l]
+        hl = hits[off[l]:off[l + 1]]
+        errP, parP, _ = O.propagate_to_r(err, par, chg, np.full(n, d["propagate_to"], np.float32), pflags=0)
+        xs, xa, wsr = oracle.select_hit_indices(d, hl, parP, errP)
+        r2 = parP[0] * parP[0] + parP[1] * parP[1]
+        for t in range(n):
+            if not stopped[t] and r2[t] < np.float32(d["rin"]) * np.float32(d["rin"]):
+                stopped[t] = True
+                snap[t] = (par[:, t].copy(), err[:, t].copy(), chg[t], chi2[t], hot[:, t].copy(), nfound[t])
+            if stopped[t]:
+                wsr[t] = 2; xs[t] = 0
+        best = np.full(n, -1); minc = np.full(n, np.float32(chi2_cut), np.float32)
+        for hc in range(16):
+            m = xs > hc
+            if not m.any():
+                break
+            ms_par = np.zeros((3, n), np.float32); ms_err = np.ones((6, n), np.float32)
+            idx = xa[hc, m]
+            ms_par[:, m] = hl[idx, :3].T; ms_err[:, m] = hl[idx, 3:].T
+            _, _, c2 = O.kalman_operation(1, errP, parP, ms_err, ms_par)
+            c2 = np.abs(c2)
+            better = m & (c2 < minc)
+            minc[better] = c2[better]; best[better] = xa[hc, better]
+        ms_par = parP[:3].copy(); ms_err = np.tile(np.array([666, 0, 666, 0, 0, 666], np.float32)[:, None], (1, n))
+        for t in range(n):
+            if wsr[t] != 2 and best[t] >= 0:
+                ms_par[:, t] = hl[best[t], :3]; ms_err[:, t] = hl[best[t], 3:]
+        err_new, par_new, _ = O.kalman_operation(2, errP, parP, ms_err, ms_par)
+        for t in range(n):
+            if wsr[t] == 2:
+                continue                         # AddBestHit adds no hit for lanes outside the layer
+            if best[t] >= 0:
+                chi2[t] += minc[t]; hot[l, t] = best[t]; nfound[t] += 1
+                ninside[t] += ntail[t]; ntail[t] = 0
+            else:
+                code = -3 if wsr[t] == 1 else (-2 if ninside[t] + ntail[t] else -1)
+                hot[l, t] = code
+                ntail[t] += code == -1
+        flip = par_new[3] < 0
+        chg = np.where(flip, -chg, chg); par_new[3] = np.abs(par_new[3])
+        par, err = par_new, err_new
+        for t in range(n):
+            if not stopped[t] and hot[l, t] == -2:
+                stopped[t] = True
+                snap[t] = (par[:, t].copy(), err[:, t].copy(), chg[t], chi2[t], hot[:, t].copy(), nfound[t])
+    for t, (p, e, c, c2, hh, nf) in snap.items():
+        par[:, t] = p; err[:, t] = e; chg[t] = c; chi2[t] = c2; hot[:, t] = hh; nfound[t] = nf
+    return dict(par=par, err=err, chg=chg, chi2=chi2, hot_idx=hot, n_found=nfound)
+
+
+@pytest.fixture(scope="module")
+def find_event():
+    geo = mk.geometry(1)
+    p = mk.default_toy_params(seed=77, simulate_material=0, pt_min=0.5, pt_max=10.0)
+    ev = mk.generate_find_event(geo, p, 400, noise_per_cm2=3.0)
+    return geo, ev
+
+
+def test_find_loop_matches_composition(find_event):
+    geo, ev = find_event
+    descs = mk.barrel_layer_descs(geo)
+    O = oracle.Oracle(bfield=3.8)
+    got = oracle.find_best_hit(ev, descs, first_layer=3, chi2_cut=15.0, bfield=3.8, n_threads=2)
+    ref = _find_composed(O, ev, descs, 3, 15.0)
+    same = np.all(got["hot_idx"] == ref["hot_idx"], axis=0)
+    assert same.mean() >= 0.995, np.nonzero(~same)[0][:10]
+    assert np.array_equal(got["n_found"][same], ref["n_found"][same])
+    assert np.array_equal(got["chg"][same], ref["chg"][same])
+    np.testing.assert_allclose(got["chi2"][same], ref["chi2"][same], rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(got["par"][:, same], ref["par"][:, same], rtol=1e-4, atol=1e-4)
+
+
+def test_find_invariants_and_threads(find_event):
+    geo, ev = find_event
+    descs = mk.barrel_layer_descs(geo)
+    a = oracle.find_best_hit(ev, descs, first_layer=3, bfield=3.8, n_threads=1)
+    b = oracle.find_best_hit(ev, descs, first_layer=3, bfield=3.8, n_threads=5)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    hot = a["hot_idx"]; nlh = np.diff(ev["hit_offset"])
+    assert np.all((hot >= -3) | (hot == -9))
+    assert np.all(hot < nlh[:, None])
+    assert np.array_equal(a["n_found"], np.sum(hot >= 0, axis=0))
+    assert np.all(a["chi2"] >= 0)
+    # after a -2 (too many holes) or a -9 (stopped) nothing more is recorded
+    for t in range(hot.shape[1]):
+        col = hot[3:, t]
+        stop = np.nonzero((col == -2) | (col == -9))[0]
+        if len(stop):
+            assert np.all(col[stop[0] + 1:] == -9)
+
+
+def test_find_noise_free_picks_true_hits():
+    geo = mk.geometry(1)
+    p = mk.default_toy_params(seed=9, simulate_material=0, pt_min=1.0, pt_max=10.0)
+    ev = mk.generate_find_event(geo, p, 2000, noise_per_cm2=0.0)
+    descs = mk.barrel_layer_descs(geo)
+    r = oracle.find_best_hit(ev, descs, first_layer=3, bfield=3.8, n_threads=4)
+    hot = r["hot_idx"][3:]; off = ev["hit_offset"][3:-1]
+    ok = hot >= 0
+    truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(hot, 0)], -5)
+    correct = truth == ev["seed_mc"][None, :]
+    assert correct.sum() / ok.sum() > 0.995
+    assert ok.sum() / hot.size > 0.97
+
+
+def test_find_with_noise_efficiency():
+    geo = mk.geometry(1)
+    p = mk.default_toy_params(seed=10, simulate_material=0, pt_min=1.0, pt_max=10.0)
+    ev = mk.generate_find_event(geo, p, 2000, noise_per_cm2=3.0)
+    descs = mk.barrel_layer_descs(geo)
+    r = oracle.find_best_hit(ev, descs, first_layer=3, bfield=3.8, n_threads=4)
+    hot = r["hot_idx"][3:]; off = ev["hit_offset"][3:-1]
+    ok = hot >= 0
+    truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(hot, 0)], -5)
+    assert (truth == ev["seed_mc"][None, :]).sum() / hot.size > 0.8   # 0.856 at 3 noise hits / cm2
