@@ -137,6 +137,90 @@ int mkf_autotune_batched(mkf_context* ctx, const mkf_fit_batch* batch, float* sc
 /* Number of variants mkf_set_batched_shape accepts. */
 int mkf_num_batched_variants(void);
 
+/* ---- Best-hit track finding -------------------------------------------------
+ *
+ * Hits are the reference's Hit (Hit.h:165-183) as 9 floats: x, y, z, then the
+ * packed 3x3 covariance xx, yx, yy, zx, zy, zz.  Layer l of an event owns the
+ * hits [hit_offset[l], hit_offset[l+1]) of one device array [total][9]. */
+
+/* One layer: LayerInfo geometry (TrackerInfo.h:35-98: m_rin, m_rout, m_zmin,
+ * m_zmax, m_propagate_to, m_q_bin, is_barrel) and the IterationLayerConfig
+ * selection settings (IterationConfig.h:44-90: min/max dphi and dq windows,
+ * dynamic-window and chi2-cut coefficients c_dp_*, c_dq_*, c_c2_*). */
+typedef struct mkf_layer_desc {
+  int is_barrel;
+  float rin, rout, zmin, zmax;
+  float propagate_to;
+  float q_bin; /* > 0: bin width in cm; < 0: number of bins (LayerOfHits::setup_bins) */
+  float min_dphi, max_dphi, min_dq, max_dq;
+  float c_dp_sf, c_dp_0, c_dp_1, c_dp_2;
+  float c_dq_sf, c_dq_0, c_dq_1, c_dq_2;
+  float c_c2_sf, c_c2_0, c_c2_1, c_c2_2;
+} mkf_layer_desc;
+
+#define MKF_MAX_FIND_LAYERS 64
+#define MKF_HIT_IDX_MAX 16 /* candidates per track and layer (MPlexHitIdxMax, MkFitter.h:18) */
+
+/* Window-selection result per track (WSR_Result, TrackerInfo.h:17-31). */
+#define MKF_WSR_INSIDE  0
+#define MKF_WSR_EDGE    1
+#define MKF_WSR_OUTSIDE 2
+
+typedef struct mkf_event_of_hits mkf_event_of_hits;
+
+/* EventOfHits (HitStructures.h:280-319): one LayerOfHits per layer, in the
+ * order the finding visits them.  Device storage grows on demand at load. */
+int mkf_event_of_hits_create(mkf_context* ctx, int n_layers, const mkf_layer_desc* layers, mkf_event_of_hits** out);
+int mkf_event_of_hits_destroy(mkf_event_of_hits* eoh);
+
+/* LayerOfHits::SuckInHits for every layer (HitStructures.cc:77-183): bins each
+ * hit by (q, phi) -- q = z on barrel layers, r on endcaps -- sorts the layer's
+ * hits stably by (q bin, fine phi bin) and records each bin's [first, end)
+ * range.  `hit_offset` is a host array [n_layers + 1]; `hits` a device array
+ * [hit_offset[n_layers]][9].  The hits are copied in sorted order, so `hits`
+ * may be reused once the stream has passed this call. */
+int mkf_event_of_hits_load(mkf_event_of_hits* eoh, const int* hit_offset, const float* hits, void* stream);
+
+/* Device views of a loaded layer (valid until the next load or destroy):
+ * ranks [n] sorted position -> hit index within the layer (LayerOfHits
+ * m_hit_ranks), bins [nq*256][2] first/end per (q bin, phi bin) (m_phi_bin_infos),
+ * phis/qs [n] per sorted hit (m_hit_phis, m_hit_qs). */
+int mkf_event_of_hits_layer(mkf_event_of_hits* eoh, int layer, int* n_hits, int* nq, const int** ranks,
+                            const int** bins, const float** phis, const float** qs);
+
+/* MkFinder::SelectHitIndices (MkFinder.cc:227-684) for n tracks whose state
+ * (par, err; device, element-major) is already at `layer`: xsize [n] (-1 when
+ * the window lies outside the layer), xarr [MKF_HIT_IDX_MAX][n] hit indices
+ * within the layer in scan order (-1 padded), wsr [n] MKF_WSR_*. */
+int mkf_select_hit_indices(mkf_event_of_hits* eoh, int layer, int n, const float* par, const float* err,
+                           int* xsize, int* xarr, int* wsr, void* stream);
+
+typedef struct mkf_find_batch {
+  int n_tracks;
+  float* par;     /* [6][n]  in: seed state; out: state when the track stopped or after the last layer */
+  float* err;     /* [21][n] */
+  int* chg;       /* [n] */
+  float* chi2;    /* [n] in: seed chi2; out: plus the chi2 of every added hit */
+  int* hot_idx;   /* [n_layers][n] in: seed hit indices on layers < first_layer; out: per layer the hit
+                     index within the layer, -1 (no hit, within the tail), -2 (no hit, stop: too many
+                     holes), -3 (no hit, window on the layer edge), or -9 (layer not reached) */
+  int* n_found;   /* [n] out: hits found, seed hits included */
+  int first_layer;/* layers before it hold the seed hits */
+  float chi2_cut; /* IterationParams::chi2Cut_min (IterationConfig.h:101) */
+  int pflags;     /* Config::finding_inter_layer_pflags: MKF_PF_* for the layer-to-layer propagation */
+} mkf_find_batch;
+
+/* MkBuilder::FindTracksBestHit (MkBuilder.cc:1362-1480) over the layers of
+ * `eoh` from first_layer on: per layer, propagate to LayerInfo::m_propagate_to
+ * (PropagateTracksToR / ToZ), stop tracks that cannot reach a barrel layer,
+ * SelectHitIndices, AddBestHit (MkFinder.cc:691-832: the smallest |chi2| below
+ * the dynamic cut among at most MKF_HIT_IDX_MAX candidates, else a fake hit
+ * code), Kalman update with the chosen or fake hit and charge flip
+ * (kalmanPropagateAndUpdate, KalmanUtilsMPlex.cc:471-504, without the
+ * propagation to the hit), and stop after a -2.  One GPU lane carries one seed
+ * through all layers. */
+int mkf_find_tracks_best_hit(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_find_batch* batch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,7 +88,26 @@ ABI_SYMBOLS = [
     "mkf_kalman_operation", "mkf_kalman_operation_endcap",
     "mkf_fit_plain", "mkf_fit_batched",
     "mkf_set_batched_shape", "mkf_get_batched_shape", "mkf_autotune_batched", "mkf_num_batched_variants",
+    "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
+    "mkf_select_hit_indices", "mkf_find_tracks_best_hit",
 ]
+
+LAYER_DESC_FIELDS = ["is_barrel", "rin", "rout", "zmin", "zmax", "propagate_to", "q_bin", "min_dphi", "max_dphi",
+                     "min_dq", "max_dq", "c_dp_sf", "c_dp_0", "c_dp_1", "c_dp_2", "c_dq_sf", "c_dq_0", "c_dq_1",
+                     "c_dq_2", "c_c2_sf", "c_c2_0", "c_c2_1", "c_c2_2"]
+MAX_FIND_LAYERS = 64
+HIT_IDX_MAX = 16
+WSR_INSIDE, WSR_EDGE, WSR_OUTSIDE = 0, 1, 2
+
+
+class MkfLayerDesc(ctypes.Structure):
+    _fields_ = [("is_barrel", ctypes.c_int)] + [(f, ctypes.c_float) for f in LAYER_DESC_FIELDS[1:]]
+
+
+class MkfFindBatch(ctypes.Structure):
+    _fields_ = [("n_tracks", ctypes.c_int), ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p),
+                ("chi2", _c_void_p), ("hot_idx", _c_void_p), ("n_found", _c_void_p), ("first_layer", ctypes.c_int),
+                ("chi2_cut", ctypes.c_float), ("pflags", ctypes.c_int)]
 
 
 def lib_path():
@@ -115,6 +134,14 @@ def lib():
         L.mkf_fit_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p]
         L.mkf_set_batched_shape.argtypes = [_c_void_p, ctypes.POINTER(MkfLaunchShape)]
         L.mkf_get_batched_shape.argtypes = [_c_void_p, ctypes.POINTER(MkfLaunchShape)]
+        L.mkf_event_of_hits_create.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(MkfLayerDesc),
+                                               ctypes.POINTER(_c_void_p)]
+        L.mkf_event_of_hits_destroy.argtypes = [_c_void_p]
+        L.mkf_event_of_hits_load.argtypes = [_c_void_p, _c_void_p, _c_void_p, _c_void_p]
+        L.mkf_event_of_hits_layer.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_int)] + [ctypes.POINTER(_c_void_p)] * 4
+        L.mkf_select_hit_indices.argtypes = [_c_void_p, ctypes.c_int, ctypes.c_int] + [_c_void_p] * 6
+        L.mkf_find_tracks_best_hit.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfFindBatch), _c_void_p]
         L.mkf_autotune_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p, ctypes.c_int,
                                            ctypes.POINTER(MkfLaunchShape), _c_float_p]
         _LIB = L
@@ -260,6 +287,126 @@ def barrel_layer_descs(geo, q_bin=2.0, min_dphi=0.01, max_dphi=0.05, min_dq=1.0,
 # Device-side harness
 # ----------------------------------------------------------------------------
 
+def layer_desc_array(descs):
+    arr = (MkfLayerDesc * len(descs))()
+    for i, d in enumerate(descs):
+        for k, v in d.items():
+            setattr(arr[i], k, v)
+    return arr
+
+
+class EventOfHits:
+    """mkf_event_of_hits: the device LayerOfHits of every layer of one event."""
+
+    def __init__(self, ctx, descs):
+        self.ctx = ctx
+        self.n_layers = len(descs)
+        self._descs = layer_desc_array(descs)
+        h = _c_void_p()
+        _check(lib().mkf_event_of_hits_create(ctx.h, self.n_layers, self._descs, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mkf_event_of_hits_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, hit_offset, hits, stream=None):
+        """hit_offset: host ints [n_layers+1]; hits: device float tensor [total, 9]."""
+        off = np.ascontiguousarray(hit_offset, np.int32)
+        if off.shape != (self.n_layers + 1,):
+            raise ValueError("hit_offset must have n_layers + 1 entries")
+        self._hits = hits
+        _check(lib().mkf_event_of_hits_load(self.h, off.ctypes.data_as(_c_void_p), _ptr(hits),
+                                            self.ctx._stream(stream)))
+
+    def layer_views(self, layer):
+        """Copies of (ranks, bins [nq*256, 2], phis, qs) of one loaded layer, as numpy arrays."""
+        import torch
+        n = ctypes.c_int(); nq = ctypes.c_int()
+        pr, pb, pp, pq = _c_void_p(), _c_void_p(), _c_void_p(), _c_void_p()
+        _check(lib().mkf_event_of_hits_layer(self.h, int(layer), ctypes.byref(n), ctypes.byref(nq), ctypes.byref(pr),
+                                             ctypes.byref(pb), ctypes.byref(pp), ctypes.byref(pq)))
+        torch.cuda.synchronize()
+        dev = torch.device("cuda", self.ctx.device)
+
+        def grab(ptr, count, dtype):
+            out = torch.empty(count, dtype=dtype, device=dev)
+            if count:
+                _check_hip(out.data_ptr(), ptr.value, count * out.element_size())
+            return out.cpu().numpy()
+        ranks = grab(pr, n.value, torch.int32)
+        bins = grab(pb, nq.value * 256 * 2, torch.int32).reshape(-1, 2)
+        phis = grab(pp, n.value, torch.float32)
+        qs = grab(pq, n.value, torch.float32)
+        return ranks, bins, phis, qs, nq.value
+
+    def select_hit_indices(self, layer, par, err, xsize, xarr, wsr, stream=None):
+        n = par.shape[-1]
+        _check(lib().mkf_select_hit_indices(self.h, int(layer), n, _ptr(par), _ptr(err), _ptr(xsize), _ptr(xarr),
+                                            _ptr(wsr), self.ctx._stream(stream)))
+
+
+def _check_hip(dst, src, nbytes):
+    """Device-to-device copy through the HIP runtime torch already loaded."""
+    global _HIPRT
+    if _HIPRT is None:
+        _HIPRT = ctypes.CDLL("libamdhip64.so")
+        _HIPRT.hipMemcpy.argtypes = [_c_void_p, _c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _HIPRT.hipDeviceSynchronize.argtypes = []
+    rc = _HIPRT.hipMemcpy(_c_void_p(dst), _c_void_p(src), ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise MkfError("hipMemcpy failed (%d)" % rc)
+
+
+_HIPRT = None
+
+
+class FindBatch:
+    """Device arrays of one best-hit finding batch (seeds in, found tracks out)."""
+
+    def __init__(self, ev, n_layers, first_layer, device="cuda:0", chi2_cut=15.0, pflags=0):
+        import torch
+        n = int(ev["n_seeds"])
+        dev = torch.device(device)
+        self.n = n
+        self.par = torch.from_numpy(np.ascontiguousarray(ev["seed_par"][:, :n], np.float32)).to(dev)
+        self.err = torch.from_numpy(np.ascontiguousarray(ev["seed_err"][:, :n], np.float32)).to(dev)
+        self.chg = torch.from_numpy(np.ascontiguousarray(ev["seed_chg"][:n], np.int32)).to(dev)
+        self.chi2 = torch.zeros(n, dtype=torch.float32, device=dev)
+        hot = np.full((n_layers, n), -9, np.int32)
+        hot[:first_layer] = ev["seed_hits"][:first_layer, :n]
+        self.hot_idx = torch.from_numpy(hot).to(dev)
+        self.n_found = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.first_layer = int(first_layer)
+        self.chi2_cut = float(chi2_cut)
+        self.pflags = int(pflags)
+        self._seed = [t.clone() for t in (self.par, self.err, self.chg, self.chi2, self.hot_idx)]
+
+    def reset(self):
+        for dst, src in zip((self.par, self.err, self.chg, self.chi2, self.hot_idx), self._seed):
+            dst.copy_(src)
+
+    def c_struct(self):
+        b = MkfFindBatch()
+        b.n_tracks = self.n
+        b.par, b.err, b.chg, b.chi2 = _ptr(self.par), _ptr(self.err), _ptr(self.chg), _ptr(self.chi2)
+        b.hot_idx, b.n_found = _ptr(self.hot_idx), _ptr(self.n_found)
+        b.first_layer, b.chi2_cut, b.pflags = self.first_layer, self.chi2_cut, self.pflags
+        return b
+
+    def numpy(self):
+        return dict(par=self.par.cpu().numpy(), err=self.err.cpu().numpy(), chg=self.chg.cpu().numpy(),
+                    chi2=self.chi2.cpu().numpy(), hot_idx=self.hot_idx.cpu().numpy(),
+                    n_found=self.n_found.cpu().numpy())
+
+
 class Context:
     """Owns an mkf_context (B field + material grid on one device)."""
 
@@ -319,6 +466,9 @@ class Context:
 
     def fit_batched(self, batch, stream=None):
         _check(lib().mkf_fit_batched(self.h, ctypes.byref(batch.c_struct()), self._stream(stream)))
+
+    def find_tracks_best_hit(self, eoh, batch, stream=None):
+        _check(lib().mkf_find_tracks_best_hit(self.h, eoh.h, ctypes.byref(batch.c_struct()), self._stream(stream)))
 
     def set_batched_shape(self, block, blocks_per_cu, variant):
         s = MkfLaunchShape(int(block), int(blocks_per_cu), int(variant))

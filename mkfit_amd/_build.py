@@ -29,7 +29,7 @@ def _run(cmd, cwd=None):
 
 def build_hip():
     out = os.path.join(PKG, "libmkfit_amd.so")
-    srcs = [os.path.join(CSRC, "mkf_fit.hip")]
+    srcs = [os.path.join(CSRC, "mkf_fit.hip"), os.path.join(CSRC, "mkf_find.hip")]
     _run([HIPCC] + HIP_FLAGS + ["-o", out + ".tmp"] + srcs)
     os.replace(out + ".tmp", out)
     return out

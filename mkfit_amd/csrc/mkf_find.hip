@@ -1,0 +1,657 @@
+// mkf_find.hip -- best-hit track finding on gfx950.
+//
+//   LayerOfHits build   k_hit_keys -> stable LSD radix sort (hipCUB) -> k_loh_fill
+//   SelectHitIndices    k_select (per-operation entry point)
+//   FindTracksBestHit   k_find_best_hit: one lane per seed carries its track
+//                       through every layer with the state in VGPRs; hit
+//                       selection, the chi2 of each candidate and the update
+//                       are fused, so per layer only the bin table, the
+//                       (phi, q) of scanned hits and the 9 floats of each
+//                       candidate are read.
+//
+// Event layout in HBM (one event, all layers back to back in visiting order):
+//   sorted hits   [total][9]  hits of every layer in LayerOfHits order
+//   phis, qs      [total]     phi and q (z on barrel, r on endcaps) per sorted hit
+//   ranks         [total]     sorted position -> hit index within its layer
+//   bins          [sum nq*256] int2 {first, end} per (layer, q bin, phi bin),
+//                             positions relative to the layer's first hit
+// Every layer's rows start at its hit offset, so a lane addresses a layer
+// with one scalar base and no per-layer allocations.
+//
+// Sort key: layer << 26 | q bin << 11 | fine phi bin.  Within one layer this is
+// the order of the reference's key fine_phi + (q_bin << 16)
+// (HitStructures.cc:114-128) because the fine phi bin is at most 1024; the
+// layer bits keep layers apart so one sort over the event serves all layers,
+// and LSD radix sort is stable like the reference's radix sort of the keys.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/mkfit_amd.h"
+#include "mkf_internal.h"
+#include "mkf_track_math.h"
+
+using namespace mkf;
+using namespace mkf_detail;
+
+namespace {
+
+constexpr int kNPhi = 256;                      // Config::m_nphi
+constexpr float kFPhiFine = 1024 / kTwoPI;      // HitStructures.h:153-158 (float arithmetic)
+constexpr float kPIOver2 = kPI / 2.0f;          // Config.h:79
+constexpr int kHitIdxMax = MKF_HIT_IDX_MAX;
+constexpr int kLayerShift = 26, kQShift = 11;
+constexpr int kMaxQBins = 1 << 15;
+
+struct LayerDev {
+  int is_barrel, nq, hit_off, n_hits, bin_off;
+  float rin, rout, zmin, zmax, propagate_to;
+  float qmin, fq;                               // bins after setup_bins (extended range, scale)
+  float min_dphi, max_dphi, min_dq, max_dq;
+  float c_dp_sf, c_dp_0, c_dp_1, c_dp_2;
+  float c_dq_sf, c_dq_0, c_dq_1, c_dq_2;
+  float c_c2_sf, c_c2_0, c_c2_1, c_c2_2;
+};
+
+// LayerOfHits::setup_bins (HitStructures.cc:11-32), same float arithmetic.
+void setup_bins(float qmin, float qmax, float dq, int& nq, float& qmin_ext, float& fq) {
+  if (dq < 0) {
+    nq = (int)-dq;
+    qmin_ext = qmin;
+  } else {
+    const float extent = qmax - qmin;
+    nq = (int)std::ceil(extent / dq);
+    const float extra = 0.5f * (nq * dq - extent);
+    qmin_ext = qmin - extra;
+  }
+  fq = nq / (qmax - qmin);
+}
+
+__device__ __forceinline__ int q_bin_checked(const LayerDev& d, float q) {   // HitStructures.h:237-245
+  const int b = (int)((q - d.qmin) * d.fq);
+  return min(max(b, 0), d.nq - 1);
+}
+__device__ __forceinline__ int phi_bin_fine(float phi) { return (int)floorf(kFPhiFine * (phi + kPI)); }
+__device__ __forceinline__ int phi_bin(float phi) { return phi_bin_fine(phi) >> 2; }
+__device__ __forceinline__ float cdist(float a) { return a > kPI ? kTwoPI - a : a; }   // Config.h:441
+
+struct OffsetsArg {
+  int off[MKF_MAX_FIND_LAYERS + 1];
+};
+
+// Per-load hit offsets into the layer table (stream-ordered; no host staging).
+__global__ void k_set_offsets(OffsetsArg o, int n_layers, LayerDev* __restrict__ L) {
+  const int l = threadIdx.x;
+  if (l < n_layers) {
+    L[l].hit_off = o.off[l];
+    L[l].n_hits = o.off[l + 1] - o.off[l];
+  }
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, long n, int v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Hit::phi / Hit::z / Hit::r (Hit.h:64-80) and the sort key of every hit.
+__global__ void __launch_bounds__(256)
+k_hit_keys(int total, int n_layers, const float* __restrict__ hits, const LayerDev* __restrict__ L,
+           uint32_t* __restrict__ keys, int* __restrict__ vals, float* __restrict__ phi_o, float* __restrict__ q_o) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int lo = 0, hi = n_layers - 1;   // layer owning hit i: the last one starting at or before it
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L[mid].hit_off <= i) lo = mid; else hi = mid - 1;
+  }
+  const LayerDev& d = L[lo];
+  const float x = hits[(size_t)i * 9 + 0], y = hits[(size_t)i * 9 + 1], z = hits[(size_t)i * 9 + 2];
+  const float phi = atan2f(y, x);
+  const float q = d.is_barrel ? z : sqrtf(x * x + y * y);
+  const int qb = q_bin_checked(d, q);
+  const int pf = phi_bin_fine(phi);
+  keys[i] = ((uint32_t)lo << kLayerShift) | ((uint32_t)qb << kQShift) | (uint32_t)pf;
+  vals[i] = i;
+  phi_o[i] = phi;
+  q_o[i] = q;
+}
+
+// Sorted copies and bin ranges (HitStructures.cc:130-183).  The fine phi bin
+// 1024 (phi == +pi exactly) shares coarse bin 0 with the row's first bins but
+// sorts last in the row; the reference's scan then overwrites that bin's range
+// with the later run.  Pass 0 writes every other run, pass 1 the 1024 runs, so
+// the later run wins here as well.
+__global__ void __launch_bounds__(256)
+k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* __restrict__ vals_s,
+           const LayerDev* __restrict__ L, const float* __restrict__ hits, const float* __restrict__ phi_o,
+           const float* __restrict__ q_o, int* __restrict__ ranks, int2* __restrict__ bins,
+           float* __restrict__ shits, float* __restrict__ phis, float* __restrict__ qs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint32_t key = keys_s[i];
+  const LayerDev& d = L[key >> kLayerShift];
+  const int local = i - d.hit_off;
+  if (pass == 0) {
+    const int j = vals_s[i];
+    ranks[i] = j - d.hit_off;
+    phis[i] = phi_o[j];
+    qs[i] = q_o[j];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) shits[(size_t)i * 9 + e] = hits[(size_t)j * 9 + e];
+  }
+  const bool fine1024 = (key & 0x400u) != 0;
+  if ((pass == 1) != fine1024) return;
+  const uint32_t jq = key & ~3u;
+  const bool first = i == 0 || (keys_s[i - 1] & ~3u) != jq;
+  const bool last = i == total - 1 || (keys_s[i + 1] & ~3u) != jq;
+  const int qb = (int)((key >> kQShift) & (kMaxQBins - 1));
+  const int pb = (int)((key & 0x3ffu) >> 2);
+  int2* b = bins + d.bin_off + qb * kNPhi + pb;
+  if (first) b->x = local;
+  if (last) b->y = local + 1;
+}
+
+// ---- SelectHitIndices window (MkFinder.cc:166-195, 227-430) -----------------
+
+struct Window {
+  float q, phi, dq, dphi;
+  int qb1, qb2, pb1, pb2, wsr;
+};
+
+__device__ __forceinline__ int wsr_z(const LayerDev& d, float z, float dz) {   // TrackerInfo.h:116-123
+  if (z > d.zmax + dz || z < d.zmin - dz) return MKF_WSR_OUTSIDE;
+  if (z < d.zmax - dz && z > d.zmin + dz) return MKF_WSR_INSIDE;
+  return MKF_WSR_EDGE;
+}
+__device__ __forceinline__ int wsr_r(const LayerDev& d, float r, float dr) {   // TrackerInfo.h:125-138, no hole
+  if (r > d.rout + dr || r < d.rin - dr) return MKF_WSR_OUTSIDE;
+  if (r < d.rout - dr && r > d.rin + dr) return MKF_WSR_INSIDE;
+  return MKF_WSR_EDGE;
+}
+
+__device__ __forceinline__ void hit_window(const LayerDev& d, const float (&par)[6], const float (&err)[21],
+                                           Window& w) {
+  float min_dq = d.min_dq, max_dq = d.max_dq, min_dphi = d.min_dphi, max_dphi = d.max_dphi;
+  const float invpt = par[3];
+  const float theta = fabsf(par[5] - kPIOver2);
+  {                                              // getHitSelDynamicWindows
+    const float max_invpt = invpt > 10.0f ? 10.0f : invpt;
+    const float this_dq = d.c_dq_0 * max_invpt + d.c_dq_1 * theta + d.c_dq_2;
+    if (d.c_dq_sf * this_dq > 0.f) {
+      min_dq = d.c_dq_sf * this_dq;
+      max_dq = 2.0f * min_dq;
+    }
+    const float this_dphi = d.c_dp_0 * max_invpt + d.c_dp_1 * theta + d.c_dp_2;
+    if (d.c_dp_sf * this_dphi > min_dphi) {
+      min_dphi = d.c_dp_sf * this_dphi;
+      max_dphi = 2.0f * min_dphi;
+    }
+  }
+  const float x = par[0], y = par[1];
+  const float r2 = x * x + y * y;
+  const float dphidx = -y / r2, dphidy = x / r2;
+  const float dphi2 = dphidx * dphidx * err[S(0, 0)] + dphidy * dphidy * err[S(1, 1)] +
+                      2 * dphidx * dphidy * err[S(0, 1)];
+  const float phi = atan2f(y, x);
+  float dphi = fmaxf(3.f * sqrtf(fabsf(dphi2)), min_dphi);
+  float q, dq;
+  if (d.is_barrel) {
+    const float z = par[2];
+    dq = fabsf(3.f * sqrtf(err[S(2, 2)]));
+    const float edge = fabsf(0.5f * (d.rout - d.rin) / tanf(par[5]));
+    w.wsr = wsr_z(d, z, sqrtf(dq * dq + edge * edge));
+    q = z;
+  } else {
+    const float r = sqrtf(r2);
+    dq = 3.f * sqrtf(fabsf(x * x * err[S(0, 0)] + y * y * err[S(1, 1)] + 2 * x * y * err[S(0, 1)]) / r2);
+    const float edge = fabsf(0.5f * (d.zmax - d.zmin) * tanf(par[5]));
+    w.wsr = wsr_r(d, r, sqrtf(dq * dq + edge * edge));
+    q = r;
+  }
+  dphi = fminf(fmaxf(fabsf(dphi), min_dphi), max_dphi);
+  dq = fminf(fmaxf(dq, min_dq), max_dq);
+  w.q = q;
+  w.phi = phi;
+  w.dq = dq;
+  w.dphi = dphi;
+  w.qb1 = q_bin_checked(d, q - dq);
+  w.qb2 = q_bin_checked(d, q + dq) + 1;
+  w.pb1 = phi_bin(phi - dphi);
+  w.pb2 = phi_bin(phi + dphi) + 1;
+  // A window of at most one full turn spans at most kNPhi + 2 phi bins; wider
+  // dynamic windows are cut there so the scan stays bounded.
+  w.pb2 = min(w.pb2, w.pb1 + kNPhi + 2);
+}
+
+// Scan of the window's bins in the reference's order (q bins outer, phi bins
+// inner, sorted hits within a bin), at most kHitIdxMax accepted hits; calls
+// on_hit(global sorted position) for each (MkFinder.cc:433-684).
+template <class F>
+__device__ __forceinline__ int scan_window(const LayerDev& d, const Window& w, const int2* __restrict__ bins,
+                                           const float* __restrict__ phis, const float* __restrict__ qs, F&& on_hit) {
+  int cnt = 0;
+  for (int qi = w.qb1; qi < w.qb2 && cnt < kHitIdxMax; ++qi)
+    for (int pi = w.pb1; pi < w.pb2 && cnt < kHitIdxMax; ++pi) {
+      const int2 be = bins[d.bin_off + qi * kNPhi + (pi & 0xff)];
+      for (int hi = be.x; hi < be.y && cnt < kHitIdxMax; ++hi) {
+        const int gi = d.hit_off + hi;
+        const float ddq = fabsf(w.q - qs[gi]);
+        const float ddphi = cdist(fabsf(w.phi - phis[gi]));
+        if (ddq >= w.dq || ddphi >= w.dphi) continue;
+        on_hit(gi);
+        ++cnt;
+      }
+    }
+  return cnt;
+}
+
+__global__ void __launch_bounds__(256)
+k_select(LayerDev d, int n, const float* __restrict__ par, const float* __restrict__ err,
+         const int2* __restrict__ bins, const float* __restrict__ phis, const float* __restrict__ qs,
+         const int* __restrict__ ranks, int* __restrict__ xsize, int* __restrict__ xarr, int* __restrict__ wsr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float p[6], e[21];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) p[k] = par[(size_t)k * n + t];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) e[k] = err[(size_t)k * n + t];
+  Window w;
+  hit_window(d, p, e, w);
+  wsr[t] = w.wsr;
+  int cnt = -1;
+  if (w.wsr != MKF_WSR_OUTSIDE) {
+    int c = 0;
+    scan_window(d, w, bins, phis, qs, [&](int gi) { xarr[(size_t)(c++) * n + t] = ranks[gi]; });
+    cnt = c;
+  }
+  xsize[t] = cnt;
+  for (int k = max(cnt, 0); k < kHitIdxMax; ++k) xarr[(size_t)k * n + t] = -1;
+}
+
+// ---- FindTracksBestHit -------------------------------------------------------
+
+struct FindArgs {
+  int n, n_layers, first_layer;
+  float chi2_cut;
+  int pflags;
+  float bfield;
+  Material mat;
+  float* __restrict__ par;
+  float* __restrict__ err;
+  int* __restrict__ chg;
+  float* __restrict__ chi2;
+  int* __restrict__ hot;
+  int* __restrict__ n_found;
+  const LayerDev* __restrict__ layers;
+  const int2* __restrict__ bins;
+  const float* __restrict__ phis;
+  const float* __restrict__ qs;
+  const float* __restrict__ shits;
+  const int* __restrict__ ranks;
+};
+
+// getHitSelDynamicChi2Cut (MkFinder.cc:202-220)
+__device__ __forceinline__ float dyn_chi2_cut(const LayerDev& d, float cut_min, float invpt, float theta_in) {
+  const float theta = fabsf(theta_in - kPIOver2);
+  const float max_invpt = invpt > 10.0f ? 10.0f : invpt;
+  const float this_c2 = d.c_c2_0 * max_invpt + d.c_c2_1 * theta + d.c_c2_2;
+  return d.c_c2_sf * this_c2 > cut_min ? d.c_c2_sf * this_c2 : cut_min;
+}
+
+__device__ __forceinline__ void load_hit9(const float* __restrict__ h, float (&mp)[3], float (&me)[6]) {
+#pragma unroll
+  for (int e = 0; e < 3; ++e) mp[e] = h[e];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) me[e] = h[3 + e];
+}
+
+// The reference's block loop (MkBuilder.cc:1395-1478) per lane.  A lane that
+// stops leaves the loop with the state it had at that point, which is the
+// state the reference outputs for it (OutputTrackAndHitIdx at stop time).
+// HitOnTrack bookkeeping as MkFinder::add_hit (MkFinder.h:312-354); with at
+// most MKF_MAX_FIND_LAYERS layers a track never exceeds Config::nMaxTrkHits.
+__global__ void __launch_bounds__(256) k_find_best_hit(FindArgs a) {
+  const int stride = gridDim.x * blockDim.x;
+  const size_t n = (size_t)a.n;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += stride) {
+    float par[6], err[21];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) par[e] = a.par[e * n + t];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) err[e] = a.err[e * n + t];
+    int chg = a.chg[t];
+    float chi2 = a.chi2[t];
+    int nfound = 0, ninside = 0, ntail = 0;
+    for (int l = 0; l < a.first_layer; ++l) nfound += a.hot[l * n + t] >= 0;
+    int layer = a.first_layer;
+    for (; layer < a.n_layers; ++layer) {
+      const LayerDev& d = a.layers[layer];
+      float pp[6], pe[21];
+#pragma unroll
+      for (int e = 0; e < 6; ++e) pp[e] = par[e];
+#pragma unroll
+      for (int e = 0; e < 21; ++e) pe[e] = err[e];
+      if (d.is_barrel)
+        propagate_to_r(pp, pe, chg, d.propagate_to, a.pflags, false, a.bfield, a.mat);
+      else
+        propagate_to_z(pp, pe, chg, d.propagate_to, a.pflags, false, a.bfield, a.mat);
+      // Tracks that cannot reach a barrel layer stop here (MkBuilder.cc:1431-1455).
+      if (d.is_barrel && pp[0] * pp[0] + pp[1] * pp[1] < d.rin * d.rin) break;
+      Window w;
+      hit_window(d, pp, pe, w);
+      float mp[3] = {pp[0], pp[1], pp[2]};
+      float me[6] = {666.f, 0.f, 666.f, 0.f, 0.f, 666.f};   // fake hit (MkFinder.cc:776-800)
+      int code = -9;
+      if (w.wsr != MKF_WSR_OUTSIDE) {
+        float min_c2 = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
+        int best = -1;
+        scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) {
+          float hp[3], he[6];
+          load_hit9(a.shits + (size_t)gi * 9, hp, he);
+          const float c2 = fabsf(d.is_barrel ? kalman_barrel<true, false>(pp, pe, hp, he)
+                                             : kalman_endcap<true, false>(pp, pe, hp, he));
+          if (c2 < min_c2) {
+            min_c2 = c2;
+            best = gi;
+          }
+        });
+        if (best >= 0) {
+          load_hit9(a.shits + (size_t)best * 9, mp, me);
+          chi2 += min_c2;
+          code = a.ranks[best];
+          ++nfound;
+          ninside += ntail;
+          ntail = 0;
+        } else {
+          code = w.wsr == MKF_WSR_EDGE ? -3 : (ninside + ntail ? -2 : -1);
+          ntail += code == -1;
+        }
+      }
+      a.hot[layer * n + t] = code;
+      if (d.is_barrel)
+        kalman_barrel<false, true>(pp, pe, mp, me);
+      else
+        kalman_endcap<false, true>(pp, pe, mp, me);
+      if (pp[3] < 0.f) {                                      // KalmanUtilsMPlex.cc:497-504
+        chg = -chg;
+        pp[3] = -pp[3];
+      }
+#pragma unroll
+      for (int e = 0; e < 6; ++e) par[e] = pp[e];
+#pragma unroll
+      for (int e = 0; e < 21; ++e) err[e] = pe[e];
+      if (code == -2) {                                       // MkBuilder.cc:1465-1473
+        ++layer;
+        break;
+      }
+    }
+    for (; layer < a.n_layers; ++layer) a.hot[layer * n + t] = -9;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) a.par[e * n + t] = par[e];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) a.err[e * n + t] = err[e];
+    a.chg[t] = chg;
+    a.chi2[t] = chi2;
+    a.n_found[t] = nfound;
+  }
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+
+struct mkf_event_of_hits {
+  mkf_context* ctx = nullptr;
+  int n_layers = 0;
+  std::vector<LayerDev> layers;   // host copy; hit_off / n_hits as of the last load
+  LayerDev* d_layers = nullptr;
+  int total_bins = 0;
+  int2* d_bins = nullptr;
+  size_t cap = 0;                 // hit capacity of the arrays below
+  uint32_t* d_keys = nullptr;
+  uint32_t* d_keys_s = nullptr;
+  int* d_vals = nullptr;
+  int* d_vals_s = nullptr;
+  float* d_phi_o = nullptr;
+  float* d_q_o = nullptr;
+  float* d_phis = nullptr;
+  float* d_qs = nullptr;
+  float* d_shits = nullptr;
+  int* d_ranks = nullptr;
+  void* d_temp = nullptr;
+  size_t temp_bytes = 0;
+  bool loaded = false;
+};
+
+namespace {
+
+void free_hit_arrays(mkf_event_of_hits* h) {
+  void* ps[] = {h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s, h->d_phi_o, h->d_q_o,
+                h->d_phis, h->d_qs,     h->d_shits, h->d_ranks, h->d_temp};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  h->d_keys = h->d_keys_s = nullptr;
+  h->d_vals = h->d_vals_s = h->d_ranks = nullptr;
+  h->d_phi_o = h->d_q_o = h->d_phis = h->d_qs = h->d_shits = nullptr;
+  h->d_temp = nullptr;
+  h->temp_bytes = 0;
+  h->cap = 0;
+}
+
+int grow(mkf_event_of_hits* h, size_t total, hipStream_t st) {
+  if (total <= h->cap && h->d_temp) return MKF_OK;
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipDeviceSynchronize());
+  free_hit_arrays(h);
+  const size_t cap = std::max<size_t>(total, 1024);
+  HIP_TRY(hipMalloc(&h->d_keys, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_keys_s, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_vals, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_vals_s, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_phi_o, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_q_o, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_phis, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_qs, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_ranks, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_shits, cap * 9 * 4));
+  size_t tb = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s, (int)cap,
+                                             0, 32, st));
+  HIP_TRY(hipMalloc(&h->d_temp, tb));
+  h->temp_bytes = tb;
+  h->cap = cap;
+  return MKF_OK;
+}
+
+int blocks_for(size_t n, int block) { return (int)std::max<size_t>(1, (n + block - 1) / block); }
+
+}  // namespace
+
+extern "C" {
+
+int mkf_event_of_hits_create(mkf_context* c, int n_layers, const mkf_layer_desc* desc, mkf_event_of_hits** out) {
+  CHECK(set_device(c));
+  if (!out || !desc) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  *out = nullptr;
+  if (n_layers <= 0 || n_layers > MKF_MAX_FIND_LAYERS)
+    return fail(MKF_ERR_INVALID_ARG, "n_layers must be in [1, %d]", MKF_MAX_FIND_LAYERS);
+  std::vector<LayerDev> L(n_layers);
+  int bin_off = 0;
+  for (int l = 0; l < n_layers; ++l) {
+    const mkf_layer_desc& s = desc[l];
+    LayerDev& d = L[l];
+    std::memset(&d, 0, sizeof(d));
+    if (!(s.q_bin != 0.f) || !(s.rout >= s.rin) || !(s.zmax >= s.zmin))
+      return fail(MKF_ERR_INVALID_ARG, "layer %d: invalid limits or q_bin", l);
+    if (!(s.min_dphi >= 0.f && s.min_dphi <= s.max_dphi && s.max_dphi <= kPI) ||
+        !(s.min_dq >= 0.f && s.min_dq <= s.max_dq && std::isfinite(s.max_dq)))
+      return fail(MKF_ERR_INVALID_ARG, "layer %d: selection limits need 0 <= min <= max (max_dphi <= pi)", l);
+    d.is_barrel = s.is_barrel ? 1 : 0;
+    d.rin = s.rin; d.rout = s.rout; d.zmin = s.zmin; d.zmax = s.zmax;
+    d.propagate_to = s.propagate_to;
+    if (d.is_barrel)
+      setup_bins(s.zmin, s.zmax, s.q_bin, d.nq, d.qmin, d.fq);
+    else
+      setup_bins(s.rin, s.rout, s.q_bin, d.nq, d.qmin, d.fq);
+    if (d.nq <= 0 || d.nq > kMaxQBins || !std::isfinite(d.fq))
+      return fail(MKF_ERR_INVALID_ARG, "layer %d: %d q bins (1..%d allowed)", l, d.nq, kMaxQBins);
+    d.min_dphi = s.min_dphi; d.max_dphi = s.max_dphi; d.min_dq = s.min_dq; d.max_dq = s.max_dq;
+    d.c_dp_sf = s.c_dp_sf; d.c_dp_0 = s.c_dp_0; d.c_dp_1 = s.c_dp_1; d.c_dp_2 = s.c_dp_2;
+    d.c_dq_sf = s.c_dq_sf; d.c_dq_0 = s.c_dq_0; d.c_dq_1 = s.c_dq_1; d.c_dq_2 = s.c_dq_2;
+    d.c_c2_sf = s.c_c2_sf; d.c_c2_0 = s.c_c2_0; d.c_c2_1 = s.c_c2_1; d.c_c2_2 = s.c_c2_2;
+    d.bin_off = bin_off;
+    if ((long)bin_off + (long)d.nq * kNPhi > (1L << 30)) return fail(MKF_ERR_INVALID_ARG, "too many bins");
+    bin_off += d.nq * kNPhi;
+  }
+  auto* h = new mkf_event_of_hits();
+  h->ctx = c;
+  h->n_layers = n_layers;
+  h->layers = L;
+  h->total_bins = bin_off;
+  hipError_t e = hipMalloc(&h->d_layers, sizeof(LayerDev) * n_layers);
+  if (e == hipSuccess) e = hipMemcpy(h->d_layers, L.data(), sizeof(LayerDev) * n_layers, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&h->d_bins, sizeof(int2) * (size_t)bin_off);
+  if (e != hipSuccess) {
+    int rc = fail(MKF_ERR_HIP, "event-of-hits setup: %s", hipGetErrorString(e));
+    mkf_event_of_hits_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return MKF_OK;
+}
+
+int mkf_event_of_hits_destroy(mkf_event_of_hits* h) {
+  if (!h) return MKF_OK;
+  (void)hipSetDevice(h->ctx->device);
+  (void)hipDeviceSynchronize();
+  free_hit_arrays(h);
+  if (h->d_layers) (void)hipFree(h->d_layers);
+  if (h->d_bins) (void)hipFree(h->d_bins);
+  delete h;
+  return MKF_OK;
+}
+
+int mkf_event_of_hits_load(mkf_event_of_hits* h, const int* hit_offset, const float* hits, void* stream) {
+  if (!h) return fail(MKF_ERR_INVALID_ARG, "event of hits is NULL");
+  CHECK(set_device(h->ctx));
+  if (!hit_offset) return fail(MKF_ERR_INVALID_ARG, "hit_offset is NULL");
+  const int L = h->n_layers;
+  if (hit_offset[0] != 0) return fail(MKF_ERR_INVALID_ARG, "hit_offset[0] must be 0");
+  for (int l = 0; l < L; ++l)
+    if (hit_offset[l + 1] < hit_offset[l]) return fail(MKF_ERR_INVALID_ARG, "hit_offset decreases at layer %d", l);
+  const size_t total = (size_t)hit_offset[L];
+  if (total * 9 * 4 >= ((size_t)1 << 32)) return fail(MKF_ERR_INVALID_ARG, "too many hits in one event");
+  if (total) CHECK(check_dev(hits, total * 9 * 4, "hits"));
+  hipStream_t st = (hipStream_t)stream;
+  CHECK(grow(h, total, st));
+  OffsetsArg o;
+  std::memset(&o, 0, sizeof(o));
+  for (int l = 0; l <= L; ++l) o.off[l] = hit_offset[l];
+  for (int l = 0; l < L; ++l) {
+    h->layers[l].hit_off = hit_offset[l];
+    h->layers[l].n_hits = hit_offset[l + 1] - hit_offset[l];
+  }
+  hipLaunchKernelGGL(k_set_offsets, dim3(1), dim3(MKF_MAX_FIND_LAYERS), 0, st, o, L, h->d_layers);
+  HIP_TRY(hipGetLastError());
+  const int* bins_i = reinterpret_cast<const int*>(h->d_bins);
+  hipLaunchKernelGGL(k_fill_i32, dim3(std::min(blocks_for((size_t)h->total_bins * 2, 256), 4096)), dim3(256), 0, st,
+                     const_cast<int*>(bins_i), (long)h->total_bins * 2, 0);
+  HIP_TRY(hipGetLastError());
+  if (total) {
+    const int nt = (int)total;
+    hipLaunchKernelGGL(k_hit_keys, dim3(blocks_for(total, 256)), dim3(256), 0, st, nt, L, hits, h->d_layers,
+                       h->d_keys, h->d_vals, h->d_phi_o, h->d_q_o);
+    HIP_TRY(hipGetLastError());
+    size_t tb = h->temp_bytes;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(h->d_temp, tb, h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s, nt, 0,
+                                               32, st));
+    for (int pass = 0; pass < 2; ++pass) {
+      hipLaunchKernelGGL(k_loh_fill, dim3(blocks_for(total, 256)), dim3(256), 0, st, nt, pass, h->d_keys_s,
+                         h->d_vals_s, h->d_layers, hits, h->d_phi_o, h->d_q_o, h->d_ranks, h->d_bins, h->d_shits,
+                         h->d_phis, h->d_qs);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  h->loaded = true;
+  return MKF_OK;
+}
+
+int mkf_event_of_hits_layer(mkf_event_of_hits* h, int layer, int* n_hits, int* nq, const int** ranks,
+                            const int** bins, const float** phis, const float** qs) {
+  if (!h) return fail(MKF_ERR_INVALID_ARG, "event of hits is NULL");
+  if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
+  if (layer < 0 || layer >= h->n_layers) return fail(MKF_ERR_INVALID_ARG, "layer %d out of range", layer);
+  const LayerDev& d = h->layers[layer];
+  if (n_hits) *n_hits = d.n_hits;
+  if (nq) *nq = d.nq;
+  if (ranks) *ranks = h->d_ranks ? h->d_ranks + d.hit_off : nullptr;
+  if (bins) *bins = reinterpret_cast<const int*>(h->d_bins + d.bin_off);
+  if (phis) *phis = h->d_phis ? h->d_phis + d.hit_off : nullptr;
+  if (qs) *qs = h->d_qs ? h->d_qs + d.hit_off : nullptr;
+  return MKF_OK;
+}
+
+int mkf_select_hit_indices(mkf_event_of_hits* h, int layer, int n, const float* par, const float* err, int* xsize,
+                           int* xarr, int* wsr, void* stream) {
+  if (!h) return fail(MKF_ERR_INVALID_ARG, "event of hits is NULL");
+  CHECK(set_device(h->ctx));
+  if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
+  if (layer < 0 || layer >= h->n_layers) return fail(MKF_ERR_INVALID_ARG, "layer %d out of range", layer);
+  if (n < 0) return fail(MKF_ERR_INVALID_ARG, "n < 0");
+  if (n == 0) return MKF_OK;
+  const size_t N = n;
+  CHECK(check_dev(par, 6 * N * 4, "par"));
+  CHECK(check_dev(err, 21 * N * 4, "err"));
+  CHECK(check_dev(xsize, N * 4, "xsize"));
+  CHECK(check_dev(xarr, kHitIdxMax * N * 4, "xarr"));
+  CHECK(check_dev(wsr, N * 4, "wsr"));
+  hipLaunchKernelGGL(k_select, dim3(blocks_for(N, 256)), dim3(256), 0, (hipStream_t)stream, h->layers[layer], n, par,
+                     err, h->d_bins, h->d_phis, h->d_qs, h->d_ranks, xsize, xarr, wsr);
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}
+
+int mkf_find_tracks_best_hit(mkf_context* c, mkf_event_of_hits* h, const mkf_find_batch* b, void* stream) {
+  CHECK(set_device(c));
+  if (!h || !b) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  if (h->ctx != c) return fail(MKF_ERR_INVALID_ARG, "event of hits belongs to another context");
+  if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
+  if (b->n_tracks < 0) return fail(MKF_ERR_INVALID_ARG, "n_tracks < 0");
+  if (b->first_layer < 0 || b->first_layer > h->n_layers)
+    return fail(MKF_ERR_INVALID_ARG, "first_layer %d outside [0, %d]", b->first_layer, h->n_layers);
+  if ((b->pflags & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
+    return fail(MKF_ERR_INVALID_ARG, "material requested but the context has no material grid");
+  if (b->n_tracks == 0) return MKF_OK;
+  const size_t N = b->n_tracks;
+  if (21 * N * 4 >= ((size_t)1 << 31) || (size_t)h->n_layers * N >= ((size_t)1 << 31))
+    return fail(MKF_ERR_INVALID_ARG, "batch too large");
+  CHECK(check_dev(b->par, 6 * N * 4, "par"));
+  CHECK(check_dev(b->err, 21 * N * 4, "err"));
+  CHECK(check_dev(b->chg, N * 4, "chg"));
+  CHECK(check_dev(b->chi2, N * 4, "chi2"));
+  CHECK(check_dev(b->hot_idx, (size_t)h->n_layers * N * 4, "hot_idx"));
+  CHECK(check_dev(b->n_found, N * 4, "n_found"));
+  FindArgs a;
+  a.n = b->n_tracks;
+  a.n_layers = h->n_layers;
+  a.first_layer = b->first_layer;
+  a.chi2_cut = b->chi2_cut;
+  a.pflags = b->pflags;
+  a.bfield = c->bfield;
+  a.mat = mat_of(c);
+  a.par = b->par; a.err = b->err; a.chg = b->chg; a.chi2 = b->chi2;
+  a.hot = b->hot_idx; a.n_found = b->n_found;
+  a.layers = h->d_layers;
+  a.bins = h->d_bins;
+  a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits; a.ranks = h->d_ranks;
+  hipLaunchKernelGGL(k_find_best_hit, dim3(blocks_for(N, 256)), dim3(256), 0, (hipStream_t)stream, a);
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}
+
+}  // extern "C"

@@ -27,97 +27,11 @@
 #include <vector>
 
 #include "../../include/mkfit_amd.h"
+#include "mkf_internal.h"
 #include "mkf_track_math.h"
 
 using namespace mkf;
-
-// ============================================================================
-// Context and error reporting
-// ============================================================================
-
-struct GraphKey {
-  const void* ptrs[12];
-  int n, slots, pflags, fflags, block, bpc, variant;
-  bool operator<(const GraphKey& o) const {
-    int c = std::memcmp(ptrs, o.ptrs, sizeof(ptrs));
-    if (c) return c < 0;
-    return std::tie(n, slots, pflags, fflags, block, bpc, variant) <
-           std::tie(o.n, o.slots, o.pflags, o.fflags, o.block, o.bpc, o.variant);
-  }
-};
-
-struct mkf_context {
-  int device = 0;
-  int num_cus = 256;
-  float bfield = 3.8112f;
-  float* d_rl = nullptr;
-  float* d_xi = nullptr;
-  mkf_launch_shape shape{256, 8, 0};
-  hipStream_t capture_stream = nullptr;
-  std::map<GraphKey, hipGraphExec_t> graphs;
-  std::mutex mu;
-};
-
-namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_last_error = buf;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                                            \
-  do {                                                                                           \
-    hipError_t e_ = (expr);                                                                      \
-    if (e_ != hipSuccess) return fail(MKF_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));      \
-  } while (0)
-
-// A device pointer must be non-null, device (or managed) memory, and the
-// allocation it lies in must extend `bytes` past it.
-int check_dev(const void* p, size_t bytes, const char* name) {
-  if (!p) return fail(MKF_ERR_INVALID_ARG, "%s is NULL", name);
-  hipPointerAttribute_t attr;
-  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(MKF_ERR_INVALID_ARG, "%s is not a HIP allocation", name);
-  }
-  if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)
-    return fail(MKF_ERR_INVALID_ARG, "%s is not device memory", name);
-  hipDeviceptr_t base = nullptr;
-  size_t size = 0;
-  if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(MKF_ERR_INVALID_ARG, "%s: cannot query its allocation", name);
-  }
-  const char* b = (const char*)base;
-  const char* q = (const char*)p;
-  if (q < b || (size_t)(q - b) + bytes > size)
-    return fail(MKF_ERR_BOUNDS, "%s: %zu bytes needed past the pointer, allocation has %zu", name, bytes,
-                size - (size_t)(q - b));
-  return MKF_OK;
-}
-
-#define CHECK(expr)            \
-  do {                         \
-    int rc_ = (expr);          \
-    if (rc_) return rc_;       \
-  } while (0)
-
-int set_device(mkf_context* c) {
-  if (!c) return fail(MKF_ERR_INVALID_ARG, "context is NULL");
-  HIP_TRY(hipSetDevice(c->device));
-  return MKF_OK;
-}
-
-Material mat_of(const mkf_context* c) { return Material{c->d_rl, c->d_xi}; }
-
-}  // namespace
+using namespace mkf_detail;
 
 // ============================================================================
 // Kernels: single operations (the MPlex free functions)
@@ -197,33 +111,6 @@ k_kalman(int op, int n, const float* __restrict__ ps_err, const float* __restric
 // Fit kernels
 // ============================================================================
 
-// Buffer addressing for the fit kernel.  Every array gets one buffer resource
-// (base, byte size) held in SGPRs; element e of track t is read with the
-// per-lane byte offset t*4 in a VGPR and the element stride e*n*4 in an SGPR
-// soffset, so no per-lane address arithmetic is spent on the 27 state and 9
-// hit elements.  The byte size doubles as a hardware bounds check: an access
-// past it returns 0 instead of touching other memory.
-constexpr int kBufFlags = 0x00020000;  // gfx9-family raw buffer: 32-bit dword data
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_buf(const void* p, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, kBufFlags);
-}
-__device__ __forceinline__ float bld_f(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
-}
-__device__ __forceinline__ int bld_i(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
-  return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0);
-}
-__device__ __forceinline__ unsigned bld_u8(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
-  return (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r, (int)vo, (int)so, 0);
-}
-__device__ __forceinline__ void bst_f(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)vo, (int)so, 0);
-}
-__device__ __forceinline__ void bst_i(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, int v) {
-  __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)vo, (int)so, 0);
-}
-
 struct FitArgs {
   int n, n_slots;
   float* __restrict__ par;
@@ -245,9 +132,11 @@ struct FitArgs {
 // One hit slot of one track: the body of the reference's per-layer loop
 // (MkFinder::BkFitFitTracks, MkFinder.cc:1612-1740; FitTracksSteered,
 // MkFitter.cc:477-500).
-// Buffer resources of one fit launch (uniform across the grid).
+// Buffer resources for the arrays read on every step (uniform across the grid);
+// the once-per-track state goes through ordinary global loads so that only
+// four descriptors (16 SGPRs) stay live in the slot loop.
 struct FitBufs {
-  __amdgpu_buffer_rsrc_t par, err, chg, chi2, hit_pos, hit_err, n_hits, hit_flags, seed_par, seed_err, seed_chg;
+  __amdgpu_buffer_rsrc_t hit_pos, hit_err, n_hits, hit_flags;
   uint32_t nb;  // n * 4: byte stride between elements
 };
 
@@ -255,37 +144,45 @@ __device__ __forceinline__ FitBufs make_bufs(const FitArgs& a) {
   FitBufs b;
   const uint32_t n = (uint32_t)a.n, nb = n * 4u, S = (uint32_t)a.n_slots;
   b.nb = nb;
-  b.par = make_buf(a.par, 6u * nb);
-  b.err = make_buf(a.err, 21u * nb);
-  b.chg = make_buf(a.chg, nb);
-  b.chi2 = make_buf(a.chi2, a.chi2 ? nb : 0u);
   b.hit_pos = make_buf(a.hit_pos, 3u * nb * S);
   b.hit_err = make_buf(a.hit_err, 6u * nb * S);
   b.n_hits = make_buf(a.n_hits, nb);
   b.hit_flags = make_buf(a.hit_flags, a.hit_flags ? n * S : 0u);
-  b.seed_par = make_buf(a.seed_par, a.seed_par ? 6u * nb : 0u);
-  b.seed_err = make_buf(a.seed_err, a.seed_par ? 21u * nb : 0u);
-  b.seed_chg = make_buf(a.seed_chg, a.seed_par ? nb : 0u);
   return b;
 }
 
+// One hit slot as read from HBM.
+struct HitIn {
+  float mp[3], me[6];
+  unsigned flags;
+};
+
+// Loads slot s of track tb/4.  Slots at or past n_slots fall outside the
+// buffers and read as zeros, so the next-slot prefetch needs no guard.
+__device__ __forceinline__ void load_hit(const FitArgs& a, const FitBufs& b, int s, uint32_t tb, HitIn& h) {
+  h.flags = a.hit_flags ? bld_u8(b.hit_flags, tb >> 2, (uint32_t)s * (uint32_t)a.n) : 0u;
+  const uint32_t sp = (uint32_t)s * 3u * b.nb, se = (uint32_t)s * 6u * b.nb;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) h.mp[e] = bld_f(b.hit_pos, tb, sp + (uint32_t)e * b.nb);
+#pragma unroll
+  for (int e = 0; e < 6; ++e) h.me[e] = bld_f(b.hit_err, tb, se + (uint32_t)e * b.nb);
+}
+
 __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int& chg, float& chi2, const FitArgs& a,
-                                         const FitBufs& b, int s, uint32_t tb) {
-  const unsigned flags = a.hit_flags ? bld_u8(b.hit_flags, tb >> 2, (uint32_t)s * (uint32_t)a.n) : 0u;
-  const bool missing = flags & MKF_HIT_MISSING;
+                                         const HitIn& h) {
+  const bool missing = h.flags & MKF_HIT_MISSING;
   float mp[3], me[6];
   if (missing) {
     mp[0] = par[0]; mp[1] = par[1]; mp[2] = par[2];
     me[0] = 666.f; me[1] = 0.f; me[2] = 666.f; me[3] = 0.f; me[4] = 0.f; me[5] = 666.f;
   } else {
-    const uint32_t sp = (uint32_t)s * 3u * b.nb, se = (uint32_t)s * 6u * b.nb;
 #pragma unroll
-    for (int e = 0; e < 3; ++e) mp[e] = bld_f(b.hit_pos, tb, sp + (uint32_t)e * b.nb);
+    for (int e = 0; e < 3; ++e) mp[e] = h.mp[e];
 #pragma unroll
-    for (int e = 0; e < 6; ++e) me[e] = bld_f(b.hit_err, tb, se + (uint32_t)e * b.nb);
+    for (int e = 0; e < 6; ++e) me[e] = h.me[e];
   }
   float c2;
-  if (!(flags & MKF_HIT_ENDCAP)) {
+  if (!(h.flags & MKF_HIT_ENDCAP)) {
     const float r = fhypot(mp[0], mp[1]);                                  // MkBase.h:55
     propagate_to_r(par, err, chg, r, a.pflags, missing, a.bfield, a.mat);
     c2 = kalman_barrel<true, true>(par, err, mp, me);
@@ -301,26 +198,30 @@ __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int&
 }
 
 // from_seed: read the seed arrays (first touch of an out-of-place fit).
-__device__ __forceinline__ void load_state(const FitArgs& a, const FitBufs& b, uint32_t tb, bool from_seed,
-                                           float (&par)[6], float (&err)[21], int& chg, float& chi2) {
-  const __amdgpu_buffer_rsrc_t P = from_seed ? b.seed_par : b.par;
-  const __amdgpu_buffer_rsrc_t E = from_seed ? b.seed_err : b.err;
+__device__ __forceinline__ void load_state(const FitArgs& a, int t, bool from_seed, float (&par)[6], float (&err)[21],
+                                           int& chg, float& chi2) {
+  const size_t n = (size_t)a.n;
+  const float* P = (from_seed ? a.seed_par : a.par) + t;
+  const float* E = (from_seed ? a.seed_err : a.err) + t;
 #pragma unroll
-  for (int e = 0; e < 6; ++e) par[e] = bld_f(P, tb, (uint32_t)e * b.nb);
+  for (int e = 0; e < 6; ++e) par[e] = P[e * n];
 #pragma unroll
-  for (int e = 0; e < 21; ++e) err[e] = bld_f(E, tb, (uint32_t)e * b.nb);
-  chg = bld_i(from_seed ? b.seed_chg : b.chg, tb, 0);
-  chi2 = ((a.fit_flags & MKF_FIT_CHI2) && !from_seed) ? bld_f(b.chi2, tb, 0) : 0.f;
+  for (int e = 0; e < 21; ++e) err[e] = E[e * n];
+  chg = from_seed ? a.seed_chg[t] : a.chg[t];
+  chi2 = ((a.fit_flags & MKF_FIT_CHI2) && !from_seed) ? a.chi2[t] : 0.f;
 }
 
-__device__ __forceinline__ void store_state(const FitArgs& a, const FitBufs& b, uint32_t tb, const float (&par)[6],
-                                            const float (&err)[21], int chg, float chi2) {
+__device__ __forceinline__ void store_state(const FitArgs& a, int t, const float (&par)[6], const float (&err)[21],
+                                            int chg, float chi2) {
+  const size_t n = (size_t)a.n;
+  float* P = a.par + t;
+  float* E = a.err + t;
 #pragma unroll
-  for (int e = 0; e < 6; ++e) bst_f(b.par, tb, (uint32_t)e * b.nb, par[e]);
+  for (int e = 0; e < 6; ++e) P[e * n] = par[e];
 #pragma unroll
-  for (int e = 0; e < 21; ++e) bst_f(b.err, tb, (uint32_t)e * b.nb, err[e]);
-  if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) || a.seed_par) bst_i(b.chg, tb, 0, chg);
-  if (a.fit_flags & MKF_FIT_CHI2) bst_f(b.chi2, tb, 0, chi2);
+  for (int e = 0; e < 21; ++e) E[e * n] = err[e];
+  if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) || a.seed_par) a.chg[t] = chg;
+  if (a.fit_flags & MKF_FIT_CHI2) a.chi2[t] = chi2;
 }
 
 // One kernel body serves both fit forms, so they execute identical machine
@@ -346,9 +247,16 @@ k_fit(FitArgs a, int s_begin, int s_end) {
     if (s_begin >= s1 && !from_seed) continue;
     float par[6], err[21], chi2;
     int chg;
-    load_state(a, b, tb, from_seed, par, err, chg, chi2);
-    for (int s = s_begin; s < s1; ++s) fit_step(par, err, chg, chi2, a, b, s, tb);
-    store_state(a, b, tb, par, err, chg, chi2);
+    load_state(a, t, from_seed, par, err, chg, chi2);
+    // Software pipeline: the hit of slot s+1 is in flight while slot s computes.
+    HitIn cur, nxt;
+    if (s_begin < s1) load_hit(a, b, s_begin, tb, nxt);
+    for (int s = s_begin; s < s1; ++s) {
+      cur = nxt;
+      if (s + 1 < s1) load_hit(a, b, s + 1, tb, nxt);
+      fit_step(par, err, chg, chi2, a, cur);
+    }
+    store_state(a, t, par, err, chg, chi2);
   }
 }
 

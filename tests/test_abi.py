@@ -58,4 +58,15 @@ def test_null_arguments_rejected():
     assert L.mkf_fit_batched(None, None, None) == 1
     assert L.mkf_set_batched_shape(None, None) == 1
     assert L.mkf_context_destroy(None) == 0
+    assert L.mkf_event_of_hits_create(None, 1, None, None) == 1
+    assert L.mkf_event_of_hits_load(None, None, None, None) == 1
+    assert L.mkf_select_hit_indices(None, 0, 0, None, None, None, None, None, None) == 1
+    assert L.mkf_find_tracks_best_hit(None, None, None, None) == 1
+    assert L.mkf_event_of_hits_destroy(None) == 0
     assert L.mkf_num_batched_variants() >= 1
+
+
+def test_struct_layouts():
+    import mkfit_amd as mk
+    assert ctypes.sizeof(mk.MkfLayerDesc) == 23 * 4
+    assert ctypes.sizeof(mk.MkfFindBatch) == 4 + 4 + 6 * 8 + 4 + 4 + 4 + 4   # n, pad, 6 pointers, 3 scalars, pad

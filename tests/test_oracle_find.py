@@ -15,42 +15,7 @@ import pytest
 import mkfit_amd as mk
 import oracle
 
-TWO_PI = np.float32(2 * np.pi)
-F_PHI_FINE = np.float32(1024 / np.float32(2 * np.pi))
-
-
-def _layer(**kw):
-    d = dict(is_barrel=1, rin=19.99, rout=20.01, zmin=-60.0, zmax=60.0, propagate_to=20.0, q_bin=2.0,
-             min_dphi=0.002, max_dphi=0.1, min_dq=0.3, max_dq=2.0, c_dp_sf=1.1, c_dq_sf=1.1, c_c2_sf=1.1)
-    d.update(kw)
-    return d
-
-
-def _random_hits(rng, n, r=20.0, zmax=62.0, ties=0):
-    phi = rng.uniform(-np.pi, np.pi, n)
-    z = rng.uniform(-zmax, zmax, n)
-    if ties:   # duplicate positions so that equal keys must keep their input order
-        k = rng.integers(0, n, ties)
-        phi[-ties:] = phi[k]
-        z[-ties:] = z[k]
-    h = np.zeros((n, 9), np.float32)
-    h[:, 0] = r * np.cos(phi); h[:, 1] = r * np.sin(phi); h[:, 2] = z
-    h[:, 3] = 1e-6; h[:, 5] = 1e-6; h[:, 8] = 1e-4
-    return h
-
-
-def _keys(desc, phis, qs):
-    nq = np.int32(0)
-    _, _, _, _, nq = oracle.loh_build(desc, np.zeros((0, 9), np.float32))
-    # setup_bins (HitStructures.cc:11-32), bin width q_bin > 0
-    qmin, qmax, dq = np.float32(desc["zmin"]), np.float32(desc["zmax"]), np.float32(desc["q_bin"])
-    extent = qmax - qmin
-    nqb = int(np.ceil(extent / dq))
-    extra = np.float32(0.5) * (np.float32(nqb) * dq - extent)
-    fq = np.float32(nqb) / (qmax - qmin)
-    qb = np.clip(((qs - (qmin - extra)) * fq).astype(np.int64), 0, nqb - 1)
-    pf = np.floor(F_PHI_FINE * (phis + np.float32(np.pi))).astype(np.int64)
-    return pf + (qb << 16), nqb
+from _find_common import F_PHI_FINE, TWO_PI, _brute_select, _cdist, _keys, _layer, _random_hits, edge_explained
 
 
 def test_loh_matches_stable_sort():
@@ -91,37 +56,6 @@ def test_loh_q_bins_clamped_and_counted():
     assert bins[:, 1].max() == len(h)
 
 
-def _cdist(a):
-    return np.where(a > np.float32(np.pi), TWO_PI - a, a)
-
-
-def _brute_select(desc, h, par, err, ranks, phis, qs):
-    """Window test over every hit; returns per-track lists in the reference's scan order."""
-    n = par.shape[1]
-    key, nqb = _keys(desc, phis, qs)
-    pos_of = np.empty(len(h), np.int64)
-    pos_of[ranks] = np.arange(len(h))
-    out, wins = [], []
-    for t in range(n):
-        x, y, z = par[0, t], par[1, t], par[2, t]
-        r2 = x * x + y * y
-        dphidx, dphidy = -y / r2, x / r2
-        dphi2 = dphidx * dphidx * err[0, t] + dphidy * dphidy * err[2, t] + 2 * dphidx * dphidy * err[1, t]
-        phi = np.arctan2(y, x)
-        dphi = np.clip(max(3 * np.sqrt(abs(dphi2)), desc["min_dphi"]), desc["min_dphi"], desc["max_dphi"])
-        dq = np.clip(abs(3 * np.sqrt(err[5, t])), desc["min_dq"], desc["max_dq"])
-        ddq = np.abs(z - h[:, 2])
-        ddphi = _cdist(np.abs(phi - np.arctan2(h[:, 1], h[:, 0])))
-        inside = np.nonzero((ddq < dq) & (ddphi < dphi))[0]
-        # scan order: q bin, then phi bin counted from the window's first phi bin, then sorted position
-        pb1 = int(np.floor(F_PHI_FINE * (phi - dphi + np.float32(np.pi)))) >> 2
-        kk = key[pos_of[inside]]
-        order = np.lexsort((pos_of[inside], (((kk & 0x3ff) >> 2) - pb1) % 256, kk >> 16))
-        out.append(inside[order])
-        wins.append((dq, dphi))
-    return out, wins
-
-
 def test_select_hit_indices_matches_brute_force():
     rng = np.random.default_rng(5)
     desc = _layer()
@@ -146,12 +80,7 @@ def test_select_hit_indices_matches_brute_force():
         exp = ref[t][:16]
         if not np.array_equal(got, exp):
             # only hits on the window edge (float32 rounding of the limits) may differ
-            dq, dphi = wins[t]
-            diff = np.setxor1d(got, exp)
-            ph = np.arctan2(h[diff, 1], h[diff, 0])
-            near = (np.abs(np.abs(par[2, t] - h[diff, 2]) - dq) < 1e-5) | \
-                   (np.abs(_cdist(np.abs(np.arctan2(par[1, t], par[0, t]) - ph)) - dphi) < 1e-5)
-            assert near.all(), (t, got, exp)
+            assert edge_explained(wins[t], par[:, t], h, np.setxor1d(got, exp), eps=1e-5), (t, got, exp)
             edge += 1
         capped += len(ref[t]) > 16
     assert capped > 50 and edge <= 3
