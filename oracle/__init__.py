@@ -231,3 +231,44 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     if rc:
         raise RuntimeError("orc_find_best_hit failed (%d)" % rc)
     return dict(par=par, err=err, chg=chg, chi2=chi2, hot_idx=hot, n_found=nfound)
+
+
+class _CEParams(ctypes.Structure):
+    _fields_ = [("max_cands", ctypes.c_int), ("max_holes", ctypes.c_int), ("max_consec_holes", ctypes.c_int),
+                ("prop_to_hit", ctypes.c_int), ("intra_pflags", ctypes.c_int)]
+
+
+class _CETracks(ctypes.Structure):
+    _fields_ = [("n_seeds", ctypes.c_int)] + [(k, ctypes.c_void_p) for k in (
+        "seed_par", "seed_err", "seed_chg", "seed_hits", "n_cands", "par", "err", "chg", "chi2", "score", "n_found",
+        "hot_idx")]
+
+
+def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.8, max_cands=6, max_holes=4,
+                      max_consec_holes=1, prop_to_hit=True, intra_pflags=0, n_threads=1):
+    """Clone-engine finding over a toy event dict; per seed up to max_cands candidates, best first."""
+    L = lib()
+    layers = layer_array(descs)
+    n = ev["n_seeds"]
+    K = int(max_cands)
+    nl = len(descs)
+    sp = np.ascontiguousarray(ev["seed_par"][:, :n], np.float32)
+    se = np.ascontiguousarray(ev["seed_err"][:, :n], np.float32)
+    sc = np.ascontiguousarray(ev["seed_chg"][:n], np.int32)
+    sh = np.ascontiguousarray(ev["seed_hits"][:first_layer, :n], np.int32)
+    out = dict(n_cands=np.zeros(n, np.int32), par=np.zeros((K, 6, n), np.float32),
+               err=np.zeros((K, 21, n), np.float32), chg=np.zeros((K, n), np.int32), chi2=np.zeros((K, n), np.float32),
+               score=np.zeros((K, n), np.float32), n_found=np.zeros((K, n), np.int32),
+               hot_idx=np.zeros((K, nl, n), np.int32))
+    off = np.ascontiguousarray(ev["hit_offset"], np.int32)
+    hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
+    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags)
+    pr = _CEParams(K, int(max_holes), int(max_consec_holes), int(bool(prop_to_hit)), int(intra_pflags))
+    tr = _CETracks(n, _p(sp), _p(se), _p(sc), _p(sh), *[_p(out[k]) for k in (
+        "n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx")])
+    ctx = OrcCtx(float(bfield), None, None)
+    rc = L.orc_find_clone_engine(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr),
+                                 ctypes.byref(tr), int(n_threads))
+    if rc:
+        raise RuntimeError("orc_find_clone_engine failed (%d)" % rc)
+    return out

@@ -131,6 +131,36 @@ int orc_select_hit_indices(const orc_layer_desc* d, int n_hits, const float* hit
                            const float* err, int* xsize, int* xarr, int* wsr);
 int orc_find_best_hit(const orc_ctx* ctx, const orc_find_event* ev, orc_find_tracks* tracks, int n_threads);
 
+/* ---- Clone-engine track finding (oracle_ce.cc) ---------------------------- */
+
+typedef struct orc_ce_params {
+  int max_cands;        /* IterationParams::maxCandsPerSeed (<= 16) */
+  int max_holes;        /* maxHolesPerCand */
+  int max_consec_holes; /* maxConsecHoles */
+  int prop_to_hit;      /* Config::finding_requires_propagation_to_hit_pos */
+  int intra_pflags;     /* Config::finding_intra_layer_pflags (chi2 propagation to the hit) */
+} orc_ce_params;
+
+typedef struct orc_ce_tracks {
+  int n_seeds;
+  const float* seed_par; /* [6][n] */
+  const float* seed_err; /* [21][n] */
+  const int* seed_chg;   /* [n] */
+  const int* seed_hits;  /* [first_layer][n] */
+  /* out, per seed the candidates after the final merge, best first (K = max_cands) */
+  int* n_cands;          /* [n] */
+  float* par;            /* [K][6][n] */
+  float* err;            /* [K][21][n] */
+  int* chg;              /* [K][n] */
+  float* chi2;           /* [K][n] */
+  float* score;          /* [K][n] */
+  int* n_found;          /* [K][n] */
+  int* hot_idx;          /* [K][n_layers][n]: hit index per layer, fake codes, -9 none */
+} orc_ce_tracks;
+
+int orc_find_clone_engine(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, orc_ce_tracks* t,
+                          int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

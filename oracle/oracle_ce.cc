@@ -1,0 +1,474 @@
+// oracle_ce.cc -- CPU restatement of mkFit's clone-engine track finding.
+//
+// TEST INFRASTRUCTURE ONLY (checker and CPU baseline).  Restates, for a barrel
+// layer plan visited in order:
+//   MkBuilder::find_tracks_in_layers (CE)       mkFit/MkBuilder.cc:1887-2060
+//   MkBuilder::find_tracks_unroll_candidates    mkFit/MkBuilder.cc:1503-1566
+//   MkBuilder::find_tracks_handle_missed_layers mkFit/MkBuilder.cc:1568-1630
+//   MkFinder::FindCandidatesCloneEngine         mkFit/MkFinder.cc:1114-1284
+//   isStripQCompatible                          mkFit/MkFinder.cc:837-862
+//   CandCloner::ProcessSeedRange                mkFit/CandCloner.cc:37-191
+//   MkFinder::UpdateWithLastHit                 mkFit/MkFinder.cc:1290-1308
+//   kalmanPropagateAnd{ComputeChi2,Update}      mkFit/KalmanUtilsMPlex.cc:471-543
+//   TrackCand::addHitIdx, HoT nodes             mkFit/HitStructures.h:873-889, CombCandidate::AddHit
+//   CombCandidate::ImportSeed / MergeCandsAndBestShortOne  mkFit/HitStructures.cc:485-552
+//   getScoreCalc / getScoreStruct / getScoreCand Track.h:648-697, HitStructures.h:589-601
+//
+// What the toy events do not exercise, and how it is handled here:
+//   * hit cluster charge: toy hits carry none; they are taken as the reference
+//     takes pixel-like clusters (charge code 0xff = Hit::maxChargePerCM), for
+//     which passStripChargePCMfromTrack returns true (MkFinder.cc:872-873);
+//   * overlap hits: every toy hit of a layer has module id 0, and with a single
+//     module id HitMatchPair::find_overlap (HitStructures.h:372-394) never
+//     returns a match, so no overlap hit is ever added;
+//   * pixel layers, dead-module gaps (-7): the toy geometry has neither.
+// Both sorts of the reference are std::sort (not stable); this restatement uses
+// stable sorts, i.e. ties in score keep generation order.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "oracle.h"
+#include "oracle_core.h"
+#include "oracle_find_core.h"
+
+using namespace orc;
+using namespace orc_find;
+
+namespace {
+
+// Config.h:355-359
+constexpr float kValidHitBonus = 4, kValidHitSlope = 0.2f, kOverlapHitBonus = 0, kMissingHitPenalty = 8,
+                kTailMissingHitPenalty = 3;
+constexpr float kScoreWorst = -1e16f;
+
+float score_calc(int nfound, int ntail, int noverlap, int nmiss, float chi2, float pt, bool in_find) {
+  const float maxBonus = 8.0;
+  float bonus = kValidHitSlope * nfound + kValidHitBonus;
+  float penalty = kMissingHitPenalty;
+  const float tailPenalty = kTailMissingHitPenalty;
+  const float overlapBonus = kOverlapHitBonus;
+  if (pt < 0.9) {
+    penalty *= in_find ? 1.7f : 1.5f;
+    bonus = std::min(bonus * (in_find ? 0.9f : 1.0f), maxBonus);
+  }
+  return bonus * nfound + overlapBonus * noverlap - penalty * nmiss - tailPenalty * ntail - chi2;
+}
+
+struct HoTNode {
+  int index, layer;
+  float chi2;
+  int prev;
+};
+
+struct Cand {
+  float par[6], err[21];
+  int chg = 0;
+  float chi2 = 0, score = kScoreWorst;
+  int last = -1;                        // lastHitIdx_: node index in the seed's HoT store
+  int nfound = 0, nmissing = 0, noverlap = 0, ninside = 0, ntail = 0;
+  bool valid = false;                   // combCandidate() != nullptr (best-short bookkeeping)
+  float pT() const { return std::abs(1.f / par[3]); }
+};
+
+struct IdxChi2 {                        // IdxChi2List (Track.h)
+  int trkIdx, hitIdx, nhits, ntailholes, noverlaps, nholes;
+  float pt, chi2, chi2_hit, score;
+};
+
+float score_struct(const IdxChi2& e) {
+  float c = e.chi2;
+  if (c < 0) c = 0.f;
+  return score_calc(e.nhits, e.ntailholes, e.noverlaps, e.nholes, c, e.pt, true);
+}
+float score_cand(const Cand& c) {
+  float chi2 = c.chi2;
+  if (chi2 < 0) chi2 = 0.f;
+  return score_calc(c.nfound, 0, c.noverlap, c.ninside, chi2, c.pT(), false);
+}
+
+struct Comb {
+  std::vector<Cand> cands;
+  Cand best_short;
+  int state = 0;                        // 0 Dormant, 1 Finding, 2 Finished
+  int pickup_layer = -1;
+  std::vector<HoTNode> hots;
+
+  void add_hit(Cand& c, int idx, int lyr, float chi2) {    // TrackCand::addHitIdx
+    hots.push_back({idx, lyr, chi2, c.last});
+    c.last = (int)hots.size() - 1;
+    if (idx >= 0 || idx == -9) {
+      ++c.nfound;
+      c.chi2 += chi2;
+      c.ninside += c.ntail;
+      c.ntail = 0;
+    } else {
+      ++c.nmissing;
+      if (idx == -1) ++c.ntail;
+    }
+  }
+  int last_idx(const Cand& c) const { return c.last >= 0 ? hots[c.last].index : -1; }
+};
+
+// isStripQCompatible (MkFinder.cc:837-862), one lane.
+bool strip_q_compatible(bool barrel, const float* pErr, const float* pPar, const float* mErr, const float* mPar,
+                        int it) {
+  auto E = [&](int i, int j) { return pErr[sym(i, j) * NN + it]; };
+  auto P = [&](int i) { return pPar[i * NN + it]; };
+  auto ME = [&](int i, int j) { return mErr[sym(i, j) * NN + it]; };
+  auto MP = [&](int i) { return mPar[i * NN + it]; };
+  if (barrel) {
+    const float res = std::abs(MP(2) - P(2));
+    const float hitHL = std::sqrt(ME(2, 2) * 3.f);
+    const float qErr = std::sqrt(E(2, 2));
+    return hitHL + std::max(3.f * qErr, 0.5f) > res;
+  }
+  const float res[2] = {MP(0) - P(0), MP(1) - P(1)};
+  const float hitT2 = ME(0, 0) + ME(1, 1);
+  const float hitT2inv = 1.f / hitT2;
+  const float proj[3] = {ME(0, 0) * hitT2inv, ME(0, 1) * hitT2inv, ME(1, 1) * hitT2inv};
+  const float qErr = std::sqrt(std::abs(E(0, 0) * proj[0] + 2.f * E(0, 1) * proj[1] + E(1, 1) * proj[2]));
+  const float resProj = std::sqrt(res[0] * proj[0] * res[0] + 2.f * res[1] * proj[1] * res[0] + res[1] * proj[2] * res[1]);
+  return std::sqrt(hitT2 * 3.f) + std::max(3.f * qErr, 0.5f) > resProj;
+}
+
+struct CEEvent {
+  const orc_find_event* ev;
+  const orc_ce_params* p;
+  std::vector<LoH> loh;
+  Ctx ctx;
+};
+
+// Block of up to NN candidates of one seed (MkFinder's NN lanes).
+struct Block {
+  alignas(64) float par[6 * NN], err[21 * NN], parP[6 * NN], errP[21 * NN];
+  int chg[NN];
+};
+
+void load_block(Block& B, const std::vector<Cand>& cs, const std::vector<int>& idx) {
+  const int n = (int)idx.size();
+  for (int l = 0; l < NN; ++l) {
+    const Cand& c = cs[idx[l < n ? l : 0]];
+    for (int k = 0; k < 6; ++k) B.par[k * NN + l] = c.par[k];
+    for (int k = 0; k < 21; ++k) B.err[k * NN + l] = c.err[k];
+    B.chg[l] = c.chg;
+  }
+}
+
+// kalmanPropagateAndComputeChi2 for one hit per lane (KalmanUtilsMPlex.cc:518-543 barrel, :744-770 endcap).
+void prop_and_chi2(const CEEvent& E, bool barrel, const float* psErr, const float* psPar, const int* chg,
+                   const float* msErr, const float* msPar, int n_proc, float* outChi2, float* propPar) {
+  std::memcpy(propPar, psPar, sizeof(float) * 6 * NN);
+  if (E.p->prop_to_hit) {
+    alignas(64) float propErr[21 * NN], ms[NN];
+    if (barrel) {
+      for (int l = 0; l < NN; ++l) ms[l] = std::hypot(msPar[0 * NN + l], msPar[1 * NN + l]);
+      propagate_helix_to_r(psErr, psPar, chg, ms, propErr, propPar, n_proc, E.p->intra_pflags, nullptr, nullptr, E.ctx);
+      alignas(64) float dE[21 * NN], dP[6 * NN];
+      kalman_operation(KFO_Calculate_Chi2, propErr, propPar, msErr, msPar, dE, dP, outChi2);
+    } else {
+      for (int l = 0; l < NN; ++l) ms[l] = msPar[2 * NN + l];
+      propagate_helix_to_z(psErr, psPar, chg, ms, propErr, propPar, n_proc, E.p->intra_pflags, nullptr, E.ctx);
+      alignas(64) float dE[21 * NN], dP[6 * NN];
+      kalman_operation_endcap(KFO_Calculate_Chi2, propErr, propPar, msErr, msPar, dE, dP, outChi2);
+    }
+  } else {
+    alignas(64) float dE[21 * NN], dP[6 * NN];
+    if (barrel)
+      kalman_operation(KFO_Calculate_Chi2, psErr, psPar, msErr, msPar, dE, dP, outChi2);
+    else
+      kalman_operation_endcap(KFO_Calculate_Chi2, psErr, psPar, msErr, msPar, dE, dP, outChi2);
+  }
+}
+
+// kalmanPropagateAndUpdate (KalmanUtilsMPlex.cc:471-504; endcap :697-730), with the charge flip.
+void prop_and_update(const CEEvent& E, bool barrel, const float* psErr, const float* psPar, int* chg,
+                     const float* msErr, const float* msPar, int n_proc, float* outErr, float* outPar) {
+  alignas(64) float c2[NN];
+  if (E.p->prop_to_hit) {
+    alignas(64) float propErr[21 * NN], propPar[6 * NN], ms[NN];
+    if (barrel) {
+      for (int l = 0; l < NN; ++l) ms[l] = std::hypot(msPar[0 * NN + l], msPar[1 * NN + l]);
+      propagate_helix_to_r(psErr, psPar, chg, ms, propErr, propPar, n_proc, E.ev->pflags, nullptr, nullptr, E.ctx);
+      kalman_operation(KFO_Update_Params, propErr, propPar, msErr, msPar, outErr, outPar, c2);
+    } else {
+      for (int l = 0; l < NN; ++l) ms[l] = msPar[2 * NN + l];
+      propagate_helix_to_z(psErr, psPar, chg, ms, propErr, propPar, n_proc, E.ev->pflags, nullptr, E.ctx);
+      kalman_operation_endcap(KFO_Update_Params, propErr, propPar, msErr, msPar, outErr, outPar, c2);
+    }
+  } else {
+    if (barrel)
+      kalman_operation(KFO_Update_Params, psErr, psPar, msErr, msPar, outErr, outPar, c2);
+    else
+      kalman_operation_endcap(KFO_Update_Params, psErr, psPar, msErr, msPar, outErr, outPar, c2);
+  }
+  for (int n = 0; n < NN; ++n)
+    if (outPar[3 * NN + n] < 0) {
+      chg[n] = -chg[n];
+      outPar[3 * NN + n] = -outPar[3 * NN + n];
+    }
+}
+
+// One seed through every layer of the plan.
+void find_seed(const CEEvent& E, Comb& cc, int layer0) {
+  const orc_find_event& ev = *E.ev;
+  const orc_ce_params& P = *E.p;
+  const int max_cands = P.max_cands;
+  int prev_layer = layer0 - 1;
+  for (int layer = layer0; layer < ev.n_layers; prev_layer = layer, ++layer) {
+    const orc_layer_desc& d = ev.layers[layer];
+    const bool barrel = d.is_barrel != 0;
+    const float* hits = ev.hits + (size_t)ev.hit_offset[layer] * 9;
+    // find_tracks_unroll_candidates
+    if (cc.state == 0 && cc.pickup_layer == prev_layer) cc.state = 1;
+    if (cc.state != 1) continue;
+    std::vector<int> active;
+    for (int ic = 0; ic < (int)cc.cands.size(); ++ic) {
+      Cand& c = cc.cands[ic];
+      if (cc.last_idx(c) == -2) continue;
+      if (c.pT() < 1.2) {                               // near the apex of a low-pT helix
+        const float dphi = std::abs(std::atan2(c.par[1], c.par[0]) - c.par[4]);
+        if (c.par[0] * c.par[0] + c.par[1] * c.par[1] > 625.f && dphi > 1.371f && dphi < 4.512f) {
+          cc.add_hit(c, -2, layer, 0.0f);
+          continue;
+        }
+      }
+      active.push_back(ic);
+    }
+    if (active.empty()) {
+      cc.state = 2;
+      continue;
+    }
+    std::vector<IdxChi2> entries;
+    std::vector<Cand> extras;
+    for (size_t b0 = 0; b0 < active.size(); b0 += NN) {
+      std::vector<int> idx(active.begin() + b0, active.begin() + std::min(active.size(), b0 + NN));
+      const int n_proc = (int)idx.size();
+      Block B;
+      load_block(B, cc.cands, idx);
+      alignas(64) float rad[NN];
+      for (int l = 0; l < NN; ++l) rad[l] = d.propagate_to;
+      int fail[NN];
+      if (barrel)
+        propagate_helix_to_r(B.err, B.par, B.chg, rad, B.errP, B.parP, n_proc, ev.pflags, nullptr, fail, E.ctx);
+      else
+        propagate_helix_to_z(B.err, B.par, B.chg, rad, B.errP, B.parP, n_proc, ev.pflags, nullptr, E.ctx);
+      int xs[NN], xa[kHitIdxMax * NN], wsr[NN];
+      select_hit_indices(d, E.loh[layer], B.parP, B.errP, n_proc, xs, xa, wsr);
+      // find_tracks_handle_missed_layers
+      for (int i = 0; i < n_proc; ++i) {
+        const Cand& c = cc.cands[idx[i]];
+        const float cand_r = std::hypot(B.parP[0 * NN + i], B.parP[1 * NN + i]);
+        if (barrel && cand_r < d.rin) {
+          xs[i] = 0;
+          wsr[i] = WSR_Outside;
+          extras.push_back(c);
+          cc.add_hit(extras.back(), -2, layer, 0);      // barrel region: stopped replica
+        } else if (wsr[i] == WSR_Outside) {
+          extras.push_back(c);
+        }
+      }
+      // FindCandidatesCloneEngine
+      int maxSize = 0;
+      for (int i = 0; i < n_proc; ++i)
+        if (xs[i] > 0) maxSize = std::max(maxSize, xs[i]);
+      int nAdded[NN] = {};
+      alignas(64) float msPar[3 * NN], msErr[6 * NN];
+      for (int l = 0; l < 3 * NN; ++l) msPar[l] = 0.f;
+      for (int l = 0; l < 6 * NN; ++l) msErr[l] = 1.f;
+      for (int hc = 0; hc < maxSize; ++hc) {
+        for (int i = 0; i < n_proc; ++i)
+          if (hc < xs[i]) {
+            const float* h = hits + (size_t)xa[hc * NN + i] * 9;
+            for (int e = 0; e < 3; ++e) msPar[e * NN + i] = h[e];
+            for (int e = 0; e < 6; ++e) msErr[e * NN + i] = h[3 + e];
+          }
+        alignas(64) float chi2[NN], propPar[6 * NN];
+        prop_and_chi2(E, barrel, B.errP, B.parP, B.chg, msErr, msPar, n_proc, chi2, propPar);
+        for (int i = 0; i < n_proc; ++i) {
+          const float max_c2 = dyn_chi2_cut(d, ev.chi2_cut, B.parP[3 * NN + i], B.parP[5 * NN + i]);
+          if (hc >= xs[i]) continue;
+          const float c2 = std::abs(chi2[i]);
+          if (!(c2 < max_c2)) continue;
+          if (!strip_q_compatible(barrel, B.errP, propPar, msErr, msPar, i)) continue;
+          ++nAdded[i];
+          const Cand& c = cc.cands[idx[i]];
+          IdxChi2 e;
+          e.trkIdx = idx[i];
+          e.hitIdx = xa[hc * NN + i];
+          e.nhits = c.nfound + 1;
+          e.ntailholes = 0;
+          e.noverlaps = c.noverlap;
+          e.nholes = c.ninside + c.ntail;
+          e.pt = std::abs(1.0f / B.parP[3 * NN + i]);
+          e.chi2 = c.chi2 + c2;
+          e.chi2_hit = c2;
+          e.score = score_struct(e);
+          entries.push_back(e);
+        }
+      }
+      for (int i = 0; i < n_proc; ++i) {                // the invalid-hit entry
+        if (wsr[i] == WSR_Outside) continue;
+        const Cand& c = cc.cands[idx[i]];
+        int fake = (c.ninside + c.ntail < P.max_holes && c.ntail < P.max_consec_holes) ? -1 : -2;
+        if (wsr[i] == WSR_Edge) fake = -3;
+        IdxChi2 e;
+        e.trkIdx = idx[i];
+        e.hitIdx = fake;
+        e.nhits = c.nfound;
+        e.ntailholes = fake == -1 ? c.ntail + 1 : c.ntail;
+        e.noverlaps = c.noverlap;
+        e.nholes = c.ninside;
+        e.pt = std::abs(1.0f / B.parP[3 * NN + i]);
+        e.chi2 = c.chi2;
+        e.chi2_hit = 0;
+        e.score = score_struct(e);
+        entries.push_back(e);
+      }
+    }
+    // CandCloner::ProcessSeedRange
+    std::vector<Cand> cv;
+    std::vector<int> to_update;                         // positions in cv updated with their last hit
+    if (!entries.empty()) {
+      std::stable_sort(entries.begin(), entries.end(),
+                       [](const IdxChi2& a, const IdxChi2& b) { return a.score > b.score; });
+      const int num_hits = std::min((int)entries.size(), max_cands);
+      size_t xi = 0;
+      int n_pushed = 0;
+      for (int ih = 0; ih < num_hits; ++ih) {
+        const IdxChi2& h = entries[ih];
+        Cand tc = cc.cands[h.trkIdx];
+        cc.add_hit(tc, h.hitIdx, layer, h.chi2_hit);
+        tc.score = h.score;
+        tc.valid = true;
+        if (h.hitIdx == -2) {
+          if (h.score > cc.best_short.score) cc.best_short = tc;
+          continue;
+        }
+        while (xi < extras.size() && extras[xi].score > tc.score && n_pushed < max_cands) {
+          cv.push_back(extras[xi++]);
+          ++n_pushed;
+        }
+        if (n_pushed >= max_cands) break;
+        cv.push_back(tc);
+        ++n_pushed;
+        if (h.hitIdx >= 0) to_update.push_back(n_pushed - 1);
+      }
+      while (xi < extras.size() && n_pushed < max_cands) {
+        cv.push_back(extras[xi++]);
+        ++n_pushed;
+      }
+      cc.cands = cv;
+    } else {
+      cc.cands = extras;                                // state is Finding here
+    }
+    // UpdateWithLastHit over the candidates that took a hit on this layer
+    for (size_t b0 = 0; b0 < to_update.size(); b0 += NN) {
+      std::vector<int> idx(to_update.begin() + b0, to_update.begin() + std::min(to_update.size(), b0 + NN));
+      const int n_proc = (int)idx.size();
+      Block B;
+      load_block(B, cc.cands, idx);
+      alignas(64) float msPar[3 * NN], msErr[6 * NN], oErr[21 * NN], oPar[6 * NN];
+      for (int l = 0; l < NN; ++l) {
+        const HoTNode& hn = cc.hots[cc.cands[idx[l < n_proc ? l : 0]].last];
+        const float* h = hits + (size_t)hn.index * 9;
+        for (int e = 0; e < 3; ++e) msPar[e * NN + l] = h[e];
+        for (int e = 0; e < 6; ++e) msErr[e * NN + l] = h[3 + e];
+      }
+      prop_and_update(E, barrel, B.err, B.par, B.chg, msErr, msPar, n_proc, oErr, oPar);
+      for (int i = 0; i < n_proc; ++i) {
+        Cand& c = cc.cands[idx[i]];
+        for (int k = 0; k < 6; ++k) c.par[k] = oPar[k * NN + i];
+        for (int k = 0; k < 21; ++k) c.err[k] = oErr[k * NN + i];
+        c.chg = B.chg[i];
+      }
+    }
+  }
+  // MergeCandsAndBestShortOne(update_score = true, sort_cands = true)
+  Cand* best_short = cc.best_short.valid ? &cc.best_short : nullptr;
+  if (!cc.cands.empty()) {
+    for (auto& c : cc.cands) c.score = score_cand(c);
+    if (best_short) best_short->score = score_cand(*best_short);
+    std::stable_sort(cc.cands.begin(), cc.cands.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
+    if (best_short && best_short->score > cc.cands.back().score) {
+      size_t ci = 0;
+      while (cc.cands[ci].score > best_short->score) ++ci;
+      if ((int)cc.cands.size() >= max_cands) cc.cands.pop_back();
+      cc.cands.insert(cc.cands.begin() + ci, *best_short);
+    }
+  } else if (best_short) {
+    cc.cands.push_back(*best_short);
+  }
+}
+
+}  // namespace
+
+extern "C" int orc_find_clone_engine(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p,
+                                     orc_ce_tracks* tr, int n_threads) {
+  if (!ev || !p || !tr || ev->n_layers <= 0 || ev->first_layer < 0 || ev->first_layer > ev->n_layers ||
+      p->max_cands < 1 || p->max_cands > 16)
+    return 1;
+  CEEvent E;
+  E.ev = ev;
+  E.p = p;
+  E.ctx.bfield = c ? c->bfield : 3.8112f;
+  E.ctx.rl_grid = c ? c->rl_grid : nullptr;
+  E.ctx.xi_grid = c ? c->xi_grid : nullptr;
+  E.loh.resize(ev->n_layers);
+  for (int l = 0; l < ev->n_layers; ++l)
+    build_loh(ev->layers[l], ev->hit_offset[l + 1] - ev->hit_offset[l], ev->hits + (size_t)ev->hit_offset[l] * 9,
+              E.loh[l]);
+  const int n = tr->n_seeds, L = ev->n_layers, K = p->max_cands;
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (;;) {
+      const int s = next.fetch_add(1);
+      if (s >= n) break;
+      Comb cc;
+      Cand seed;
+      for (int k = 0; k < 6; ++k) seed.par[k] = tr->seed_par[(size_t)k * n + s];
+      for (int k = 0; k < 21; ++k) seed.err[k] = tr->seed_err[(size_t)k * n + s];
+      seed.chg = tr->seed_chg[s];
+      seed.chi2 = 0;
+      seed.valid = true;
+      for (int l = 0; l < ev->first_layer; ++l) cc.add_hit(seed, tr->seed_hits[(size_t)l * n + s], l, 0.0f);
+      seed.score = score_cand(seed);
+      cc.cands.push_back(seed);                         // CombCandidate::ImportSeed
+      cc.pickup_layer = ev->first_layer - 1;
+      cc.best_short.score = kScoreWorst;
+      find_seed(E, cc, ev->first_layer);
+      const int nc = std::min((int)cc.cands.size(), K);
+      tr->n_cands[s] = nc;
+      for (int ic = 0; ic < K; ++ic) {
+        const size_t o = (size_t)ic;
+        for (int l = 0; l < L; ++l) tr->hot_idx[(o * L + l) * n + s] = -9;
+        if (ic >= nc) {
+          tr->score[o * n + s] = kScoreWorst;
+          tr->n_found[o * n + s] = 0;
+          continue;
+        }
+        const Cand& cd = cc.cands[ic];
+        for (int k = 0; k < 6; ++k) tr->par[(o * 6 + k) * n + s] = cd.par[k];
+        for (int k = 0; k < 21; ++k) tr->err[(o * 21 + k) * n + s] = cd.err[k];
+        tr->chg[o * n + s] = cd.chg;
+        tr->chi2[o * n + s] = cd.chi2;
+        tr->score[o * n + s] = cd.score;
+        tr->n_found[o * n + s] = cd.nfound;
+        for (int node = cd.last; node >= 0; node = cc.hots[node].prev) {
+          const HoTNode& h = cc.hots[node];
+          if (h.layer >= 0 && h.layer < L && tr->hot_idx[(o * L + h.layer) * n + s] == -9)
+            tr->hot_idx[(o * L + h.layer) * n + s] = h.index;
+        }
+      }
+    }
+  };
+  n_threads = std::max(1, n_threads);
+  std::vector<std::thread> pool;
+  for (int i = 1; i < n_threads; ++i) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  return 0;
+}
