@@ -221,6 +221,45 @@ typedef struct mkf_find_batch {
  * through all layers. */
 int mkf_find_tracks_best_hit(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_find_batch* batch, void* stream);
 
+#define MKF_CE_MAX_CANDS 8
+
+typedef struct mkf_ce_batch {
+  int n_seeds;
+  const float* seed_par;  /* [6][n] */
+  const float* seed_err;  /* [21][n] */
+  const int* seed_chg;    /* [n] */
+  const int* seed_hits;   /* [first_layer][n] hit index of the seed on each layer before first_layer */
+  int first_layer;        /* >= 1 */
+  float chi2_cut;         /* IterationParams::chi2Cut_min */
+  int pflags;             /* finding_inter_layer_pflags: layer-to-layer and update propagation */
+  int intra_pflags;       /* finding_intra_layer_pflags: propagation to a hit for its chi2 */
+  int prop_to_hit;        /* finding_requires_propagation_to_hit_pos (Config.cc:52) */
+  int max_cands;          /* IterationParams::maxCandsPerSeed, 1..MKF_CE_MAX_CANDS (IterationConfig.h:97) */
+  int max_holes;          /* maxHolesPerCand */
+  int max_consec_holes;   /* maxConsecHoles */
+  /* out: per seed its candidates after the final merge, best first (K = max_cands) */
+  int* n_cands;           /* [n] */
+  float* par;             /* [K][6][n] */
+  float* err;             /* [K][21][n] */
+  int* chg;               /* [K][n] */
+  float* chi2;            /* [K][n] */
+  float* score;           /* [K][n]  getScoreCand of each candidate; -1e16 for empty slots */
+  int* n_found;           /* [K][n] */
+  int* hot_idx;           /* [K][n_layers][n] hit index or fake code per layer, -9 none */
+} mkf_ce_batch;
+
+/* MkBuilder::FindTracksCloneEngine / find_tracks_in_layers (MkBuilder.cc:1847-2060)
+ * over the layers of `eoh` from first_layer on: up to max_cands candidates per
+ * seed; per layer every candidate is propagated, its hits selected and scored
+ * (FindCandidatesCloneEngine, MkFinder.cc:1114-1284), the best max_cands
+ * (hit, candidate) combinations per seed become the next candidates
+ * (CandCloner::ProcessSeedRange, CandCloner.cc:37-191) and are updated with
+ * their new hit (UpdateWithLastHit, MkFinder.cc:1290-1308); the final order is
+ * CombCandidate::MergeCandsAndBestShortOne (HitStructures.cc:511-552).  Hit
+ * cluster charge and overlap (second-module) hits are not modelled: hits carry
+ * no charge or module id. */
+int mkf_find_tracks_clone_engine(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_ce_batch* batch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,8 +89,9 @@ ABI_SYMBOLS = [
     "mkf_fit_plain", "mkf_fit_batched",
     "mkf_set_batched_shape", "mkf_get_batched_shape", "mkf_autotune_batched", "mkf_num_batched_variants",
     "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
-    "mkf_select_hit_indices", "mkf_find_tracks_best_hit",
+    "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
 ]
+CE_MAX_CANDS = 8
 
 LAYER_DESC_FIELDS = ["is_barrel", "rin", "rout", "zmin", "zmax", "propagate_to", "q_bin", "min_dphi", "max_dphi",
                      "min_dq", "max_dq", "c_dp_sf", "c_dp_0", "c_dp_1", "c_dp_2", "c_dq_sf", "c_dq_0", "c_dq_1",
@@ -102,6 +103,15 @@ WSR_INSIDE, WSR_EDGE, WSR_OUTSIDE = 0, 1, 2
 
 class MkfLayerDesc(ctypes.Structure):
     _fields_ = [("is_barrel", ctypes.c_int)] + [(f, ctypes.c_float) for f in LAYER_DESC_FIELDS[1:]]
+
+
+class MkfCEBatch(ctypes.Structure):
+    _fields_ = [("n_seeds", ctypes.c_int), ("seed_par", _c_void_p), ("seed_err", _c_void_p), ("seed_chg", _c_void_p),
+                ("seed_hits", _c_void_p), ("first_layer", ctypes.c_int), ("chi2_cut", ctypes.c_float),
+                ("pflags", ctypes.c_int), ("intra_pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int),
+                ("max_cands", ctypes.c_int), ("max_holes", ctypes.c_int), ("max_consec_holes", ctypes.c_int),
+                ("n_cands", _c_void_p), ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p),
+                ("chi2", _c_void_p), ("score", _c_void_p), ("n_found", _c_void_p), ("hot_idx", _c_void_p)]
 
 
 class MkfFindBatch(ctypes.Structure):
@@ -142,6 +152,7 @@ def lib():
                                               ctypes.POINTER(ctypes.c_int)] + [ctypes.POINTER(_c_void_p)] * 4
         L.mkf_select_hit_indices.argtypes = [_c_void_p, ctypes.c_int, ctypes.c_int] + [_c_void_p] * 6
         L.mkf_find_tracks_best_hit.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfFindBatch), _c_void_p]
+        L.mkf_find_tracks_clone_engine.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfCEBatch), _c_void_p]
         L.mkf_autotune_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p, ctypes.c_int,
                                            ctypes.POINTER(MkfLaunchShape), _c_float_p]
         _LIB = L
@@ -407,6 +418,53 @@ class FindBatch:
                     n_found=self.n_found.cpu().numpy())
 
 
+class CEBatch:
+    """Device arrays of one clone-engine finding batch: seeds in, up to max_cands candidates per seed out."""
+
+    def __init__(self, ev, n_layers, first_layer, device="cuda:0", chi2_cut=30.0, pflags=0, intra_pflags=0,
+                 prop_to_hit=True, max_cands=6, max_holes=4, max_consec_holes=1):
+        import torch
+        n = int(ev["n_seeds"])
+        dev = torch.device(device)
+        K, L = int(max_cands), int(n_layers)
+        self.n, self.K, self.L = n, K, L
+        f = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).to(dev)
+        self.seed_par = f(ev["seed_par"][:, :n], np.float32)
+        self.seed_err = f(ev["seed_err"][:, :n], np.float32)
+        self.seed_chg = f(ev["seed_chg"][:n], np.int32)
+        self.seed_hits = f(ev["seed_hits"][:first_layer, :n], np.int32)
+        z = lambda shape, dt: torch.zeros(shape, dtype=dt, device=dev)
+        self.n_cands = z(n, torch.int32)
+        self.par = z((K, 6, n), torch.float32)
+        self.err = z((K, 21, n), torch.float32)
+        self.chg = z((K, n), torch.int32)
+        self.chi2 = z((K, n), torch.float32)
+        self.score = z((K, n), torch.float32)
+        self.n_found = z((K, n), torch.int32)
+        self.hot_idx = z((K, L, n), torch.int32)
+        self.first_layer, self.chi2_cut, self.pflags, self.intra_pflags = int(first_layer), float(chi2_cut), \
+            int(pflags), int(intra_pflags)
+        self.prop_to_hit, self.max_holes, self.max_consec_holes = int(bool(prop_to_hit)), int(max_holes), \
+            int(max_consec_holes)
+
+    def c_struct(self):
+        b = MkfCEBatch()
+        b.n_seeds = self.n
+        b.seed_par, b.seed_err, b.seed_chg, b.seed_hits = (_ptr(self.seed_par), _ptr(self.seed_err),
+                                                           _ptr(self.seed_chg), _ptr(self.seed_hits))
+        b.first_layer, b.chi2_cut, b.pflags, b.intra_pflags = self.first_layer, self.chi2_cut, self.pflags, \
+            self.intra_pflags
+        b.prop_to_hit, b.max_cands, b.max_holes, b.max_consec_holes = self.prop_to_hit, self.K, self.max_holes, \
+            self.max_consec_holes
+        for k in ("n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx"):
+            setattr(b, k, _ptr(getattr(self, k)))
+        return b
+
+    def numpy(self):
+        return {k: getattr(self, k).cpu().numpy()
+                for k in ("n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx")}
+
+
 class Context:
     """Owns an mkf_context (B field + material grid on one device)."""
 
@@ -469,6 +527,10 @@ class Context:
 
     def find_tracks_best_hit(self, eoh, batch, stream=None):
         _check(lib().mkf_find_tracks_best_hit(self.h, eoh.h, ctypes.byref(batch.c_struct()), self._stream(stream)))
+
+    def find_tracks_clone_engine(self, eoh, batch, stream=None):
+        _check(lib().mkf_find_tracks_clone_engine(self.h, eoh.h, ctypes.byref(batch.c_struct()),
+                                                  self._stream(stream)))
 
     def set_batched_shape(self, block, blocks_per_cu, variant):
         s = MkfLaunchShape(int(block), int(blocks_per_cu), int(variant))

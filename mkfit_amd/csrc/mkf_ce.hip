@@ -1,0 +1,561 @@
+// mkf_ce.hip -- clone-engine track finding on gfx950.
+//
+// One 64-lane workgroup carries 8 seeds through every layer: lane g*8 + c holds
+// candidate c of seed g (at most 8 candidates per seed) with its state in VGPRs.
+// Per layer:
+//   (a) lanes propagate their candidate to the layer, select hits and compute
+//       each hit's chi2 (after propagating to the hit, as the reference does
+//       with finding_requires_propagation_to_hit_pos); accepted hits and the
+//       invalid-hit entry go to LDS as IdxChi2List records;
+//   (b) one lane per seed keeps the best max_cands records (a stable top-K by
+//       score in the reference's generation order) and lays out the seed's
+//       next candidate list -- records interleaved with held-back extras, the
+//       best short candidate set aside (CandCloner::ProcessSeedRange);
+//   (c) every lane builds its new candidate from its parent's state (exchanged
+//       through LDS), appends the HitOnTrack node and, for a real hit, applies
+//       the Kalman update after propagating to the hit (UpdateWithLastHit).
+// Hit lists live in a per-seed append-only node store in HBM, like the
+// reference's CombCandidate::m_hots: cloning a candidate copies one index.
+// After the last layer MergeCandsAndBestShortOne orders the candidates, and
+// each lane walks its node chain to write the per-layer hit indices.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/mkfit_amd.h"
+#include "mkf_find_common.h"
+#include "mkf_internal.h"
+#include "mkf_track_math.h"
+
+using namespace mkf;
+using namespace mkf_detail;
+using namespace mkf_find;
+
+namespace {
+
+constexpr int kS = 8;                 // candidate slots per seed
+constexpr int kG = 8;                 // seeds per workgroup
+constexpr int kLanes = kS * kG;       // 64: one wavefront
+constexpr int kW = 37;                // LDS words per exchanged candidate (36 used, odd stride)
+constexpr float kScoreWorst = -1e16f;
+
+// Config.h:355-359 and getScoreCalc (Track.h:648-672)
+__device__ __forceinline__ float score_calc(int nfound, int ntail, int noverlap, int nmiss, float chi2, float pt,
+                                            bool in_find) {
+  const float maxBonus = 8.0f;
+  float bonus = 0.2f * nfound + 4.0f;
+  float penalty = 8.0f;
+  if (pt < 0.9f) {
+    penalty *= in_find ? 1.7f : 1.5f;
+    bonus = fminf(bonus * (in_find ? 0.9f : 1.0f), maxBonus);
+  }
+  return bonus * nfound + 0.0f * noverlap - penalty * nmiss - 3.0f * ntail - chi2;
+}
+
+// std::hypot(float, float) of the reference: the sum of squares in double (exact
+// for float inputs up to one rounding) and a correctly rounded root, rounded to float.
+__device__ __forceinline__ float hypot_ref(float x, float y) {
+  const double xd = x, yd = y;
+  return (float)__builtin_sqrt(xd * xd + yd * yd);
+}
+
+struct Cand {
+  float par[6], err[21];
+  int chg;
+  float chi2, score;
+  int nfound, nmissing, ninside, ntail;
+  int last;       // node index of the last HitOnTrack (CombCandidate::m_hots position)
+  int last_hit;   // hit index stored in that node
+};
+
+__device__ __forceinline__ void add_hit_counts(Cand& c, int idx, float chi2) {   // TrackCand::addHitIdx
+  if (idx >= 0 || idx == -9) {
+    ++c.nfound;
+    c.chi2 += chi2;
+    c.ninside += c.ntail;
+    c.ntail = 0;
+  } else {
+    ++c.nmissing;
+    if (idx == -1) ++c.ntail;
+  }
+  c.last_hit = idx;
+}
+
+__device__ __forceinline__ float cand_score(const Cand& c) {                       // getScoreCand
+  const float chi2 = c.chi2 < 0 ? 0.f : c.chi2;
+  return score_calc(c.nfound, 0, 0, c.ninside, chi2, fabsf(1.f / c.par[3]), false);
+}
+
+__device__ __forceinline__ void put(float* X, const Cand& c) {
+#pragma unroll
+  for (int e = 0; e < 6; ++e) X[e] = c.par[e];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) X[6 + e] = c.err[e];
+  X[27] = __int_as_float(c.chg);
+  X[28] = c.chi2;
+  X[29] = c.score;
+  X[30] = __int_as_float(c.nfound);
+  X[31] = __int_as_float(c.nmissing);
+  X[32] = __int_as_float(c.ninside);
+  X[33] = __int_as_float(c.ntail);
+  X[34] = __int_as_float(c.last);
+  X[35] = __int_as_float(c.last_hit);
+}
+__device__ __forceinline__ void get(const float* X, Cand& c) {
+#pragma unroll
+  for (int e = 0; e < 6; ++e) c.par[e] = X[e];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) c.err[e] = X[6 + e];
+  c.chg = __float_as_int(X[27]);
+  c.chi2 = X[28];
+  c.score = X[29];
+  c.nfound = __float_as_int(X[30]);
+  c.nmissing = __float_as_int(X[31]);
+  c.ninside = __float_as_int(X[32]);
+  c.ntail = __float_as_int(X[33]);
+  c.last = __float_as_int(X[34]);
+  c.last_hit = __float_as_int(X[35]);
+}
+
+// isStripQCompatible (MkFinder.cc:837-862) for one lane: track z / xy from the
+// state propagated to the hit, its uncertainty from the state at the layer.
+__device__ __forceinline__ bool strip_q_compatible(bool barrel, const float (&pe)[21], const float (&q)[6],
+                                                   const float (&mp)[3], const float (&me)[6]) {
+  if (barrel) {
+    const float res = fabsf(mp[2] - q[2]);
+    const float hitHL = sqrtf(me[5] * 3.f);
+    const float qErr = sqrtf(pe[S(2, 2)]);
+    return hitHL + fmaxf(3.f * qErr, 0.5f) > res;
+  }
+  const float r0 = mp[0] - q[0], r1 = mp[1] - q[1];
+  const float hitT2 = me[0] + me[2];
+  const float inv = 1.f / hitT2;
+  const float p0 = me[0] * inv, p1 = me[1] * inv, p2 = me[2] * inv;
+  const float qErr = sqrtf(fabsf(pe[S(0, 0)] * p0 + 2.f * pe[S(0, 1)] * p1 + pe[S(1, 1)] * p2));
+  const float resProj = sqrtf(r0 * p0 * r0 + 2.f * r1 * p1 * r0 + r1 * p2 * r1);
+  return sqrtf(hitT2 * 3.f) + fmaxf(3.f * qErr, 0.5f) > resProj;
+}
+
+struct CeArgs {
+  int n, n_layers, first_layer;
+  int max_cands, max_holes, max_consec, prop_to_hit, intra_pflags, inter_pflags;
+  float chi2_cut, bfield;
+  Material mat;
+  const float* __restrict__ seed_par;
+  const float* __restrict__ seed_err;
+  const int* __restrict__ seed_chg;
+  const int* __restrict__ seed_hits;
+  int* __restrict__ n_cands;
+  float* __restrict__ par;
+  float* __restrict__ err;
+  int* __restrict__ chg;
+  float* __restrict__ chi2;
+  float* __restrict__ score;
+  int* __restrict__ n_found;
+  int* __restrict__ hot;
+  int4* __restrict__ nodes;   // [n][node_cap] {hit index, layer, prev node, 0}
+  int node_cap;
+  const LayerDev* __restrict__ layers;
+  const int2* __restrict__ bins;
+  const float* __restrict__ phis;
+  const float* __restrict__ qs;
+  const float* __restrict__ shits;
+  const int* __restrict__ ranks;
+};
+
+// Appends a node to seed `s`'s store; returns its index, or -1 when the store
+// is full (never for the store size the host allocates: 3 nodes per candidate
+// slot and layer plus the seed hits).
+__device__ __forceinline__ int append_node(const CeArgs& a, int s, int* cnt, int hit, int layer, int prev) {
+  const int k = atomicAdd(cnt, 1);
+  if (k >= a.node_cap) return -1;
+  a.nodes[(size_t)s * a.node_cap + k] = make_int4(hit, layer, prev, 0);
+  return k;
+}
+
+__device__ __forceinline__ void propagate(const LayerDev& d, float (&p)[6], float (&e)[21], int chg, float target,
+                                          int pflags, const CeArgs& a) {
+  if (d.is_barrel)
+    propagate_to_r(p, e, chg, target, pflags, false, a.bfield, a.mat);
+  else
+    propagate_to_z(p, e, chg, target, pflags, false, a.bfield, a.mat);
+}
+
+__global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
+  __shared__ float X[kLanes * kW];          // candidate state exchange
+  __shared__ float4 E[kLanes][kHitIdxMax];  // accepted-hit records {score, chi2_hit, sorted pos, hit_cnt}
+  __shared__ int ecnt[kLanes];
+  __shared__ float fscore[kLanes];          // invalid-hit record score
+  __shared__ int fcode[kLanes];             // its hit code, 0 = none
+  __shared__ int xkind[kLanes];             // 1 = held back unchanged, 2 = stopped replica
+  __shared__ int4 D[kG][kS];                // next candidates {kind, parent slot, value, node}
+  __shared__ float Dsc[kG][kS], Dc2[kG][kS];
+  __shared__ float BS[kG][kW];              // best short candidate
+  __shared__ int s_bs[kG], s_state[kG], s_ncand[kG], s_nnodes[kG], s_any[kG], s_nnew[kG], s_merge[kG];
+  __shared__ int O[kG][kS + 1];             // final order (kS = best short)
+  __shared__ int s_nout[kG];
+
+  const int lane = threadIdx.x;
+  const int g = lane / kS, c = lane % kS;
+  const int seed = blockIdx.x * kG + g;
+  const bool gvalid = seed < a.n;
+  const size_t N = (size_t)a.n;
+  const int K = a.max_cands;
+
+  Cand st = {};
+  if (c == 0) {
+    s_state[g] = 0;
+    s_ncand[g] = gvalid ? 1 : 0;
+    s_nnodes[g] = 0;
+    s_bs[g] = 0;
+    BS[g][29] = kScoreWorst;
+  }
+  __syncthreads();
+  if (gvalid && c == 0) {                                    // CombCandidate::ImportSeed
+#pragma unroll
+    for (int e = 0; e < 6; ++e) st.par[e] = a.seed_par[e * N + seed];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) st.err[e] = a.seed_err[e * N + seed];
+    st.chg = a.seed_chg[seed];
+    st.chi2 = 0.f;
+    st.nfound = st.nmissing = st.ninside = st.ntail = 0;
+    st.last = -1;
+    st.last_hit = -1;
+    for (int l = 0; l < a.first_layer; ++l) {
+      const int h = a.seed_hits[l * N + seed];
+      const int k = append_node(a, seed, &s_nnodes[g], h, l, st.last);
+      if (k >= 0) st.last = k;
+      add_hit_counts(st, h, 0.f);
+    }
+    st.score = cand_score(st);
+  }
+
+  for (int layer = a.first_layer; layer < a.n_layers; ++layer) {
+    const LayerDev& d = a.layers[layer];
+    const bool barrel = d.is_barrel != 0;
+    // ---- (a) find_tracks_unroll_candidates
+    if (c == 0 && gvalid) {
+      if (s_state[g] == 0 && layer - 1 == a.first_layer - 1) s_state[g] = 1;
+      s_any[g] = 0;
+    }
+    ecnt[lane] = 0;
+    fcode[lane] = 0;
+    xkind[lane] = 0;
+    __syncthreads();
+    const bool finding = gvalid && s_state[g] == 1;
+    const bool valid = finding && c < s_ncand[g];
+    bool active = valid && st.last_hit != -2;
+    if (active && fabsf(1.f / st.par[3]) < 1.2f) {          // near the apex of a low-pT helix
+      const float dphi = fabsf(atan2f(st.par[1], st.par[0]) - st.par[4]);
+      if (st.par[0] * st.par[0] + st.par[1] * st.par[1] > 625.f && dphi > 1.371f && dphi < 4.512f) {
+        const int k = append_node(a, seed, &s_nnodes[g], -2, layer, st.last);
+        if (k >= 0) st.last = k;
+        add_hit_counts(st, -2, 0.f);
+        active = false;
+      }
+    }
+    if (active) s_any[g] = 1;
+    __syncthreads();
+    if (c == 0) s_merge[g] = finding && s_any[g];
+    if (c == 0 && finding && !s_any[g]) s_state[g] = 2;
+    if (active) {
+      float pp[6], pe[21];
+#pragma unroll
+      for (int e = 0; e < 6; ++e) pp[e] = st.par[e];
+#pragma unroll
+      for (int e = 0; e < 21; ++e) pe[e] = st.err[e];
+      propagate(d, pp, pe, st.chg, d.propagate_to, a.inter_pflags, a);
+      Window w;
+      hit_window(d, pp, pe, w);
+      // find_tracks_handle_missed_layers (MkBuilder.cc:1568-1630)
+      if (barrel && hypot_ref(pp[0], pp[1]) < d.rin) {
+        const int k = append_node(a, seed, &s_nnodes[g], -2, layer, st.last);
+        if (k >= 0) st.last = k;
+        add_hit_counts(st, -2, 0.f);
+        xkind[lane] = 2;
+      } else if (w.wsr == MKF_WSR_OUTSIDE) {
+        xkind[lane] = 1;
+      } else {
+        // FindCandidatesCloneEngine (MkFinder.cc:1114-1284)
+        const float max_c2 = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
+        const float pt = fabsf(1.0f / pp[3]);
+        int hc = 0, ne = 0;
+        scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) {
+          float mp[3], me[6];
+          load_hit9(a.shits + (size_t)gi * 9, mp, me);
+          float q[6], qe[21];
+#pragma unroll
+          for (int e = 0; e < 6; ++e) q[e] = pp[e];
+#pragma unroll
+          for (int e = 0; e < 21; ++e) qe[e] = pe[e];
+          if (a.prop_to_hit) propagate(d, q, qe, st.chg, barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a);
+          const float c2 = fabsf(barrel ? kalman_barrel<true, false>(q, qe, mp, me)
+                                        : kalman_endcap<true, false>(q, qe, mp, me));
+          if (c2 < max_c2 && strip_q_compatible(barrel, pe, q, mp, me)) {
+            const float tot = st.chi2 + c2;
+            const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot, pt, true);
+            E[lane][ne++] = make_float4(sc, c2, __int_as_float(gi), __int_as_float(hc));
+          }
+          ++hc;
+        });
+        ecnt[lane] = ne;
+        int fake = (st.ninside + st.ntail < a.max_holes && st.ntail < a.max_consec) ? -1 : -2;
+        if (w.wsr == MKF_WSR_EDGE) fake = -3;
+        const int nt = fake == -1 ? st.ntail + 1 : st.ntail;
+        fcode[lane] = fake;
+        fscore[lane] = score_calc(st.nfound, nt, 0, st.ninside, st.chi2 < 0 ? 0.f : st.chi2, pt, true);
+      }
+    }
+    put(&X[lane * kW], st);
+    __syncthreads();
+    // ---- (b) CandCloner::ProcessSeedRange, one lane per seed
+    if (c == 0 && s_merge[g]) {
+      float tsc[kS];
+      int tkind[kS], tslot[kS], tval[kS];
+      float tc2[kS];
+      int nt = 0, nent = 0;
+      auto insert = [&](float sc, int kind, int slot, int val, float cc2) {
+        ++nent;
+        if (nt == K && !(sc > tsc[K - 1])) return;
+        int pos = nt < K ? nt : K - 1;
+        while (pos > 0 && tsc[pos - 1] < sc) {
+          tsc[pos] = tsc[pos - 1]; tkind[pos] = tkind[pos - 1]; tslot[pos] = tslot[pos - 1];
+          tval[pos] = tval[pos - 1]; tc2[pos] = tc2[pos - 1];
+          --pos;
+        }
+        tsc[pos] = sc; tkind[pos] = kind; tslot[pos] = slot; tval[pos] = val; tc2[pos] = cc2;
+        if (nt < K) ++nt;
+      };
+      int ptr[kS];
+#pragma unroll
+      for (int s = 0; s < kS; ++s) ptr[s] = 0;
+      for (int hc = 0; hc < kHitIdxMax; ++hc)                // generation order: hit_cnt, then lane
+        for (int s = 0; s < kS; ++s) {
+          const int l = g * kS + s;
+          if (ptr[s] < ecnt[l]) {
+            const float4 r = E[l][ptr[s]];
+            if (__float_as_int(r.w) == hc) {
+              insert(r.x, 2, s, __float_as_int(r.z), r.y);
+              ++ptr[s];
+            }
+          }
+        }
+      for (int s = 0; s < kS; ++s) {
+        const int l = g * kS + s;
+        if (fcode[l]) insert(fscore[l], 3, s, fcode[l], 0.f);
+      }
+      int np = 0;
+      int xs[kS], nx = 0;
+      for (int s = 0; s < kS; ++s)
+        if (xkind[g * kS + s]) xs[nx++] = s;
+      if (nent == 0) {
+        for (int i = 0; i < nx; ++i) {
+          D[g][np] = make_int4(1, xs[i], 0, -1);
+          Dsc[g][np] = X[(g * kS + xs[i]) * kW + 29];
+          ++np;
+        }
+      } else {
+        int xi = 0;
+        for (int ih = 0; ih < nt; ++ih) {
+          if (tkind[ih] == 3 && tval[ih] == -2) {
+            if (tsc[ih] > BS[g][29]) {                      // best short candidate
+              Cand b;
+              get(&X[(g * kS + tslot[ih]) * kW], b);
+              const int k = append_node(a, seed, &s_nnodes[g], -2, layer, b.last);
+              if (k >= 0) b.last = k;
+              add_hit_counts(b, -2, 0.f);
+              b.score = tsc[ih];
+              put(BS[g], b);
+              s_bs[g] = 1;
+            }
+            continue;
+          }
+          while (xi < nx && X[(g * kS + xs[xi]) * kW + 29] > tsc[ih] && np < K) {
+            D[g][np] = make_int4(1, xs[xi], 0, -1);
+            Dsc[g][np] = X[(g * kS + xs[xi]) * kW + 29];
+            ++np; ++xi;
+          }
+          if (np >= K) break;
+          D[g][np] = make_int4(tkind[ih], tslot[ih], tval[ih], 0);
+          Dsc[g][np] = tsc[ih];
+          Dc2[g][np] = tc2[ih];
+          ++np;
+        }
+        while (xi < nx && np < K) {
+          D[g][np] = make_int4(1, xs[xi], 0, -1);
+          Dsc[g][np] = X[(g * kS + xs[xi]) * kW + 29];
+          ++np; ++xi;
+        }
+      }
+      s_nnew[g] = np;
+    }
+    __syncthreads();
+    // ---- (c) new candidates; UpdateWithLastHit for those that took a hit
+    if (s_merge[g] && gvalid) {
+      if (c < s_nnew[g]) {
+        const int4 dd = D[g][c];
+        get(&X[(g * kS + dd.y) * kW], st);
+        if (dd.x != 1) {
+          const int hit = dd.x == 2 ? a.ranks[dd.z] : dd.z;
+          const int k = append_node(a, seed, &s_nnodes[g], hit, layer, st.last);
+          if (k >= 0) st.last = k;
+          add_hit_counts(st, hit, Dc2[g][c]);
+          st.score = Dsc[g][c];
+          if (dd.x == 2) {
+            float mp[3], me[6];
+            load_hit9(a.shits + (size_t)dd.z * 9, mp, me);
+            if (a.prop_to_hit)
+              propagate(d, st.par, st.err, st.chg, barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.inter_pflags, a);
+            if (barrel)
+              kalman_barrel<false, true>(st.par, st.err, mp, me);
+            else
+              kalman_endcap<false, true>(st.par, st.err, mp, me);
+            if (st.par[3] < 0.f) {
+              st.chg = -st.chg;
+              st.par[3] = -st.par[3];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (c == 0 && s_merge[g]) s_ncand[g] = s_nnew[g];
+  }
+
+  // ---- MergeCandsAndBestShortOne(update_score, sort) and output
+  __syncthreads();
+  const bool valid = gvalid && c < s_ncand[g];
+  if (valid) st.score = cand_score(st);
+  put(&X[lane * kW], st);
+  __syncthreads();
+  if (c == 0 && gvalid) {
+    const int nc = s_ncand[g];
+    int ord[kS + 1];
+    int m = 0;
+    for (int i = 0; i < nc; ++i) {                            // stable insertion sort by score
+      const float sc = X[(g * kS + i) * kW + 29];
+      int pos = m;
+      while (pos > 0 && X[(g * kS + ord[pos - 1]) * kW + 29] < sc) {
+        ord[pos] = ord[pos - 1];
+        --pos;
+      }
+      ord[pos] = i;
+      ++m;
+    }
+    if (s_bs[g]) {
+      Cand b;
+      get(BS[g], b);
+      b.score = cand_score(b);
+      put(BS[g], b);
+      if (m > 0) {
+        if (b.score > X[(g * kS + ord[m - 1]) * kW + 29]) {
+          int ci = 0;
+          while (X[(g * kS + ord[ci]) * kW + 29] > b.score) ++ci;
+          if (m >= K) --m;
+          for (int j = m; j > ci; --j) ord[j] = ord[j - 1];
+          ord[ci] = kS;
+          ++m;
+        }
+      } else {
+        ord[m++] = kS;
+      }
+    }
+    for (int i = 0; i < m; ++i) O[g][i] = ord[i];
+    s_nout[g] = m;
+    a.n_cands[seed] = m;
+  }
+  __syncthreads();
+  if (!gvalid || c >= K) return;
+  const int L = a.n_layers;
+  int* hot = a.hot + ((size_t)c * L) * N + seed;
+  for (int l = 0; l < L; ++l) hot[(size_t)l * N] = -9;
+  if (c >= s_nout[g]) {
+    a.score[c * N + seed] = kScoreWorst;
+    a.n_found[c * N + seed] = 0;
+    return;
+  }
+  Cand o;
+  get(O[g][c] == kS ? BS[g] : &X[(g * kS + O[g][c]) * kW], o);
+#pragma unroll
+  for (int e = 0; e < 6; ++e) a.par[((size_t)c * 6 + e) * N + seed] = o.par[e];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) a.err[((size_t)c * 21 + e) * N + seed] = o.err[e];
+  a.chg[c * N + seed] = o.chg;
+  a.chi2[c * N + seed] = o.chi2;
+  a.score[c * N + seed] = o.score;
+  a.n_found[c * N + seed] = o.nfound;
+  const int4* nodes = a.nodes + (size_t)seed * a.node_cap;
+  for (int k = o.last, steps = 0; k >= 0 && steps < a.node_cap; k = nodes[k].z, ++steps) {
+    const int4 nd = nodes[k];
+    if (nd.y >= 0 && nd.y < L && hot[(size_t)nd.y * N] == -9) hot[(size_t)nd.y * N] = nd.x;
+  }
+}
+
+}  // namespace
+
+extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch* b,
+                                            void* stream) {
+  CHECK(set_device(c));
+  if (!h || !b) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  if (h->ctx != c) return fail(MKF_ERR_INVALID_ARG, "event of hits belongs to another context");
+  if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
+  if (b->n_seeds < 0) return fail(MKF_ERR_INVALID_ARG, "n_seeds < 0");
+  if (b->first_layer < 1 || b->first_layer > h->n_layers)
+    return fail(MKF_ERR_INVALID_ARG, "first_layer %d outside [1, %d]", b->first_layer, h->n_layers);
+  if (b->max_cands < 1 || b->max_cands > MKF_CE_MAX_CANDS)
+    return fail(MKF_ERR_INVALID_ARG, "max_cands must be in [1, %d]", MKF_CE_MAX_CANDS);
+  if (((b->pflags | b->intra_pflags) & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
+    return fail(MKF_ERR_INVALID_ARG, "material requested but the context has no material grid");
+  if (b->n_seeds == 0) return MKF_OK;
+  const size_t N = b->n_seeds, K = b->max_cands, L = h->n_layers;
+  if (K * 21 * N * 4 >= ((size_t)1 << 31) || K * L * N >= ((size_t)1 << 31))
+    return fail(MKF_ERR_INVALID_ARG, "batch too large");
+  CHECK(check_dev(b->seed_par, 6 * N * 4, "seed_par"));
+  CHECK(check_dev(b->seed_err, 21 * N * 4, "seed_err"));
+  CHECK(check_dev(b->seed_chg, N * 4, "seed_chg"));
+  CHECK(check_dev(b->seed_hits, (size_t)b->first_layer * N * 4, "seed_hits"));
+  CHECK(check_dev(b->n_cands, N * 4, "n_cands"));
+  CHECK(check_dev(b->par, K * 6 * N * 4, "par"));
+  CHECK(check_dev(b->err, K * 21 * N * 4, "err"));
+  CHECK(check_dev(b->chg, K * N * 4, "chg"));
+  CHECK(check_dev(b->chi2, K * N * 4, "chi2"));
+  CHECK(check_dev(b->score, K * N * 4, "score"));
+  CHECK(check_dev(b->n_found, K * N * 4, "n_found"));
+  CHECK(check_dev(b->hot_idx, K * L * N * 4, "hot_idx"));
+  const int node_cap = b->first_layer + 3 * kS * (h->n_layers - b->first_layer) + kS;
+  const size_t need = (size_t)node_cap * N;
+  hipStream_t st = (hipStream_t)stream;
+  if (need > h->ce_node_cap) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (h->d_ce_nodes) HIP_TRY(hipFree(h->d_ce_nodes));
+    h->d_ce_nodes = nullptr;
+    h->ce_node_cap = 0;
+    HIP_TRY(hipMalloc(&h->d_ce_nodes, need * sizeof(int4)));
+    h->ce_node_cap = need;
+  }
+  CeArgs a;
+  a.n = b->n_seeds;
+  a.n_layers = h->n_layers;
+  a.first_layer = b->first_layer;
+  a.max_cands = b->max_cands;
+  a.max_holes = b->max_holes;
+  a.max_consec = b->max_consec_holes;
+  a.prop_to_hit = b->prop_to_hit;
+  a.intra_pflags = b->intra_pflags;
+  a.inter_pflags = b->pflags;
+  a.chi2_cut = b->chi2_cut;
+  a.bfield = c->bfield;
+  a.mat = mat_of(c);
+  a.seed_par = b->seed_par; a.seed_err = b->seed_err; a.seed_chg = b->seed_chg; a.seed_hits = b->seed_hits;
+  a.n_cands = b->n_cands; a.par = b->par; a.err = b->err; a.chg = b->chg; a.chi2 = b->chi2; a.score = b->score;
+  a.n_found = b->n_found; a.hot = b->hot_idx;
+  a.nodes = reinterpret_cast<int4*>(h->d_ce_nodes);
+  a.node_cap = node_cap;
+  a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
+  a.ranks = h->d_ranks;
+  const int grid = (int)((N + kG - 1) / kG);
+  hipLaunchKernelGGL(k_find_clone_engine, dim3(grid), dim3(kLanes), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}

@@ -33,50 +33,13 @@
 #include "../../include/mkfit_amd.h"
 #include "mkf_internal.h"
 #include "mkf_track_math.h"
+#include "mkf_find_common.h"
 
 using namespace mkf;
 using namespace mkf_detail;
+using namespace mkf_find;
 
 namespace {
-
-constexpr int kNPhi = 256;                      // Config::m_nphi
-constexpr float kFPhiFine = 1024 / kTwoPI;      // HitStructures.h:153-158 (float arithmetic)
-constexpr float kPIOver2 = kPI / 2.0f;          // Config.h:79
-constexpr int kHitIdxMax = MKF_HIT_IDX_MAX;
-constexpr int kLayerShift = 26, kQShift = 11;
-constexpr int kMaxQBins = 1 << 15;
-
-struct LayerDev {
-  int is_barrel, nq, hit_off, n_hits, bin_off;
-  float rin, rout, zmin, zmax, propagate_to;
-  float qmin, fq;                               // bins after setup_bins (extended range, scale)
-  float min_dphi, max_dphi, min_dq, max_dq;
-  float c_dp_sf, c_dp_0, c_dp_1, c_dp_2;
-  float c_dq_sf, c_dq_0, c_dq_1, c_dq_2;
-  float c_c2_sf, c_c2_0, c_c2_1, c_c2_2;
-};
-
-// LayerOfHits::setup_bins (HitStructures.cc:11-32), same float arithmetic.
-void setup_bins(float qmin, float qmax, float dq, int& nq, float& qmin_ext, float& fq) {
-  if (dq < 0) {
-    nq = (int)-dq;
-    qmin_ext = qmin;
-  } else {
-    const float extent = qmax - qmin;
-    nq = (int)std::ceil(extent / dq);
-    const float extra = 0.5f * (nq * dq - extent);
-    qmin_ext = qmin - extra;
-  }
-  fq = nq / (qmax - qmin);
-}
-
-__device__ __forceinline__ int q_bin_checked(const LayerDev& d, float q) {   // HitStructures.h:237-245
-  const int b = (int)((q - d.qmin) * d.fq);
-  return min(max(b, 0), d.nq - 1);
-}
-__device__ __forceinline__ int phi_bin_fine(float phi) { return (int)floorf(kFPhiFine * (phi + kPI)); }
-__device__ __forceinline__ int phi_bin(float phi) { return phi_bin_fine(phi) >> 2; }
-__device__ __forceinline__ float cdist(float a) { return a > kPI ? kTwoPI - a : a; }   // Config.h:441
 
 struct OffsetsArg {
   int off[MKF_MAX_FIND_LAYERS + 1];
@@ -153,100 +116,6 @@ k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* 
   if (last) b->y = local + 1;
 }
 
-// ---- SelectHitIndices window (MkFinder.cc:166-195, 227-430) -----------------
-
-struct Window {
-  float q, phi, dq, dphi;
-  int qb1, qb2, pb1, pb2, wsr;
-};
-
-__device__ __forceinline__ int wsr_z(const LayerDev& d, float z, float dz) {   // TrackerInfo.h:116-123
-  if (z > d.zmax + dz || z < d.zmin - dz) return MKF_WSR_OUTSIDE;
-  if (z < d.zmax - dz && z > d.zmin + dz) return MKF_WSR_INSIDE;
-  return MKF_WSR_EDGE;
-}
-__device__ __forceinline__ int wsr_r(const LayerDev& d, float r, float dr) {   // TrackerInfo.h:125-138, no hole
-  if (r > d.rout + dr || r < d.rin - dr) return MKF_WSR_OUTSIDE;
-  if (r < d.rout - dr && r > d.rin + dr) return MKF_WSR_INSIDE;
-  return MKF_WSR_EDGE;
-}
-
-__device__ __forceinline__ void hit_window(const LayerDev& d, const float (&par)[6], const float (&err)[21],
-                                           Window& w) {
-  float min_dq = d.min_dq, max_dq = d.max_dq, min_dphi = d.min_dphi, max_dphi = d.max_dphi;
-  const float invpt = par[3];
-  const float theta = fabsf(par[5] - kPIOver2);
-  {                                              // getHitSelDynamicWindows
-    const float max_invpt = invpt > 10.0f ? 10.0f : invpt;
-    const float this_dq = d.c_dq_0 * max_invpt + d.c_dq_1 * theta + d.c_dq_2;
-    if (d.c_dq_sf * this_dq > 0.f) {
-      min_dq = d.c_dq_sf * this_dq;
-      max_dq = 2.0f * min_dq;
-    }
-    const float this_dphi = d.c_dp_0 * max_invpt + d.c_dp_1 * theta + d.c_dp_2;
-    if (d.c_dp_sf * this_dphi > min_dphi) {
-      min_dphi = d.c_dp_sf * this_dphi;
-      max_dphi = 2.0f * min_dphi;
-    }
-  }
-  const float x = par[0], y = par[1];
-  const float r2 = x * x + y * y;
-  const float dphidx = -y / r2, dphidy = x / r2;
-  const float dphi2 = dphidx * dphidx * err[S(0, 0)] + dphidy * dphidy * err[S(1, 1)] +
-                      2 * dphidx * dphidy * err[S(0, 1)];
-  const float phi = atan2f(y, x);
-  float dphi = fmaxf(3.f * sqrtf(fabsf(dphi2)), min_dphi);
-  float q, dq;
-  if (d.is_barrel) {
-    const float z = par[2];
-    dq = fabsf(3.f * sqrtf(err[S(2, 2)]));
-    const float edge = fabsf(0.5f * (d.rout - d.rin) / tanf(par[5]));
-    w.wsr = wsr_z(d, z, sqrtf(dq * dq + edge * edge));
-    q = z;
-  } else {
-    const float r = sqrtf(r2);
-    dq = 3.f * sqrtf(fabsf(x * x * err[S(0, 0)] + y * y * err[S(1, 1)] + 2 * x * y * err[S(0, 1)]) / r2);
-    const float edge = fabsf(0.5f * (d.zmax - d.zmin) * tanf(par[5]));
-    w.wsr = wsr_r(d, r, sqrtf(dq * dq + edge * edge));
-    q = r;
-  }
-  dphi = fminf(fmaxf(fabsf(dphi), min_dphi), max_dphi);
-  dq = fminf(fmaxf(dq, min_dq), max_dq);
-  w.q = q;
-  w.phi = phi;
-  w.dq = dq;
-  w.dphi = dphi;
-  w.qb1 = q_bin_checked(d, q - dq);
-  w.qb2 = q_bin_checked(d, q + dq) + 1;
-  w.pb1 = phi_bin(phi - dphi);
-  w.pb2 = phi_bin(phi + dphi) + 1;
-  // A window of at most one full turn spans at most kNPhi + 2 phi bins; wider
-  // dynamic windows are cut there so the scan stays bounded.
-  w.pb2 = min(w.pb2, w.pb1 + kNPhi + 2);
-}
-
-// Scan of the window's bins in the reference's order (q bins outer, phi bins
-// inner, sorted hits within a bin), at most kHitIdxMax accepted hits; calls
-// on_hit(global sorted position) for each (MkFinder.cc:433-684).
-template <class F>
-__device__ __forceinline__ int scan_window(const LayerDev& d, const Window& w, const int2* __restrict__ bins,
-                                           const float* __restrict__ phis, const float* __restrict__ qs, F&& on_hit) {
-  int cnt = 0;
-  for (int qi = w.qb1; qi < w.qb2 && cnt < kHitIdxMax; ++qi)
-    for (int pi = w.pb1; pi < w.pb2 && cnt < kHitIdxMax; ++pi) {
-      const int2 be = bins[d.bin_off + qi * kNPhi + (pi & 0xff)];
-      for (int hi = be.x; hi < be.y && cnt < kHitIdxMax; ++hi) {
-        const int gi = d.hit_off + hi;
-        const float ddq = fabsf(w.q - qs[gi]);
-        const float ddphi = cdist(fabsf(w.phi - phis[gi]));
-        if (ddq >= w.dq || ddphi >= w.dphi) continue;
-        on_hit(gi);
-        ++cnt;
-      }
-    }
-  return cnt;
-}
-
 __global__ void __launch_bounds__(256)
 k_select(LayerDev d, int n, const float* __restrict__ par, const float* __restrict__ err,
          const int2* __restrict__ bins, const float* __restrict__ phis, const float* __restrict__ qs,
@@ -292,21 +161,6 @@ struct FindArgs {
   const float* __restrict__ shits;
   const int* __restrict__ ranks;
 };
-
-// getHitSelDynamicChi2Cut (MkFinder.cc:202-220)
-__device__ __forceinline__ float dyn_chi2_cut(const LayerDev& d, float cut_min, float invpt, float theta_in) {
-  const float theta = fabsf(theta_in - kPIOver2);
-  const float max_invpt = invpt > 10.0f ? 10.0f : invpt;
-  const float this_c2 = d.c_c2_0 * max_invpt + d.c_c2_1 * theta + d.c_c2_2;
-  return d.c_c2_sf * this_c2 > cut_min ? d.c_c2_sf * this_c2 : cut_min;
-}
-
-__device__ __forceinline__ void load_hit9(const float* __restrict__ h, float (&mp)[3], float (&me)[6]) {
-#pragma unroll
-  for (int e = 0; e < 3; ++e) mp[e] = h[e];
-#pragma unroll
-  for (int e = 0; e < 6; ++e) me[e] = h[3 + e];
-}
 
 // The reference's block loop (MkBuilder.cc:1395-1478) per lane.  A lane that
 // stops leaves the loop with the state it had at that point, which is the
@@ -405,28 +259,6 @@ __global__ void __launch_bounds__(256) k_find_best_hit(FindArgs a) {
 // C ABI
 // ============================================================================
 
-struct mkf_event_of_hits {
-  mkf_context* ctx = nullptr;
-  int n_layers = 0;
-  std::vector<LayerDev> layers;   // host copy; hit_off / n_hits as of the last load
-  LayerDev* d_layers = nullptr;
-  int total_bins = 0;
-  int2* d_bins = nullptr;
-  size_t cap = 0;                 // hit capacity of the arrays below
-  uint32_t* d_keys = nullptr;
-  uint32_t* d_keys_s = nullptr;
-  int* d_vals = nullptr;
-  int* d_vals_s = nullptr;
-  float* d_phi_o = nullptr;
-  float* d_q_o = nullptr;
-  float* d_phis = nullptr;
-  float* d_qs = nullptr;
-  float* d_shits = nullptr;
-  int* d_ranks = nullptr;
-  void* d_temp = nullptr;
-  size_t temp_bytes = 0;
-  bool loaded = false;
-};
 
 namespace {
 
@@ -530,6 +362,7 @@ int mkf_event_of_hits_destroy(mkf_event_of_hits* h) {
   (void)hipSetDevice(h->ctx->device);
   (void)hipDeviceSynchronize();
   free_hit_arrays(h);
+  if (h->d_ce_nodes) (void)hipFree(h->d_ce_nodes);
   if (h->d_layers) (void)hipFree(h->d_layers);
   if (h->d_bins) (void)hipFree(h->d_bins);
   delete h;

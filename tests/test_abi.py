@@ -62,6 +62,7 @@ def test_null_arguments_rejected():
     assert L.mkf_event_of_hits_load(None, None, None, None) == 1
     assert L.mkf_select_hit_indices(None, 0, 0, None, None, None, None, None, None) == 1
     assert L.mkf_find_tracks_best_hit(None, None, None, None) == 1
+    assert L.mkf_find_tracks_clone_engine(None, None, None, None) == 1
     assert L.mkf_event_of_hits_destroy(None) == 0
     assert L.mkf_num_batched_variants() >= 1
 
@@ -70,3 +71,4 @@ def test_struct_layouts():
     import mkfit_amd as mk
     assert ctypes.sizeof(mk.MkfLayerDesc) == 23 * 4
     assert ctypes.sizeof(mk.MkfFindBatch) == 4 + 4 + 6 * 8 + 4 + 4 + 4 + 4   # n, pad, 6 pointers, 3 scalars, pad
+    assert ctypes.sizeof(mk.MkfCEBatch) == 8 + 4 * 8 + 8 * 4 + 8 * 8      # n + pad, 4 ptrs, 8 scalars, 8 ptrs

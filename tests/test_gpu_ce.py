@@ -1,0 +1,119 @@
+"""GPU parity of clone-engine finding (mkf_find_tracks_clone_engine) against the CPU oracle.
+
+Per seed the candidate lists (count, order, hit index per layer of every candidate) must match the
+oracle's exactly, except for seeds where a float32 rounding difference reorders two candidates of
+nearly equal score or moves a hit across a window edge; those seeds are counted and bounded.
+"""
+import numpy as np
+import pytest
+
+import mkfit_amd as mk
+import oracle
+from _common import par_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _event(seed, n, noise, pt_min=0.5):
+    geo = mk.geometry(1)
+    p = mk.default_toy_params(seed=seed, simulate_material=0, pt_min=pt_min, pt_max=10.0)
+    return mk.generate_find_event(geo, p, n, noise_per_cm2=noise), mk.barrel_layer_descs(geo)
+
+
+def _run(ev, descs, dev, K, first_layer=3, chi2_cut=30.0):
+    import torch
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    hits = _t(ev["hits_flat"], dev) if len(ev["hits_flat"]) else torch.zeros((1, 9), device=dev)
+    eoh.load(ev["hit_offset"], hits)
+    b = mk.CEBatch(ev, len(descs), first_layer, device=dev, chi2_cut=chi2_cut, max_cands=K)
+    ctx.find_tracks_clone_engine(eoh, b)
+    torch.cuda.synchronize()
+    return b.numpy()
+
+
+def _compare(g, c, min_same=0.99):
+    n = len(c["n_cands"])
+    same = (g["n_cands"] == c["n_cands"]) & np.all(g["hot_idx"] == c["hot_idx"], axis=(0, 1))
+    assert same.mean() >= min_same, (same.mean(), np.nonzero(~same)[0][:8])
+    for k in range(c["par"].shape[0]):
+        m = same & (c["n_cands"] > k)
+        assert np.array_equal(g["n_found"][k, m], c["n_found"][k, m])
+        assert np.array_equal(g["chg"][k, m], c["chg"][k, m])
+        if m.any():
+            pr = par_rel(g["par"][k][:, m], c["par"][k][:, m])
+            assert np.percentile(pr, 99) < 1e-4 and pr.max() < 1e-2, (k, np.percentile(pr, 99), pr.max())
+            ds = np.abs(g["score"][k, m] - c["score"][k, m]) / np.maximum(1.0, np.abs(c["score"][k, m]))
+            assert np.percentile(ds, 99) < 1e-4, np.percentile(ds, 99)
+    return same
+
+
+def _true_fraction(hot, ev, first_layer):
+    h = hot[first_layer:]; off = ev["hit_offset"][first_layer:-1]
+    ok = h >= 0
+    truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(h, 0)], -5)
+    return (truth == ev["seed_mc"][None, :]).sum() / h.size
+
+
+@pytest.mark.parametrize("K,noise", [(6, 2.0), (1, 2.0), (8, 3.0)])
+def test_ce_matches_oracle(gpu_device, K, noise):
+    ev, descs = _event(100 + K, 4000, noise)
+    g = _run(ev, descs, gpu_device, K)
+    c = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=K, n_threads=8)
+    _compare(g, c)
+    assert abs(_true_fraction(g["hot_idx"][0], ev, 3) - _true_fraction(c["hot_idx"][0], ev, 3)) < 3e-3
+
+
+def test_ce_noise_free(gpu_device):
+    ev, descs = _event(7, 3000, 0.0, pt_min=1.0)
+    g = _run(ev, descs, gpu_device, 6)
+    c = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=8)
+    _compare(g, c, min_same=0.995)
+    assert _true_fraction(g["hot_idx"][0], ev, 3) > 0.985
+
+
+def test_ce_edge_cases(gpu_device):
+    ev, descs = _event(8, 500, 2.0)
+    L = len(descs)
+    g = _run(ev, descs, gpu_device, 6, first_layer=L)          # nothing to search: the seed itself
+    c = oracle.find_clone_engine(ev, descs, L, chi2_cut=30.0, max_cands=6, n_threads=4)
+    assert np.array_equal(g["n_cands"], np.ones(ev["n_seeds"], np.int32))
+    assert np.array_equal(g["hot_idx"], c["hot_idx"])
+    off = ev["hit_offset"]
+    keep = np.ones(off[-1], bool); keep[off[5]:off[6]] = False
+    cnt = np.diff(off).copy(); cnt[5] = 0
+    ev2 = dict(ev, hits_flat=ev["hits_flat"][keep], hit_offset=np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32),
+               hit_mc=ev["hit_mc"][keep])
+    g = _run(ev2, descs, gpu_device, 6)
+    c = oracle.find_clone_engine(ev2, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=4)
+    _compare(g, c)
+    ev3 = dict(ev, hits_flat=np.zeros((0, 9), np.float32), hit_offset=np.zeros(L + 1, np.int32),
+               hit_mc=np.zeros(0, np.int32))
+    g = _run(ev3, descs, gpu_device, 6)
+    c = oracle.find_clone_engine(ev3, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=4)
+    _compare(g, c, min_same=1.0)
+
+
+def test_ce_rejects_bad_arguments(gpu_device):
+    import torch
+    ev, descs = _event(9, 50, 1.0)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+    b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, max_cands=6)
+    b.K = 9
+    with pytest.raises(mk.MkfError, match="max_cands"):
+        ctx.find_tracks_clone_engine(eoh, b)
+    b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, max_cands=6)
+    b.first_layer = 0
+    with pytest.raises(mk.MkfError, match="first_layer"):
+        ctx.find_tracks_clone_engine(eoh, b)
+    b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, max_cands=6)
+    b.hot_idx = torch.zeros((6, 2, b.n), dtype=torch.int32, device=gpu_device)
+    with pytest.raises(mk.MkfError, match="hot_idx"):
+        ctx.find_tracks_clone_engine(eoh, b)
