@@ -41,6 +41,9 @@ extern "C" {
 /* Fit flags. */
 #define MKF_FIT_CHI2        1 /* accumulate chi2 of every hit into chi2[] (MkFinder.cc:1740) */
 #define MKF_FIT_CHARGE_FLIP 2 /* flip charge and 1/pT when 1/pT < 0 after an update (MkFinder.cc:1671-1678) */
+#define MKF_FIT_FP64_FALLBACK 4 /* no counterpart in the reference: where an updated covariance fails the
+                                   conditioning test (negative variance, or C_ij^2 > (1+1e-5) C_ii C_jj), recompute
+                                   it in FP64 in the Joseph form (I-KH)C(I-KH)^T + KVK^T */
 
 /* Per-slot hit flags (hit_flags[slot][track]). */
 #define MKF_HIT_ENDCAP  1 /* propagate to the hit z, endcap update (MkFitter.cc:488-497) */

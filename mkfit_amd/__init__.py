@@ -21,6 +21,7 @@ KFO_CALCULATE_CHI2 = 1
 KFO_UPDATE_PARAMS = 2
 FIT_CHI2 = 1
 FIT_CHARGE_FLIP = 2
+FIT_FP64_FALLBACK = 4
 HIT_ENDCAP = 1
 HIT_MISSING = 2
 NBINS_Z, NBINS_R = 300, 120

@@ -168,6 +168,79 @@ __device__ __forceinline__ void load_hit(const FitArgs& a, const FitBufs& b, int
   for (int e = 0; e < 6; ++e) h.me[e] = bld_f(b.hit_err, tb, se + (uint32_t)e * b.nb);
 }
 
+// FP64 fallback of the covariance update (MKF_FIT_FP64_FALLBACK).  The test
+// flags a negative or NaN variance or a covariance beyond what the variances
+// allow by more than rounding, C_ij^2 > (1 + 1e-5) C_ii C_jj, with the
+// products formed in double (exact for float factors).
+__device__ __forceinline__ bool ill_conditioned(const float (&C)[21]) {
+  bool bad = false;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) bad |= !(C[S(i, i)] >= 0.f);
+#pragma unroll
+  for (int i = 1; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      const double cij = C[S(i, j)];
+      bad |= cij * cij > (double)C[S(i, i)] * (double)C[S(j, j)] * 1.00001;
+    }
+  return bad;
+}
+
+// The updated covariance in the Joseph form (I - KH) C (I - KH)^T + K V K^T,
+// evaluated in double from the FP32 pre-update covariance C and rounded to
+// float; H is the barrel (tangent-plane rotation and z) or endcap (x, y)
+// projection and V the hit covariance projected the same way.  Symmetric and
+// positive semi-definite by construction for a positive semi-definite C.
+__device__ __attribute__((noinline)) void joseph_fp64(const float* __restrict__ pre, float px, float py,
+                                                      const float* __restrict__ mp, const float* __restrict__ me,
+                                                      bool endcap, float* __restrict__ out) {
+  double C[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) C[i][j] = pre[S(i, j)];
+  const double vxx = me[0], vyx = me[1], vyy = me[2], vzx = me[3], vzy = me[4], vzz = me[5];
+  double h0[6] = {0, 0, 0, 0, 0, 0}, h1[6] = {0, 0, 0, 0, 0, 0}, V00, V01, V11;
+  if (endcap) {
+    h0[0] = 1; h1[1] = 1;
+    V00 = vxx; V01 = vyx; V11 = vyy;
+  } else {
+    const double mx = mp[0], my = mp[1];
+    const double r = __builtin_sqrt(mx * mx + my * my);
+    const double a00 = -(my + (double)py) / (2 * r), a01 = (mx + (double)px) / (2 * r);
+    h0[0] = a00; h0[1] = a01; h1[2] = 1;
+    V00 = a00 * a00 * vxx + 2 * a00 * a01 * vyx + a01 * a01 * vyy;
+    V01 = a00 * vzx + a01 * vzy;
+    V11 = vzz;
+  }
+  double CH0[6], CH1[6];
+  for (int i = 0; i < 6; ++i) {
+    CH0[i] = 0; CH1[i] = 0;
+    for (int k = 0; k < 6; ++k) { CH0[i] += C[i][k] * h0[k]; CH1[i] += C[i][k] * h1[k]; }
+  }
+  double S00 = V00, S01 = V01, S11 = V11;
+  for (int k = 0; k < 6; ++k) { S00 += h0[k] * CH0[k]; S01 += h0[k] * CH1[k]; S11 += h1[k] * CH1[k]; }
+  const double det = S00 * S11 - S01 * S01;
+  const double i00 = S11 / det, i01 = -S01 / det, i11 = S00 / det;
+  double K0[6], K1[6], A[6][6], AC[6][6];
+  for (int i = 0; i < 6; ++i) {
+    K0[i] = CH0[i] * i00 + CH1[i] * i01;
+    K1[i] = CH0[i] * i01 + CH1[i] * i11;
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - K0[i] * h0[j] - K1[i] * h1[j];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      AC[i][j] = 0;
+      for (int k = 0; k < 6; ++k) AC[i][j] += A[i][k] * C[k][j];
+    }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double v = K0[i] * (V00 * K0[j] + V01 * K1[j]) + K1[i] * (V01 * K0[j] + V11 * K1[j]);
+      for (int k = 0; k < 6; ++k) v += AC[i][k] * A[j][k];
+      out[S(i, j)] = (float)v;
+    }
+}
+
+template <bool FB>
 __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int& chg, float& chi2, const FitArgs& a,
                                          const HitIn& h) {
   const bool missing = h.flags & MKF_HIT_MISSING;
@@ -182,14 +255,22 @@ __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int&
     for (int e = 0; e < 6; ++e) me[e] = h.me[e];
   }
   float c2;
-  if (!(h.flags & MKF_HIT_ENDCAP)) {
+  const bool endcap = h.flags & MKF_HIT_ENDCAP;
+  float pre[FB ? 21 : 1], px = 0.f, py = 0.f;
+  if (!endcap) {
     const float r = fhypot(mp[0], mp[1]);                                  // MkBase.h:55
     propagate_to_r(par, err, chg, r, a.pflags, missing, a.bfield, a.mat);
-    c2 = kalman_barrel<true, true>(par, err, mp, me);
   } else {
     propagate_to_z(par, err, chg, mp[2], a.pflags, missing, a.bfield, a.mat);
-    c2 = kalman_endcap<true, true>(par, err, mp, me);
   }
+  if (FB) {
+#pragma unroll
+    for (int e = 0; e < 21; ++e) pre[e] = err[e];
+    px = par[0];
+    py = par[1];
+  }
+  c2 = endcap ? kalman_endcap<true, true>(par, err, mp, me) : kalman_barrel<true, true>(par, err, mp, me);
+  if (FB && ill_conditioned(err)) joseph_fp64(pre, px, py, mp, me, endcap, err);
   if (a.fit_flags & MKF_FIT_CHI2) chi2 += c2;
   if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) && par[3] < 0.f) {
     chg = -chg;
@@ -235,7 +316,7 @@ __device__ __forceinline__ void store_state(const FitArgs& a, int t, const float
 // SIMD requested from the register allocator (the autotuner's occupancy knob).
 // FORM only names the instantiation (0 = plain, 1 = batched) so profiles keep
 // the two forms apart; the body does not depend on it.
-template <int MINW, int FORM>
+template <int MINW, int FORM, bool FB>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 512), amdgpu_waves_per_eu(MINW, 8)))
 k_fit(FitArgs a, int s_begin, int s_end) {
   const int stride = gridDim.x * blockDim.x;
@@ -254,7 +335,7 @@ k_fit(FitArgs a, int s_begin, int s_end) {
     for (int s = s_begin; s < s1; ++s) {
       cur = nxt;
       if (s + 1 < s1) load_hit(a, b, s + 1, tb, nxt);
-      fit_step(par, err, chg, chi2, a, cur);
+      fit_step<FB>(par, err, chg, chi2, a, cur);
     }
     store_state(a, t, par, err, chg, chi2);
   }
@@ -264,9 +345,11 @@ using FitKernel = void (*)(FitArgs, int, int);
 // Register budgets above 128 VGPRs/lane (4 waves) make the compiler demote the
 // private arrays to scratch, which changes FMA contraction and hence the bits;
 // the variants stop there so every choice gives identical results.
-static const FitKernel kFusedVariants[] = {k_fit<1, 1>, k_fit<2, 1>, k_fit<4, 1>};
-static const FitKernel kPlainKernel = k_fit<1, 0>;
-static constexpr int kNumFusedVariants = sizeof(kFusedVariants) / sizeof(kFusedVariants[0]);
+// [fp64 fallback][variant]
+static const FitKernel kFusedVariants[2][3] = {{k_fit<1, 1, false>, k_fit<2, 1, false>, k_fit<4, 1, false>},
+                                               {k_fit<1, 1, true>, k_fit<2, 1, true>, k_fit<4, 1, true>}};
+static const FitKernel kPlainKernel[2] = {k_fit<1, 0, false>, k_fit<1, 0, true>};
+static constexpr int kNumFusedVariants = 3;
 
 // ============================================================================
 // C ABI
@@ -328,7 +411,8 @@ int launch_fused(mkf_context* c, const FitArgs& a, const mkf_launch_shape& sh, h
   const int block = sh.block;
   int grid = sh.blocks_per_cu > 0 ? sh.blocks_per_cu * c->num_cus : grid_for(a.n, block);
   grid = std::max(1, std::min(grid, grid_for(a.n, block)));
-  hipLaunchKernelGGL(kFusedVariants[sh.variant], dim3(grid), dim3(block), 0, st, a, 0, a.n_slots);
+  const int fb = (a.fit_flags & MKF_FIT_FP64_FALLBACK) ? 1 : 0;
+  hipLaunchKernelGGL(kFusedVariants[fb][sh.variant], dim3(grid), dim3(block), 0, st, a, 0, a.n_slots);
   HIP_TRY(hipGetLastError());
   return MKF_OK;
 }
@@ -480,7 +564,8 @@ int mkf_fit_plain(mkf_context* c, const mkf_fit_batch* b, void* stream) {
   if (a.n == 0) return MKF_OK;
   const int launches = std::max(a.n_slots, a.seed_par ? 1 : 0);
   for (int s = 0; s < launches; ++s) {
-    hipLaunchKernelGGL(kPlainKernel, dim3(grid_for(a.n, 256)), dim3(256), 0, (hipStream_t)stream, a, s, s + 1);
+    hipLaunchKernelGGL(kPlainKernel[(a.fit_flags & MKF_FIT_FP64_FALLBACK) ? 1 : 0], dim3(grid_for(a.n, 256)),
+                       dim3(256), 0, (hipStream_t)stream, a, s, s + 1);
     HIP_TRY(hipGetLastError());
   }
   return MKF_OK;

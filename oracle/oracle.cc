@@ -170,6 +170,85 @@ int kalman_batch(int op, int endcap, int n, const T* ps_err, const T* ps_par, co
   return 0;
 }
 
+// Conditioning test of an updated covariance (packed 6x6, one lane): a negative
+// or NaN variance, or a covariance beyond what the variances allow by more
+// than rounding, C_ij^2 > (1 + 1e-5) C_ii C_jj (this also catches a nonzero
+// covariance of a zero variance; a zero row, as the z row after an endcap step,
+// passes).  Healthy track covariances reach correlations of 1 - 1e-7, which
+// FP32 rounding alone can push to 1 + 3e-7; the margin keeps those out.  The
+// products are formed in double, where the product of two floats is exact, so
+// the decision is the same on every platform for the same input bits.
+template <class T>
+bool ill_conditioned(const T* C, int lane) {
+  for (int i = 0; i < 6; ++i)
+    if (!(C[sym(i, i) * NN + lane] >= T(0))) return true;
+  for (int i = 1; i < 6; ++i)
+    for (int j = 0; j < i; ++j) {
+      const double cij = (double)C[sym(i, j) * NN + lane];
+      const double lim = (double)C[sym(i, i) * NN + lane] * (double)C[sym(j, j) * NN + lane];
+      if (cij * cij > lim * 1.00001) return true;
+    }
+  return false;
+}
+
+// The FP64 fallback: the updated covariance in the Joseph form
+//   C' = (I - K H) C (I - K H)^T + K V K^T,   K = C H^T (H C H^T + V)^-1,
+// which is symmetric and positive semi-definite by construction, evaluated in
+// double from the pre-update state and rounded to float.  H is the 2x6
+// measurement projection of the barrel (rotation onto the tangent plane,
+// KalmanUtilsMPlex.cc:585-591, and z) or the endcap (x, y) update; V the hit
+// covariance projected the same way.  No counterpart in the reference: an
+// extension for covariances the FP32 form (I - KH)C leaves non-positive.
+template <class T>
+void joseph_fp64(const T* psErr, const T* psPar, const T* msErr, const T* msPar, bool endcap, T* outErr, int lane) {
+  double C[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) C[i][j] = (double)psErr[sym(i, j) * NN + lane];
+  const double vxx = msErr[0 * NN + lane], vyx = msErr[1 * NN + lane], vyy = msErr[2 * NN + lane];
+  const double vzx = msErr[3 * NN + lane], vzy = msErr[4 * NN + lane], vzz = msErr[5 * NN + lane];
+  double h0[6] = {0, 0, 0, 0, 0, 0}, h1[6] = {0, 0, 0, 0, 0, 0}, V00, V01, V11;
+  if (endcap) {
+    h0[0] = 1; h1[1] = 1;
+    V00 = vxx; V01 = vyx; V11 = vyy;
+  } else {
+    const double mx = msPar[0 * NN + lane], my = msPar[1 * NN + lane];
+    const double px = psPar[0 * NN + lane], py = psPar[1 * NN + lane];
+    const double r = std::sqrt(mx * mx + my * my);
+    const double a00 = -(my + py) / (2 * r), a01 = (mx + px) / (2 * r);
+    h0[0] = a00; h0[1] = a01; h1[2] = 1;
+    V00 = a00 * a00 * vxx + 2 * a00 * a01 * vyx + a01 * a01 * vyy;
+    V01 = a00 * vzx + a01 * vzy;
+    V11 = vzz;
+  }
+  double CH0[6], CH1[6];
+  for (int i = 0; i < 6; ++i) {
+    CH0[i] = 0; CH1[i] = 0;
+    for (int k = 0; k < 6; ++k) { CH0[i] += C[i][k] * h0[k]; CH1[i] += C[i][k] * h1[k]; }
+  }
+  double S00 = V00, S01 = V01, S11 = V11;
+  for (int k = 0; k < 6; ++k) { S00 += h0[k] * CH0[k]; S01 += h0[k] * CH1[k]; S11 += h1[k] * CH1[k]; }
+  const double det = S00 * S11 - S01 * S01;
+  const double i00 = S11 / det, i01 = -S01 / det, i11 = S00 / det;
+  double K0[6], K1[6], A[6][6], AC[6][6];
+  for (int i = 0; i < 6; ++i) {
+    K0[i] = CH0[i] * i00 + CH1[i] * i01;
+    K1[i] = CH0[i] * i01 + CH1[i] * i11;
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - K0[i] * h0[j] - K1[i] * h1[j];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      AC[i][j] = 0;
+      for (int k = 0; k < 6; ++k) AC[i][j] += A[i][k] * C[k][j];
+    }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double v = K0[i] * (V00 * K0[j] + V01 * K1[j]) + K1[i] * (V01 * K0[j] + V11 * K1[j]);
+      for (int k = 0; k < 6; ++k) v += AC[i][k] * A[j][k];
+      outErr[sym(i, j) * NN + lane] = T(v);
+    }
+}
+
 // One NN block of the fit: BkFitFitTracks-style per-layer loop with per-lane
 // hit counts, per-lane barrel/endcap choice (FitTracksSteered generalised to a
 // per-lane pattern: both operations run where a block mixes them, each lane
@@ -179,7 +258,7 @@ void fit_block(const Ctx& ctx, const B& bt, int t0, int cnt) {
   const int n = bt.n_tracks;
   alignas(64) T err[21 * NN], par[6 * NN], chi2[NN];
   alignas(64) T errP[21 * NN], parP[6 * NN], errU[21 * NN], parU[6 * NN], c2[NN];
-  alignas(64) T errE[21 * NN], parE[6 * NN], c2E[NN];
+  alignas(64) T errE[21 * NN], parE[6 * NN], c2E[NN], errPE[21 * NN], parPE[6 * NN];
   alignas(64) T msPar[3 * NN], msErr[6 * NN], tgt[NN];
   alignas(64) int chg[NN], nh[NN], nomat[NN], fail[NN];
   unsigned char fl[NN];
@@ -224,14 +303,16 @@ void fit_block(const Ctx& ctx, const B& bt, int t0, int cnt) {
     }
     if (any_e) {
       for (int l = 0; l < NN; ++l) tgt[l] = msPar[2 * NN + l];                                      // MkBase.h:84
-      propagate_helix_to_z(err, par, chg, tgt, errP, parP, cnt, bt.pflags, nomat, ctx);
-      kalman_operation_endcap(kfop, errP, parP, msErr, msPar, errE, parE, c2E);
+      propagate_helix_to_z(err, par, chg, tgt, errPE, parPE, cnt, bt.pflags, nomat, ctx);
+      kalman_operation_endcap(kfop, errPE, parPE, msErr, msPar, errE, parE, c2E);
     }
     for (int l = 0; l < cnt; ++l) {
       if (fl[l] & 0x80) continue;
       const bool ec = fl[l] & ORC_HIT_ENDCAP;
-      const T* se = ec ? errE : errU;
+      T* se = ec ? errE : errU;
       const T* sp = ec ? parE : parU;
+      if ((bt.fit_flags & ORC_FIT_FP64_FALLBACK) && ill_conditioned(se, l))
+        joseph_fp64(ec ? errPE : errP, ec ? parPE : parP, msErr, msPar, ec, se, l);
       for (int i = 0; i < 21; ++i) err[i * NN + l] = se[i * NN + l];
       for (int i = 0; i < 6; ++i) par[i * NN + l] = sp[i * NN + l];
       if (bt.fit_flags & ORC_FIT_CHI2) chi2[l] += ec ? c2E[l] : c2[l];                              // MkFinder.cc:1740

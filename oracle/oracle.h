@@ -25,6 +25,7 @@ typedef struct orc_ctx {
 /* Fit flags. */
 #define ORC_FIT_CHI2        1 /* accumulate chi2 per hit (MkFinder::BkFitFitTracks) */
 #define ORC_FIT_CHARGE_FLIP 2 /* flip charge when 1/pT turns negative after update */
+#define ORC_FIT_FP64_FALLBACK 4 /* recompute an ill-conditioned updated covariance in FP64 (Joseph form) */
 
 /* Hit-slot flags. */
 #define ORC_HIT_ENDCAP  1 /* propagate to the hit z and use the endcap update */
