@@ -140,6 +140,38 @@ int mkf_autotune_batched(mkf_context* ctx, const mkf_fit_batch* batch, float* sc
 /* Number of variants mkf_set_batched_shape accepts. */
 int mkf_num_batched_variants(void);
 
+/* ---- Gradient of the fit ------------------------------------------------------
+ *
+ * No counterpart in the reference (whose fit has no derivative code): the
+ * reverse-mode gradient of the fit's total chi2 with respect to per-layer
+ * material and alignment, for calibration.  The fit is the barrel chi2 fit of
+ * mkf_fit_batched (one hit per layer, slot l = layer l, uniform field) with
+ * two inputs made parameters theta[l] = (x/X0, d_rphi, d_z) of layer l:
+ *   material   the propagation to layer l uses radL = x/X0 and
+ *              xi = xi_per_x0 * x/X0 instead of the material grid;
+ *   alignment  layer l's hits are moved by d_rphi (cm) along their local r-phi
+ *              direction and d_z (cm) along z before the propagation and update:
+ *              x' = x - d_rphi y/r, y' = y + d_rphi x/r, z' = z + d_z.
+ * The forward states are kept only every `checkpoint_every` layers and are
+ * recomputed within each segment on the way back. */
+typedef struct mkf_adjoint_batch {
+  int n_tracks, n_layers;
+  const float* seed_par;  /* [6][n] */
+  const float* seed_err;  /* [21][n] */
+  const int* seed_chg;    /* [n] */
+  const float* hit_pos;   /* [n_layers][3][n] */
+  const float* hit_err;   /* [n_layers][6][n] */
+  const int* n_hits;      /* [n] hits on layers 0 .. n_hits-1 */
+  const float* theta;     /* [n_layers][3] device */
+  int pflags;             /* MKF_PF_APPLY_MATERIAL: material from theta (MKF_PF_USE_PARAM_B_FIELD unsupported) */
+  float xi_per_x0;        /* energy-loss scale per unit x/X0 (GeV) */
+  int checkpoint_every;   /* layers per recomputed segment, >= 1 */
+  float* chi2;            /* [n] out: the track's total chi2 */
+  float* grad;            /* [n_layers][3] out: d(sum over tracks of chi2)/d theta */
+} mkf_adjoint_batch;
+
+int mkf_fit_adjoint(mkf_context* ctx, const mkf_adjoint_batch* batch, void* stream);
+
 /* ---- Best-hit track finding -------------------------------------------------
  *
  * Hits are the reference's Hit (Hit.h:165-183) as 9 floats: x, y, z, then the

@@ -91,6 +91,7 @@ ABI_SYMBOLS = [
     "mkf_set_batched_shape", "mkf_get_batched_shape", "mkf_autotune_batched", "mkf_num_batched_variants",
     "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
+    "mkf_fit_adjoint",
 ]
 CE_MAX_CANDS = 8
 
@@ -113,6 +114,13 @@ class MkfCEBatch(ctypes.Structure):
                 ("max_cands", ctypes.c_int), ("max_holes", ctypes.c_int), ("max_consec_holes", ctypes.c_int),
                 ("n_cands", _c_void_p), ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p),
                 ("chi2", _c_void_p), ("score", _c_void_p), ("n_found", _c_void_p), ("hot_idx", _c_void_p)]
+
+
+class MkfAdjointBatch(ctypes.Structure):
+    _fields_ = [("n_tracks", ctypes.c_int), ("n_layers", ctypes.c_int)] + [
+        (k, _c_void_p) for k in ("seed_par", "seed_err", "seed_chg", "hit_pos", "hit_err", "n_hits", "theta")] + [
+        ("pflags", ctypes.c_int), ("xi_per_x0", ctypes.c_float), ("checkpoint_every", ctypes.c_int),
+        ("chi2", _c_void_p), ("grad", _c_void_p)]
 
 
 class MkfFindBatch(ctypes.Structure):
@@ -154,6 +162,7 @@ def lib():
         L.mkf_select_hit_indices.argtypes = [_c_void_p, ctypes.c_int, ctypes.c_int] + [_c_void_p] * 6
         L.mkf_find_tracks_best_hit.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfFindBatch), _c_void_p]
         L.mkf_find_tracks_clone_engine.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfCEBatch), _c_void_p]
+        L.mkf_fit_adjoint.argtypes = [_c_void_p, ctypes.POINTER(MkfAdjointBatch), _c_void_p]
         L.mkf_autotune_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p, ctypes.c_int,
                                            ctypes.POINTER(MkfLaunchShape), _c_float_p]
         _LIB = L
@@ -464,6 +473,38 @@ class CEBatch:
     def numpy(self):
         return {k: getattr(self, k).cpu().numpy()
                 for k in ("n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx")}
+
+
+class AlignmentData:
+    """Device copy of a fit batch for the gradient with respect to per-layer material and alignment.
+
+    Slot l of every track must be its hit on layer l (the barrel generator's layout).
+    """
+
+    def __init__(self, ctx, ev, device="cuda:0", pflags=PF_APPLY_MATERIAL, xi_per_x0=2.2e-3, checkpoint_every=5):
+        import torch
+        dev = torch.device(device)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.ctx = ctx
+        self.n = int(ev["seed_par"].shape[1])
+        self.n_layers = int(ev["hit_pos"].shape[0])
+        self.seed_par, self.seed_err, self.seed_chg = t(ev["seed_par"]), t(ev["seed_err"]), t(ev["chg"])
+        self.hit_pos, self.hit_err, self.n_hits = t(ev["hit_pos"]), t(ev["hit_err"]), t(ev["n_hits"])
+        self.chi2 = torch.zeros(self.n, dtype=torch.float32, device=dev)
+        self.grad = torch.zeros((self.n_layers, 3), dtype=torch.float32, device=dev)
+        self.pflags, self.xi_per_x0, self.checkpoint_every = int(pflags), float(xi_per_x0), int(checkpoint_every)
+
+    def run(self, theta, stream=None):
+        """Per-track chi2 [n] and d(sum chi2)/d theta [n_layers, 3] at theta (device float32 [n_layers, 3])."""
+        th = theta.detach().to(self.grad.device, dtype=self.grad.dtype).contiguous()
+        b = MkfAdjointBatch()
+        b.n_tracks, b.n_layers = self.n, self.n_layers
+        b.seed_par, b.seed_err, b.seed_chg = _ptr(self.seed_par), _ptr(self.seed_err), _ptr(self.seed_chg)
+        b.hit_pos, b.hit_err, b.n_hits, b.theta = _ptr(self.hit_pos), _ptr(self.hit_err), _ptr(self.n_hits), _ptr(th)
+        b.pflags, b.xi_per_x0, b.checkpoint_every = self.pflags, self.xi_per_x0, self.checkpoint_every
+        b.chi2, b.grad = _ptr(self.chi2), _ptr(self.grad)
+        _check(lib().mkf_fit_adjoint(self.ctx.h, ctypes.byref(b), self.ctx._stream(stream)))
+        return self.chi2.clone(), self.grad.clone()
 
 
 class Context:

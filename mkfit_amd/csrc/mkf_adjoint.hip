@@ -1,0 +1,222 @@
+// mkf_adjoint.hip -- reverse-mode gradient of the fit's total chi2 with respect
+// to per-layer material (x/X0) and per-layer alignment (r-phi and z offsets).
+//
+// One lane per track.  Forward: the fit of the alignment model (hits moved by
+// their layer's offsets, material from the layer's x/X0; mkf_adjoint_math.h)
+// in FP32, saving the 27-float state every `checkpoint_every` layers.
+// Reverse: the adjoint lambda of the state (27 floats, zero after the last
+// hit) is carried back layer by layer.  At layer l the state entering the
+// layer is recomputed from the segment's checkpoint, and the step's
+// vector-Jacobian product
+//     v = lambda_out^T d(state_out)/d(in) + d(chi2_l)/d(in),
+//     in = (state_in, theta_l)  (30 directions)
+// is evaluated from forward tangents, Dual<D> carrying D input directions per
+// pass.  v[0:27] is the new lambda, v[27:30] the track's gradient for layer l.
+// Per-track gradient rows go to HBM and a second kernel sums them over tracks
+// in double, in a fixed order (deterministic result).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/mkfit_amd.h"
+#include "mkf_adjoint_math.h"
+#include "mkf_internal.h"
+
+using namespace mkf;
+using namespace mkf_adj;
+using namespace mkf_detail;
+
+namespace {
+
+constexpr int kDirs = 30;       // 27 state entries + 3 layer parameters
+constexpr int kTanD = 3;        // tangents per Dual pass
+constexpr int kBlock = 128;
+
+struct AdjArgs {
+  int n, n_layers, ck, pflags;
+  float xi_per_x0, bfield;
+  const float* __restrict__ seed_par;
+  const float* __restrict__ seed_err;
+  const int* __restrict__ seed_chg;
+  const float* __restrict__ hit_pos;
+  const float* __restrict__ hit_err;
+  const int* __restrict__ n_hits;
+  const float* __restrict__ theta;
+  float* __restrict__ chi2;
+  float* __restrict__ ckpt;     // [n_ck][27][n]
+  float* __restrict__ gtrack;   // [n_layers * 3][n]
+};
+
+__device__ __forceinline__ void load_hit(const AdjArgs& a, int l, int t, float (&hp)[3], float (&he)[6],
+                                         float& rh_inv) {
+  const size_t n = (size_t)a.n;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) hp[e] = a.hit_pos[((size_t)l * 3 + e) * n + t];
+#pragma unroll
+  for (int e = 0; e < 6; ++e) he[e] = a.hit_err[((size_t)l * 6 + e) * n + t];
+  const double x = hp[0], y = hp[1];
+  rh_inv = 1.f / (float)__builtin_sqrt(x * x + y * y);
+}
+
+__global__ void __launch_bounds__(kBlock) k_adjoint(AdjArgs a) {
+  __shared__ float vbuf[kBlock][kDirs + 1];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  const size_t n = (size_t)a.n;
+  const int nh = min(a.n_hits[t], a.n_layers);
+  const int chg = a.seed_chg[t];
+  float par[6], err[21];
+  // ---- forward, with checkpoints
+#pragma unroll
+  for (int e = 0; e < 6; ++e) par[e] = a.seed_par[e * n + t];
+#pragma unroll
+  for (int e = 0; e < 21; ++e) err[e] = a.seed_err[e * n + t];
+  float chi2 = 0.f;
+  for (int l = 0; l < nh; ++l) {
+    if (l % a.ck == 0) {
+      float* ck = a.ckpt + (size_t)(l / a.ck) * 27 * n + t;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) ck[e * n] = par[e];
+#pragma unroll
+      for (int e = 0; e < 21; ++e) ck[(6 + e) * n] = err[e];
+    }
+    float hp[3], he[6], rhi;
+    load_hit(a, l, t, hp, he, rhi);
+    chi2 += align_step<float>(par, err, chg, hp, he, a.theta[3 * l], a.theta[3 * l + 1], a.theta[3 * l + 2],
+                              a.xi_per_x0, a.bfield, a.pflags, rhi);
+  }
+  a.chi2[t] = chi2;
+  for (int l = nh; l < a.n_layers; ++l)
+    for (int q = 0; q < 3; ++q) a.gtrack[((size_t)l * 3 + q) * n + t] = 0.f;
+  // ---- reverse sweep
+  float lam[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) lam[e] = 0.f;
+  float* vb = vbuf[threadIdx.x];
+  for (int l = nh - 1; l >= 0; --l) {
+    const int c0 = (l / a.ck) * a.ck;
+    const float* ck = a.ckpt + (size_t)(l / a.ck) * 27 * n + t;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) par[e] = ck[e * n];
+#pragma unroll
+    for (int e = 0; e < 21; ++e) err[e] = ck[(6 + e) * n];
+    for (int j = c0; j < l; ++j) {                      // recompute the state entering layer l
+      float hp[3], he[6], rhi;
+      load_hit(a, j, t, hp, he, rhi);
+      (void)align_step<float>(par, err, chg, hp, he, a.theta[3 * j], a.theta[3 * j + 1], a.theta[3 * j + 2],
+                              a.xi_per_x0, a.bfield, a.pflags, rhi);
+    }
+    float hp[3], he[6], rhi;
+    load_hit(a, l, t, hp, he, rhi);
+    const float th0 = a.theta[3 * l], th1 = a.theta[3 * l + 1], th2 = a.theta[3 * l + 2];
+    for (int d0 = 0; d0 < kDirs; d0 += kTanD) {
+      using DV = Dual<kTanD>;
+      DV dp[6], de[21], m, dr, dz;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) {
+        dp[e].v = par[e];
+#pragma unroll
+        for (int k = 0; k < kTanD; ++k) dp[e].d[k] = (e == d0 + k) ? 1.f : 0.f;
+      }
+#pragma unroll
+      for (int e = 0; e < 21; ++e) {
+        de[e].v = err[e];
+#pragma unroll
+        for (int k = 0; k < kTanD; ++k) de[e].d[k] = (6 + e == d0 + k) ? 1.f : 0.f;
+      }
+      m.v = th0; dr.v = th1; dz.v = th2;
+#pragma unroll
+      for (int k = 0; k < kTanD; ++k) {
+        m.d[k] = (27 == d0 + k) ? 1.f : 0.f;
+        dr.d[k] = (28 == d0 + k) ? 1.f : 0.f;
+        dz.d[k] = (29 == d0 + k) ? 1.f : 0.f;
+      }
+      const DV c2 = align_step<DV>(dp, de, chg, hp, he, m, dr, dz, a.xi_per_x0, a.bfield, a.pflags, rhi);
+#pragma unroll
+      for (int k = 0; k < kTanD; ++k) {
+        float v = c2.d[k];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) v += lam[e] * dp[e].d[k];
+#pragma unroll
+        for (int e = 0; e < 21; ++e) v += lam[6 + e] * de[e].d[k];
+        if (d0 + k < kDirs) vb[d0 + k] = v;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 27; ++e) lam[e] = vb[e];
+    for (int q = 0; q < 3; ++q) a.gtrack[((size_t)l * 3 + q) * n + t] = vb[27 + q];
+  }
+}
+
+// Sum over tracks of each parameter's row, in double; one block per parameter.
+__global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ rows, int n, float* __restrict__ out) {
+  __shared__ double part[256];
+  const float* r = rows + (size_t)blockIdx.x * n;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += (double)r[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = (float)part[0];
+}
+
+}  // namespace
+
+extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void* stream) {
+  CHECK(set_device(c));
+  if (!b) return fail(MKF_ERR_INVALID_ARG, "batch is NULL");
+  if (b->n_tracks < 0 || b->n_layers <= 0 || b->n_layers > 1024)
+    return fail(MKF_ERR_INVALID_ARG, "need n_tracks >= 0 and 1 <= n_layers <= 1024");
+  if (b->checkpoint_every < 1) return fail(MKF_ERR_INVALID_ARG, "checkpoint_every must be >= 1");
+  if (b->pflags & MKF_PF_USE_PARAM_B_FIELD)
+    return fail(MKF_ERR_INVALID_ARG, "the adjoint supports the uniform field only");
+  const size_t N = b->n_tracks, L = b->n_layers;
+  CHECK(check_dev(b->theta, L * 3 * 4, "theta"));
+  CHECK(check_dev(b->grad, L * 3 * 4, "grad"));
+  hipStream_t st = (hipStream_t)stream;
+  if (N == 0) {
+    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)(L * 3)), dim3(256), 0, st, (const float*)nullptr, 0, b->grad);
+    HIP_TRY(hipGetLastError());
+    return MKF_OK;
+  }
+  if (L * 6 * N * 4 >= ((size_t)1 << 32)) return fail(MKF_ERR_INVALID_ARG, "batch too large");
+  CHECK(check_dev(b->seed_par, 6 * N * 4, "seed_par"));
+  CHECK(check_dev(b->seed_err, 21 * N * 4, "seed_err"));
+  CHECK(check_dev(b->seed_chg, N * 4, "seed_chg"));
+  CHECK(check_dev(b->hit_pos, L * 3 * N * 4, "hit_pos"));
+  CHECK(check_dev(b->hit_err, L * 6 * N * 4, "hit_err"));
+  CHECK(check_dev(b->n_hits, N * 4, "n_hits"));
+  CHECK(check_dev(b->chi2, N * 4, "chi2"));
+  const size_t n_ck = (L + b->checkpoint_every - 1) / b->checkpoint_every;
+  const size_t need = (n_ck * 27 + L * 3) * N * 4;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (need > c->adj_bytes) {
+    HIP_TRY(hipDeviceSynchronize());
+    if (c->adj_scratch) HIP_TRY(hipFree(c->adj_scratch));
+    c->adj_scratch = nullptr;
+    c->adj_bytes = 0;
+    HIP_TRY(hipMalloc(&c->adj_scratch, need));
+    c->adj_bytes = need;
+  }
+  AdjArgs a;
+  a.n = b->n_tracks;
+  a.n_layers = b->n_layers;
+  a.ck = b->checkpoint_every;
+  a.pflags = b->pflags;
+  a.xi_per_x0 = b->xi_per_x0;
+  a.bfield = c->bfield;
+  a.seed_par = b->seed_par; a.seed_err = b->seed_err; a.seed_chg = b->seed_chg;
+  a.hit_pos = b->hit_pos; a.hit_err = b->hit_err; a.n_hits = b->n_hits;
+  a.theta = b->theta;
+  a.chi2 = b->chi2;
+  a.ckpt = static_cast<float*>(c->adj_scratch);
+  a.gtrack = a.ckpt + n_ck * 27 * N;
+  hipLaunchKernelGGL(k_adjoint, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)(L * 3)), dim3(256), 0, st, a.gtrack, a.n, b->grad);
+  HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}

@@ -468,6 +468,7 @@ int mkf_context_destroy(mkf_context* c) {
   (void)hipSetDevice(c->device);
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
+  if (c->adj_scratch) (void)hipFree(c->adj_scratch);
   if (c->d_rl) (void)hipFree(c->d_rl);
   if (c->d_xi) (void)hipFree(c->d_xi);
   delete c;

@@ -42,6 +42,8 @@ struct mkf_context {
   hipStream_t capture_stream = nullptr;
   std::map<GraphKey, hipGraphExec_t> graphs;
   std::mutex mu;
+  void* adj_scratch = nullptr;   // adjoint checkpoints and per-track gradient rows
+  size_t adj_bytes = 0;
 };
 
 namespace mkf_detail {

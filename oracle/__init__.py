@@ -272,3 +272,25 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
     if rc:
         raise RuntimeError("orc_find_clone_engine failed (%d)" % rc)
     return out
+
+
+class _AlignBatch(ctypes.Structure):
+    _fields_ = [("n_tracks", ctypes.c_int), ("n_layers", ctypes.c_int)] + [
+        (k, ctypes.c_void_p) for k in ("seed_par", "seed_err", "seed_chg", "hit_pos", "hit_err", "n_hits")] + [
+        ("pflags", ctypes.c_int), ("xi_per_x0", ctypes.c_float)]
+
+
+def align_chi2(ev, theta, pflags=2, xi_per_x0=2.2e-3, dtype=np.float64, bfield=3.8, n_threads=1):
+    """Per-track sum of update chi2 of the fit with per-layer theta[l] = (x/X0, d_rphi, d_z)."""
+    L = lib()
+    th = np.ascontiguousarray(theta, np.float64 if np.dtype(dtype) == np.float64 else np.float32)
+    nl = th.shape[0]
+    n = ev["seed_par"].shape[1]
+    arrs = [np.ascontiguousarray(ev[k]) for k in ("seed_par", "seed_err", "chg", "hit_pos", "hit_err", "n_hits")]
+    b = _AlignBatch(n, nl, *[_p(a) for a in arrs], int(pflags), float(xi_per_x0))
+    out = np.zeros(n, np.float64)
+    f = L.orc_align_chi2_f64 if th.dtype == np.float64 else L.orc_align_chi2_f32
+    rc = f(ctypes.byref(OrcCtx(float(bfield), None, None)), ctypes.byref(b), _p(th), _p(out), int(n_threads))
+    if rc:
+        raise RuntimeError("orc_align_chi2 failed (%d)" % rc)
+    return out

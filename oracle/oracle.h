@@ -89,6 +89,25 @@ int orc_kalman_op_f64(int op, int endcap, int n, const double* ps_err, const dou
 int orc_fit_f32(const orc_ctx* ctx, const orc_fit_batch_f32* b, int n_threads);
 int orc_fit_f64(const orc_ctx* ctx, const orc_fit_batch_f64* b, int n_threads);
 
+/* ---- Fit as a function of per-layer material and alignment (oracle_align.cc) */
+
+typedef struct orc_align_batch {
+  int n_tracks, n_layers;  /* slot l of every track is its hit on barrel layer l */
+  const float* seed_par;   /* [6][n] */
+  const float* seed_err;   /* [21][n] */
+  const int* seed_chg;     /* [n] */
+  const float* hit_pos;    /* [n_layers][3][n] */
+  const float* hit_err;    /* [n_layers][6][n] */
+  const int* n_hits;       /* [n] */
+  int pflags;              /* ORC PF flags; material from theta when PF_apply_material */
+  float xi_per_x0;         /* xi = xi_per_x0 * x/X0 */
+} orc_align_batch;
+
+/* theta [n_layers][3] = (x/X0, d_rphi, d_z); chi2 [n] = sum of the update chi2 of each track. */
+int orc_align_chi2_f32(const orc_ctx* ctx, const orc_align_batch* b, const float* theta, double* chi2, int n_threads);
+int orc_align_chi2_f64(const orc_ctx* ctx, const orc_align_batch* b, const double* theta, double* chi2,
+                       int n_threads);
+
 /* ---- Best-hit track finding (oracle_find.cc) ------------------------------ */
 
 typedef struct orc_layer_desc {

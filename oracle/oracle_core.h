@@ -434,9 +434,11 @@ template <class T> inline void lookup_material(const Ctx& ctx, T zq, T rq, T& rl
 
 // ---- propagateHelixToRMPlex (PropagationMPlex.cc:408-505) -------------------
 template <class T>
+// matRl / matXi, when given, replace the grid lookup with per-lane material
+// (the per-layer x/X0 of the alignment model).
 inline void propagate_helix_to_r(const T* inErr, const T* inPar, const int* inChg, const T* msRad, T* outErr,
                                  T* outPar, int N_proc, int pflags, const int* noMatEff, int* failOut,
-                                 const Ctx& ctx) {
+                                 const Ctx& ctx, const T* matRl = nullptr, const T* matXi = nullptr) {
   std::memcpy(outErr, inErr, sizeof(T) * 21 * NN);
   std::memcpy(outPar, inPar, sizeof(T) * 6 * NN);
   alignas(64) T errorProp[36 * NN];
@@ -448,8 +450,14 @@ inline void propagate_helix_to_r(const T* inErr, const T* inPar, const int* inCh
     for (int n = 0; n < NN; ++n) {
       hitsRl[n] = 0; hitsXi[n] = 0; propSign[n] = 1;
       if (n >= N_proc) continue;
-      if (!(failFlag[n] || (noMatEff && noMatEff[n])))
-        lookup_material(ctx, outPar[2 * NN + n], msRad[n], hitsRl[n], hitsXi[n]);
+      if (!(failFlag[n] || (noMatEff && noMatEff[n]))) {
+        if (matRl) {
+          hitsRl[n] = matRl[n];
+          hitsXi[n] = matXi[n];
+        } else {
+          lookup_material(ctx, outPar[2 * NN + n], msRad[n], hitsRl[n], hitsXi[n]);
+        }
+      }
       const T r0 = hipo(inPar[0 * NN + n], inPar[1 * NN + n]);
       const T r = msRad[n];
       propSign[n] = (r > r0 ? T(1) : T(-1));

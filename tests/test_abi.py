@@ -63,6 +63,7 @@ def test_null_arguments_rejected():
     assert L.mkf_select_hit_indices(None, 0, 0, None, None, None, None, None, None) == 1
     assert L.mkf_find_tracks_best_hit(None, None, None, None) == 1
     assert L.mkf_find_tracks_clone_engine(None, None, None, None) == 1
+    assert L.mkf_fit_adjoint(None, None, None) == 1
     assert L.mkf_event_of_hits_destroy(None) == 0
     assert L.mkf_num_batched_variants() >= 1
 
@@ -72,3 +73,4 @@ def test_struct_layouts():
     assert ctypes.sizeof(mk.MkfLayerDesc) == 23 * 4
     assert ctypes.sizeof(mk.MkfFindBatch) == 4 + 4 + 6 * 8 + 4 + 4 + 4 + 4   # n, pad, 6 pointers, 3 scalars, pad
     assert ctypes.sizeof(mk.MkfCEBatch) == 8 + 4 * 8 + 8 * 4 + 8 * 8      # n + pad, 4 ptrs, 8 scalars, 8 ptrs
+    assert ctypes.sizeof(mk.MkfAdjointBatch) == 8 + 7 * 8 + 12 + 4 + 2 * 8  # 2 ints, 7 ptrs, 3 scalars + pad, 2 ptrs

@@ -1,0 +1,112 @@
+"""The adjoint of the fit (mkf_fit_adjoint): forward chi2 against the oracle's FP32 model, the
+gradient against FP64 central finite differences of the oracle's model, checkpoint independence,
+the torch.autograd harness, and recovery of injected alignment offsets by gradient descent."""
+import numpy as np
+import pytest
+
+import mkfit_amd as mk
+import oracle
+from mkfit_amd.alignment import make_event, weak_mode_residual
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def setup(gpu_device):
+    ev, geo, th_true = make_event(4000, seed=21)
+    ctx = mk.Context(bfield=3.8)
+    return ev, geo, th_true, ctx
+
+
+def _theta(th, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(th, np.float32)).to(dev)
+
+
+def test_forward_matches_oracle_model(setup, gpu_device):
+    ev, geo, th, ctx = setup
+    data = mk.AlignmentData(ctx, ev, gpu_device)
+    chi2, _ = data.run(_theta(th, gpu_device))
+    c32 = oracle.align_chi2(ev, th.astype(np.float32), dtype=np.float32, n_threads=8)
+    c64 = oracle.align_chi2(ev, th, n_threads=8)
+    g = chi2.cpu().numpy().astype(np.float64)
+    rel = np.abs(g - c32) / np.maximum(1.0, c32)
+    assert np.median(rel) < 1e-5 and np.percentile(rel, 99) < 1e-3, (np.median(rel), np.percentile(rel, 99))
+    assert abs(g.sum() - c64.sum()) / c64.sum() < 1e-4
+
+
+def test_gradient_matches_fp64_finite_differences(setup, gpu_device):
+    ev, geo, th, ctx = setup
+    th0 = th.copy()
+    th0[:, 1:] += np.random.default_rng(3).normal(0, 20e-4, (th.shape[0], 2))   # off the optimum
+    data = mk.AlignmentData(ctx, ev, gpu_device)
+    _, grad = data.run(_theta(th0, gpu_device))
+    grad = grad.cpu().numpy().astype(np.float64)
+    L = th.shape[0]
+    h = 1e-6
+    fd = np.zeros_like(grad)
+    for l in range(L):
+        for k in range(3):
+            tp = th0.copy(); tp[l, k] += h
+            tm = th0.copy(); tm[l, k] -= h
+            fd[l, k] = (oracle.align_chi2(ev, tp, n_threads=8).sum() - oracle.align_chi2(ev, tm, n_threads=8).sum()) / (2 * h)
+    for k in range(3):
+        scale = np.abs(fd[:, k]).max()
+        err = np.abs(grad[:, k] - fd[:, k]) / np.maximum(np.abs(fd[:, k]), 1e-2 * scale)
+        assert err.max() < 1e-3, (k, err.max(), np.argmax(err))
+
+
+def test_checkpointing_does_not_change_the_result(setup, gpu_device):
+    ev, geo, th, ctx = setup
+    out = []
+    for ck in (1, 5, 25):
+        data = mk.AlignmentData(ctx, ev, gpu_device, checkpoint_every=ck)
+        c, g = data.run(_theta(th, gpu_device))
+        out.append((c.cpu().numpy(), g.cpu().numpy()))
+    for c, g in out[1:]:
+        assert np.array_equal(c, out[0][0]) and np.array_equal(g, out[0][1])
+
+
+def test_autograd_harness(setup, gpu_device):
+    import torch
+    from mkfit_amd.autograd import fit_chi2
+    ev, geo, th, ctx = setup
+    data = mk.AlignmentData(ctx, ev, gpu_device)
+    theta = _theta(th, gpu_device).requires_grad_(True)
+    loss = fit_chi2(theta, data)
+    loss.backward()
+    chi2, grad = data.run(theta.detach())
+    assert torch.allclose(theta.grad, grad)
+    assert abs(loss.item() - chi2.double().sum().item()) <= 1e-6 * abs(loss.item())
+
+
+def test_gradient_descent_recovers_offsets(gpu_device):
+    """20 diagonally preconditioned descent steps on the offsets, material held at its true value."""
+    import torch
+    ev, geo, th_true = make_event(50000, seed=22)
+    ctx = mk.Context(bfield=3.8)
+    data = mk.AlignmentData(ctx, ev, gpu_device)
+    L = th_true.shape[0]
+    radii = np.array([geo.barrel_r[l] for l in range(L)])
+    th = th_true.copy()
+    th[:, 1:] = 0.0
+    # diagonal curvature from differences of the adjoint gradient
+    H = np.zeros((L, 3))
+    d = 1e-4
+    for l in range(L):
+        for k in (1, 2):
+            tp = th.copy(); tp[l, k] += d
+            tm = th.copy(); tm[l, k] -= d
+            gp = data.run(_theta(tp, gpu_device))[1].cpu().numpy()
+            gm = data.run(_theta(tm, gpu_device))[1].cpu().numpy()
+            H[l, k] = (gp[l, k] - gm[l, k]) / (2 * d)
+    assert np.all(H[:, 1:] > 0)
+    for _ in range(20):
+        g = data.run(_theta(th, gpu_device))[1].cpu().numpy().astype(np.float64)
+        th[:, 1:] -= 0.5 * g[:, 1:] / H[:, 1:]
+    r_rphi = weak_mode_residual(th[:, 1], th_true[:, 1], radii, "rphi")
+    r_z = weak_mode_residual(th[:, 2], th_true[:, 2], radii, "z")
+    s_rphi = weak_mode_residual(np.zeros(L), th_true[:, 1], radii, "rphi")
+    s_z = weak_mode_residual(np.zeros(L), th_true[:, 2], radii, "z")
+    assert r_rphi.std() < 0.1 * s_rphi.std(), (r_rphi.std(), s_rphi.std())
+    assert r_z.std() < 0.3 * s_z.std(), (r_z.std(), s_z.std())
