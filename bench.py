@@ -53,6 +53,10 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--shape", type=str, default=None,
                     help="block,blocks_per_cu,variant for the batched kernel (skips the autotuner)")
+    ap.add_argument("--workload", default="fit", choices=["fit", "find-ce", "find-bh", "adjoint"],
+                    help="fit (default: the metric's workload); find-ce / find-bh: clone-engine / best-hit "
+                         "finding per event; adjoint: gradient of the fit's chi2 (secondary lines)")
+    ap.add_argument("--events", type=int, default=4, help="distinct events generated for the find workloads")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration run for rocprofv3 --pmc: one 256 MiB device copy, then "
                          "--steps batched and plain fits; no timing line")
@@ -75,6 +79,10 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if args.workload in ("find-ce", "find-bh"):
+        return bench_find(args, mk, torch, dev)
+    if args.workload == "adjoint":
+        return bench_adjoint(args, mk, torch, dev)
     geo = mk.geometry(2)
     rl, xi = mk.material_grid(geo)
     params = mk.default_toy_params(seed=args.seed, bfield=3.8)
@@ -206,6 +214,108 @@ def main():
                      "algorithmic_GBps": alg_bytes_b / (kern_b * 1e-3) / 1e9},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+def _timed(torch, stream, fn, steps, warmup):
+    for _ in range(warmup):
+        fn(0)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t_start = time.perf_counter()
+    for i, (e0, e1) in enumerate(evs):
+        e0.record(stream)
+        fn(i)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t_start, [e0.elapsed_time(e1) for e0, e1 in evs]
+
+
+def bench_find(args, mk, torch, dev):
+    """Track finding per event: 17-layer barrel (r = 4..100 cm), U(5k, 15k) seeds per event, noise hits
+    at 2 per cm^2 on every layer, 3-hit seeds; clone engine with up to 6 candidates (chi2 cut 30) or
+    best-hit (chi2 cut 15).  A step is one event: LayerOfHits build + finding, inputs in HBM."""
+    import oracle
+    ce = args.workload == "find-ce"
+    geo = mk.geometry(1)
+    descs = mk.barrel_layer_descs(geo)
+    rng = np.random.default_rng(args.seed)
+    events = []
+    t0 = time.time()
+    for e in range(args.events):
+        p = mk.default_toy_params(seed=args.seed + e, simulate_material=0)
+        ev = mk.generate_find_event(geo, p, int(rng.integers(5000, 15001)), noise_per_cm2=2.0)
+        events.append(ev)
+    log("generated %d events in %.1f s; seeds %s, hits %s" % (len(events), time.time() - t0,
+        [e["n_seeds"] for e in events], [int(e["hit_offset"][-1]) for e in events]))
+    ctx = mk.Context(bfield=3.8)
+    L = len(descs)
+    eohs, batches, hits = [], [], []
+    for ev in events:
+        eoh = mk.EventOfHits(ctx, descs)
+        h = torch.from_numpy(ev["hits_flat"]).to(dev)
+        hits.append(h)
+        eohs.append(eoh)
+        batches.append(mk.CEBatch(ev, L, 3, device=dev, chi2_cut=30.0, max_cands=6) if ce else
+                       mk.FindBatch(ev, L, 3, device=dev, chi2_cut=15.0))
+    stream = torch.cuda.current_stream()
+
+    def step(i):
+        k = i % len(events)
+        eohs[k].load(events[k]["hit_offset"], hits[k])
+        if ce:
+            ctx.find_tracks_clone_engine(eohs[k], batches[k])
+        else:
+            batches[k].reset()
+            ctx.find_tracks_best_hit(eohs[k], batches[k])
+
+    wall, per = _timed(torch, stream, step, args.steps, args.warmup)
+    seeds = sum(events[i % len(events)]["n_seeds"] for i in range(args.steps))
+    ms = wall * 1e3 / args.steps
+    cpu = None
+    if not args.no_cpu:
+        ncpu = min(16, os.cpu_count() or 1)
+        ev = events[0]
+        t = time.perf_counter()
+        if ce:
+            oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=ncpu)
+        else:
+            oracle.find_best_hit(ev, descs, 3, chi2_cut=15.0, n_threads=ncpu)
+        dt = time.perf_counter() - t
+        cpu = {"value": ev["n_seeds"] / dt, "unit": "seeds/s", "cores": ncpu, "kind": "port",
+               "sample": "one event (%d seeds), oracle %s restatement incl. its LayerOfHits build, %d threads" %
+                         (ev["n_seeds"], "clone-engine" if ce else "best-hit", ncpu)}
+    res = {"metric": "seeds processed/sec, %s track finding on 1 MI355X" % ("clone-engine" if ce else "best-hit"),
+           "value": seeds / wall, "unit": "seeds/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic: toy MC finding events (signal + uniform noise hits), seeds %d.." % args.seed,
+           "config": {"workload": "find per event: 17 barrel layers, U(5k,15k) seeds, noise 2/cm^2, %s" %
+                      ("up to 6 candidates, chi2 cut 30" if ce else "best hit, chi2 cut 15"),
+                      "events": len(events)},
+           "event_ms_events": float(np.mean(per)), "cpu_baseline": cpu}
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+def bench_adjoint(args, mk, torch, dev):
+    """Gradient of the fit's total chi2 w.r.t. per-layer material and alignment: 200,000 tracks on the
+    25-layer barrel, checkpoints every 5 layers.  A step is one forward fit + reverse sweep."""
+    from mkfit_amd.alignment import make_event
+    n = 200_000
+    ev, geo, th = make_event(n, seed=args.seed)
+    ctx = mk.Context(bfield=3.8)
+    data = mk.AlignmentData(ctx, ev, dev, checkpoint_every=5)
+    theta = torch.from_numpy(th.astype(np.float32)).to(dev)
+    stream = torch.cuda.current_stream()
+    wall, per = _timed(torch, stream, lambda i: data.run(theta), args.steps, args.warmup)
+    res = {"metric": "tracks/sec through fit + adjoint (d sum chi2 / d per-layer x/X0 and alignment) on 1 MI355X",
+           "value": n * args.steps / wall, "unit": "tracks/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic: toy MC with per-layer x/X0 in [0.01, 0.05] and N(0, 50 um) offsets",
+           "config": {"workload": "adjoint: %d tracks, 25 barrel layers, checkpoint every 5 layers" % n},
+           "cpu_baseline": None}
     print(json.dumps(res), flush=True)
     ctx.close()
 

@@ -44,6 +44,17 @@ def build_toymc():
     return out
 
 
+def build_shim_demo():
+    """tests/cpp/fit_shim_demo: the C++ interface (include/mkfit_amd.hpp) driven like MkFitter."""
+    out = os.path.join(ROOT, "tests", "cpp", "fit_shim_demo")
+    _run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+          "-I", "/opt/rocm/include", "-o", out + ".tmp", os.path.join(ROOT, "tests", "cpp", "fit_shim_demo.cc"),
+          "-L", PKG, "-lmkfit_amd", "-lmkfit_toymc", "-L", "/opt/rocm/lib", "-lamdhip64",
+          "-Wl,-rpath," + PKG, "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "-B", "all"])
@@ -53,6 +64,7 @@ def build_all():
     build_oracle()
     build_toymc()
     build_hip()
+    build_shim_demo()
 
 
 if __name__ == "__main__":
