@@ -243,6 +243,9 @@ typedef struct mkf_find_batch {
   int first_layer;/* layers before it hold the seed hits */
   float chi2_cut; /* IterationParams::chi2Cut_min (IterationConfig.h:101) */
   int pflags;     /* Config::finding_inter_layer_pflags: MKF_PF_* for the layer-to-layer propagation */
+  int prop_to_hit;  /* Config::finding_requires_propagation_to_hit_pos (Config.cc:52): chi2 and update after
+                       propagating to each hit (kalmanPropagateAndComputeChi2 / ...AndUpdate) */
+  int intra_pflags; /* Config::finding_intra_layer_pflags for that propagation */
 } mkf_find_batch;
 
 /* MkBuilder::FindTracksBestHit (MkBuilder.cc:1362-1480) over the layers of

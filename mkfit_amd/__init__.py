@@ -126,7 +126,8 @@ class MkfAdjointBatch(ctypes.Structure):
 class MkfFindBatch(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p),
                 ("chi2", _c_void_p), ("hot_idx", _c_void_p), ("n_found", _c_void_p), ("first_layer", ctypes.c_int),
-                ("chi2_cut", ctypes.c_float), ("pflags", ctypes.c_int)]
+                ("chi2_cut", ctypes.c_float), ("pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int),
+                ("intra_pflags", ctypes.c_int)]
 
 
 def lib_path():
@@ -392,7 +393,8 @@ _HIPRT = None
 class FindBatch:
     """Device arrays of one best-hit finding batch (seeds in, found tracks out)."""
 
-    def __init__(self, ev, n_layers, first_layer, device="cuda:0", chi2_cut=15.0, pflags=0):
+    def __init__(self, ev, n_layers, first_layer, device="cuda:0", chi2_cut=15.0, pflags=0, prop_to_hit=False,
+                 intra_pflags=0):
         import torch
         n = int(ev["n_seeds"])
         dev = torch.device(device)
@@ -408,6 +410,7 @@ class FindBatch:
         self.first_layer = int(first_layer)
         self.chi2_cut = float(chi2_cut)
         self.pflags = int(pflags)
+        self.prop_to_hit, self.intra_pflags = int(bool(prop_to_hit)), int(intra_pflags)
         self._seed = [t.clone() for t in (self.par, self.err, self.chg, self.chi2, self.hot_idx)]
 
     def reset(self):
@@ -420,6 +423,7 @@ class FindBatch:
         b.par, b.err, b.chg, b.chi2 = _ptr(self.par), _ptr(self.err), _ptr(self.chg), _ptr(self.chi2)
         b.hot_idx, b.n_found = _ptr(self.hot_idx), _ptr(self.n_found)
         b.first_layer, b.chi2_cut, b.pflags = self.first_layer, self.chi2_cut, self.pflags
+        b.prop_to_hit, b.intra_pflags = self.prop_to_hit, self.intra_pflags
         return b
 
     def numpy(self):

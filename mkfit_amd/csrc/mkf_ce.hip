@@ -53,13 +53,6 @@ __device__ __forceinline__ float score_calc(int nfound, int ntail, int noverlap,
   return bonus * nfound + 0.0f * noverlap - penalty * nmiss - 3.0f * ntail - chi2;
 }
 
-// std::hypot(float, float) of the reference: the sum of squares in double (exact
-// for float inputs up to one rounding) and a correctly rounded root, rounded to float.
-__device__ __forceinline__ float hypot_ref(float x, float y) {
-  const double xd = x, yd = y;
-  return (float)__builtin_sqrt(xd * xd + yd * yd);
-}
-
 struct Cand {
   float par[6], err[21];
   int chg;

@@ -145,7 +145,7 @@ k_select(LayerDev d, int n, const float* __restrict__ par, const float* __restri
 struct FindArgs {
   int n, n_layers, first_layer;
   float chi2_cut;
-  int pflags;
+  int pflags, prop_to_hit, intra_pflags;
   float bfield;
   Material mat;
   float* __restrict__ par;
@@ -205,8 +205,19 @@ __global__ void __launch_bounds__(256) k_find_best_hit(FindArgs a) {
         scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) {
           float hp[3], he[6];
           load_hit9(a.shits + (size_t)gi * 9, hp, he);
-          const float c2 = fabsf(d.is_barrel ? kalman_barrel<true, false>(pp, pe, hp, he)
-                                             : kalman_endcap<true, false>(pp, pe, hp, he));
+          float q[6], qe[21];
+#pragma unroll
+          for (int e = 0; e < 6; ++e) q[e] = pp[e];
+#pragma unroll
+          for (int e = 0; e < 21; ++e) qe[e] = pe[e];
+          if (a.prop_to_hit) {                                // kalmanPropagateAndComputeChi2
+            if (d.is_barrel)
+              propagate_to_r(q, qe, chg, hypot_ref(hp[0], hp[1]), a.intra_pflags, false, a.bfield, a.mat);
+            else
+              propagate_to_z(q, qe, chg, hp[2], a.intra_pflags, false, a.bfield, a.mat);
+          }
+          const float c2 = fabsf(d.is_barrel ? kalman_barrel<true, false>(q, qe, hp, he)
+                                             : kalman_endcap<true, false>(q, qe, hp, he));
           if (c2 < min_c2) {
             min_c2 = c2;
             best = gi;
@@ -225,6 +236,12 @@ __global__ void __launch_bounds__(256) k_find_best_hit(FindArgs a) {
         }
       }
       a.hot[layer * n + t] = code;
+      if (a.prop_to_hit) {                                    // kalmanPropagateAndUpdate
+        if (d.is_barrel)
+          propagate_to_r(pp, pe, chg, hypot_ref(mp[0], mp[1]), a.intra_pflags, false, a.bfield, a.mat);
+        else
+          propagate_to_z(pp, pe, chg, mp[2], a.intra_pflags, false, a.bfield, a.mat);
+      }
       if (d.is_barrel)
         kalman_barrel<false, true>(pp, pe, mp, me);
       else
@@ -457,7 +474,7 @@ int mkf_find_tracks_best_hit(mkf_context* c, mkf_event_of_hits* h, const mkf_fin
   if (b->n_tracks < 0) return fail(MKF_ERR_INVALID_ARG, "n_tracks < 0");
   if (b->first_layer < 0 || b->first_layer > h->n_layers)
     return fail(MKF_ERR_INVALID_ARG, "first_layer %d outside [0, %d]", b->first_layer, h->n_layers);
-  if ((b->pflags & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
+  if (((b->pflags | (b->prop_to_hit ? b->intra_pflags : 0)) & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
     return fail(MKF_ERR_INVALID_ARG, "material requested but the context has no material grid");
   if (b->n_tracks == 0) return MKF_OK;
   const size_t N = b->n_tracks;
@@ -475,6 +492,8 @@ int mkf_find_tracks_best_hit(mkf_context* c, mkf_event_of_hits* h, const mkf_fin
   a.first_layer = b->first_layer;
   a.chi2_cut = b->chi2_cut;
   a.pflags = b->pflags;
+  a.prop_to_hit = b->prop_to_hit;
+  a.intra_pflags = b->intra_pflags;
   a.bfield = c->bfield;
   a.mat = mat_of(c);
   a.par = b->par; a.err = b->err; a.chg = b->chg; a.chi2 = b->chi2;

@@ -164,6 +164,13 @@ __device__ __forceinline__ void load_hit9(const float* __restrict__ h, float (&m
   for (int e = 0; e < 6; ++e) me[e] = h[3 + e];
 }
 
+// std::hypot(float, float) of the reference: the sum of squares in double (exact
+// for float inputs up to one rounding) and a correctly rounded root, rounded to float.
+__device__ __forceinline__ float hypot_ref(float x, float y) {
+  const double xd = x, yd = y;
+  return (float)__builtin_sqrt(xd * xd + yd * yd);
+}
+
 }  // namespace mkf_find
 
 struct mkf_event_of_hits {

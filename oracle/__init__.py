@@ -169,7 +169,7 @@ class LayerDesc(ctypes.Structure):
 class _FindEvent(ctypes.Structure):
     _fields_ = [("n_layers", ctypes.c_int), ("layers", ctypes.c_void_p), ("hit_offset", ctypes.c_void_p),
                 ("hits", ctypes.c_void_p), ("first_layer", ctypes.c_int), ("chi2_cut", ctypes.c_float),
-                ("pflags", ctypes.c_int)]
+                ("pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int), ("intra_pflags", ctypes.c_int)]
 
 
 class _FindTracks(ctypes.Structure):
@@ -211,7 +211,8 @@ def select_hit_indices(desc, hits, par, err):
     return xs, xa, w
 
 
-def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n_threads=1):
+def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n_threads=1, prop_to_hit=False,
+                  intra_pflags=0):
     """Best-hit finding over a toy event dict (see mkfit_amd.generate_find_event)."""
     L = lib()
     layers = layer_array(descs)
@@ -224,7 +225,8 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     nfound = np.zeros(n, np.int32)
     off = np.ascontiguousarray(ev["hit_offset"], np.int32)
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
-    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags)
+    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags,
+                    int(bool(prop_to_hit)), int(intra_pflags))
     ft = _FindTracks(n, _p(par), _p(err), _p(chg), _p(chi2), _p(hot), _p(nfound))
     ctx = OrcCtx(float(bfield), None, None)
     rc = L.orc_find_best_hit(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(ft), int(n_threads))
@@ -262,7 +264,7 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
                hot_idx=np.zeros((K, nl, n), np.int32))
     off = np.ascontiguousarray(ev["hit_offset"], np.int32)
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
-    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags)
+    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags, 0, 0)
     pr = _CEParams(K, int(max_holes), int(max_consec_holes), int(bool(prop_to_hit)), int(intra_pflags))
     tr = _CETracks(n, _p(sp), _p(se), _p(sc), _p(sh), *[_p(out[k]) for k in (
         "n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx")])

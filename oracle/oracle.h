@@ -129,6 +129,8 @@ typedef struct orc_find_event {
   int first_layer;               /* first layer searched (the seed hits fill the ones before) */
   float chi2_cut;                /* IterationParams::chi2Cut_min */
   int pflags;                    /* finding_inter_layer_pflags */
+  int prop_to_hit;               /* best hit: finding_requires_propagation_to_hit_pos (clone engine: see orc_ce_params) */
+  int intra_pflags;              /* best hit: finding_intra_layer_pflags */
 } orc_find_event;
 
 typedef struct orc_find_tracks {

@@ -70,11 +70,28 @@ struct EventLayers {
   std::vector<LoH> loh;
   const int* hit_off;
   const float* hits;  // [total][9]
+  int prop_to_hit = 0, intra_pflags = 0;
+  Ctx ctx;
 };
 
 // AddBestHit (MkFinder.cc:691-832) for one block, propagated state in parP/errP.
 void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock& B, const float* parP,
                   const float* errP, const int* xsize, const int* xarr, const int* wsr, int n_proc) {
+  // With prop_to_hit (Config::finding_requires_propagation_to_hit_pos) the chi2 and the update are
+  // taken after propagating the layer state to each hit's radius (barrel) or z (endcap) with the
+  // intra-layer flags: kalmanPropagateAndComputeChi2 / kalmanPropagateAndUpdate
+  // (KalmanUtilsMPlex.cc:471-543, 697-770).
+  auto propagate_to_hits = [&](const float* msPar, float* eOut, float* pOut) {
+    alignas(64) float ms[NN];
+    if (E.desc[layer].is_barrel) {
+      for (int l = 0; l < NN; ++l) ms[l] = std::hypot(msPar[0 * NN + l], msPar[1 * NN + l]);
+      propagate_helix_to_r(errP, parP, B.chg, ms, eOut, pOut, n_proc, E.intra_pflags, nullptr, nullptr, E.ctx);
+    } else {
+      for (int l = 0; l < NN; ++l) ms[l] = msPar[2 * NN + l];
+      propagate_helix_to_z(errP, parP, B.chg, ms, eOut, pOut, n_proc, E.intra_pflags, nullptr, E.ctx);
+    }
+  };
+  alignas(64) float errH[21 * NN], parH[6 * NN];
   const orc_layer_desc& d = E.desc[layer];
   const float* hits = E.hits + (size_t)E.hit_off[layer] * 9;
   float minChi2[NN];
@@ -96,10 +113,17 @@ void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock
         for (int e = 0; e < 3; ++e) msPar[e * NN + it] = h[e];
         for (int e = 0; e < 6; ++e) msErr[e * NN + it] = h[3 + e];
       }
+    const float* eC = errP;
+    const float* pC = parP;
+    if (E.prop_to_hit) {
+      propagate_to_hits(msPar, errH, parH);
+      eC = errH;
+      pC = parH;
+    }
     if (d.is_barrel)
-      kalman_operation(KFO_Calculate_Chi2, errP, parP, msErr, msPar, dErr, dPar, outChi2);
+      kalman_operation(KFO_Calculate_Chi2, eC, pC, msErr, msPar, dErr, dPar, outChi2);
     else
-      kalman_operation_endcap(KFO_Calculate_Chi2, errP, parP, msErr, msPar, dErr, dPar, outChi2);
+      kalman_operation_endcap(KFO_Calculate_Chi2, eC, pC, msErr, msPar, dErr, dPar, outChi2);
     for (int it = 0; it < n_proc; ++it)
       if (hc < xsize[it]) {
         const float c2 = std::abs(outChi2[it]);
@@ -135,11 +159,18 @@ void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock
       add_hit(B, it, fake_idx, layer);
     }
   }
-  // kalmanPropagateAndUpdate without re-propagation (finding_requires_propagation_to_hit_pos = false)
+  // kalmanPropagateAndUpdate
+  const float* eU = errP;
+  const float* pU = parP;
+  if (E.prop_to_hit) {
+    propagate_to_hits(msPar, errH, parH);
+    eU = errH;
+    pU = parH;
+  }
   if (d.is_barrel)
-    kalman_operation(KFO_Update_Params, errP, parP, msErr, msPar, B.err, B.par, outChi2);
+    kalman_operation(KFO_Update_Params, eU, pU, msErr, msPar, B.err, B.par, outChi2);
   else
-    kalman_operation_endcap(KFO_Update_Params, errP, parP, msErr, msPar, B.err, B.par, outChi2);
+    kalman_operation_endcap(KFO_Update_Params, eU, pU, msErr, msPar, B.err, B.par, outChi2);
   for (int it = 0; it < NN; ++it)
     if (B.par[3 * NN + it] < 0) {
       B.chg[it] = -B.chg[it];
@@ -212,6 +243,9 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
   E.desc = ev->layers;
   E.hit_off = ev->hit_offset;
   E.hits = ev->hits;
+  E.prop_to_hit = ev->prop_to_hit;
+  E.intra_pflags = ev->intra_pflags;
+  E.ctx = ctx;
   E.loh.resize(ev->n_layers);
   for (int l = 0; l < ev->n_layers; ++l)
     build_loh(ev->layers[l], ev->hit_offset[l + 1] - ev->hit_offset[l], ev->hits + (size_t)ev->hit_offset[l] * 9,

@@ -164,6 +164,16 @@ def test_find_best_hit_matches_oracle(setup, gpu_device):
     assert _true_fraction(gpu, ev, 3) > 0.8
 
 
+def test_find_best_hit_propagate_to_hit(setup, gpu_device):
+    import torch
+    ev, descs, ctx, eoh = setup
+    b = mk.FindBatch(ev, len(descs), 3, device=gpu_device, chi2_cut=15.0, prop_to_hit=True)
+    ctx.find_tracks_best_hit(eoh, b)
+    torch.cuda.synchronize()
+    cpu = oracle.find_best_hit(ev, descs, first_layer=3, chi2_cut=15.0, bfield=3.8, n_threads=8, prop_to_hit=True)
+    _compare_find(b.numpy(), cpu)
+
+
 def test_find_noise_free_exact_path(gpu_device):
     geo, ev, descs = _event(31, 3000, 0.0, pt_min=1.0)
     ctx = mk.Context(bfield=3.8)

@@ -217,3 +217,23 @@ def test_find_with_noise_efficiency():
     ok = hot >= 0
     truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(hot, 0)], -5)
     assert (truth == ev["seed_mc"][None, :]).sum() / hot.size > 0.8   # 0.856 at 3 noise hits / cm2
+
+
+def test_find_propagate_to_hit_option():
+    """finding_requires_propagation_to_hit_pos: chi2 and update after propagating to each hit.  The toy
+    hits lie on the layer radius where the state already is, so the choice changes little; with the
+    propagation the chosen hits stay the generating track's."""
+    geo = mk.geometry(1)
+    p = mk.default_toy_params(seed=12, simulate_material=0, pt_min=1.0, pt_max=10.0)
+    ev = mk.generate_find_event(geo, p, 1500, noise_per_cm2=2.0)
+    descs = mk.barrel_layer_descs(geo)
+    a = oracle.find_best_hit(ev, descs, first_layer=3, bfield=3.8, n_threads=4)
+    b = oracle.find_best_hit(ev, descs, first_layer=3, bfield=3.8, n_threads=4, prop_to_hit=True)
+    same = np.all(a["hot_idx"] == b["hot_idx"], axis=0).mean()
+    assert 0.9 < same <= 1.0
+    assert not np.array_equal(a["par"], b["par"])             # the extra propagation does act
+    for r in (a, b):
+        hot = r["hot_idx"][3:]; off = ev["hit_offset"][3:-1]
+        ok = hot >= 0
+        truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(hot, 0)], -5)
+        assert (truth == ev["seed_mc"][None, :]).sum() / hot.size > 0.9
