@@ -41,6 +41,9 @@ extern "C" {
 /* Fit flags. */
 #define MKF_FIT_CHI2        1 /* accumulate chi2 of every hit into chi2[] (MkFinder.cc:1740) */
 #define MKF_FIT_CHARGE_FLIP 2 /* flip charge and 1/pT when 1/pT < 0 after an update (MkFinder.cc:1671-1678) */
+#define MKF_FIT_PROP_TO_PCA 8 /* after the last hit, propagate to the point of closest approach to the origin in
+                                 the r-z plane (MkBase::PropagateTracksToPCAZ via BkFitPropTracksToPCA,
+                                 MkBase.h:91-104, MkFinder.cc:1746-1749) with pca_pflags */
 #define MKF_FIT_FP64_FALLBACK 4 /* no counterpart in the reference: where an updated covariance fails the
                                    conditioning test (negative variance, or C_ij^2 > (1+1e-5) C_ii C_jj), recompute
                                    it in FP64 in the Joseph form (I-KH)C(I-KH)^T + KVK^T */
@@ -109,6 +112,7 @@ typedef struct mkf_fit_batch {
   const float* seed_par;          /* [6][n] or NULL */
   const float* seed_err;          /* [21][n] */
   const int* seed_chg;            /* [n] */
+  int pca_pflags;                 /* MKF_PF_* of the propagation to the PCA (Config::pca_prop_pflags) */
 } mkf_fit_batch;
 
 /* Plain fit: one kernel launch per hit slot, state round-trips through HBM

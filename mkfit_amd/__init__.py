@@ -22,6 +22,7 @@ KFO_UPDATE_PARAMS = 2
 FIT_CHI2 = 1
 FIT_CHARGE_FLIP = 2
 FIT_FP64_FALLBACK = 4
+FIT_PROP_TO_PCA = 8
 HIT_ENDCAP = 1
 HIT_MISSING = 2
 NBINS_Z, NBINS_R = 300, 120
@@ -42,7 +43,8 @@ class MkfFitBatch(ctypes.Structure):
                 ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p), ("chi2", _c_void_p),
                 ("hit_pos", _c_void_p), ("hit_err", _c_void_p), ("n_hits", _c_void_p),
                 ("hit_flags", _c_void_p), ("pflags", ctypes.c_int), ("fit_flags", ctypes.c_int),
-                ("seed_par", _c_void_p), ("seed_err", _c_void_p), ("seed_chg", _c_void_p)]
+                ("seed_par", _c_void_p), ("seed_err", _c_void_p), ("seed_chg", _c_void_p),
+                ("pca_pflags", ctypes.c_int)]
 
 
 class MkfLaunchShape(ctypes.Structure):
@@ -602,7 +604,8 @@ class FitBatch:
     """Device-resident batch (torch tensors) in the layout of mkf_fit_batch."""
 
     def __init__(self, par, err, chg, chi2, hit_pos, hit_err, n_hits, hit_flags=None, pflags=0,
-                 fit_flags=FIT_CHI2, seed=None):
+                 fit_flags=FIT_CHI2, seed=None, pca_pflags=0):
+        self.pca_pflags = int(pca_pflags)
         self.par, self.err, self.chg, self.chi2 = par, err, chg, chi2
         self.seed = seed  # optional (seed_par, seed_err, seed_chg) for an out-of-place fit
         self.hit_pos, self.hit_err, self.n_hits, self.hit_flags = hit_pos, hit_err, n_hits, hit_flags
@@ -611,7 +614,7 @@ class FitBatch:
         self.n_slots = int(hit_pos.shape[0])
 
     @classmethod
-    def from_numpy(cls, ev, device, pflags=0, fit_flags=FIT_CHI2, with_flags=True, out_of_place=False):
+    def from_numpy(cls, ev, device, pflags=0, fit_flags=FIT_CHI2, with_flags=True, out_of_place=False, pca_pflags=0):
         import torch
         d = torch.device(device)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(d)
@@ -624,7 +627,8 @@ class FitBatch:
             par, err, chg, seed = sp, se, sc, None
         return cls(par, err, chg, torch.zeros(n, dtype=torch.float32, device=d),
                    t(ev["hit_pos"]), t(ev["hit_err"]), t(ev["n_hits"]),
-                   t(ev["hit_flags"]) if with_flags else None, pflags=pflags, fit_flags=fit_flags, seed=seed)
+                   t(ev["hit_flags"]) if with_flags else None, pflags=pflags, fit_flags=fit_flags, seed=seed,
+                   pca_pflags=pca_pflags)
 
     def clone_state(self):
         return [x.clone() for x in (self.par, self.err, self.chg, self.chi2)]
@@ -639,7 +643,7 @@ class FitBatch:
         b.par, b.err, b.chg, b.chi2 = _ptr(self.par), _ptr(self.err), _ptr(self.chg), _ptr(self.chi2)
         b.hit_pos, b.hit_err, b.n_hits = _ptr(self.hit_pos), _ptr(self.hit_err), _ptr(self.n_hits)
         b.hit_flags = _ptr(self.hit_flags)
-        b.pflags, b.fit_flags = self.pflags, self.fit_flags
+        b.pflags, b.fit_flags, b.pca_pflags = self.pflags, self.fit_flags, self.pca_pflags
         if self.seed is not None:
             b.seed_par, b.seed_err, b.seed_chg = (_ptr(x) for x in self.seed)
         return b

@@ -124,7 +124,7 @@ struct FitArgs {
   const float* __restrict__ seed_par;
   const float* __restrict__ seed_err;
   const int* __restrict__ seed_chg;
-  int pflags, fit_flags;
+  int pflags, fit_flags, pca_pflags;
   float bfield;
   Material mat;
 };
@@ -278,6 +278,18 @@ __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int&
   }
 }
 
+// MkBase::PropagateTracksToPCAZ (MkBase.h:91-104): the z of the point of
+// closest approach to the origin of the track's straight-line r-z projection,
+// then propagateHelixToZ there.  std::hypot of the reference is formed in double
+// and rounded, as glibc's hypotf does.
+__device__ __forceinline__ void propagate_to_pca(float (&par)[6], float (&err)[21], int chg, const FitArgs& a) {
+  const float slope = tanf(par[5]);
+  const double xd = par[0], yd = par[1];
+  const float r = (float)__builtin_sqrt(xd * xd + yd * yd);
+  const float z = (slope * (slope * par[2] - r)) / (1 + slope * slope);
+  propagate_to_z(par, err, chg, z, a.pca_pflags, false, a.bfield, a.mat);
+}
+
 // from_seed: read the seed arrays (first touch of an out-of-place fit).
 __device__ __forceinline__ void load_state(const FitArgs& a, int t, bool from_seed, float (&par)[6], float (&err)[21],
                                            int& chg, float& chi2) {
@@ -322,10 +334,12 @@ k_fit(FitArgs a, int s_begin, int s_end) {
   const int stride = gridDim.x * blockDim.x;
   const bool from_seed = s_begin == 0 && a.seed_par != nullptr;
   const FitBufs b = make_bufs(a);
+  // The launch that ends at the last slot also takes every track to its PCA.
+  const bool to_pca = (a.fit_flags & MKF_FIT_PROP_TO_PCA) && s_end >= a.n_slots;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += stride) {
     const uint32_t tb = (uint32_t)t * 4u;  // lane byte offset in a 4-byte element array
     const int s1 = min(min(bld_i(b.n_hits, tb, 0), a.n_slots), s_end);
-    if (s_begin >= s1 && !from_seed) continue;
+    if (s_begin >= s1 && !from_seed && !to_pca) continue;
     float par[6], err[21], chi2;
     int chg;
     load_state(a, t, from_seed, par, err, chg, chi2);
@@ -337,6 +351,7 @@ k_fit(FitArgs a, int s_begin, int s_end) {
       if (s + 1 < s1) load_hit(a, b, s + 1, tb, nxt);
       fit_step<FB>(par, err, chg, chi2, a, cur);
     }
+    if (to_pca) propagate_to_pca(par, err, chg, a);
     store_state(a, t, par, err, chg, chi2);
   }
 }
@@ -379,6 +394,7 @@ int validate_batch(const mkf_fit_batch* b, FitArgs& a, const mkf_context* c) {
   a.seed_chg = b->seed_chg;
   a.pflags = b->pflags;
   a.fit_flags = b->fit_flags;
+  a.pca_pflags = b->pca_pflags;
   a.bfield = c->bfield;
   a.mat = mat_of(c);
   if (n == 0) return MKF_OK;
@@ -399,7 +415,7 @@ int validate_batch(const mkf_fit_batch* b, FitArgs& a, const mkf_context* c) {
     CHECK(check_dev(b->hit_err, 6 * n * S * 4, "hit_err"));
     if (b->hit_flags) CHECK(check_dev(b->hit_flags, n * S, "hit_flags"));
   }
-  if ((b->pflags & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
+  if (((b->pflags | ((b->fit_flags & MKF_FIT_PROP_TO_PCA) ? b->pca_pflags : 0)) & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
     return fail(MKF_ERR_INVALID_ARG, "material requested but the context has no material grid");
   return MKF_OK;
 }
@@ -563,7 +579,7 @@ int mkf_fit_plain(mkf_context* c, const mkf_fit_batch* b, void* stream) {
   FitArgs a;
   CHECK(validate_batch(b, a, c));
   if (a.n == 0) return MKF_OK;
-  const int launches = std::max(a.n_slots, a.seed_par ? 1 : 0);
+  const int launches = std::max(a.n_slots, (a.seed_par || (a.fit_flags & MKF_FIT_PROP_TO_PCA)) ? 1 : 0);
   for (int s = 0; s < launches; ++s) {
     hipLaunchKernelGGL(kPlainKernel[(a.fit_flags & MKF_FIT_FP64_FALLBACK) ? 1 : 0], dim3(grid_for(a.n, 256)),
                        dim3(256), 0, (hipStream_t)stream, a, s, s + 1);
@@ -584,6 +600,7 @@ int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
   std::memcpy(key.ptrs, ptrs, sizeof(ptrs));
   key.n = a.n; key.slots = a.n_slots; key.pflags = a.pflags; key.fflags = a.fit_flags;
   key.block = c->shape.block; key.bpc = c->shape.blocks_per_cu; key.variant = c->shape.variant;
+  key.pca_pflags = a.pca_pflags;
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     hipGraph_t g = nullptr;

@@ -23,12 +23,12 @@
 
 struct GraphKey {
   const void* ptrs[12];
-  int n, slots, pflags, fflags, block, bpc, variant;
+  int n, slots, pflags, fflags, block, bpc, variant, pca_pflags;
   bool operator<(const GraphKey& o) const {
     int c = std::memcmp(ptrs, o.ptrs, sizeof(ptrs));
     if (c) return c < 0;
-    return std::tie(n, slots, pflags, fflags, block, bpc, variant) <
-           std::tie(o.n, o.slots, o.pflags, o.fflags, o.block, o.bpc, o.variant);
+    return std::tie(n, slots, pflags, fflags, block, bpc, variant, pca_pflags) <
+           std::tie(o.n, o.slots, o.pflags, o.fflags, o.block, o.bpc, o.variant, o.pca_pflags);
   }
 };
 

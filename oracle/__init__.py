@@ -16,6 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBS = {}
 
 FIT_CHI2 = 1
+FIT_PROP_TO_PCA = 8
 FIT_CHARGE_FLIP = 2
 
 
@@ -28,7 +29,7 @@ class _Batch32(ctypes.Structure):
                 ("par", ctypes.c_void_p), ("err", ctypes.c_void_p), ("chg", ctypes.c_void_p),
                 ("chi2", ctypes.c_void_p), ("hit_pos", ctypes.c_void_p), ("hit_err", ctypes.c_void_p),
                 ("n_hits", ctypes.c_void_p), ("hit_flags", ctypes.c_void_p),
-                ("pflags", ctypes.c_int), ("fit_flags", ctypes.c_int)]
+                ("pflags", ctypes.c_int), ("fit_flags", ctypes.c_int), ("pca_pflags", ctypes.c_int)]
 
 
 _Batch64 = _Batch32  # same field layout (pointers differ only in pointee type)
@@ -135,7 +136,7 @@ class Oracle:
                                                _p(oe), _p(opar), _p(c2))
         return oe, opar, c2
 
-    def fit(self, ev, pflags=0, fit_flags=FIT_CHI2, dtype=np.float32, n_threads=1, with_flags=True):
+    def fit(self, ev, pflags=0, fit_flags=FIT_CHI2, dtype=np.float32, n_threads=1, with_flags=True, pca_pflags=0):
         """Whole fit of a toy-MC event dict; returns (par, err, chg, chi2)."""
         sfx, T = self._dt(dtype)
         n = ev["seed_par"].shape[1]
@@ -146,7 +147,7 @@ class Oracle:
         nh = np.ascontiguousarray(ev["n_hits"], np.int32)
         hf = np.ascontiguousarray(ev["hit_flags"], np.uint8) if with_flags else None
         b = _Batch32(n, S, _p(par), _p(err), _p(chg), _p(chi2), _p(hp), _p(he), _p(nh), _p(hf), int(pflags),
-                     int(fit_flags))
+                     int(fit_flags), int(pca_pflags))
         rc = getattr(self.L, "orc_fit_" + sfx)(ctypes.byref(self.ctx), ctypes.byref(b), int(n_threads))
         if rc != 0:
             raise RuntimeError("oracle fit failed (%d)" % rc)

@@ -322,6 +322,15 @@ void fit_block(const Ctx& ctx, const B& bt, int t0, int cnt) {
       }
     }
   }
+  if (bt.fit_flags & ORC_FIT_PROP_TO_PCA) {                                   // MkBase.h:91-104
+    for (int l = 0; l < NN; ++l) {
+      const T slope = std::tan(par[5 * NN + l]);
+      tgt[l] = (slope * (slope * par[2 * NN + l] - std::hypot(par[0 * NN + l], par[1 * NN + l]))) / (1 + slope * slope);
+    }
+    propagate_helix_to_z(err, par, chg, tgt, errP, parP, cnt, bt.pca_pflags, nullptr, ctx);
+    std::memcpy(err, errP, sizeof(err));
+    std::memcpy(par, parP, sizeof(par));
+  }
   unpack(bt.err, err, 21, n, t0, cnt);
   unpack(bt.par, par, 6, n, t0, cnt);
   unpack(bt.chi2, chi2, 1, n, t0, cnt);

@@ -25,7 +25,8 @@ typedef struct orc_ctx {
 /* Fit flags. */
 #define ORC_FIT_CHI2        1 /* accumulate chi2 per hit (MkFinder::BkFitFitTracks) */
 #define ORC_FIT_CHARGE_FLIP 2 /* flip charge when 1/pT turns negative after update */
-#define ORC_FIT_FP64_FALLBACK 4 /* recompute an ill-conditioned updated covariance in FP64 (Joseph form) */
+#define ORC_FIT_FP64_FALLBACK 4
+#define ORC_FIT_PROP_TO_PCA 8 /* MkBase::PropagateTracksToPCAZ after the last hit (BkFitPropTracksToPCA) */ /* recompute an ill-conditioned updated covariance in FP64 (Joseph form) */
 
 /* Hit-slot flags. */
 #define ORC_HIT_ENDCAP  1 /* propagate to the hit z and use the endcap update */
@@ -43,6 +44,7 @@ typedef struct orc_fit_batch_f32 {
   const unsigned char* hit_flags; /* [n_slots][n] or NULL */
   int pflags;                     /* 1 = parametrised B, 2 = material */
   int fit_flags;                  /* ORC_FIT_* */
+  int pca_pflags;                 /* flags of the propagation to the PCA (ORC_FIT_PROP_TO_PCA) */
 } orc_fit_batch_f32;
 
 typedef struct orc_fit_batch_f64 {
@@ -57,6 +59,7 @@ typedef struct orc_fit_batch_f64 {
   const unsigned char* hit_flags;
   int pflags;
   int fit_flags;
+  int pca_pflags;
 } orc_fit_batch_f64;
 
 int orc_simd_width(void);
