@@ -29,6 +29,7 @@ METRIC = "tracks fitted/sec on 1 MI355X, batched vs plain kernel, at ≤1e-5 rel
 # material + chi2 + update), counted on the oracle restatement by
 # oracle/flopcount.cc (+,-,*,/ = 1; sqrt, sin, cos, log = 1).
 FLOP_PER_TRACK_LAYER = 1394
+FLOP_PER_ENDCAP_STEP = 413 + 168
 # Algorithmic HBM bytes per track-layer of the batched kernel: the hit (3 + 6
 # floats) and its flag byte; per track: seed in (27 floats + charge), result out
 # (27 floats + charge + chi2) and the hit count.
@@ -53,8 +54,9 @@ def parse():
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--shape", type=str, default=None,
                     help="block,blocks_per_cu,variant for the batched kernel (skips the autotuner)")
-    ap.add_argument("--workload", default="fit", choices=["fit", "find-ce", "find-bh", "adjoint"],
-                    help="fit (default: the metric's workload); find-ce / find-bh: clone-engine / best-hit "
+    ap.add_argument("--workload", default="fit", choices=["fit", "fit-uneven", "find-ce", "find-bh", "adjoint"],
+                    help="fit (default: the metric's workload); fit-uneven: 2M tracks, barrel + 2x9 disks, "
+                         "8-30 hits, 5%% missing, FP64 fallback on; find-ce / find-bh: clone-engine / best-hit "
                          "finding per event; adjoint: gradient of the fit's chi2 (secondary lines)")
     ap.add_argument("--events", type=int, default=4, help="distinct events generated for the find workloads")
     ap.add_argument("--probe", action="store_true",
@@ -83,15 +85,23 @@ def main():
         return bench_find(args, mk, torch, dev)
     if args.workload == "adjoint":
         return bench_adjoint(args, mk, torch, dev)
-    geo = mk.geometry(2)
+    uneven = args.workload == "fit-uneven"
+    geo = mk.geometry(3 if uneven else 2)
     rl, xi = mk.material_grid(geo)
-    params = mk.default_toy_params(seed=args.seed, bfield=3.8)
-    n = args.n_tracks
+    if uneven:
+        params = mk.default_toy_params(seed=args.seed, bfield=3.8, eta_max=2.5, hits_min=8, hits_max=30,
+                                       missing_frac=0.05)
+        n = 2_000_000 if args.n_tracks == 1_000_000 else args.n_tracks
+        n_slots = 30
+    else:
+        params = mk.default_toy_params(seed=args.seed, bfield=3.8)
+        n = args.n_tracks
+        n_slots = geo.n_barrel
     t0 = time.time()
-    ev = mk.generate(geo, params, n, geo.n_barrel)
+    ev = mk.generate(geo, params, n, n_slots)
     log("generated %d tracks in %.2f s; mean hits %.2f" % (n, time.time() - t0, ev["n_hits"].mean()))
     pflags = mk.PF_APPLY_MATERIAL
-    fflags = mk.FIT_CHI2
+    fflags = mk.FIT_CHI2 | (mk.FIT_FP64_FALLBACK if uneven else 0)
     ctx = mk.Context(bfield=params.bfield, rl_grid=rl, xi_grid=xi, device=0)
     batch = mk.FitBatch.from_numpy(ev, dev, pflags=pflags, fit_flags=fflags, out_of_place=True)
     plain_out = mk.FitBatch.from_numpy(ev, dev, pflags=pflags, fit_flags=fflags, out_of_place=True)
@@ -181,6 +191,10 @@ def main():
         log("cpu baseline %.3g tracks/s (%.2f s per pass)" % (n / best, best))
 
     flops = FLOP_PER_TRACK_LAYER * n_track_layers
+    if uneven:   # endcap steps: propagation to z + endcap update (oracle/flopcount.cc)
+        valid = np.arange(n_slots)[:, None] < ev["n_hits"][None, :]
+        n_endcap = int(np.sum(valid & ((ev["hit_flags"] & 1) != 0)))
+        flops = FLOP_PER_TRACK_LAYER * (n_track_layers - n_endcap) + FLOP_PER_ENDCAP_STEP * n_endcap
     achieved_tflops = flops / (kern_b * 1e-3) / 1e12
     alg_bytes_b = n_track_layers * HIT_BYTES + n * TRACK_BYTES
     res = {
@@ -197,9 +211,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded toy MC (helix + multiple scattering + energy loss), seed %d" % args.seed,
         "config": {
-            "workload": "plain-fit-at-scale: %d tracks, 25 barrel layers r=3-110 cm, |z|<=120 cm, Bz=3.8 T, "
-                        "x/X0=0.02 per layer, material on, chi2 accumulated" % n,
-            "n_tracks": n, "layers": 25, "track_layers": n_track_layers,
+            "workload": ("batched-fit-uneven: %d tracks, 25 barrel layers + 2x9 endcap disks |z|=30-270 cm, "
+                         "|eta|<2.5, 8-30 hits per track, 5%% missing, Bz=3.8 T, x/X0=0.02, material on, chi2 "
+                         "accumulated, FP64 covariance fallback" % n) if uneven else
+                        ("plain-fit-at-scale: %d tracks, 25 barrel layers r=3-110 cm, |z|<=120 cm, Bz=3.8 T, "
+                         "x/X0=0.02 per layer, material on, chi2 accumulated" % n),
+            "n_tracks": n, "layers": n_slots, "track_layers": n_track_layers,
             "batched_shape": {"block": shape[0], "blocks_per_cu": shape[1], "variant": shape[2]},
         },
         "plain": {"value": tps_p, "ms_per_step": ms_p, "unit": "tracks/s",
