@@ -72,7 +72,8 @@ typedef struct mkf_toymc_output {
  * (Hit.h:165-183) -- and one seed per track built from its hits on the first
  * `seed_layers` layers (state: truth at the last seed hit, smeared as for the
  * fit seeds).  Tracks that do not cross all seed layers get no seed.
- * Barrel-only geometries. */
+ * Barrel-only geometries (layers = barrel layers) or disk-only ones (layers =
+ * the +z disks; preset 4). */
 typedef struct mkf_toymc_find_output {
   int n_tracks;            /* generated tracks (capacity of the seed arrays) */
   int layer_capacity;      /* hits per layer the caller allocated */
@@ -94,7 +95,8 @@ int mkf_toymc_noise_count(const mkf_geometry* geo, int layer, float noise_per_cm
 
 /* Presets: 1 = 17 barrel layers r = 4 + 6i cm, |z| <= 120;
  *          2 = 25 barrel layers r = 3 + (107/24)i cm, |z| <= 120;
- *          3 = the 25 barrel layers with staged lengths plus 2 x 9 disks at |z| = 30..270 cm.
+ *          3 = the 25 barrel layers with staged lengths plus 2 x 9 disks at |z| = 30..270 cm;
+ *          4 = 2 x 9 disks only, |z| = 30..270 cm, r from |z|/sinh(2.5) to 110 cm.
  * x_over_x0 = 0.02 and xi = 0.02 * 2.2e-3 GeV on every layer. */
 int mkf_toymc_geometry_preset(int which, mkf_geometry* geo);
 void mkf_toymc_default_params(mkf_toymc_params* p);

@@ -262,7 +262,7 @@ def generate_find_event(geo, params, n_tracks, noise_per_cm2=2.0, seed_layers=3,
     Returns a dict: hits_flat [total][9], hit_offset [L+1], hit_mc [total], n_seeds,
     seed_par [6][n], seed_err [21][n], seed_chg [n], seed_hits [seed_layers][n], seed_mc [n].
     """
-    L = geo.n_barrel
+    L = geo.n_barrel if geo.n_barrel else geo.n_disk     # barrel layers, or the +z disks of a disk-only geometry
     cap = n_tracks + max(toy().mkf_toymc_noise_count(ctypes.byref(geo), l, float(noise_per_cm2)) for l in range(L))
     hits = np.zeros((L, cap, 9), np.float32)
     nlh = np.zeros(L, np.int32)
@@ -288,6 +288,21 @@ def generate_find_event(geo, params, n_tracks, noise_per_cm2=2.0, seed_layers=3,
     return dict(hits_flat=flat, hit_offset=off, hit_mc=mcf, n_seeds=ns, seed_par=np.ascontiguousarray(sp[:, :ns]),
                 seed_err=np.ascontiguousarray(se[:, :ns]), seed_chg=sc[:ns].copy(),
                 seed_hits=np.ascontiguousarray(sh[:, :ns]), seed_mc=smc[:ns].copy())
+
+
+def disk_layer_descs(geo, q_bin=1.5, min_dphi=0.01, max_dphi=0.05, min_dq=1.0, max_dq=2.0, half_thickness=0.01):
+    """LayerInfo + selection limits for the +z disks of a disk-only geometry (endcap layers: q = r).
+
+    Defaults are the toy cylinder geometry's endcap settings (Geoms/CylCowWLids.cc:97-123: q_bin 1.5 cm,
+    selection limits 0.01/0.05 rad and 1/2 cm); the track is propagated to the disk's z, where the toy
+    hits lie."""
+    out = []
+    for l in range(geo.n_disk):
+        z = geo.disk_z[l]
+        out.append(dict(is_barrel=0, rin=geo.disk_rmin[l], rout=geo.disk_rmax[l], zmin=z - half_thickness,
+                        zmax=z + half_thickness, propagate_to=z, q_bin=q_bin, min_dphi=min_dphi, max_dphi=max_dphi,
+                        min_dq=min_dq, max_dq=max_dq, c_dp_sf=1.1, c_dq_sf=1.1, c_c2_sf=1.1))
+    return out
 
 
 def barrel_layer_descs(geo, q_bin=2.0, min_dphi=0.01, max_dphi=0.05, min_dq=1.0, max_dq=2.0, half_thickness=0.01):

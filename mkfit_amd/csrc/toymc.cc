@@ -344,6 +344,15 @@ int mkf_toymc_geometry_preset(int which, mkf_geometry* g) {
         else { g->disk_rmin[i] = std::max(16.f, rlo); g->disk_rmax[i] = 110.f; }
       }
     }
+  } else if (which == 4) {
+    // Forward disks only: 9 disks at z = +-(30..270) cm, r from |z|/sinh(2.5) to 110 cm.
+    g->n_disk = 9;
+    for (int i = 0; i < 9; ++i) {
+      const float z = 30.f * (i + 1);
+      g->disk_z[i] = z;
+      g->disk_rmin[i] = float(z / std::sinh(2.5));
+      g->disk_rmax[i] = 110.f;
+    }
   } else {
     return 2;
   }
@@ -372,32 +381,46 @@ int mkf_toymc_generate(const mkf_geometry* geo, const mkf_toymc_params* p, mkf_t
   return 0;
 }
 
+// Finding events use either a barrel-only geometry (layers = the barrel layers)
+// or a disk-only one (layers = the +z disks).
+static int find_layers(const mkf_geometry* g) {
+  if (g->n_barrel > 0 && g->n_disk == 0) return g->n_barrel;
+  if (g->n_barrel == 0 && g->n_disk > 0) return g->n_disk;
+  return 0;
+}
+
 int mkf_toymc_noise_count(const mkf_geometry* g, int layer, float noise_per_cm2) {
-  if (!g || layer < 0 || layer >= g->n_barrel || noise_per_cm2 <= 0) return 0;
-  const double area = 2 * kPi * g->barrel_r[layer] * 2 * g->barrel_zmax[layer];
+  if (!g || layer < 0 || layer >= find_layers(g) || noise_per_cm2 <= 0) return 0;
+  const double area = g->n_barrel > 0 ? 2 * kPi * g->barrel_r[layer] * 2 * g->barrel_zmax[layer]
+                                      : kPi * (double(g->disk_rmax[layer]) * g->disk_rmax[layer] -
+                                               double(g->disk_rmin[layer]) * g->disk_rmin[layer]);
   return (int)std::llround(area * noise_per_cm2);
 }
 
 int mkf_toymc_find_event(const mkf_geometry* g, const mkf_toymc_params* p, float noise_per_cm2, int seed_layers,
                          mkf_toymc_find_output* o, int n_threads) {
-  if (!g || !p || !o || o->n_tracks < 0 || g->n_disk != 0 || seed_layers < 1 || seed_layers > g->n_barrel) return 1;
-  const int n = o->n_tracks, L = g->n_barrel, cap = o->layer_capacity;
+  if (!g || !p || !o || o->n_tracks < 0 || find_layers(g) == 0 || seed_layers < 1 || seed_layers > find_layers(g))
+    return 1;
+  const int n = o->n_tracks, L = find_layers(g), cap = o->layer_capacity;
+  const bool disks = g->n_barrel == 0;
+  const int n_gen_layers = g->n_barrel + 2 * g->n_disk;
   for (int l = 0; l < L; ++l)
     if (n + mkf_toymc_noise_count(g, l, noise_per_cm2) > cap) return 2;
   // Tracks through the fit generator (one hit per crossed layer, in layer order).
-  std::vector<float> seed_par(6 * (size_t)n), seed_err(21 * (size_t)n), hit_pos((size_t)L * 3 * n),
-      hit_err((size_t)L * 6 * n), truth((size_t)L * 6 * n);
-  std::vector<int> chg(n), nh(n), hit_layer((size_t)L * n);
+  std::vector<float> seed_par(6 * (size_t)n), seed_err(21 * (size_t)n), hit_pos((size_t)n_gen_layers * 3 * n),
+      hit_err((size_t)n_gen_layers * 6 * n), truth((size_t)n_gen_layers * 6 * n);
+  std::vector<int> chg(n), nh(n), hit_layer((size_t)n_gen_layers * n);
+  std::vector<unsigned char> flags((size_t)n_gen_layers * n);
   mkf_toymc_output to;
   to.n_tracks = n;
-  to.n_slots = L;
+  to.n_slots = n_gen_layers;
   to.seed_par = seed_par.data();
   to.seed_err = seed_err.data();
   to.chg = chg.data();
   to.hit_pos = hit_pos.data();
   to.hit_err = hit_err.data();
   to.n_hits = nh.data();
-  to.hit_flags = nullptr;
+  to.hit_flags = flags.data();
   to.hit_layer = hit_layer.data();
   to.truth_par = truth.data();
   mkf_toymc_params pp = *p;
@@ -410,7 +433,8 @@ int mkf_toymc_find_event(const mkf_geometry* g, const mkf_toymc_params* p, float
   for (int l = 0; l < L; ++l) o->n_layer_hits[l] = 0;
   for (int t = 0; t < n; ++t)
     for (int k = 0; k < nh[t]; ++k) {
-      const int l = hit_layer[(size_t)k * n + t];
+      const int l = hit_layer[(size_t)k * n + t];   // disk mode: +z disks are layers 0..n_disk-1
+      if (l < 0 || l >= L) continue;
       const int i = o->n_layer_hits[l]++;
       float* h = o->hits + ((size_t)l * cap + i) * 9;
       for (int e = 0; e < 3; ++e) h[e] = hit_pos[((size_t)k * 3 + e) * n + t];
@@ -421,15 +445,29 @@ int mkf_toymc_find_event(const mkf_geometry* g, const mkf_toymc_params* p, float
   for (int l = 0; l < L; ++l) {
     Rng rng(p->seed ^ 0x5bd1e995u, (uint64_t)(1u << 30) + l);
     const int nn = mkf_toymc_noise_count(g, l, noise_per_cm2);
-    const double r = g->barrel_r[l], zmax = g->barrel_zmax[l];
     const double vr = double(p->sig_rphi) * p->sig_rphi, vz = double(p->sig_z) * p->sig_z;
     for (int k = 0; k < nn; ++k) {
-      const double ph = -kPi + 2 * kPi * rng.uniform(), z = -zmax + 2 * zmax * rng.uniform();
+      const double ph = -kPi + 2 * kPi * rng.uniform(), u = rng.uniform();
+      double r, z;
+      if (disks) {   // uniform on the annulus
+        const double r0 = g->disk_rmin[l], r1 = g->disk_rmax[l];
+        r = std::sqrt(r0 * r0 + u * (r1 * r1 - r0 * r0));
+        z = g->disk_z[l];
+      } else {
+        r = g->barrel_r[l];
+        z = -g->barrel_zmax[l] + 2 * g->barrel_zmax[l] * u;
+      }
       const int i = o->n_layer_hits[l]++;
       float* h = o->hits + ((size_t)l * cap + i) * 9;
       const double sp = std::sin(ph), cp = std::cos(ph);
       h[0] = float(r * cp); h[1] = float(r * sp); h[2] = float(z);
-      h[3] = float(vr * sp * sp); h[4] = float(-vr * sp * cp); h[5] = float(vr * cp * cp);
+      if (disks) {   // radial and r-phi resolution in the disk plane, as the signal hits
+        const double vrr = double(p->sig_r_endcap) * p->sig_r_endcap;
+        h[3] = float(vrr * cp * cp + vr * sp * sp); h[4] = float((vrr - vr) * sp * cp);
+        h[5] = float(vrr * sp * sp + vr * cp * cp);
+      } else {
+        h[3] = float(vr * sp * sp); h[4] = float(-vr * sp * cp); h[5] = float(vr * cp * cp);
+      }
       h[6] = 0.f; h[7] = 0.f; h[8] = float(vz);
       o->hit_mc[(size_t)l * cap + i] = -1;
     }
@@ -438,9 +476,13 @@ int mkf_toymc_find_event(const mkf_geometry* g, const mkf_toymc_params* p, float
   // hit, smeared with the fit-seed widths.
   int ns = 0;
   for (int t = 0; t < n; ++t) {
-    if (nh[t] < seed_layers) continue;
+    bool ok = true;
+    for (int l = 0; l < seed_layers; ++l) ok = ok && sig_index[l][t] >= 0;
+    if (!ok) continue;
     Rng rng(p->seed ^ 0x27d4eb2fu, (uint64_t)t);
-    const int k = seed_layers - 1;
+    int k = -1;   // slot of the hit on the last seed layer
+    for (int s = 0; s < nh[t]; ++s)
+      if (hit_layer[(size_t)s * n + t] == seed_layers - 1) k = s;
     double sd[6];
     for (int e = 0; e < 6; ++e) sd[e] = truth[((size_t)k * 6 + e) * n + t];
     const double sig[6] = {p->seed_sig_pos, p->seed_sig_pos, p->seed_sig_z, p->seed_rel_ipt * sd[3], p->seed_sig_ang,

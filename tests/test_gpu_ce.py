@@ -77,6 +77,22 @@ def test_ce_noise_free(gpu_device):
     assert _true_fraction(g["hot_idx"][0], ev, 3) > 0.985
 
 
+def test_ce_on_endcap_disks(gpu_device):
+    import torch
+    geo = mk.geometry(4)
+    p = mk.default_toy_params(seed=15, simulate_material=0, eta_max=2.5, pt_min=1.0)
+    ev = mk.generate_find_event(geo, p, 4000, noise_per_cm2=1.0)
+    descs = mk.disk_layer_descs(geo)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+    b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, chi2_cut=30.0, max_cands=6)
+    ctx.find_tracks_clone_engine(eoh, b)
+    torch.cuda.synchronize()
+    c = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=8)
+    _compare(b.numpy(), c)
+
+
 def test_ce_edge_cases(gpu_device):
     ev, descs = _event(8, 500, 2.0)
     L = len(descs)

@@ -240,3 +240,21 @@ def test_find_rejects_bad_arguments(setup, gpu_device):
         ctx.find_tracks_best_hit(fresh, mk.FindBatch(ev, len(descs), 3, device=gpu_device))
     with pytest.raises(mk.MkfError, match="hit_offset"):
         fresh.load(np.array([0, 5, 3] + [3] * (len(descs) - 2), np.int32), torch.zeros((5, 9), device=gpu_device))
+
+
+def test_find_on_endcap_disks(gpu_device):
+    geo = mk.geometry(4)
+    p = mk.default_toy_params(seed=14, simulate_material=0, eta_max=2.5, pt_min=1.0)
+    ev = mk.generate_find_event(geo, p, 4000, noise_per_cm2=1.0)
+    descs = mk.disk_layer_descs(geo)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+    for l in (0, 4, 8):
+        r_c, b_c, _, _, _ = oracle.loh_build(descs[l], ev["hits_flat"][ev["hit_offset"][l]:ev["hit_offset"][l + 1]])
+        r_g, b_g, _, _, _ = eoh.layer_views(l)
+        assert np.mean(r_g == r_c) > 0.999
+    _, gpu = _run_find(ctx, eoh, ev, len(descs), 3, gpu_device)
+    cpu = oracle.find_best_hit(ev, descs, first_layer=3, chi2_cut=15.0, bfield=3.8, n_threads=8)
+    _compare_find(gpu, cpu)
+

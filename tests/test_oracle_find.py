@@ -237,3 +237,19 @@ def test_find_propagate_to_hit_option():
         ok = hot >= 0
         truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(hot, 0)], -5)
         assert (truth == ev["seed_mc"][None, :]).sum() / hot.size > 0.9
+
+
+def test_find_on_endcap_disks():
+    """Disk-only geometry: endcap layers (q = r bins, propagation to z, the endcap update)."""
+    geo = mk.geometry(4)
+    p = mk.default_toy_params(seed=13, simulate_material=0, eta_max=2.5, pt_min=1.0)
+    ev = mk.generate_find_event(geo, p, 3000, noise_per_cm2=1.0)
+    descs = mk.disk_layer_descs(geo)
+    assert ev["n_seeds"] > 500
+    for r in (oracle.find_best_hit(ev, descs, 3, chi2_cut=15.0, n_threads=4),
+              {"hot_idx": oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, n_threads=4)["hot_idx"][0]}):
+        hot = r["hot_idx"][3:]; off = ev["hit_offset"][3:-1]
+        ok = hot >= 0
+        truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(hot, 0)], -5)
+        assert (truth == ev["seed_mc"][None, :]).sum() / ok.sum() > 0.9
+        assert ok.sum() / hot.size > 0.5
