@@ -37,7 +37,24 @@ def _run(ev, descs, dev, K, first_layer=3, chi2_cut=30.0):
     return b.numpy()
 
 
-def _compare(g, c, min_same=0.99):
+def _dump(name, **arrays):
+    """Keeps the arrays of a failed comparison when MKF_TEST_DUMP names a directory."""
+    import os
+    d = os.environ.get("MKF_TEST_DUMP")
+    if d:
+        os.makedirs(d, exist_ok=True)
+        np.savez_compressed(os.path.join(d, name + ".npz"), **arrays)
+
+
+def _compare(g, c, min_same=0.99, name="ce"):
+    try:
+        return _compare_impl(g, c, min_same)
+    except AssertionError:
+        _dump(name, **{"g_" + k: v for k, v in g.items()}, **{"c_" + k: v for k, v in c.items()})
+        raise
+
+
+def _compare_impl(g, c, min_same):
     n = len(c["n_cands"])
     same = (g["n_cands"] == c["n_cands"]) & np.all(g["hot_idx"] == c["hot_idx"], axis=(0, 1))
     assert same.mean() >= min_same, (same.mean(), np.nonzero(~same)[0][:8])
@@ -65,7 +82,7 @@ def test_ce_matches_oracle(gpu_device, K, noise):
     ev, descs = _event(100 + K, 4000, noise)
     g = _run(ev, descs, gpu_device, K)
     c = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=K, n_threads=8)
-    _compare(g, c)
+    _compare(g, c, name="ce_K%d" % K)
     assert abs(_true_fraction(g["hot_idx"][0], ev, 3) - _true_fraction(c["hot_idx"][0], ev, 3)) < 3e-3
 
 

@@ -59,7 +59,10 @@ def test_loh_build_matches_oracle(setup):
         phi_g = np.empty_like(p_g); phi_g[r_g] = p_g
         q_o = np.empty_like(q_g); q_o[r_g] = q_g
         assert np.array_equal(q_o, hl[:, 2])
-        assert np.all(np.abs(phi_g - phi_c) <= 2 * np.spacing(np.abs(phi_c)))
+        # device and host atan2f agree to a few ulp (both within their documented error bounds)
+        dphi = np.abs(phi_g - phi_c)
+        dphi = np.minimum(dphi, np.abs(dphi - np.float32(2 * np.pi)))
+        assert np.all(dphi <= 4 * np.spacing(np.maximum(np.abs(phi_c), np.float32(1e-6)))), dphi.max()
         k_c, _ = _keys(descs[l], phi_c, hl[:, 2])
         k_g, _ = _keys(descs[l], phi_g, hl[:, 2])
         moved = k_c != k_g
