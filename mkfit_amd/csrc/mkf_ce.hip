@@ -437,9 +437,11 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       ++m;
     }
     if (s_bs[g]) {
+      // The best short candidate's score is recomputed only alongside the
+      // candidates' (HitStructures.cc:515-521); alone it keeps its finding score.
       Cand b;
       get(BS[g], b);
-      b.score = cand_score(b);
+      if (m > 0) b.score = cand_score(b);
       put(BS[g], b);
       if (m > 0) {
         if (b.score > X[(g * kS + ord[m - 1]) * kW + 29]) {
