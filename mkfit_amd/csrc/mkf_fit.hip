@@ -328,7 +328,10 @@ __device__ __forceinline__ void store_state(const FitArgs& a, int t, const float
 // SIMD requested from the register allocator (the autotuner's occupancy knob).
 // FORM only names the instantiation (0 = plain, 1 = batched) so profiles keep
 // the two forms apart; the body does not depend on it.
-template <int MINW, int FORM, bool FB>
+// PF: software-pipeline the hit loads (slot s+1 in flight while slot s
+// computes) at the cost of 9 more live VGPRs; the loads do not change the
+// arithmetic, so both settings give the same bits.
+template <int MINW, int FORM, bool FB, bool PF = true>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 512), amdgpu_waves_per_eu(MINW, 8)))
 k_fit(FitArgs a, int s_begin, int s_end) {
   const int stride = gridDim.x * blockDim.x;
@@ -343,13 +346,20 @@ k_fit(FitArgs a, int s_begin, int s_end) {
     float par[6], err[21], chi2;
     int chg;
     load_state(a, t, from_seed, par, err, chg, chi2);
-    // Software pipeline: the hit of slot s+1 is in flight while slot s computes.
-    HitIn cur, nxt;
-    if (s_begin < s1) load_hit(a, b, s_begin, tb, nxt);
-    for (int s = s_begin; s < s1; ++s) {
-      cur = nxt;
-      if (s + 1 < s1) load_hit(a, b, s + 1, tb, nxt);
-      fit_step<FB>(par, err, chg, chi2, a, cur);
+    if (PF) {
+      HitIn cur, nxt;
+      if (s_begin < s1) load_hit(a, b, s_begin, tb, nxt);
+      for (int s = s_begin; s < s1; ++s) {
+        cur = nxt;
+        if (s + 1 < s1) load_hit(a, b, s + 1, tb, nxt);
+        fit_step<FB>(par, err, chg, chi2, a, cur);
+      }
+    } else {
+      for (int s = s_begin; s < s1; ++s) {
+        HitIn cur;
+        load_hit(a, b, s, tb, cur);
+        fit_step<FB>(par, err, chg, chi2, a, cur);
+      }
     }
     if (to_pca) propagate_to_pca(par, err, chg, a);
     store_state(a, t, par, err, chg, chi2);
@@ -361,10 +371,13 @@ using FitKernel = void (*)(FitArgs, int, int);
 // private arrays to scratch, which changes FMA contraction and hence the bits;
 // the variants stop there so every choice gives identical results.
 // [fp64 fallback][variant]
-static const FitKernel kFusedVariants[2][3] = {{k_fit<1, 1, false>, k_fit<2, 1, false>, k_fit<4, 1, false>},
-                                               {k_fit<1, 1, true>, k_fit<2, 1, true>, k_fit<4, 1, true>}};
+static const FitKernel kFusedVariants[2][6] = {
+    {k_fit<1, 1, false>, k_fit<2, 1, false>, k_fit<4, 1, false>,
+     k_fit<1, 1, false, false>, k_fit<2, 1, false, false>, k_fit<4, 1, false, false>},
+    {k_fit<1, 1, true>, k_fit<2, 1, true>, k_fit<4, 1, true>,
+     k_fit<1, 1, true, false>, k_fit<2, 1, true, false>, k_fit<4, 1, true, false>}};
 static const FitKernel kPlainKernel[2] = {k_fit<1, 0, false>, k_fit<1, 0, true>};
-static constexpr int kNumFusedVariants = 3;
+static constexpr int kNumFusedVariants = 6;
 
 // ============================================================================
 // C ABI
