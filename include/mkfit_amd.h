@@ -234,6 +234,71 @@ int mkf_event_of_hits_layer(mkf_event_of_hits* eoh, int layer, int* n_hits, int*
 int mkf_select_hit_indices(mkf_event_of_hits* eoh, int layer, int n, const float* par, const float* err,
                            int* xsize, int* xarr, int* wsr, void* stream);
 
+/* ---- Per-layer finding steps --------------------------------------------
+ * The MkFinder operations a caller's own layer loop applies to one layer
+ * (MkBuilder.cc:1395-1478, 1926-2060), on the per-lane MkFinder members
+ * (MkFinder.h:238-300) held as device arrays, element-major.  xsize / xarr /
+ * wsr are what mkf_select_hit_indices wrote for the same lanes and layer;
+ * xarr entries outside [0, hits on the layer) are skipped. */
+typedef struct mkf_finder_lanes {
+  int n;
+  const float* prop_par;   /* [6][n]  Par[iP]: the state propagated to the layer */
+  const float* prop_err;   /* [21][n] Err[iP] */
+  float* par;              /* [6][n]  Par[iC]: out, the state updated on the layer */
+  float* err;              /* [21][n] Err[iC] */
+  int* chg;                /* [n] Chg, in/out: flipped with 1/pT when an update leaves 1/pT < 0 */
+  float* chi2;             /* [n] Chi2 */
+  int* n_found;            /* [n] NFoundHits */
+  int* n_inside_minus_one; /* [n] NInsideMinusOneHits */
+  int* n_tail_minus_one;   /* [n] NTailMinusOneHits */
+} mkf_finder_lanes;
+
+/* MkFinder::AddBestHit (MkFinder.cc:691-832) with add_hit (MkFinder.h:312-354):
+ * per lane the smallest |chi2| below getHitSelDynamicChi2Cut among its xarr
+ * hits, else a fake hit (-3 window on the layer edge, -2 after an earlier -1,
+ * else -1); chi2 and the hit counters advance; then the Kalman update
+ * (m_update_param_foo = kalmanPropagateAndUpdate[Endcap], KalmanUtilsMPlex.cc:
+ * 471-504, 697-730) of prop_* with the chosen or fake hit into par / err, with
+ * the charge flip.  hit_code [n] out: the HitOnTrack index added for the layer,
+ * or -9 when the window lies outside the layer (no HitOnTrack; the update
+ * still uses the fake hit).  prop_to_hit / intra_pflags: as in mkf_find_batch. */
+int mkf_add_best_hit(mkf_context* ctx, mkf_event_of_hits* eoh, int layer, const mkf_finder_lanes* lanes,
+                     const int* xsize, const int* xarr, const int* wsr, float chi2_cut, int prop_to_hit,
+                     int intra_pflags, int* hit_code, void* stream);
+
+/* IdxChi2List (Track.h:28-42), plus the entry's position in its lane's xarr. */
+typedef struct mkf_idx_chi2 {
+  int trk_idx;      /* lane */
+  int hit_idx;      /* hit index within the layer, or the fake code -1 / -2 / -3 */
+  unsigned module;  /* 0 for hits (no module ids), 0xffffffff for the fake entry */
+  int nhits, ntailholes, noverlaps, nholes;
+  float pt, chi2, chi2_hit, score;
+  int xhit_pos;     /* position in xarr; -1 for the fake entry */
+} mkf_idx_chi2;
+
+#define MKF_CE_ENTRIES_PER_LANE (MKF_HIT_IDX_MAX + 1)
+
+/* MkFinder::FindCandidatesCloneEngine (MkFinder.cc:1114-1284): per lane the
+ * IdxChi2List records its hits pass (|chi2| below the dynamic cut and
+ * isStripQCompatible), in xarr order, then the invalid-hit record (-1, or -2
+ * past max_holes / max_consec_holes, -3 on the layer edge), none when the
+ * window lies outside the layer.  Reads prop_*, chg, chi2 and the counters of
+ * `lanes`; writes n_entries [n] and entries [n][MKF_CE_ENTRIES_PER_LANE].
+ * The reference hands each record to CandCloner::add_cand in the order
+ * (xarr position, lane) for hits, then lane order for invalid records. */
+int mkf_find_candidates_clone_engine(mkf_context* ctx, mkf_event_of_hits* eoh, int layer,
+                                     const mkf_finder_lanes* lanes, const int* xsize, const int* xarr,
+                                     const int* wsr, float chi2_cut, int prop_to_hit, int intra_pflags,
+                                     int max_holes, int max_consec_holes, int* n_entries, mkf_idx_chi2* entries,
+                                     void* stream);
+
+/* MkFinder::UpdateWithLastHit (MkFinder.cc:1290-1308): per lane the Kalman
+ * update of prop_* with hit hit_idx[t] of the layer (kalmanPropagateAndUpdate
+ * with the inter-layer flags `pflags`, charge flip) into par / err.  Lanes with
+ * hit_idx < 0 or past the layer's hits copy prop_* unchanged. */
+int mkf_update_with_last_hit(mkf_context* ctx, mkf_event_of_hits* eoh, int layer, const mkf_finder_lanes* lanes,
+                             const int* hit_idx, int prop_to_hit, int pflags, void* stream);
+
 typedef struct mkf_find_batch {
   int n_tracks;
   float* par;     /* [6][n]  in: seed state; out: state when the track stopped or after the last layer */

@@ -93,6 +93,7 @@ ABI_SYMBOLS = [
     "mkf_set_batched_shape", "mkf_get_batched_shape", "mkf_autotune_batched", "mkf_num_batched_variants",
     "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
+    "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
     "mkf_fit_adjoint",
 ]
 CE_MAX_CANDS = 8
@@ -123,6 +124,19 @@ class MkfAdjointBatch(ctypes.Structure):
         (k, _c_void_p) for k in ("seed_par", "seed_err", "seed_chg", "hit_pos", "hit_err", "n_hits", "theta")] + [
         ("pflags", ctypes.c_int), ("xi_per_x0", ctypes.c_float), ("checkpoint_every", ctypes.c_int),
         ("chi2", _c_void_p), ("grad", _c_void_p)]
+
+
+class MkfFinderLanes(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int)] + [(k, ctypes.c_void_p) for k in (
+        "prop_par", "prop_err", "par", "err", "chg", "chi2", "n_found", "n_inside_minus_one", "n_tail_minus_one")]
+
+
+# mkf_idx_chi2: IdxChi2List (Track.h:28-42) plus the record's position in its lane's hit list.
+IDX_CHI2_FIELDS = [("trk_idx", "i4"), ("hit_idx", "i4"), ("module", "u4"), ("nhits", "i4"), ("ntailholes", "i4"),
+                   ("noverlaps", "i4"), ("nholes", "i4"), ("pt", "f4"), ("chi2", "f4"), ("chi2_hit", "f4"),
+                   ("score", "f4"), ("xhit_pos", "i4")]
+IDX_CHI2_DTYPE = np.dtype([(k, "<" + t) for k, t in IDX_CHI2_FIELDS])
+CE_ENTRIES_PER_LANE = 17
 
 
 class MkfFindBatch(ctypes.Structure):
@@ -166,6 +180,13 @@ def lib():
         L.mkf_find_tracks_best_hit.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfFindBatch), _c_void_p]
         L.mkf_find_tracks_clone_engine.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfCEBatch), _c_void_p]
         L.mkf_fit_adjoint.argtypes = [_c_void_p, ctypes.POINTER(MkfAdjointBatch), _c_void_p]
+        L.mkf_add_best_hit.argtypes = [_c_void_p, _c_void_p, ctypes.c_int, ctypes.POINTER(MkfFinderLanes)] + \
+            [_c_void_p] * 3 + [ctypes.c_float, ctypes.c_int, ctypes.c_int, _c_void_p, _c_void_p]
+        L.mkf_find_candidates_clone_engine.argtypes = [_c_void_p, _c_void_p, ctypes.c_int,
+                                                       ctypes.POINTER(MkfFinderLanes)] + [_c_void_p] * 3 + \
+            [ctypes.c_float] + [ctypes.c_int] * 4 + [_c_void_p] * 3
+        L.mkf_update_with_last_hit.argtypes = [_c_void_p, _c_void_p, ctypes.c_int, ctypes.POINTER(MkfFinderLanes),
+                                               _c_void_p, ctypes.c_int, ctypes.c_int, _c_void_p]
         L.mkf_autotune_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p, ctypes.c_int,
                                            ctypes.POINTER(MkfLaunchShape), _c_float_p]
         _LIB = L
@@ -392,6 +413,37 @@ class EventOfHits:
                                             _ptr(wsr), self.ctx._stream(stream)))
 
 
+class FinderLanes:
+    """Per-lane MkFinder state on the device for the per-layer steps (mkf_finder_lanes): the state at the
+    layer (prop_par, prop_err), the updated state (par, err), charge, chi2 and the hit counters."""
+
+    KEYS = ("prop_par", "prop_err", "par", "err", "chg", "chi2", "n_found", "n_inside_minus_one",
+            "n_tail_minus_one")
+
+    def __init__(self, prop_par, prop_err, chg, chi2=None, n_found=None, n_inside_minus_one=None,
+                 n_tail_minus_one=None):
+        import torch
+        dev = prop_par.device
+        n = prop_par.shape[-1]
+        self.n = n
+        self.prop_par = prop_par.contiguous()
+        self.prop_err = prop_err.contiguous()
+        self.par = torch.zeros((6, n), dtype=torch.float32, device=dev)
+        self.err = torch.zeros((21, n), dtype=torch.float32, device=dev)
+        self.chg = chg.contiguous().clone()
+        zi = lambda v: v.contiguous().clone() if v is not None else torch.zeros(n, dtype=torch.int32, device=dev)
+        self.chi2 = chi2.contiguous().clone() if chi2 is not None else torch.zeros(n, dtype=torch.float32, device=dev)
+        self.n_found = zi(n_found)
+        self.n_inside_minus_one = zi(n_inside_minus_one)
+        self.n_tail_minus_one = zi(n_tail_minus_one)
+
+    def c_struct(self):
+        return MkfFinderLanes(self.n, *[_ptr(getattr(self, k)) for k in self.KEYS])
+
+    def numpy(self):
+        return {k: getattr(self, k).cpu().numpy() for k in self.KEYS}
+
+
 def _check_hip(dst, src, nbytes):
     """Device-to-device copy through the HIP runtime torch already loaded."""
     global _HIPRT
@@ -594,6 +646,27 @@ class Context:
     def find_tracks_clone_engine(self, eoh, batch, stream=None):
         _check(lib().mkf_find_tracks_clone_engine(self.h, eoh.h, ctypes.byref(batch.c_struct()),
                                                   self._stream(stream)))
+
+    def add_best_hit(self, eoh, layer, lanes, xsize, xarr, wsr, hit_code, chi2_cut=15.0, prop_to_hit=False,
+                     intra_pflags=0, stream=None):
+        st = lanes.c_struct()
+        _check(lib().mkf_add_best_hit(self.h, eoh.h, int(layer), ctypes.byref(st), _ptr(xsize), _ptr(xarr), _ptr(wsr),
+                                      float(chi2_cut), int(bool(prop_to_hit)), int(intra_pflags), _ptr(hit_code),
+                                      self._stream(stream)))
+
+    def find_candidates_clone_engine(self, eoh, layer, lanes, xsize, xarr, wsr, n_entries, entries, chi2_cut=30.0,
+                                     prop_to_hit=True, intra_pflags=0, max_holes=4, max_consec_holes=1, stream=None):
+        """entries: a device byte tensor of n * 17 records of IDX_CHI2_DTYPE (48 bytes each)."""
+        st = lanes.c_struct()
+        _check(lib().mkf_find_candidates_clone_engine(
+            self.h, eoh.h, int(layer), ctypes.byref(st), _ptr(xsize), _ptr(xarr), _ptr(wsr), float(chi2_cut),
+            int(bool(prop_to_hit)), int(intra_pflags), int(max_holes), int(max_consec_holes), _ptr(n_entries),
+            _ptr(entries), self._stream(stream)))
+
+    def update_with_last_hit(self, eoh, layer, lanes, hit_idx, prop_to_hit=True, pflags=0, stream=None):
+        st = lanes.c_struct()
+        _check(lib().mkf_update_with_last_hit(self.h, eoh.h, int(layer), ctypes.byref(st), _ptr(hit_idx),
+                                              int(bool(prop_to_hit)), int(pflags), self._stream(stream)))
 
     def set_batched_shape(self, block, blocks_per_cu, variant):
         s = MkfLaunchShape(int(block), int(blocks_per_cu), int(variant))

@@ -38,20 +38,6 @@ constexpr int kS = 8;                 // candidate slots per seed
 constexpr int kG = 8;                 // seeds per workgroup
 constexpr int kLanes = kS * kG;       // 64: one wavefront
 constexpr int kW = 37;                // LDS words per exchanged candidate (36 used, odd stride)
-constexpr float kScoreWorst = -1e16f;
-
-// Config.h:355-359 and getScoreCalc (Track.h:648-672)
-__device__ __forceinline__ float score_calc(int nfound, int ntail, int noverlap, int nmiss, float chi2, float pt,
-                                            bool in_find) {
-  const float maxBonus = 8.0f;
-  float bonus = 0.2f * nfound + 4.0f;
-  float penalty = 8.0f;
-  if (pt < 0.9f) {
-    penalty *= in_find ? 1.7f : 1.5f;
-    bonus = fminf(bonus * (in_find ? 0.9f : 1.0f), maxBonus);
-  }
-  return bonus * nfound + 0.0f * noverlap - penalty * nmiss - 3.0f * ntail - chi2;
-}
 
 struct Cand {
   float par[6], err[21];
@@ -109,25 +95,6 @@ __device__ __forceinline__ void get(const float* X, Cand& c) {
   c.ntail = __float_as_int(X[33]);
   c.last = __float_as_int(X[34]);
   c.last_hit = __float_as_int(X[35]);
-}
-
-// isStripQCompatible (MkFinder.cc:837-862) for one lane: track z / xy from the
-// state propagated to the hit, its uncertainty from the state at the layer.
-__device__ __forceinline__ bool strip_q_compatible(bool barrel, const float (&pe)[21], const float (&q)[6],
-                                                   const float (&mp)[3], const float (&me)[6]) {
-  if (barrel) {
-    const float res = fabsf(mp[2] - q[2]);
-    const float hitHL = sqrtf(me[5] * 3.f);
-    const float qErr = sqrtf(pe[S(2, 2)]);
-    return hitHL + fmaxf(3.f * qErr, 0.5f) > res;
-  }
-  const float r0 = mp[0] - q[0], r1 = mp[1] - q[1];
-  const float hitT2 = me[0] + me[2];
-  const float inv = 1.f / hitT2;
-  const float p0 = me[0] * inv, p1 = me[1] * inv, p2 = me[2] * inv;
-  const float qErr = sqrtf(fabsf(pe[S(0, 0)] * p0 + 2.f * pe[S(0, 1)] * p1 + pe[S(1, 1)] * p2));
-  const float resProj = sqrtf(r0 * p0 * r0 + 2.f * r1 * p1 * r0 + r1 * p2 * r1);
-  return sqrtf(hitT2 * 3.f) + fmaxf(3.f * qErr, 0.5f) > resProj;
 }
 
 struct CeArgs {

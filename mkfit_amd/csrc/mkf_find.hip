@@ -13,6 +13,7 @@
 //   sorted hits   [total][9]  hits of every layer in LayerOfHits order
 //   phis, qs      [total]     phi and q (z on barrel, r on endcaps) per sorted hit
 //   ranks         [total]     sorted position -> hit index within its layer
+//   inv           [total]     event hit index -> sorted position
 //   bins          [sum nq*256] int2 {first, end} per (layer, q bin, phi bin),
 //                             positions relative to the layer's first hit
 // Every layer's rows start at its hit offset, so a lane addresses a layer
@@ -89,7 +90,7 @@ k_hit_keys(int total, int n_layers, const float* __restrict__ hits, const LayerD
 __global__ void __launch_bounds__(256)
 k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* __restrict__ vals_s,
            const LayerDev* __restrict__ L, const float* __restrict__ hits, const float* __restrict__ phi_o,
-           const float* __restrict__ q_o, int* __restrict__ ranks, int2* __restrict__ bins,
+           const float* __restrict__ q_o, int* __restrict__ ranks, int* __restrict__ inv, int2* __restrict__ bins,
            float* __restrict__ shits, float* __restrict__ phis, float* __restrict__ qs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
@@ -99,6 +100,7 @@ k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* 
   if (pass == 0) {
     const int j = vals_s[i];
     ranks[i] = j - d.hit_off;
+    inv[j] = i;
     phis[i] = phi_o[j];
     qs[i] = q_o[j];
 #pragma unroll
@@ -281,11 +283,11 @@ namespace {
 
 void free_hit_arrays(mkf_event_of_hits* h) {
   void* ps[] = {h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s, h->d_phi_o, h->d_q_o,
-                h->d_phis, h->d_qs,     h->d_shits, h->d_ranks, h->d_temp};
+                h->d_phis, h->d_qs,     h->d_shits, h->d_ranks, h->d_inv,   h->d_temp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   h->d_keys = h->d_keys_s = nullptr;
-  h->d_vals = h->d_vals_s = h->d_ranks = nullptr;
+  h->d_vals = h->d_vals_s = h->d_ranks = h->d_inv = nullptr;
   h->d_phi_o = h->d_q_o = h->d_phis = h->d_qs = h->d_shits = nullptr;
   h->d_temp = nullptr;
   h->temp_bytes = 0;
@@ -307,6 +309,7 @@ int grow(mkf_event_of_hits* h, size_t total, hipStream_t st) {
   HIP_TRY(hipMalloc(&h->d_phis, cap * 4));
   HIP_TRY(hipMalloc(&h->d_qs, cap * 4));
   HIP_TRY(hipMalloc(&h->d_ranks, cap * 4));
+  HIP_TRY(hipMalloc(&h->d_inv, cap * 4));
   HIP_TRY(hipMalloc(&h->d_shits, cap * 9 * 4));
   size_t tb = 0;
   HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s, (int)cap,
@@ -422,7 +425,7 @@ int mkf_event_of_hits_load(mkf_event_of_hits* h, const int* hit_offset, const fl
                                                32, st));
     for (int pass = 0; pass < 2; ++pass) {
       hipLaunchKernelGGL(k_loh_fill, dim3(blocks_for(total, 256)), dim3(256), 0, st, nt, pass, h->d_keys_s,
-                         h->d_vals_s, h->d_layers, hits, h->d_phi_o, h->d_q_o, h->d_ranks, h->d_bins, h->d_shits,
+                         h->d_vals_s, h->d_layers, hits, h->d_phi_o, h->d_q_o, h->d_ranks, h->d_inv, h->d_bins, h->d_shits,
                          h->d_phis, h->d_qs);
       HIP_TRY(hipGetLastError());
     }

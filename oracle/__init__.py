@@ -277,6 +277,89 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
     return out
 
 
+class _FinderLanes(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int)] + [(k, ctypes.c_void_p) for k in (
+        "prop_par", "prop_err", "par", "err", "chg", "chi2", "n_found", "n_inside_minus_one", "n_tail_minus_one")]
+
+
+IDX_CHI2_DTYPE = np.dtype([("trk_idx", "<i4"), ("hit_idx", "<i4"), ("module", "<u4"), ("nhits", "<i4"),
+                           ("ntailholes", "<i4"), ("noverlaps", "<i4"), ("nholes", "<i4"), ("pt", "<f4"),
+                           ("chi2", "<f4"), ("chi2_hit", "<f4"), ("score", "<f4"), ("xhit_pos", "<i4")])
+
+LANE_KEYS = ("chg", "chi2", "n_found", "n_inside_minus_one", "n_tail_minus_one")
+
+
+def _lanes(lanes):
+    """Copies of a finder-lane dict (prop_par, prop_err, chg, chi2 and the hit counters) plus output arrays."""
+    n = lanes["prop_par"].shape[1]
+    out = dict(prop_par=np.ascontiguousarray(lanes["prop_par"], np.float32),
+               prop_err=np.ascontiguousarray(lanes["prop_err"], np.float32),
+               par=np.zeros((6, n), np.float32), err=np.zeros((21, n), np.float32))
+    for k in LANE_KEYS:
+        dt = np.float32 if k == "chi2" else np.int32
+        out[k] = np.array(lanes.get(k, np.zeros(n, dt)), dt)
+    st = _FinderLanes(n, *[_p(out[k]) for k in ("prop_par", "prop_err", "par", "err") + LANE_KEYS])
+    return out, st
+
+
+def _layer_event(ev, descs, chi2_cut, pflags, prop_to_hit, intra_pflags):
+    layers = layer_array(descs)
+    off = np.ascontiguousarray(ev["hit_offset"], np.int32)
+    hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
+    fe = _FindEvent(len(descs), ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), 0, chi2_cut, pflags,
+                    int(bool(prop_to_hit)), int(intra_pflags))
+    return fe, (layers, off, hits)
+
+
+def add_best_hit(ev, descs, layer, lanes, xsize, xarr, wsr, chi2_cut=15.0, prop_to_hit=False, intra_pflags=0,
+                 bfield=3.8):
+    """MkFinder::AddBestHit on one layer; returns the lane dict after the step plus hit_code."""
+    out, st = _lanes(lanes)
+    fe, keep = _layer_event(ev, descs, chi2_cut, 0, prop_to_hit, intra_pflags)
+    xs = np.ascontiguousarray(xsize, np.int32); xa = np.ascontiguousarray(xarr, np.int32)
+    w = np.ascontiguousarray(wsr, np.int32)
+    code = np.zeros(st.n, np.int32)
+    ctx = OrcCtx(float(bfield), None, None)
+    rc = lib().orc_add_best_hit(ctypes.byref(ctx), ctypes.byref(fe), int(layer), ctypes.byref(st), _p(xs), _p(xa),
+                                _p(w), _p(code))
+    if rc:
+        raise RuntimeError("orc_add_best_hit failed (%d)" % rc)
+    out["hit_code"] = code
+    return out
+
+
+def find_candidates_ce(ev, descs, layer, lanes, xsize, xarr, wsr, chi2_cut=30.0, max_holes=4, max_consec_holes=1,
+                       prop_to_hit=True, intra_pflags=0, bfield=3.8):
+    """MkFinder::FindCandidatesCloneEngine on one layer: (n_entries [n], entries [n][17] of IDX_CHI2_DTYPE)."""
+    keep_lanes, st = _lanes(lanes)
+    fe, keep = _layer_event(ev, descs, chi2_cut, 0, prop_to_hit, intra_pflags)
+    pr = _CEParams(1, int(max_holes), int(max_consec_holes), int(bool(prop_to_hit)), int(intra_pflags))
+    xs = np.ascontiguousarray(xsize, np.int32); xa = np.ascontiguousarray(xarr, np.int32)
+    w = np.ascontiguousarray(wsr, np.int32)
+    ne = np.zeros(st.n, np.int32)
+    ent = np.zeros((st.n, 17), IDX_CHI2_DTYPE)
+    ctx = OrcCtx(float(bfield), None, None)
+    rc = lib().orc_find_candidates_ce(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr), int(layer),
+                                      ctypes.byref(st), _p(xs), _p(xa), _p(w), _p(ne), _p(ent))
+    if rc:
+        raise RuntimeError("orc_find_candidates_ce failed (%d)" % rc)
+    return ne, ent
+
+
+def update_with_last_hit(ev, descs, layer, lanes, hit_idx, pflags=0, prop_to_hit=True, bfield=3.8):
+    """MkFinder::UpdateWithLastHit on one layer: (par, err, chg)."""
+    out, st = _lanes(lanes)
+    fe, keep = _layer_event(ev, descs, 30.0, pflags, prop_to_hit, 0)
+    pr = _CEParams(1, 4, 1, int(bool(prop_to_hit)), 0)
+    hi = np.ascontiguousarray(hit_idx, np.int32)
+    ctx = OrcCtx(float(bfield), None, None)
+    rc = lib().orc_update_with_last_hit(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr), int(layer),
+                                        ctypes.byref(st), _p(hi))
+    if rc:
+        raise RuntimeError("orc_update_with_last_hit failed (%d)" % rc)
+    return out["par"], out["err"], out["chg"]
+
+
 class _AlignBatch(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("n_layers", ctypes.c_int)] + [
         (k, ctypes.c_void_p) for k in ("seed_par", "seed_err", "seed_chg", "hit_pos", "hit_err", "n_hits")] + [

@@ -156,6 +156,33 @@ int orc_select_hit_indices(const orc_layer_desc* d, int n_hits, const float* hit
                            const float* err, int* xsize, int* xarr, int* wsr);
 int orc_find_best_hit(const orc_ctx* ctx, const orc_find_event* ev, orc_find_tracks* tracks, int n_threads);
 
+/* Per-layer steps on MkFinder lane state (host arrays, element-major [k][n]). */
+typedef struct orc_finder_lanes {
+  int n;
+  const float* prop_par;   /* [6][n]  state at the layer (Par[iP]) */
+  const float* prop_err;   /* [21][n] */
+  float* par;              /* [6][n]  out: updated state (Par[iC]) */
+  float* err;              /* [21][n] */
+  int* chg;                /* [n] in/out */
+  float* chi2;             /* [n] in/out */
+  int* n_found;            /* [n] in/out */
+  int* n_inside_minus_one; /* [n] in/out */
+  int* n_tail_minus_one;   /* [n] in/out */
+} orc_finder_lanes;
+
+typedef struct orc_idx_chi2 {  /* IdxChi2List (Track.h:28-42) + position in the lane's XHitArr (-1: invalid hit) */
+  int trk_idx, hit_idx;
+  unsigned module;
+  int nhits, ntailholes, noverlaps, nholes;
+  float pt, chi2, chi2_hit, score;
+  int xhit_pos;
+} orc_idx_chi2;
+
+/* AddBestHit on `layer` of ev (hits, chi2_cut, prop_to_hit, intra_pflags) for lanes with select results
+ * xsize [n], xarr [16][n], wsr [n]; hit_code [n] = HitOnTrack index added, -9 none (window outside). */
+int orc_add_best_hit(const orc_ctx* ctx, const orc_find_event* ev, int layer, const orc_finder_lanes* lanes,
+                     const int* xsize, const int* xarr, const int* wsr, int* hit_code);
+
 /* ---- Clone-engine track finding (oracle_ce.cc) ---------------------------- */
 
 typedef struct orc_ce_params {
@@ -185,6 +212,16 @@ typedef struct orc_ce_tracks {
 
 int orc_find_clone_engine(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, orc_ce_tracks* t,
                           int n_threads);
+
+/* FindCandidatesCloneEngine on `layer` (reads prop_*, chg, chi2 and the counters of `lanes`; the chi2 cut
+ * from ev, the rest from p): n_entries [n], entries [n][17] per lane in XHitArr order, invalid record last. */
+int orc_find_candidates_ce(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, int layer,
+                           const orc_finder_lanes* lanes, const int* xsize, const int* xarr, const int* wsr,
+                           int* n_entries, orc_idx_chi2* entries);
+/* UpdateWithLastHit on `layer`: update prop_* with hit hit_idx[t] (ev->pflags for the propagation to the hit
+ * when p->prop_to_hit) into par / err / chg; lanes with hit_idx outside the layer copy prop_* unchanged. */
+int orc_update_with_last_hit(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, int layer,
+                             const orc_finder_lanes* lanes, const int* hit_idx);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,7 @@ struct Cand {
 struct IdxChi2 {                        // IdxChi2List (Track.h)
   int trkIdx, hitIdx, nhits, ntailholes, noverlaps, nholes;
   float pt, chi2, chi2_hit, score;
+  int xpos;                             // position in the lane's XHitArr, -1 for the invalid-hit record
 };
 
 float score_struct(const IdxChi2& e) {
@@ -212,6 +213,82 @@ void prop_and_update(const CEEvent& E, bool barrel, const float* psErr, const fl
     }
 }
 
+// Per-lane MkFinder members FindCandidatesCloneEngine reads.
+struct LaneCounts {
+  float chi2;
+  int nfound, noverlap, ninside, ntail;
+};
+
+// FindCandidatesCloneEngine (MkFinder.cc:1114-1284) for one block of lanes at
+// the layer (state parP/errP): appends the IdxChi2List records in the order the
+// reference hands them to CandCloner::add_cand -- accepted hits by (XHitArr
+// position, lane), then each lane's invalid-hit record.  XHitArr entries
+// outside [0, n_hits) are skipped.
+void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const float* hits, int n_hits,
+                           const float* parP, const float* errP, const int* chg, const LaneCounts* lc,
+                           const int* trk, const int* xs, const int* xa, const int* wsr, int n_proc,
+                           std::vector<IdxChi2>& entries) {
+  const orc_find_event& ev = *E.ev;
+  const orc_ce_params& P = *E.p;
+  const bool barrel = d.is_barrel != 0;
+  int maxSize = 0;
+  for (int i = 0; i < n_proc; ++i)
+    if (xs[i] > 0) maxSize = std::max(maxSize, std::min(xs[i], kHitIdxMax));
+  alignas(64) float msPar[3 * NN], msErr[6 * NN];
+  for (int l = 0; l < 3 * NN; ++l) msPar[l] = 0.f;
+  for (int l = 0; l < 6 * NN; ++l) msErr[l] = 1.f;
+  for (int hc = 0; hc < maxSize; ++hc) {
+    bool ok[NN] = {};
+    for (int i = 0; i < n_proc; ++i)
+      if (hc < xs[i] && xa[hc * NN + i] >= 0 && xa[hc * NN + i] < n_hits) {
+        ok[i] = true;
+        const float* h = hits + (size_t)xa[hc * NN + i] * 9;
+        for (int e = 0; e < 3; ++e) msPar[e * NN + i] = h[e];
+        for (int e = 0; e < 6; ++e) msErr[e * NN + i] = h[3 + e];
+      }
+    alignas(64) float chi2[NN], propPar[6 * NN];
+    prop_and_chi2(E, barrel, errP, parP, chg, msErr, msPar, n_proc, chi2, propPar);
+    for (int i = 0; i < n_proc; ++i) {
+      const float max_c2 = dyn_chi2_cut(d, ev.chi2_cut, parP[3 * NN + i], parP[5 * NN + i]);
+      if (!ok[i]) continue;
+      const float c2 = std::abs(chi2[i]);
+      if (!(c2 < max_c2)) continue;
+      if (!strip_q_compatible(barrel, errP, propPar, msErr, msPar, i)) continue;
+      IdxChi2 e;
+      e.trkIdx = trk[i];
+      e.hitIdx = xa[hc * NN + i];
+      e.nhits = lc[i].nfound + 1;
+      e.ntailholes = 0;
+      e.noverlaps = lc[i].noverlap;
+      e.nholes = lc[i].ninside + lc[i].ntail;
+      e.pt = std::abs(1.0f / parP[3 * NN + i]);
+      e.chi2 = lc[i].chi2 + c2;
+      e.chi2_hit = c2;
+      e.score = score_struct(e);
+      e.xpos = hc;
+      entries.push_back(e);
+    }
+  }
+  for (int i = 0; i < n_proc; ++i) {                    // the invalid-hit record
+    if (wsr[i] == WSR_Outside) continue;
+    int fake = (lc[i].ninside + lc[i].ntail < P.max_holes && lc[i].ntail < P.max_consec_holes) ? -1 : -2;
+    if (wsr[i] == WSR_Edge) fake = -3;
+    IdxChi2 e;
+    e.trkIdx = trk[i];
+    e.hitIdx = fake;
+    e.nhits = lc[i].nfound;
+    e.ntailholes = fake == -1 ? lc[i].ntail + 1 : lc[i].ntail;
+    e.noverlaps = lc[i].noverlap;
+    e.nholes = lc[i].ninside;
+    e.pt = std::abs(1.0f / parP[3 * NN + i]);
+    e.chi2 = lc[i].chi2;
+    e.chi2_hit = 0;
+    e.score = score_struct(e);
+    e.xpos = -1;
+    entries.push_back(e);
+  }
+}
+
 // One seed through every layer of the plan.
 void find_seed(const CEEvent& E, Comb& cc, int layer0) {
   const orc_find_event& ev = *E.ev;
@@ -272,62 +349,15 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0) {
         }
       }
       // FindCandidatesCloneEngine
-      int maxSize = 0;
-      for (int i = 0; i < n_proc; ++i)
-        if (xs[i] > 0) maxSize = std::max(maxSize, xs[i]);
-      int nAdded[NN] = {};
-      alignas(64) float msPar[3 * NN], msErr[6 * NN];
-      for (int l = 0; l < 3 * NN; ++l) msPar[l] = 0.f;
-      for (int l = 0; l < 6 * NN; ++l) msErr[l] = 1.f;
-      for (int hc = 0; hc < maxSize; ++hc) {
-        for (int i = 0; i < n_proc; ++i)
-          if (hc < xs[i]) {
-            const float* h = hits + (size_t)xa[hc * NN + i] * 9;
-            for (int e = 0; e < 3; ++e) msPar[e * NN + i] = h[e];
-            for (int e = 0; e < 6; ++e) msErr[e * NN + i] = h[3 + e];
-          }
-        alignas(64) float chi2[NN], propPar[6 * NN];
-        prop_and_chi2(E, barrel, B.errP, B.parP, B.chg, msErr, msPar, n_proc, chi2, propPar);
-        for (int i = 0; i < n_proc; ++i) {
-          const float max_c2 = dyn_chi2_cut(d, ev.chi2_cut, B.parP[3 * NN + i], B.parP[5 * NN + i]);
-          if (hc >= xs[i]) continue;
-          const float c2 = std::abs(chi2[i]);
-          if (!(c2 < max_c2)) continue;
-          if (!strip_q_compatible(barrel, B.errP, propPar, msErr, msPar, i)) continue;
-          ++nAdded[i];
-          const Cand& c = cc.cands[idx[i]];
-          IdxChi2 e;
-          e.trkIdx = idx[i];
-          e.hitIdx = xa[hc * NN + i];
-          e.nhits = c.nfound + 1;
-          e.ntailholes = 0;
-          e.noverlaps = c.noverlap;
-          e.nholes = c.ninside + c.ntail;
-          e.pt = std::abs(1.0f / B.parP[3 * NN + i]);
-          e.chi2 = c.chi2 + c2;
-          e.chi2_hit = c2;
-          e.score = score_struct(e);
-          entries.push_back(e);
-        }
-      }
-      for (int i = 0; i < n_proc; ++i) {                // the invalid-hit entry
-        if (wsr[i] == WSR_Outside) continue;
+      LaneCounts lc[NN];
+      int trk[NN];
+      for (int i = 0; i < n_proc; ++i) {
         const Cand& c = cc.cands[idx[i]];
-        int fake = (c.ninside + c.ntail < P.max_holes && c.ntail < P.max_consec_holes) ? -1 : -2;
-        if (wsr[i] == WSR_Edge) fake = -3;
-        IdxChi2 e;
-        e.trkIdx = idx[i];
-        e.hitIdx = fake;
-        e.nhits = c.nfound;
-        e.ntailholes = fake == -1 ? c.ntail + 1 : c.ntail;
-        e.noverlaps = c.noverlap;
-        e.nholes = c.ninside;
-        e.pt = std::abs(1.0f / B.parP[3 * NN + i]);
-        e.chi2 = c.chi2;
-        e.chi2_hit = 0;
-        e.score = score_struct(e);
-        entries.push_back(e);
+        lc[i] = {c.chi2, c.nfound, c.noverlap, c.ninside, c.ntail};
+        trk[i] = idx[i];
       }
+      find_candidates_block(E, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B.parP, B.errP, B.chg, lc,
+                            trk, xs, xa, wsr, n_proc, entries);
     }
     // CandCloner::ProcessSeedRange
     std::vector<Cand> cv;
@@ -470,5 +500,111 @@ extern "C" int orc_find_clone_engine(const orc_ctx* c, const orc_find_event* ev,
   for (int i = 1; i < n_threads; ++i) pool.emplace_back(work);
   work();
   for (auto& th : pool) th.join();
+  return 0;
+}
+
+namespace {
+
+CEEvent layer_event(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p) {
+  CEEvent E;
+  E.ev = ev;
+  E.p = p;
+  E.ctx.bfield = c ? c->bfield : 3.8112f;
+  E.ctx.rl_grid = c ? c->rl_grid : nullptr;
+  E.ctx.xi_grid = c ? c->xi_grid : nullptr;
+  return E;
+}
+
+// Lanes [t0, t0 + cnt) of an element-major [k][n] array into an NN block (padding repeats lane t0).
+template <class T>
+void gather_block(T* dst, const T* src, int rows, int n, int t0, int cnt) {
+  for (int k = 0; k < rows; ++k)
+    for (int l = 0; l < NN; ++l) dst[k * NN + l] = src[(size_t)k * n + t0 + (l < cnt ? l : 0)];
+}
+
+}  // namespace
+
+extern "C" int orc_find_candidates_ce(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p, int layer,
+                                      const orc_finder_lanes* ln, const int* xsize, const int* xarr, const int* wsr,
+                                      int* n_entries, orc_idx_chi2* out) {
+  if (!ev || !p || !ln || layer < 0 || layer >= ev->n_layers || ln->n < 0) return 1;
+  const CEEvent E = layer_event(c, ev, p);
+  const orc_layer_desc& d = ev->layers[layer];
+  const float* hits = ev->hits + (size_t)ev->hit_offset[layer] * 9;
+  const int n_hits = ev->hit_offset[layer + 1] - ev->hit_offset[layer];
+  const int n = ln->n;
+  constexpr int kPer = kHitIdxMax + 1;
+  for (int t0 = 0; t0 < n; t0 += NN) {
+    const int cnt = std::min(NN, n - t0);
+    alignas(64) float parP[6 * NN], errP[21 * NN];
+    int chg[NN], xs[NN], xa[kHitIdxMax * NN], w[NN], trk[NN];
+    LaneCounts lc[NN];
+    gather_block(parP, ln->prop_par, 6, n, t0, cnt);
+    gather_block(errP, ln->prop_err, 21, n, t0, cnt);
+    gather_block(chg, ln->chg, 1, n, t0, cnt);
+    gather_block(xs, xsize, 1, n, t0, cnt);
+    gather_block(xa, xarr, kHitIdxMax, n, t0, cnt);
+    gather_block(w, wsr, 1, n, t0, cnt);
+    for (int l = 0; l < NN; ++l) {
+      const int t = t0 + (l < cnt ? l : 0);
+      lc[l] = {ln->chi2[t], ln->n_found[t], 0, ln->n_inside_minus_one[t], ln->n_tail_minus_one[t]};
+      trk[l] = t;
+    }
+    std::vector<IdxChi2> entries;
+    find_candidates_block(E, d, hits, n_hits, parP, errP, chg, lc, trk, xs, xa, w, cnt, entries);
+    for (int l = 0; l < cnt; ++l) n_entries[t0 + l] = 0;
+    for (const IdxChi2& e : entries) {
+      const int t = e.trkIdx;
+      orc_idx_chi2& o = out[(size_t)t * kPer + n_entries[t]++];
+      o.trk_idx = t;
+      o.hit_idx = e.hitIdx;
+      o.module = e.hitIdx >= 0 ? 0u : 0xffffffffu;
+      o.nhits = e.nhits;
+      o.ntailholes = e.ntailholes;
+      o.noverlaps = e.noverlaps;
+      o.nholes = e.nholes;
+      o.pt = e.pt;
+      o.chi2 = e.chi2;
+      o.chi2_hit = e.chi2_hit;
+      o.score = e.score;
+      o.xhit_pos = e.xpos;
+    }
+  }
+  return 0;
+}
+
+extern "C" int orc_update_with_last_hit(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p,
+                                        int layer, const orc_finder_lanes* ln, const int* hit_idx) {
+  if (!ev || !p || !ln || layer < 0 || layer >= ev->n_layers || ln->n < 0) return 1;
+  const CEEvent E = layer_event(c, ev, p);
+  const bool barrel = ev->layers[layer].is_barrel != 0;
+  const float* hits = ev->hits + (size_t)ev->hit_offset[layer] * 9;
+  const int n_hits = ev->hit_offset[layer + 1] - ev->hit_offset[layer];
+  const int n = ln->n;
+  for (int t0 = 0; t0 < n; t0 += NN) {
+    const int cnt = std::min(NN, n - t0);
+    alignas(64) float par[6 * NN], err[21 * NN], msPar[3 * NN], msErr[6 * NN], oErr[21 * NN], oPar[6 * NN];
+    int chg[NN];
+    gather_block(par, ln->prop_par, 6, n, t0, cnt);
+    gather_block(err, ln->prop_err, 21, n, t0, cnt);
+    gather_block(chg, ln->chg, 1, n, t0, cnt);
+    bool ok[NN];
+    for (int l = 0; l < NN; ++l) {
+      const int h = hit_idx[t0 + (l < cnt ? l : 0)];
+      ok[l] = h >= 0 && h < n_hits;
+      const float* hp = hits + (size_t)(ok[l] ? h : 0) * 9;
+      for (int e = 0; e < 3; ++e) msPar[e * NN + l] = ok[l] ? hp[e] : 0.f;
+      for (int e = 0; e < 6; ++e) msErr[e * NN + l] = ok[l] ? hp[3 + e] : 1.f;
+    }
+    int chg_u[NN];
+    std::memcpy(chg_u, chg, sizeof(chg));
+    prop_and_update(E, barrel, err, par, chg_u, msErr, msPar, cnt, oErr, oPar);
+    for (int l = 0; l < cnt; ++l) {
+      const size_t t = (size_t)t0 + l;
+      for (int k = 0; k < 6; ++k) ln->par[k * (size_t)n + t] = ok[l] ? oPar[k * NN + l] : par[k * NN + l];
+      for (int k = 0; k < 21; ++k) ln->err[k * (size_t)n + t] = ok[l] ? oErr[k * NN + l] : err[k * NN + l];
+      ln->chg[t] = ok[l] ? chg_u[l] : chg[l];
+    }
+  }
   return 0;
 }

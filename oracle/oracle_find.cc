@@ -94,11 +94,12 @@ void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock
   alignas(64) float errH[21 * NN], parH[6 * NN];
   const orc_layer_desc& d = E.desc[layer];
   const float* hits = E.hits + (size_t)E.hit_off[layer] * 9;
+  const int n_hits = E.hit_off[layer + 1] - E.hit_off[layer];
   float minChi2[NN];
   int bestHit[NN];
   int maxSize = 0;
   for (int it = 0; it < NN; ++it) {
-    if (it < n_proc && xsize[it] > 0) maxSize = std::max(maxSize, xsize[it]);
+    if (it < n_proc && xsize[it] > 0) maxSize = std::max(maxSize, std::min(xsize[it], kHitIdxMax));
     bestHit[it] = -1;
     minChi2[it] = dyn_chi2_cut(d, chi2_cut_min, parP[3 * NN + it], parP[5 * NN + it]);
   }
@@ -106,9 +107,13 @@ void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock
   alignas(64) float dErr[21 * NN], dPar[6 * NN];
   for (int l = 0; l < 3 * NN; ++l) msPar[l] = 0.f;
   for (int l = 0; l < 6 * NN; ++l) msErr[l] = 1.f;
+  // XHitArr entries outside [0, n_hits) are skipped.
+  auto valid = [&](int hc, int it) {
+    return hc < xsize[it] && xarr[hc * NN + it] >= 0 && xarr[hc * NN + it] < n_hits;
+  };
   for (int hc = 0; hc < maxSize; ++hc) {
     for (int it = 0; it < n_proc; ++it)
-      if (hc < xsize[it]) {
+      if (valid(hc, it)) {
         const float* h = hits + (size_t)xarr[hc * NN + it] * 9;
         for (int e = 0; e < 3; ++e) msPar[e * NN + it] = h[e];
         for (int e = 0; e < 6; ++e) msErr[e * NN + it] = h[3 + e];
@@ -125,7 +130,7 @@ void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock
     else
       kalman_operation_endcap(KFO_Calculate_Chi2, eC, pC, msErr, msPar, dErr, dPar, outChi2);
     for (int it = 0; it < n_proc; ++it)
-      if (hc < xsize[it]) {
+      if (valid(hc, it)) {
         const float c2 = std::abs(outChi2[it]);
         if (c2 < minChi2[it]) {
           minChi2[it] = c2;
@@ -343,6 +348,60 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
   for (int i = 1; i < n_threads; ++i) pool.emplace_back(work);
   work();
   for (auto& th : pool) th.join();
+  return 0;
+}
+
+int orc_add_best_hit(const orc_ctx* c, const orc_find_event* ev, int layer, const orc_finder_lanes* ln,
+                     const int* xsize, const int* xarr, const int* wsr, int* hit_code) {
+  if (!ev || !ln || layer < 0 || layer >= ev->n_layers || ln->n < 0) return 1;
+  EventLayers E;
+  E.n_layers = ev->n_layers;
+  E.desc = ev->layers;
+  E.hit_off = ev->hit_offset;
+  E.hits = ev->hits;
+  E.prop_to_hit = ev->prop_to_hit;
+  E.intra_pflags = ev->intra_pflags;
+  E.ctx.bfield = c ? c->bfield : 3.8112f;
+  E.ctx.rl_grid = c ? c->rl_grid : nullptr;
+  E.ctx.xi_grid = c ? c->xi_grid : nullptr;
+  const int n = ln->n;
+  for (int t0 = 0; t0 < n; t0 += NN) {
+    const int cnt = std::min(NN, n - t0);
+    auto lane = [&](int l) { return (size_t)t0 + (l < cnt ? l : 0); };
+    FindBlock B;
+    alignas(64) float parP[6 * NN], errP[21 * NN];
+    int xs[NN], xa[kHitIdxMax * NN], w[NN];
+    for (int k = 0; k < 6; ++k)
+      for (int l = 0; l < NN; ++l) parP[k * NN + l] = ln->prop_par[k * (size_t)n + lane(l)];
+    for (int k = 0; k < 21; ++k)
+      for (int l = 0; l < NN; ++l) errP[k * NN + l] = ln->prop_err[k * (size_t)n + lane(l)];
+    for (int k = 0; k < kHitIdxMax; ++k)
+      for (int l = 0; l < NN; ++l) xa[k * NN + l] = xarr[k * (size_t)n + lane(l)];
+    for (int l = 0; l < NN; ++l) {
+      const size_t t = lane(l);
+      B.chg[l] = ln->chg[t];
+      B.chi2[l] = ln->chi2[t];
+      B.nhits[l] = 0;
+      B.nfound[l] = ln->n_found[t];
+      B.ninside[l] = ln->n_inside_minus_one[t];
+      B.ntail[l] = ln->n_tail_minus_one[t];
+      B.stopped[l] = 0;
+      xs[l] = xsize[t];
+      w[l] = wsr[t];
+    }
+    add_best_hit(E, layer, ev->chi2_cut, B, parP, errP, xs, xa, w, cnt);
+    for (int l = 0; l < cnt; ++l) {
+      const size_t t = (size_t)t0 + l;
+      for (int k = 0; k < 6; ++k) ln->par[k * (size_t)n + t] = B.par[k * NN + l];
+      for (int k = 0; k < 21; ++k) ln->err[k * (size_t)n + t] = B.err[k * NN + l];
+      ln->chg[t] = B.chg[l];
+      ln->chi2[t] = B.chi2[l];
+      ln->n_found[t] = B.nfound[l];
+      ln->n_inside_minus_one[t] = B.ninside[l];
+      ln->n_tail_minus_one[t] = B.ntail[l];
+      hit_code[t] = B.nhits[l] > 0 ? B.hot_idx[0][l] : -9;
+    }
+  }
   return 0;
 }
 

@@ -123,3 +123,33 @@ def assert_parity(name, gpu, cpu32, cpu64, kind, tol, frac=0.999):
         assert np.percentile(d, q) <= 3 * np.percentile(c64, q) + tol / 10, (q, line)
         assert np.percentile(g64, q) <= 3 * np.percentile(c64, q) + tol / 10, (q, line)
     return line
+
+
+class RawDeviceBuffer:
+    """Device memory from hipMalloc itself, so that its allocation is exactly `nbytes` long.
+
+    Torch's caching allocator places small tensors inside larger segments, and the library's bounds
+    check sees the segment, not the tensor: a buffer meant to be too short for a call must come from
+    here, or the call would pass the check and write past the tensor."""
+
+    def __init__(self, nbytes):
+        import ctypes
+        self._hip = ctypes.CDLL("libamdhip64.so")
+        self._p = ctypes.c_void_p()
+        assert self._hip.hipMalloc(ctypes.byref(self._p), ctypes.c_size_t(int(nbytes))) == 0
+        self.nbytes = int(nbytes)
+
+    def data_ptr(self):
+        return self._p.value
+
+    def free(self):
+        if self._p.value:
+            self._hip.hipDeviceSynchronize()
+            self._hip.hipFree(self._p)
+            self._p = type(self._p)()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()

@@ -9,7 +9,7 @@ import pytest
 
 import mkfit_amd as mk
 import oracle
-from _common import par_rel
+from _common import RawDeviceBuffer, par_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -147,6 +147,7 @@ def test_ce_rejects_bad_arguments(gpu_device):
     with pytest.raises(mk.MkfError, match="first_layer"):
         ctx.find_tracks_clone_engine(eoh, b)
     b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, max_cands=6)
-    b.hot_idx = torch.zeros((6, 2, b.n), dtype=torch.int32, device=gpu_device)
-    with pytest.raises(mk.MkfError, match="hot_idx"):
-        ctx.find_tracks_clone_engine(eoh, b)
+    with RawDeviceBuffer(6 * 2 * b.n * 4) as short:               # room for 2 of the 17 layers
+        b.hot_idx = short
+        with pytest.raises(mk.MkfError, match="hot_idx"):
+            ctx.find_tracks_clone_engine(eoh, b)

@@ -10,7 +10,7 @@ import pytest
 
 import mkfit_amd as mk
 import oracle
-from _common import par_rel
+from _common import RawDeviceBuffer, par_rel
 from _find_common import F_PHI_FINE, _brute_select, _keys, _random_hits, edge_explained
 
 pytestmark = pytest.mark.gpu
@@ -235,9 +235,10 @@ def test_find_rejects_bad_arguments(setup, gpu_device):
     with pytest.raises(mk.MkfError, match="first_layer"):
         ctx.find_tracks_best_hit(eoh, b)
     b = mk.FindBatch(ev, len(descs), 3, device=gpu_device)
-    b.hot_idx = torch.zeros((2, b.n), dtype=torch.int32, device=gpu_device)   # too short for 17 layers
-    with pytest.raises(mk.MkfError, match="hot_idx"):
-        ctx.find_tracks_best_hit(eoh, b)
+    with RawDeviceBuffer(2 * b.n * 4) as short:                  # too short for 17 layers
+        b.hot_idx = short
+        with pytest.raises(mk.MkfError, match="hot_idx"):
+            ctx.find_tracks_best_hit(eoh, b)
     fresh = mk.EventOfHits(ctx, descs)
     with pytest.raises(mk.MkfError, match="not loaded"):
         ctx.find_tracks_best_hit(fresh, mk.FindBatch(ev, len(descs), 3, device=gpu_device))

@@ -1,0 +1,218 @@
+"""GPU parity of the per-layer MkFinder steps (mkf_add_best_hit, mkf_find_candidates_clone_engine,
+mkf_update_with_last_hit) against the oracle's, on identical inputs, and of a best-hit layer loop
+driven from the host through the per-layer entry points against the oracle's FindTracksBestHit.
+
+Integer results (hit codes, record lists) must agree exactly except where a float32 rounding
+difference decides a comparison (two chi2 equal to rounding, a chi2 on the cut); those lanes are
+counted and bounded.  Floating-point results: parameters 1e-5 and covariances 1e-4 relative at the
+99th percentile, as for the fit.
+"""
+import numpy as np
+import pytest
+
+import mkfit_amd as mk
+import oracle
+from _common import RawDeviceBuffer, cov_rel, par_rel
+from test_oracle_finder_layer import _lanes_at
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.fixture(scope="module")
+def setup(gpu_device):
+    geo = mk.geometry(1)
+    p = mk.default_toy_params(seed=321, simulate_material=0, pt_min=0.5, pt_max=10.0)
+    ev = mk.generate_find_event(geo, p, 4000, noise_per_cm2=3.0)
+    descs = mk.barrel_layer_descs(geo)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+    return ev, descs, ctx, eoh
+
+
+def _layer_hits(ev, layer):
+    off = ev["hit_offset"]
+    return ev["hits_flat"][off[layer]:off[layer + 1]]
+
+
+def _gpu_lanes(ln, dev):
+    return mk.FinderLanes(_t(ln["prop_par"], dev), _t(ln["prop_err"], dev), _t(ln["chg"], dev),
+                          chi2=_t(ln["chi2"], dev) if "chi2" in ln else None,
+                          n_found=_t(ln["n_found"], dev) if "n_found" in ln else None,
+                          n_inside_minus_one=_t(ln["n_inside_minus_one"], dev) if "n_found" in ln else None,
+                          n_tail_minus_one=_t(ln["n_tail_minus_one"], dev) if "n_found" in ln else None)
+
+
+def _state_close(gp, ge, cp, ce):
+    pr = par_rel(gp, cp)
+    cr = cov_rel(ge, ce)
+    assert np.percentile(pr, 99) < 1e-5 and pr.max() < 1e-3, (np.percentile(pr, 99), pr.max())
+    assert np.percentile(cr, 99) < 1e-4, np.percentile(cr, 99)
+
+
+@pytest.mark.parametrize("layer", [3, 8])
+def test_add_best_hit_matches_oracle(setup, gpu_device, layer):
+    import torch
+    ev, descs, ctx, eoh = setup
+    O = oracle.Oracle(bfield=3.8)
+    ln = _lanes_at(O, ev, descs, layer, seed=layer)
+    xs, xa, w = oracle.select_hit_indices(descs[layer], _layer_hits(ev, layer), ln["prop_par"], ln["prop_err"])
+    cpu = oracle.add_best_hit(ev, descs, layer, ln, xs, xa, w, chi2_cut=15.0, bfield=3.8)
+    g = _gpu_lanes(ln, gpu_device)
+    code = torch.zeros(g.n, dtype=torch.int32, device=gpu_device)
+    ctx.add_best_hit(eoh, layer, g, _t(xs, gpu_device), _t(xa, gpu_device), _t(w, gpu_device), code, chi2_cut=15.0)
+    torch.cuda.synchronize()
+    gn = g.numpy()
+    gc = code.cpu().numpy()
+    same = gc == cpu["hit_code"]
+    assert same.mean() >= 0.998, (same.mean(), np.nonzero(~same)[0][:8])
+    for k in ("chg", "n_found", "n_inside_minus_one", "n_tail_minus_one"):
+        assert np.array_equal(gn[k][same], cpu[k][same]), k
+    _state_close(gn["par"][:, same], gn["err"][:, same], cpu["par"][:, same], cpu["err"][:, same])
+    dc = np.abs(gn["chi2"][same] - cpu["chi2"][same]) / np.maximum(1.0, cpu["chi2"][same])
+    assert np.percentile(dc, 99) < 1e-4
+    assert (cpu["hit_code"] >= 0).mean() > 0.5 and (cpu["hit_code"] < 0).any()
+
+
+@pytest.mark.parametrize("layer", [3, 8])
+def test_find_candidates_matches_oracle(setup, gpu_device, layer):
+    import torch
+    ev, descs, ctx, eoh = setup
+    O = oracle.Oracle(bfield=3.8)
+    ln = _lanes_at(O, ev, descs, layer, seed=10 + layer)
+    xs, xa, w = oracle.select_hit_indices(descs[layer], _layer_hits(ev, layer), ln["prop_par"], ln["prop_err"])
+    ne_c, ent_c = oracle.find_candidates_ce(ev, descs, layer, ln, xs, xa, w, chi2_cut=30.0, max_holes=3,
+                                            max_consec_holes=1, bfield=3.8)
+    g = _gpu_lanes(ln, gpu_device)
+    n = g.n
+    ne = torch.zeros(n, dtype=torch.int32, device=gpu_device)
+    ent = torch.zeros(n * mk.CE_ENTRIES_PER_LANE * mk.IDX_CHI2_DTYPE.itemsize, dtype=torch.uint8,
+                      device=gpu_device)
+    ctx.find_candidates_clone_engine(eoh, layer, g, _t(xs, gpu_device), _t(xa, gpu_device), _t(w, gpu_device),
+                                     ne, ent, chi2_cut=30.0, max_holes=3, max_consec_holes=1)
+    torch.cuda.synchronize()
+    ne_g = ne.cpu().numpy()
+    ent_g = ent.cpu().numpy().view(mk.IDX_CHI2_DTYPE).reshape(n, mk.CE_ENTRIES_PER_LANE)
+    ints = ("trk_idx", "hit_idx", "module", "nhits", "ntailholes", "noverlaps", "nholes", "xhit_pos")
+    same = ne_g == ne_c
+    for t in np.nonzero(same)[0]:
+        for k in ints:
+            if not np.array_equal(ent_g[t, :ne_g[t]][k], ent_c[t, :ne_c[t]][k]):
+                same[t] = False
+    assert same.mean() >= 0.998, (same.mean(), np.nonzero(~same)[0][:8])
+    m = np.zeros(ent_c.shape, bool)
+    for t in np.nonzero(same)[0]:
+        m[t, :ne_c[t]] = True
+    for k in ("pt", "chi2", "chi2_hit", "score"):
+        d = np.abs(ent_g[k][m] - ent_c[k][m]) / np.maximum(1.0, np.abs(ent_c[k][m]))
+        assert np.percentile(d, 99) < 1e-4 and d.max() < 1e-2, (k, np.percentile(d, 99), d.max())
+    assert (ent_c["hit_idx"][m] >= 0).sum() > n // 2
+
+
+def test_update_with_last_hit_matches_oracle(setup, gpu_device):
+    import torch
+    ev, descs, ctx, eoh = setup
+    n = ev["n_seeds"]
+    layer = 3
+    hl = _layer_hits(ev, layer)
+    par = ev["seed_par"][:, :n].astype(F); err = ev["seed_err"][:, :n].astype(F)
+    chg = ev["seed_chg"][:n].astype(np.int32)
+    rng = np.random.default_rng(4)
+    hit = rng.integers(0, len(hl), n).astype(np.int32)
+    hit[::7] = -1
+    hit[3::11] = len(hl) + 3
+    ln = dict(prop_par=par, prop_err=err, chg=chg)
+    p_c, e_c, c_c = oracle.update_with_last_hit(ev, descs, layer, ln, hit, pflags=0, bfield=3.8)
+    g = _gpu_lanes(ln, gpu_device)
+    ctx.update_with_last_hit(eoh, layer, g, _t(hit, gpu_device), prop_to_hit=True, pflags=0)
+    torch.cuda.synchronize()
+    gn = g.numpy()
+    assert np.array_equal(gn["chg"], c_c)
+    _state_close(gn["par"], gn["err"], p_c, e_c)
+    ok = (hit >= 0) & (hit < len(hl))
+    assert np.array_equal(gn["par"][:, ~ok], par[:, ~ok]) and np.array_equal(gn["err"][:, ~ok], err[:, ~ok])
+
+
+def test_best_hit_loop_through_layer_steps(setup, gpu_device):
+    """The host drives MkBuilder's best-hit layer loop (MkBuilder.cc:1395-1478) over the per-layer
+    entry points; the result must match the oracle's whole-event FindTracksBestHit."""
+    import torch
+    ev, descs, ctx, eoh = setup
+    dev = gpu_device
+    n = ev["n_seeds"]
+    L = len(descs)
+    first = 3
+    par = _t(ev["seed_par"][:, :n].astype(F), dev); err = _t(ev["seed_err"][:, :n].astype(F), dev)
+    chg = _t(ev["seed_chg"][:n].astype(np.int32), dev)
+    hot = np.full((L, n), -9, np.int32)
+    hot[:first] = ev["seed_hits"][:first, :n]
+    g = mk.FinderLanes(par, err, chg, n_found=_t((hot[:first] >= 0).sum(0).astype(np.int32), dev))
+    out_par, out_err = par.clone(), err.clone()
+    out_chg, out_chi2, out_nf = chg.clone(), g.chi2.clone(), g.n_found.clone()
+    active = torch.ones(n, dtype=torch.bool, device=dev)
+    pe = torch.empty_like(err); pp = torch.empty_like(par)
+    xs = torch.empty(n, dtype=torch.int32, device=dev); xa = torch.empty((16, n), dtype=torch.int32, device=dev)
+    w = torch.empty(n, dtype=torch.int32, device=dev); code = torch.empty(n, dtype=torch.int32, device=dev)
+    cur_par, cur_err = par.clone(), err.clone()
+    for layer in range(first, L):
+        d = descs[layer]
+        ctx.propagate_helix_to_r(cur_err, cur_par, g.chg, torch.full((n,), d["propagate_to"], device=dev), pe, pp)
+        reach = pp[0] * pp[0] + pp[1] * pp[1] >= F(d["rin"]) * F(d["rin"])
+        stop = active & ~reach                                 # stops with the state it had
+        out_par[:, stop], out_err[:, stop] = cur_par[:, stop], cur_err[:, stop]
+        out_chg[stop], out_chi2[stop], out_nf[stop] = g.chg[stop], g.chi2[stop], g.n_found[stop]
+        active &= reach
+        g.prop_par, g.prop_err = pp, pe
+        eoh.select_hit_indices(layer, pp, pe, xs, xa, w)
+        ctx.add_best_hit(eoh, layer, g, xs, xa, w, code, chi2_cut=15.0)
+        cur_par, cur_err = g.par.clone(), g.err.clone()
+        c = code.cpu().numpy()
+        act = active.cpu().numpy()
+        hot[layer, act] = c[act]
+        stop = active & (code == -2)                           # stops after its -2
+        out_par[:, stop], out_err[:, stop] = cur_par[:, stop], cur_err[:, stop]
+        out_chg[stop], out_chi2[stop], out_nf[stop] = g.chg[stop], g.chi2[stop], g.n_found[stop]
+        active &= code != -2
+    out_par[:, active], out_err[:, active] = cur_par[:, active], cur_err[:, active]
+    out_chg[active], out_chi2[active], out_nf[active] = g.chg[active], g.chi2[active], g.n_found[active]
+    torch.cuda.synchronize()
+    gpu = dict(par=out_par.cpu().numpy(), chg=out_chg.cpu().numpy(), chi2=out_chi2.cpu().numpy(), hot_idx=hot,
+               n_found=out_nf.cpu().numpy())
+    cpu = oracle.find_best_hit(ev, descs, first_layer=first, chi2_cut=15.0, bfield=3.8, n_threads=8)
+    same = np.all(gpu["hot_idx"] == cpu["hot_idx"], axis=0)
+    assert same.mean() >= 0.99, (same.mean(), np.nonzero(~same)[0][:8])
+    assert np.array_equal(gpu["n_found"][same], cpu["n_found"][same])
+    assert np.array_equal(gpu["chg"][same], cpu["chg"][same])
+    pr = par_rel(gpu["par"][:, same], cpu["par"][:, same])
+    assert np.percentile(pr, 99) < 1e-4 and pr.max() < 1e-2, (np.percentile(pr, 99), pr.max())
+
+
+def test_layer_steps_reject_bad_arguments(setup, gpu_device):
+    import torch
+    ev, descs, ctx, eoh = setup
+    O = oracle.Oracle(bfield=3.8)
+    ln = _lanes_at(O, ev, descs, 3)
+    g = _gpu_lanes(ln, gpu_device)
+    n = g.n
+    xs = torch.zeros(n, dtype=torch.int32, device=gpu_device)
+    xa = torch.zeros((16, n), dtype=torch.int32, device=gpu_device)
+    w = torch.zeros(n, dtype=torch.int32, device=gpu_device)
+    code = torch.zeros(n, dtype=torch.int32, device=gpu_device)
+    with pytest.raises(mk.MkfError, match="layer"):
+        ctx.add_best_hit(eoh, len(descs), g, xs, xa, w, code)
+    with RawDeviceBuffer(8 * n * 4) as short:
+        with pytest.raises(mk.MkfError, match="xarr"):
+            ctx.add_best_hit(eoh, 3, g, xs, short, w, code)
+    ne = torch.zeros(n, dtype=torch.int32, device=gpu_device)
+    with RawDeviceBuffer(n * 48) as short:
+        with pytest.raises(mk.MkfError, match="entries"):
+            ctx.find_candidates_clone_engine(eoh, 3, g, xs, xa, w, ne, short)
+    with RawDeviceBuffer(n // 2 * 4) as short:
+        with pytest.raises(mk.MkfError, match="hit_idx"):
+            ctx.update_with_last_hit(eoh, 3, g, short)
