@@ -4,9 +4,12 @@
 //
 //   MkFitter::InputTracksForFit / FitTracks / OutputFittedTracks  (MkFitter.h:57-88)
 //   EventOfHits::SuckInHits                                       (HitStructures.h:280-319)
-//   MkFinder::SelectHitIndices, FindTracksBestHit,
-//            FindTracksCloneEngine, OutputTracksAndHitIdx          (MkFinder.h:126-228,
-//                                                                   MkBuilder.cc:1362-2060)
+//   MkFinder::FindTracksBestHit, FindTracksCloneEngine,
+//            OutputTracksAndHitIdx                                 (MkBuilder.cc:1362-2060)
+//   MkFinder per-layer steps: PropagateTracksToR/Z, SelectHitIndices,
+//            AddBestHit, FindCandidatesCloneEngine, UpdateWithLastHit
+//                                                                  (MkBase.h:14-106,
+//                                                                   MkFinder.h:126-228)
 //
 // Records mirror the reference's: TrackState (Track.h:74-143: x, y, z, 1/pT,
 // phi, theta; packed 6x6 covariance; charge), Hit (Hit.h:165-183: position and
@@ -86,6 +89,10 @@ class DeviceArray {
   }
   T* get() const { return p_; }
   size_t size() const { return n_; }
+  void swap(DeviceArray& o) {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+  }
 
  private:
   void release() {
@@ -321,6 +328,134 @@ class MkFinder {
     K_ = max_cands;
   }
 
+  // ---- per-layer steps, for callers that run the layer loop themselves --------
+  // The lanes are the seeds of InputTracksAndHitIdx.  As in the reference,
+  // Par[iC] is a lane's current state and Par[iP] its state propagated to the
+  // layer (MkBase.h:14-106).
+
+  // Lanes <- seeds: Par[iC] = seed state, chi2 0, NFoundHits = seed hits found.
+  void BeginLayerSteps() {
+    const size_t n = n_;
+    hipStream_t st = c_.stream();
+    std::vector<int> hot, nf(n, 0);
+    hot_seed_.download(hot, st);
+    c_.sync();
+    lane_hots_.assign(n, {});
+    for (size_t t = 0; t < n; ++t)
+      for (int l = 0; l < first_; ++l) {
+        const int h = hot[(size_t)l * n + t];
+        lane_hots_[t].push_back({h, l});
+        nf[t] += h >= 0;
+      }
+    cur_par_.resize(6 * n); cur_err_.resize(21 * n); prop_par_.resize(6 * n); prop_err_.resize(21 * n);
+    lane_chg_.resize(n); lane_chi2_.resize(n);
+    if (n) {
+      check_hip(hipMemcpyAsync(cur_par_.get(), par_.get(), 6 * n * 4, hipMemcpyDeviceToDevice, st), "copy");
+      check_hip(hipMemcpyAsync(cur_err_.get(), err_.get(), 21 * n * 4, hipMemcpyDeviceToDevice, st), "copy");
+      check_hip(hipMemcpyAsync(lane_chg_.get(), chg_.get(), n * 4, hipMemcpyDeviceToDevice, st), "copy");
+    }
+    lane_chi2_.upload(std::vector<float>(n, 0.f), st);
+    lane_nf_.upload(nf, st);
+    lane_ninside_.upload(std::vector<int>(n, 0), st);
+    lane_ntail_.upload(std::vector<int>(n, 0), st);
+    xsize_.resize(n); xarr_.resize((size_t)MKF_HIT_IDX_MAX * n); wsr_.resize(n); code_.resize(n);
+    c_.sync();
+  }
+
+  // MkBase::PropagateTracksToR / ToZ with one target for all lanes: Par[iC] -> Par[iP].
+  void PropagateTracksToR(float r, int pflags = 0) {
+    target_.upload(std::vector<float>(n_, r), c_.stream());
+    check(mkf_propagate_helix_to_r(c_.get(), n_, cur_err_.get(), cur_par_.get(), lane_chg_.get(), target_.get(),
+                                   prop_err_.get(), prop_par_.get(), pflags, nullptr, nullptr, c_.stream()));
+  }
+  void PropagateTracksToZ(float z, int pflags = 0) {
+    target_.upload(std::vector<float>(n_, z), c_.stream());
+    check(mkf_propagate_helix_to_z(c_.get(), n_, cur_err_.get(), cur_par_.get(), lane_chg_.get(), target_.get(),
+                                   prop_err_.get(), prop_par_.get(), pflags, nullptr, c_.stream()));
+  }
+
+  // Par[iP] of every lane, element-major [6][n].
+  void PropagatedParameters(std::vector<float>& par) {
+    prop_par_.download(par, c_.stream());
+    c_.sync();
+  }
+
+  // Hit candidates of every lane on `layer` from Par[iP].
+  void SelectHitIndices(EventOfHits& eoh, int layer) {
+    check(mkf_select_hit_indices(eoh.get(), layer, n_, prop_par_.get(), prop_err_.get(), xsize_.get(), xarr_.get(),
+                                 wsr_.get(), c_.stream()));
+  }
+
+  // Par[iP] -> Par[iC] with the best hit or a fake hit; returns each lane's
+  // HitOnTrack index for the layer (-9 when none was added).
+  const std::vector<int>& AddBestHit(EventOfHits& eoh, int layer, float chi2_cut, bool prop_to_hit = false,
+                                     int intra_pflags = 0) {
+    mkf_finder_lanes ln = lanes(true);
+    check(mkf_add_best_hit(c_.get(), eoh.get(), layer, &ln, xsize_.get(), xarr_.get(), wsr_.get(), chi2_cut,
+                           prop_to_hit, intra_pflags, code_.get(), c_.stream()));
+    code_.download(codes_, c_.stream());
+    c_.sync();
+    for (int t = 0; t < n_; ++t)
+      if (codes_[t] != -9) lane_hots_[t].push_back({codes_[t], layer});
+    return codes_;
+  }
+
+  // IdxChi2List records of every lane for the layer, from Par[iP].
+  void FindCandidatesCloneEngine(EventOfHits& eoh, int layer, float chi2_cut,
+                                 std::vector<std::vector<mkf_idx_chi2>>& records, int max_holes = 4,
+                                 int max_consec_holes = 1, bool prop_to_hit = true, int intra_pflags = 0) {
+    const size_t n = n_;
+    n_entries_.resize(n);
+    entries_.resize(n * MKF_CE_ENTRIES_PER_LANE);
+    mkf_finder_lanes ln = lanes(false);
+    check(mkf_find_candidates_clone_engine(c_.get(), eoh.get(), layer, &ln, xsize_.get(), xarr_.get(), wsr_.get(),
+                                           chi2_cut, prop_to_hit, intra_pflags, max_holes, max_consec_holes,
+                                           n_entries_.get(), entries_.get(), c_.stream()));
+    std::vector<int> ne;
+    std::vector<mkf_idx_chi2> all;
+    n_entries_.download(ne, c_.stream());
+    entries_.download(all, c_.stream());
+    c_.sync();
+    records.assign(n, {});
+    for (size_t t = 0; t < n; ++t)
+      records[t].assign(all.begin() + t * MKF_CE_ENTRIES_PER_LANE,
+                        all.begin() + t * MKF_CE_ENTRIES_PER_LANE + ne[t]);
+  }
+
+  // Par[iC] updated with hit hit_idx[lane] of the layer (lanes with < 0 keep theirs).
+  void UpdateWithLastHit(EventOfHits& eoh, int layer, const std::vector<int>& hit_idx, bool prop_to_hit = true,
+                         int pflags = 0) {
+    target_idx_.upload(hit_idx, c_.stream());
+    mkf_finder_lanes ln = lanes(false);
+    ln.prop_par = cur_par_.get(); ln.prop_err = cur_err_.get();
+    ln.par = prop_par_.get(); ln.err = prop_err_.get();
+    check(mkf_update_with_last_hit(c_.get(), eoh.get(), layer, &ln, target_idx_.get(), prop_to_hit, pflags,
+                                   c_.stream()));
+    cur_par_.swap(prop_par_);
+    cur_err_.swap(prop_err_);
+  }
+
+  // Each lane's current state, chi2, found-hit count and HitOnTrack list.
+  void OutputLanes(std::vector<Track>& tracks) {
+    std::vector<float> par, err, chi2;
+    std::vector<int> chg, nf;
+    hipStream_t st = c_.stream();
+    cur_par_.download(par, st); cur_err_.download(err, st); lane_chg_.download(chg, st);
+    lane_chi2_.download(chi2, st); lane_nf_.download(nf, st);
+    c_.sync();
+    const size_t n = n_;
+    tracks.assign(n, Track());
+    for (size_t t = 0; t < n; ++t) {
+      Track& tr = tracks[t];
+      for (int e = 0; e < 6; ++e) tr.state.par[e] = par[e * n + t];
+      for (int e = 0; e < 21; ++e) tr.state.err[e] = err[e * n + t];
+      tr.state.charge = chg[t];
+      tr.chi2 = chi2[t];
+      tr.n_found = nf[t];
+      tr.hots = lane_hots_[t];
+    }
+  }
+
   // The best candidate of every seed (OutputTracksAndHitIdx / the first
   // candidate after MergeCandsAndBestShortOne), hits in layer order.
   void OutputTracksAndHitIdx(std::vector<Track>& tracks) {
@@ -349,10 +484,28 @@ class MkFinder {
   }
 
  private:
+  mkf_finder_lanes lanes(bool writes_current) {
+    mkf_finder_lanes ln{};
+    ln.n = n_;
+    ln.prop_par = prop_par_.get(); ln.prop_err = prop_err_.get();
+    ln.par = writes_current ? cur_par_.get() : nullptr;
+    ln.err = writes_current ? cur_err_.get() : nullptr;
+    ln.chg = lane_chg_.get(); ln.chi2 = lane_chi2_.get(); ln.n_found = lane_nf_.get();
+    ln.n_inside_minus_one = lane_ninside_.get(); ln.n_tail_minus_one = lane_ntail_.get();
+    return ln;
+  }
+
   Context& c_;
   int n_ = 0, first_ = 0, L_ = 0, K_ = 1;
   DeviceArray<float> par_, err_, out_par_, out_err_, out_chi2_, out_score_;
   DeviceArray<int> chg_, hot_seed_, seed_hits_, out_chg_, out_nf_, out_hot_, out_nc_;
+  // per-layer step state
+  DeviceArray<float> cur_par_, cur_err_, prop_par_, prop_err_, lane_chi2_, target_;
+  DeviceArray<int> lane_chg_, lane_nf_, lane_ninside_, lane_ntail_, xsize_, xarr_, wsr_, code_, n_entries_,
+      target_idx_;
+  DeviceArray<mkf_idx_chi2> entries_;
+  std::vector<int> codes_;
+  std::vector<std::vector<HitOnTrack>> lane_hots_;
 };
 
 }  // namespace mkfit_amd

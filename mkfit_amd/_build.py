@@ -46,14 +46,18 @@ def build_toymc():
 
 
 def build_shim_demo():
-    """tests/cpp/fit_shim_demo: the C++ interface (include/mkfit_amd.hpp) driven like MkFitter."""
-    out = os.path.join(ROOT, "tests", "cpp", "fit_shim_demo")
-    _run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
-          "-I", "/opt/rocm/include", "-o", out + ".tmp", os.path.join(ROOT, "tests", "cpp", "fit_shim_demo.cc"),
-          "-L", PKG, "-lmkfit_amd", "-lmkfit_toymc", "-L", "/opt/rocm/lib", "-lamdhip64",
-          "-Wl,-rpath," + PKG, "-Wl,-rpath,/opt/rocm/lib"])
-    os.replace(out + ".tmp", out)
-    return out
+    """tests/cpp/{fit,find}_shim_demo: the C++ interface (include/mkfit_amd.hpp) driven like MkFitter and
+    like MkBuilder's best-hit layer loop."""
+    outs = []
+    for name in ("fit_shim_demo", "find_shim_demo"):
+        out = os.path.join(ROOT, "tests", "cpp", name)
+        _run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+              "-I", "/opt/rocm/include", "-o", out + ".tmp", os.path.join(ROOT, "tests", "cpp", name + ".cc"),
+              "-L", PKG, "-lmkfit_amd", "-lmkfit_toymc", "-L", "/opt/rocm/lib", "-lamdhip64",
+              "-Wl,-rpath," + PKG, "-Wl,-rpath,/opt/rocm/lib"])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_oracle():
