@@ -1,6 +1,7 @@
 """Build recipe for the native libraries (no cmake; direct hipcc / g++ lines).
 
-  mkfit_amd/libmkfit_amd.so    HIP kernels + C ABI (include/mkfit_amd.h), gfx950
+  mkfit_amd/libmkfit_amd.so    HIP kernels + C ABI (include/mkfit_amd.h), gfx950, and the event-file
+                               reader/writer (include/mkfit_event_file.h, host code)
   mkfit_amd/libmkfit_toymc.so  toy Monte Carlo generator (include/mkfit_toymc.h), host C++
   oracle/liborc_avx512.so      CPU oracle, NN=16 (test infrastructure)
   oracle/liborc_avx2.so        CPU oracle, NN=8  (test infrastructure)
@@ -31,7 +32,7 @@ def build_hip():
     out = os.path.join(PKG, "libmkfit_amd.so")
     srcs = [os.path.join(CSRC, "mkf_fit.hip"), os.path.join(CSRC, "mkf_find.hip"),
             os.path.join(CSRC, "mkf_ce.hip"), os.path.join(CSRC, "mkf_finder_layer.hip"),
-            os.path.join(CSRC, "mkf_adjoint.hip")]
+            os.path.join(CSRC, "mkf_adjoint.hip"), os.path.join(CSRC, "event_file.cc")]
     _run([HIPCC] + HIP_FLAGS + ["-o", out + ".tmp"] + srcs)
     os.replace(out + ".tmp", out)
     return out

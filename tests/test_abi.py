@@ -17,6 +17,7 @@ def _declared(header, prefix):
 @pytest.mark.parametrize("header,lib,prefix", [
     ("include/mkfit_amd.h", "mkfit_amd/libmkfit_amd.so", "mkf_"),
     ("include/mkfit_toymc.h", "mkfit_amd/libmkfit_toymc.so", "mkf_toymc_"),
+    ("include/mkfit_event_file.h", "mkfit_amd/libmkfit_amd.so", "mkf_event_file_"),
 ])
 def test_exports_every_declared_symbol(header, lib, prefix):
     names = _declared(header, prefix)
@@ -65,11 +66,15 @@ def test_null_arguments_rejected():
     assert L.mkf_find_tracks_clone_engine(None, None, None, None) == 1
     assert L.mkf_fit_adjoint(None, None, None) == 1
     assert L.mkf_event_of_hits_destroy(None) == 0
+    assert L.mkf_add_best_hit(None, None, 0, None, None, None, None, ctypes.c_float(15.0), 0, 0, None, None) == 1
+    assert L.mkf_update_with_last_hit(None, None, 0, None, None, 1, 0, None) == 1
     assert L.mkf_num_batched_variants() >= 1
 
 
 def test_struct_layouts():
     import mkfit_amd as mk
+    assert ctypes.sizeof(mk.MkfFinderLanes) == 8 + 9 * 8                 # n + pad, 9 pointers
+    assert mk.IDX_CHI2_DTYPE.itemsize == 48
     assert ctypes.sizeof(mk.MkfLayerDesc) == 23 * 4
     assert ctypes.sizeof(mk.MkfFindBatch) == 4 + 4 + 6 * 8 + 5 * 4 + 4   # n, pad, 6 pointers, 5 scalars, pad
     assert ctypes.sizeof(mk.MkfCEBatch) == 8 + 4 * 8 + 8 * 4 + 8 * 8      # n + pad, 4 ptrs, 8 scalars, 8 ptrs
