@@ -123,7 +123,7 @@ class MkfAdjointBatch(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("n_layers", ctypes.c_int)] + [
         (k, _c_void_p) for k in ("seed_par", "seed_err", "seed_chg", "hit_pos", "hit_err", "n_hits", "theta")] + [
         ("pflags", ctypes.c_int), ("xi_per_x0", ctypes.c_float), ("checkpoint_every", ctypes.c_int),
-        ("chi2", _c_void_p), ("grad", _c_void_p)]
+        ("chi2", _c_void_p), ("grad", _c_void_p), ("fp64", ctypes.c_int)]
 
 
 class MkfFinderLanes(ctypes.Structure):
@@ -472,6 +472,9 @@ class FindBatch:
         self.err = torch.from_numpy(np.ascontiguousarray(ev["seed_err"][:, :n], np.float32)).to(dev)
         self.chg = torch.from_numpy(np.ascontiguousarray(ev["seed_chg"][:n], np.int32)).to(dev)
         self.chi2 = torch.zeros(n, dtype=torch.float32, device=dev)
+        if ev["seed_hits"].shape[0] < first_layer:
+            raise ValueError("seed_hits has %d rows; first_layer %d needs one per layer before it"
+                             % (ev["seed_hits"].shape[0], first_layer))
         hot = np.full((n_layers, n), -9, np.int32)
         hot[:first_layer] = ev["seed_hits"][:first_layer, :n]
         self.hot_idx = torch.from_numpy(hot).to(dev)
@@ -515,6 +518,9 @@ class CEBatch:
         self.seed_par = f(ev["seed_par"][:, :n], np.float32)
         self.seed_err = f(ev["seed_err"][:, :n], np.float32)
         self.seed_chg = f(ev["seed_chg"][:n], np.int32)
+        if ev["seed_hits"].shape[0] < first_layer:
+            raise ValueError("seed_hits has %d rows; first_layer %d needs one per layer before it"
+                             % (ev["seed_hits"].shape[0], first_layer))
         self.seed_hits = f(ev["seed_hits"][:first_layer, :n], np.int32)
         z = lambda shape, dt: torch.zeros(shape, dtype=dt, device=dev)
         self.n_cands = z(n, torch.int32)
@@ -554,7 +560,8 @@ class AlignmentData:
     Slot l of every track must be its hit on layer l (the barrel generator's layout).
     """
 
-    def __init__(self, ctx, ev, device="cuda:0", pflags=PF_APPLY_MATERIAL, xi_per_x0=2.2e-3, checkpoint_every=5):
+    def __init__(self, ctx, ev, device="cuda:0", pflags=PF_APPLY_MATERIAL, xi_per_x0=2.2e-3, checkpoint_every=5,
+                 fp64=False):
         import torch
         dev = torch.device(device)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
@@ -566,6 +573,7 @@ class AlignmentData:
         self.chi2 = torch.zeros(self.n, dtype=torch.float32, device=dev)
         self.grad = torch.zeros((self.n_layers, 3), dtype=torch.float32, device=dev)
         self.pflags, self.xi_per_x0, self.checkpoint_every = int(pflags), float(xi_per_x0), int(checkpoint_every)
+        self.fp64 = int(bool(fp64))
 
     def run(self, theta, stream=None):
         """Per-track chi2 [n] and d(sum chi2)/d theta [n_layers, 3] at theta (device float32 [n_layers, 3])."""
@@ -576,6 +584,7 @@ class AlignmentData:
         b.hit_pos, b.hit_err, b.n_hits, b.theta = _ptr(self.hit_pos), _ptr(self.hit_err), _ptr(self.n_hits), _ptr(th)
         b.pflags, b.xi_per_x0, b.checkpoint_every = self.pflags, self.xi_per_x0, self.checkpoint_every
         b.chi2, b.grad = _ptr(self.chi2), _ptr(self.grad)
+        b.fp64 = self.fp64
         _check(lib().mkf_fit_adjoint(self.ctx.h, ctypes.byref(b), self.ctx._stream(stream)))
         return self.chi2.clone(), self.grad.clone()
 

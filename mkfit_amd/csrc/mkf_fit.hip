@@ -189,55 +189,60 @@ __device__ __forceinline__ bool ill_conditioned(const float (&C)[21]) {
 // The updated covariance in the Joseph form (I - KH) C (I - KH)^T + K V K^T,
 // evaluated in double from the FP32 pre-update covariance C and rounded to
 // float; H is the barrel (tangent-plane rotation and z) or endcap (x, y)
-// projection and V the hit covariance projected the same way.  Symmetric and
-// positive semi-definite by construction for a positive semi-definite C.
-__device__ __attribute__((noinline)) void joseph_fp64(const float* __restrict__ pre, float px, float py,
-                                                      const float* __restrict__ mp, const float* __restrict__ me,
-                                                      bool endcap, float* __restrict__ out) {
-  double C[6][6];
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) C[i][j] = pre[S(i, j)];
+// projection, two rows h0 and h1, and V the hit covariance projected the same
+// way.  Symmetric and positive semi-definite by construction for a positive
+// semi-definite C.  With CH = C H^T (two columns), K = CH S^-1 and
+// s_ab = h_a . CH_b, the product expands to rank-2 corrections of C:
+//   AC   = C - K0 CH0^T - K1 CH1^T                       (A = I - K H)
+//   AC h0 = CH0 - K0 s00 - K1 s01,  AC h1 = CH1 - K0 s01 - K1 s11
+//   C'   = AC - (AC h0) K0^T - (AC h1) K1^T + K V K^T
+// so no 6x6 temporary is formed and the routine stays inline at a modest
+// register cost (about 50 VGPRs of doubles).
+__device__ __forceinline__ void joseph_fp64(const float (&pre)[21], float px, float py, const float (&mp)[3],
+                                            const float (&me)[6], bool endcap, float (&out)[21]) {
   const double vxx = me[0], vyx = me[1], vyy = me[2], vzx = me[3], vzy = me[4], vzz = me[5];
-  double h0[6] = {0, 0, 0, 0, 0, 0}, h1[6] = {0, 0, 0, 0, 0, 0}, V00, V01, V11;
+  double a00 = 1.0, a01 = 0.0, V00, V01, V11;
   if (endcap) {
-    h0[0] = 1; h1[1] = 1;
     V00 = vxx; V01 = vyx; V11 = vyy;
   } else {
     const double mx = mp[0], my = mp[1];
     const double r = __builtin_sqrt(mx * mx + my * my);
-    const double a00 = -(my + (double)py) / (2 * r), a01 = (mx + (double)px) / (2 * r);
-    h0[0] = a00; h0[1] = a01; h1[2] = 1;
+    a00 = -(my + (double)py) / (2 * r);
+    a01 = (mx + (double)px) / (2 * r);
     V00 = a00 * a00 * vxx + 2 * a00 * a01 * vyx + a01 * a01 * vyy;
     V01 = a00 * vzx + a01 * vzy;
     V11 = vzz;
   }
+  // CH0 = C h0, CH1 = C h1: barrel h0 = (a00, a01, 0..), h1 = e2; endcap h0 = e0, h1 = e1.
   double CH0[6], CH1[6];
+#pragma unroll
   for (int i = 0; i < 6; ++i) {
-    CH0[i] = 0; CH1[i] = 0;
-    for (int k = 0; k < 6; ++k) { CH0[i] += C[i][k] * h0[k]; CH1[i] += C[i][k] * h1[k]; }
+    CH0[i] = endcap ? (double)pre[S(i, 0)] : a00 * (double)pre[S(i, 0)] + a01 * (double)pre[S(i, 1)];
+    CH1[i] = endcap ? (double)pre[S(i, 1)] : (double)pre[S(i, 2)];
   }
-  double S00 = V00, S01 = V01, S11 = V11;
-  for (int k = 0; k < 6; ++k) { S00 += h0[k] * CH0[k]; S01 += h0[k] * CH1[k]; S11 += h1[k] * CH1[k]; }
+  const double s00 = endcap ? CH0[0] : a00 * CH0[0] + a01 * CH0[1];
+  const double s01 = endcap ? CH1[0] : a00 * CH1[0] + a01 * CH1[1];
+  const double s11 = endcap ? CH1[1] : CH1[2];
+  const double S00 = s00 + V00, S01 = s01 + V01, S11 = s11 + V11;
   const double det = S00 * S11 - S01 * S01;
   const double i00 = S11 / det, i01 = -S01 / det, i11 = S00 / det;
-  double K0[6], K1[6], A[6][6], AC[6][6];
+  double K0[6], K1[6];
+#pragma unroll
   for (int i = 0; i < 6; ++i) {
     K0[i] = CH0[i] * i00 + CH1[i] * i01;
     K1[i] = CH0[i] * i01 + CH1[i] * i11;
   }
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) A[i][j] = (i == j ? 1.0 : 0.0) - K0[i] * h0[j] - K1[i] * h1[j];
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) {
-      AC[i][j] = 0;
-      for (int k = 0; k < 6; ++k) AC[i][j] += A[i][k] * C[k][j];
-    }
-  for (int i = 0; i < 6; ++i)
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double g0 = CH0[i] - K0[i] * s00 - K1[i] * s01;    // (AC h0)_i
+    const double g1 = CH1[i] - K0[i] * s01 - K1[i] * s11;    // (AC h1)_i
+    const double kv0 = K0[i] * V00 + K1[i] * V01, kv1 = K0[i] * V01 + K1[i] * V11;
+#pragma unroll
     for (int j = 0; j <= i; ++j) {
-      double v = K0[i] * (V00 * K0[j] + V01 * K1[j]) + K1[i] * (V01 * K0[j] + V11 * K1[j]);
-      for (int k = 0; k < 6; ++k) v += AC[i][k] * A[j][k];
-      out[S(i, j)] = (float)v;
+      const double ac = (double)pre[S(i, j)] - K0[i] * CH0[j] - K1[i] * CH1[j];
+      out[S(i, j)] = (float)(ac - g0 * K0[j] - g1 * K1[j] + kv0 * K0[j] + kv1 * K1[j]);
     }
+  }
 }
 
 template <bool FB>
@@ -270,7 +275,9 @@ __device__ __forceinline__ void fit_step(float (&par)[6], float (&err)[21], int&
     py = par[1];
   }
   c2 = endcap ? kalman_endcap<true, true>(par, err, mp, me) : kalman_barrel<true, true>(par, err, mp, me);
-  if (FB && ill_conditioned(err)) joseph_fp64(pre, px, py, mp, me, endcap, err);
+  if constexpr (FB) {
+    if (ill_conditioned(err)) joseph_fp64(pre, px, py, mp, me, endcap, err);
+  }
   if (a.fit_flags & MKF_FIT_CHI2) chi2 += c2;
   if ((a.fit_flags & MKF_FIT_CHARGE_FLIP) && par[3] < 0.f) {
     chg = -chg;
@@ -367,9 +374,10 @@ k_fit(FitArgs a, int s_begin, int s_end) {
 }
 
 using FitKernel = void (*)(FitArgs, int, int);
-// Register budgets above 128 VGPRs/lane (4 waves) make the compiler demote the
-// private arrays to scratch, which changes FMA contraction and hence the bits;
-// the variants stop there so every choice gives identical results.
+// Register budgets below what the kernel needs (5 or 6 waves: 96 or 80
+// VGPRs) make the compiler spill to scratch, and the rematerialised
+// expressions then round differently; the variants stop at 4 waves so every
+// choice gives identical results.
 // [fp64 fallback][variant]
 static const FitKernel kFusedVariants[2][6] = {
     {k_fit<1, 1, false>, k_fit<2, 1, false>, k_fit<4, 1, false>,

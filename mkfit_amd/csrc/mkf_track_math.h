@@ -134,8 +134,11 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
     cosa = 1.f - 2.f * sinah * sinah;
     sina = 2.f * sinah * cosah;
 
-    if (i + 1 != kNiter) {
-      const float oor0 = (r0 > 0.f && fabsf(r - r0) < 0.0001f) ? frcp(r0) : 0.f;
+    const float oor0 = (r0 > 0.f && fabsf(r - r0) < 0.0001f) ? frcp(r0) : 0.f;
+    // Every derivative term is scaled by oor0: with oor0 == 0 the sums keep
+    // their values, so the block is skipped (a whole wavefront skips it on the
+    // first iterations, before the step has converged to within 1 um).
+    if (i + 1 != kNiter && oor0 != 0.f) {
       const float dadipt = id * kinv;
       const float dadx = -x * ipt * kinv * oor0;
       const float dady = -y * ipt * kinv * oor0;

@@ -221,6 +221,9 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     par = np.array(ev["seed_par"][:, :n], np.float32); err = np.array(ev["seed_err"][:, :n], np.float32)
     chg = np.array(ev["seed_chg"][:n], np.int32); chi2 = np.zeros(n, np.float32)
     nl = len(descs)
+    if ev["seed_hits"].shape[0] < first_layer:
+        raise ValueError("seed_hits has %d rows; first_layer %d needs one per layer before it"
+                         % (ev["seed_hits"].shape[0], first_layer))
     hot = np.full((nl, n), -9, np.int32)
     hot[:first_layer] = ev["seed_hits"][:first_layer, :n]
     nfound = np.zeros(n, np.int32)
@@ -258,6 +261,9 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
     sp = np.ascontiguousarray(ev["seed_par"][:, :n], np.float32)
     se = np.ascontiguousarray(ev["seed_err"][:, :n], np.float32)
     sc = np.ascontiguousarray(ev["seed_chg"][:n], np.int32)
+    if ev["seed_hits"].shape[0] < first_layer:
+        raise ValueError("seed_hits has %d rows; first_layer %d needs one per layer before it"
+                         % (ev["seed_hits"].shape[0], first_layer))
     sh = np.ascontiguousarray(ev["seed_hits"][:first_layer, :n], np.int32)
     out = dict(n_cands=np.zeros(n, np.int32), par=np.zeros((K, 6, n), np.float32),
                err=np.zeros((K, 21, n), np.float32), chg=np.zeros((K, n), np.int32), chi2=np.zeros((K, n), np.float32),

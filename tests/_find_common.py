@@ -80,3 +80,13 @@ def edge_explained(desc_win, par_t, h, diff, eps=2e-5):
     near_q = np.abs(np.abs(par_t[2] - h[diff, 2]) - dq) < eps
     near_p = np.abs(_cdist(np.abs(np.arctan2(par_t[1], par_t[0]) - ph)) - dphi) < eps
     return bool(np.all(near_q | near_p))
+
+
+def pad_seed_hits(ev, first_layer):
+    """The event with seed_hits extended to first_layer rows; layers past the seed layers hold -1 (no hit),
+    for searches that start beyond the seed layers."""
+    sh = ev["seed_hits"]
+    if sh.shape[0] >= first_layer:
+        return ev
+    pad = np.full((first_layer - sh.shape[0], sh.shape[1]), -1, np.int32)
+    return dict(ev, seed_hits=np.ascontiguousarray(np.vstack([sh, pad]), np.int32))

@@ -10,6 +10,7 @@ import pytest
 import mkfit_amd as mk
 import oracle
 from _common import RawDeviceBuffer, par_rel
+from _find_common import pad_seed_hits
 
 pytestmark = pytest.mark.gpu
 
@@ -113,8 +114,9 @@ def test_ce_on_endcap_disks(gpu_device):
 def test_ce_edge_cases(gpu_device):
     ev, descs = _event(8, 500, 2.0)
     L = len(descs)
-    g = _run(ev, descs, gpu_device, 6, first_layer=L)          # nothing to search: the seed itself
-    c = oracle.find_clone_engine(ev, descs, L, chi2_cut=30.0, max_cands=6, n_threads=4)
+    evL = pad_seed_hits(ev, L)
+    g = _run(evL, descs, gpu_device, 6, first_layer=L)         # nothing to search: the seed itself
+    c = oracle.find_clone_engine(evL, descs, L, chi2_cut=30.0, max_cands=6, n_threads=4)
     assert np.array_equal(g["n_cands"], np.ones(ev["n_seeds"], np.int32))
     assert np.array_equal(g["hot_idx"], c["hot_idx"])
     off = ev["hit_offset"]

@@ -11,7 +11,7 @@ import pytest
 import mkfit_amd as mk
 import oracle
 from _common import RawDeviceBuffer, par_rel
-from _find_common import F_PHI_FINE, _brute_select, _keys, _random_hits, edge_explained
+from _find_common import F_PHI_FINE, _brute_select, _keys, _random_hits, edge_explained, pad_seed_hits
 
 pytestmark = pytest.mark.gpu
 
@@ -73,7 +73,9 @@ def test_loh_build_matches_oracle(setup):
             assert np.array_equal(b_g, b_c), l
         else:
             assert np.array_equal(np.argsort(k_g, kind="stable"), r_g)
-    assert flips <= 5
+    # a hit lands within a few ulp of one of the 1024 fine-bin edges with probability ~1e-4, and the two
+    # atan2f implementations then disagree on the bin about a third of the time
+    assert flips <= max(5, 1e-4 * off[-1]), (flips, off[-1])
 
 
 def test_loh_reload_smaller_event(setup, gpu_device):
@@ -84,9 +86,18 @@ def test_loh_reload_smaller_event(setup, gpu_device):
     e2.load(ev2["hit_offset"], _t(ev2["hits_flat"], gpu_device))
     off = ev2["hit_offset"]
     for l in (0, 7, 16):
-        r_c, b_c, _, _, _ = oracle.loh_build(descs[l], ev2["hits_flat"][off[l]:off[l + 1]])
-        r_g, b_g, _, _, _ = e2.layer_views(l)
-        assert np.array_equal(b_g, b_c) and np.array_equal(r_g, r_c)
+        hl = ev2["hits_flat"][off[l]:off[l + 1]]
+        r_c, b_c, p_c, _, _ = oracle.loh_build(descs[l], hl)
+        r_g, b_g, p_g, _, _ = e2.layer_views(l)
+        # the layer holds exactly the small event's hits, ordered by the device's own keys
+        assert sorted(r_g.tolist()) == list(range(len(hl)))
+        phi_g = np.empty_like(p_g); phi_g[r_g] = p_g
+        k_g, _ = _keys(descs[l], phi_g, hl[:, 2])
+        assert np.array_equal(np.argsort(k_g, kind="stable"), r_g)
+        phi_c = np.empty_like(p_c); phi_c[r_c] = p_c
+        k_c, _ = _keys(descs[l], phi_c, hl[:, 2])
+        if np.array_equal(k_g, k_c):            # no fine-bin flip: identical to the oracle
+            assert np.array_equal(b_g, b_c) and np.array_equal(r_g, r_c)
 
 
 def _random_states(rng, n, r, zspan):
@@ -203,8 +214,9 @@ def test_find_edge_cases(setup, gpu_device):
     ev, descs, ctx, eoh = setup
     L = len(descs)
     # nothing to search: the seeds come back unchanged
-    b, g = _run_find(ctx, eoh, ev, L, L, gpu_device)
+    b, g = _run_find(ctx, eoh, pad_seed_hits(ev, L), L, L, gpu_device)
     assert np.array_equal(g["par"], ev["seed_par"]) and np.array_equal(g["hot_idx"][:3], ev["seed_hits"][:3])
+    assert np.all(g["hot_idx"][3:] == -1)
     assert np.array_equal(g["n_found"], np.sum(ev["seed_hits"] >= 0, axis=0))
     # no seeds
     e0 = dict(ev, n_seeds=0)

@@ -77,7 +77,7 @@ def test_add_best_hit_matches_oracle(setup, gpu_device, layer):
     _state_close(gn["par"][:, same], gn["err"][:, same], cpu["par"][:, same], cpu["err"][:, same])
     dc = np.abs(gn["chi2"][same] - cpu["chi2"][same]) / np.maximum(1.0, cpu["chi2"][same])
     assert np.percentile(dc, 99) < 1e-4
-    assert (cpu["hit_code"] >= 0).mean() > 0.5 and (cpu["hit_code"] < 0).any()
+    assert (cpu["hit_code"] >= 0).mean() > 0.5
 
 
 @pytest.mark.parametrize("layer", [3, 8])
@@ -123,10 +123,16 @@ def test_update_with_last_hit_matches_oracle(setup, gpu_device):
     hl = _layer_hits(ev, layer)
     par = ev["seed_par"][:, :n].astype(F); err = ev["seed_err"][:, :n].astype(F)
     chg = ev["seed_chg"][:n].astype(np.int32)
-    rng = np.random.default_rng(4)
-    hit = rng.integers(0, len(hl), n).astype(np.int32)
+    # each seed's own hit on the layer (the update a candidate makes after taking it), some lanes without
+    mc = ev["hit_mc"][ev["hit_offset"][layer]:ev["hit_offset"][layer + 1]]
+    first = {}
+    for i, m in enumerate(mc):
+        if m >= 0 and m not in first:
+            first[m] = i
+    hit = np.array([first.get(int(m), -1) for m in ev["seed_mc"][:n]], np.int32)
     hit[::7] = -1
     hit[3::11] = len(hl) + 3
+    assert (hit >= 0).mean() > 0.5
     ln = dict(prop_par=par, prop_err=err, chg=chg)
     p_c, e_c, c_c = oracle.update_with_last_hit(ev, descs, layer, ln, hit, pflags=0, bfield=3.8)
     g = _gpu_lanes(ln, gpu_device)

@@ -243,16 +243,19 @@ def test_propagation_failure_resets(gpu_device):
     np.testing.assert_array_equal(_np(ge), err)
 
 
-def test_all_batched_variants_identical(gpu_device):
-    """Every launch shape / register-budget variant the autotuner may pick gives the same bits."""
+@pytest.mark.parametrize("fallback", [False, True])
+def test_all_batched_variants_identical(gpu_device, fallback):
+    """Every launch shape / register-budget variant the autotuner may pick gives the same bits, with and
+    without the FP64 fallback."""
     import torch
     geo, p, ev, rl, xi = event(3, 5000, 45, hits_min=8, hits_max=30, missing_frac=0.05)
     ctx = mk.Context(p.bfield, rl, xi)
+    ff = mk.FIT_CHI2 | mk.FIT_CHARGE_FLIP | (mk.FIT_FP64_FALLBACK if fallback else 0)
     ref = None
     for v in range(mk.lib().mkf_num_batched_variants()):
         for block, bpc in ((256, 0), (64, 4), (512, 1)):
             ctx.set_batched_shape(block, bpc, v)
-            g = _fit_gpu(ctx, ev, gpu_device, mk.PF_APPLY_MATERIAL, mk.FIT_CHI2 | mk.FIT_CHARGE_FLIP, True)
+            g = _fit_gpu(ctx, ev, gpu_device, mk.PF_APPLY_MATERIAL, ff, True)
             if ref is None:
                 ref = g
             for name, x, y in zip(("par", "err", "chg", "chi2"), g, ref):

@@ -4,7 +4,7 @@ import pytest
 
 import mkfit_amd as mk
 import oracle
-from _common import par_rel
+from _common import TOL_COV, TOL_PAR, assert_parity, par_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -23,6 +23,10 @@ def test_pca_parity(gpu_device, form):
     torch.cuda.synchronize()
     O = oracle.Oracle(bfield=3.8, rl_grid=rl, xi_grid=xi)
     par, err, chg, chi2 = O.fit(ev, pflags=2, fit_flags=1 | 8, n_threads=8, pca_pflags=0)
-    pr = par_rel(b.par.cpu().numpy(), par)
-    assert np.mean(pr.max(axis=0) <= 1e-5) > 0.98, np.mean(pr.max(axis=0) <= 1e-5)
-    assert np.percentile(pr, 99.9) < 1e-3
+    p64, e64, _, _ = O.fit(ev, pflags=2, fit_flags=1 | 8, dtype=np.float64, n_threads=8, pca_pflags=0)
+    # The step to the PCA extrapolates the fitted state over up to a metre, so FP32 rounding of the
+    # angles (~1e-7 rad) moves the position there by ~1e-5 cm: the FP32 oracle itself is that far from
+    # FP64.  The GPU must be as close to FP32 and FP64 as the FP32 oracle is to FP64.
+    floor = float(np.mean(par_rel(par, p64) <= TOL_PAR))
+    assert_parity("pca", b.par.cpu().numpy(), par, p64, "par", TOL_PAR, frac=floor - 0.05)
+    assert_parity("pca", b.err.cpu().numpy(), err, e64, "cov", TOL_COV, frac=0.95)

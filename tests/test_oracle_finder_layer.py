@@ -185,8 +185,12 @@ def test_update_with_last_hit_matches_primitives(event):
     hl = _layer_hits(ev, layer)
     par = ev["seed_par"][:, :n].astype(F); err = ev["seed_err"][:, :n].astype(F)
     chg = ev["seed_chg"][:n].astype(np.int32)
-    rng = np.random.default_rng(2)
-    hit = rng.integers(0, len(hl), n).astype(np.int32)
+    mc = ev["hit_mc"][ev["hit_offset"][layer]:ev["hit_offset"][layer + 1]]
+    first = {}
+    for i, m in enumerate(mc):
+        if m >= 0 and m not in first:
+            first[m] = i
+    hit = np.array([first.get(int(m), -1) for m in ev["seed_mc"][:n]], np.int32)
     hit[::7] = -1
     hit[3::11] = len(hl) + 3
     p2, e2, c2 = oracle.update_with_last_hit(ev, descs, layer, dict(prop_par=par, prop_err=err, chg=chg), hit,
