@@ -172,6 +172,8 @@ typedef struct mkf_adjoint_batch {
   int checkpoint_every;   /* layers per recomputed segment, >= 1 */
   float* chi2;            /* [n] out: the track's total chi2 */
   float* grad;            /* [n_layers][3] out: d(sum over tracks of chi2)/d theta */
+  int fp64;               /* 0: the FP32 fit model (as mkf_fit_batched computes it); 1: the same model in FP64,
+                             forward and reverse (checkpoints twice the size) */
 } mkf_adjoint_batch;
 
 int mkf_fit_adjoint(mkf_context* ctx, const mkf_adjoint_batch* batch, void* stream);

@@ -3,7 +3,8 @@
 //
 // One lane per track.  Forward: the fit of the alignment model (hits moved by
 // their layer's offsets, material from the layer's x/X0; mkf_adjoint_math.h)
-// in FP32, saving the 27-float state every `checkpoint_every` layers.
+// in FP32 (or, with fp64 set, the same model in FP64), saving the 27-entry
+// state every `checkpoint_every` layers.
 // Reverse: the adjoint lambda of the state (27 floats, zero after the last
 // hit) is carried back layer by layer.  At layer l the state entering the
 // layer is recomputed from the segment's checkpoint, and the step's
@@ -29,8 +30,10 @@ using namespace mkf_detail;
 namespace {
 
 constexpr int kDirs = 30;       // 27 state entries + 3 layer parameters
-constexpr int kTanD = 3;        // tangents per Dual pass
 constexpr int kBlock = 128;
+// Tangents per Dual pass: what fits in registers (a double Dual is twice as wide).
+template <class T> constexpr int kTanD = 3;
+template <> constexpr int kTanD<double> = 1;
 
 struct AdjArgs {
   int n, n_layers, ck, pflags;
@@ -43,10 +46,12 @@ struct AdjArgs {
   const int* __restrict__ n_hits;
   const float* __restrict__ theta;
   float* __restrict__ chi2;
-  float* __restrict__ ckpt;     // [n_ck][27][n]
-  float* __restrict__ gtrack;   // [n_layers * 3][n]
+  void* __restrict__ ckpt;      // [n_ck][27][n] of the model's scalar type
+  void* __restrict__ gtrack;    // [n_layers * 3][n] of the model's scalar type
 };
 
+// Hit l of track t and the reciprocal of its transverse radius in the model's
+// scalar type (the FP32 model rounds the double root to float first).
 __device__ __forceinline__ void load_hit(const AdjArgs& a, int l, int t, float (&hp)[3], float (&he)[6],
                                          float& rh_inv) {
   const size_t n = (size_t)a.n;
@@ -57,84 +62,98 @@ __device__ __forceinline__ void load_hit(const AdjArgs& a, int l, int t, float (
   const double x = hp[0], y = hp[1];
   rh_inv = 1.f / (float)__builtin_sqrt(x * x + y * y);
 }
+__device__ __forceinline__ void load_hit(const AdjArgs& a, int l, int t, float (&hp)[3], float (&he)[6],
+                                         double& rh_inv) {
+  float unused;
+  load_hit(a, l, t, hp, he, unused);
+  const double x = hp[0], y = hp[1];
+  rh_inv = 1.0 / __builtin_sqrt(x * x + y * y);
+}
 
+template <class T>
 __global__ void __launch_bounds__(kBlock) k_adjoint(AdjArgs a) {
-  __shared__ float vbuf[kBlock][kDirs + 1];
+  __shared__ T vbuf[kBlock][kDirs + 1];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n) return;
   const size_t n = (size_t)a.n;
+  T* __restrict__ ckpt = static_cast<T*>(a.ckpt);
+  T* __restrict__ gtrack = static_cast<T*>(a.gtrack);
   const int nh = min(a.n_hits[t], a.n_layers);
   const int chg = a.seed_chg[t];
-  float par[6], err[21];
+  T par[6], err[21];
   // ---- forward, with checkpoints
 #pragma unroll
   for (int e = 0; e < 6; ++e) par[e] = a.seed_par[e * n + t];
 #pragma unroll
   for (int e = 0; e < 21; ++e) err[e] = a.seed_err[e * n + t];
-  float chi2 = 0.f;
+  T chi2 = T(0);
   for (int l = 0; l < nh; ++l) {
     if (l % a.ck == 0) {
-      float* ck = a.ckpt + (size_t)(l / a.ck) * 27 * n + t;
+      T* ck = ckpt + (size_t)(l / a.ck) * 27 * n + t;
 #pragma unroll
       for (int e = 0; e < 6; ++e) ck[e * n] = par[e];
 #pragma unroll
       for (int e = 0; e < 21; ++e) ck[(6 + e) * n] = err[e];
     }
-    float hp[3], he[6], rhi;
+    float hp[3], he[6];
+    T rhi;
     load_hit(a, l, t, hp, he, rhi);
-    chi2 += align_step<float>(par, err, chg, hp, he, a.theta[3 * l], a.theta[3 * l + 1], a.theta[3 * l + 2],
-                              a.xi_per_x0, a.bfield, a.pflags, rhi);
+    chi2 += align_step<T>(par, err, chg, hp, he, T(a.theta[3 * l]), T(a.theta[3 * l + 1]), T(a.theta[3 * l + 2]),
+                          a.xi_per_x0, a.bfield, a.pflags, rhi);
   }
-  a.chi2[t] = chi2;
+  a.chi2[t] = (float)chi2;
   for (int l = nh; l < a.n_layers; ++l)
-    for (int q = 0; q < 3; ++q) a.gtrack[((size_t)l * 3 + q) * n + t] = 0.f;
+    for (int q = 0; q < 3; ++q) gtrack[((size_t)l * 3 + q) * n + t] = T(0);
   // ---- reverse sweep
-  float lam[27];
+  T lam[27];
 #pragma unroll
-  for (int e = 0; e < 27; ++e) lam[e] = 0.f;
-  float* vb = vbuf[threadIdx.x];
+  for (int e = 0; e < 27; ++e) lam[e] = T(0);
+  T* vb = vbuf[threadIdx.x];
   for (int l = nh - 1; l >= 0; --l) {
     const int c0 = (l / a.ck) * a.ck;
-    const float* ck = a.ckpt + (size_t)(l / a.ck) * 27 * n + t;
+    const T* ck = ckpt + (size_t)(l / a.ck) * 27 * n + t;
 #pragma unroll
     for (int e = 0; e < 6; ++e) par[e] = ck[e * n];
 #pragma unroll
     for (int e = 0; e < 21; ++e) err[e] = ck[(6 + e) * n];
     for (int j = c0; j < l; ++j) {                      // recompute the state entering layer l
-      float hp[3], he[6], rhi;
+      float hp[3], he[6];
+      T rhi;
       load_hit(a, j, t, hp, he, rhi);
-      (void)align_step<float>(par, err, chg, hp, he, a.theta[3 * j], a.theta[3 * j + 1], a.theta[3 * j + 2],
-                              a.xi_per_x0, a.bfield, a.pflags, rhi);
+      (void)align_step<T>(par, err, chg, hp, he, T(a.theta[3 * j]), T(a.theta[3 * j + 1]), T(a.theta[3 * j + 2]),
+                          a.xi_per_x0, a.bfield, a.pflags, rhi);
     }
-    float hp[3], he[6], rhi;
+    float hp[3], he[6];
+    T rhi;
     load_hit(a, l, t, hp, he, rhi);
-    const float th0 = a.theta[3 * l], th1 = a.theta[3 * l + 1], th2 = a.theta[3 * l + 2];
-    for (int d0 = 0; d0 < kDirs; d0 += kTanD) {
-      using DV = Dual<kTanD>;
+    const T th0 = a.theta[3 * l], th1 = a.theta[3 * l + 1], th2 = a.theta[3 * l + 2];
+    constexpr int TD = kTanD<T>;
+    for (int d0 = 0; d0 < kDirs; d0 += TD) {
+      using DV = Dual<T, TD>;
       DV dp[6], de[21], m, dr, dz;
 #pragma unroll
       for (int e = 0; e < 6; ++e) {
         dp[e].v = par[e];
 #pragma unroll
-        for (int k = 0; k < kTanD; ++k) dp[e].d[k] = (e == d0 + k) ? 1.f : 0.f;
+        for (int k = 0; k < TD; ++k) dp[e].d[k] = (e == d0 + k) ? T(1) : T(0);
       }
 #pragma unroll
       for (int e = 0; e < 21; ++e) {
         de[e].v = err[e];
 #pragma unroll
-        for (int k = 0; k < kTanD; ++k) de[e].d[k] = (6 + e == d0 + k) ? 1.f : 0.f;
+        for (int k = 0; k < TD; ++k) de[e].d[k] = (6 + e == d0 + k) ? T(1) : T(0);
       }
       m.v = th0; dr.v = th1; dz.v = th2;
 #pragma unroll
-      for (int k = 0; k < kTanD; ++k) {
-        m.d[k] = (27 == d0 + k) ? 1.f : 0.f;
-        dr.d[k] = (28 == d0 + k) ? 1.f : 0.f;
-        dz.d[k] = (29 == d0 + k) ? 1.f : 0.f;
+      for (int k = 0; k < TD; ++k) {
+        m.d[k] = (27 == d0 + k) ? T(1) : T(0);
+        dr.d[k] = (28 == d0 + k) ? T(1) : T(0);
+        dz.d[k] = (29 == d0 + k) ? T(1) : T(0);
       }
       const DV c2 = align_step<DV>(dp, de, chg, hp, he, m, dr, dz, a.xi_per_x0, a.bfield, a.pflags, rhi);
 #pragma unroll
-      for (int k = 0; k < kTanD; ++k) {
-        float v = c2.d[k];
+      for (int k = 0; k < TD; ++k) {
+        T v = c2.d[k];
 #pragma unroll
         for (int e = 0; e < 6; ++e) v += lam[e] * dp[e].d[k];
 #pragma unroll
@@ -144,14 +163,15 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(AdjArgs a) {
     }
 #pragma unroll
     for (int e = 0; e < 27; ++e) lam[e] = vb[e];
-    for (int q = 0; q < 3; ++q) a.gtrack[((size_t)l * 3 + q) * n + t] = vb[27 + q];
+    for (int q = 0; q < 3; ++q) gtrack[((size_t)l * 3 + q) * n + t] = vb[27 + q];
   }
 }
 
 // Sum over tracks of each parameter's row, in double; one block per parameter.
-__global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ rows, int n, float* __restrict__ out) {
+template <class T>
+__global__ void __launch_bounds__(256) k_sum_rows(const T* __restrict__ rows, int n, float* __restrict__ out) {
   __shared__ double part[256];
-  const float* r = rows + (size_t)blockIdx.x * n;
+  const T* r = rows + (size_t)blockIdx.x * n;
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) s += (double)r[i];
   part[threadIdx.x] = s;
@@ -178,7 +198,8 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   CHECK(check_dev(b->grad, L * 3 * 4, "grad"));
   hipStream_t st = (hipStream_t)stream;
   if (N == 0) {
-    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)(L * 3)), dim3(256), 0, st, (const float*)nullptr, 0, b->grad);
+    hipLaunchKernelGGL(k_sum_rows<float>, dim3((unsigned)(L * 3)), dim3(256), 0, st, (const float*)nullptr, 0,
+                       b->grad);
     HIP_TRY(hipGetLastError());
     return MKF_OK;
   }
@@ -191,7 +212,8 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   CHECK(check_dev(b->n_hits, N * 4, "n_hits"));
   CHECK(check_dev(b->chi2, N * 4, "chi2"));
   const size_t n_ck = (L + b->checkpoint_every - 1) / b->checkpoint_every;
-  const size_t need = (n_ck * 27 + L * 3) * N * 4;
+  const size_t word = b->fp64 ? 8 : 4;
+  const size_t need = (n_ck * 27 + L * 3) * N * word;
   std::lock_guard<std::mutex> lk(c->mu);
   if (need > c->adj_bytes) {
     HIP_TRY(hipDeviceSynchronize());
@@ -212,11 +234,20 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   a.hit_pos = b->hit_pos; a.hit_err = b->hit_err; a.n_hits = b->n_hits;
   a.theta = b->theta;
   a.chi2 = b->chi2;
-  a.ckpt = static_cast<float*>(c->adj_scratch);
-  a.gtrack = a.ckpt + n_ck * 27 * N;
-  hipLaunchKernelGGL(k_adjoint, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a);
-  HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)(L * 3)), dim3(256), 0, st, a.gtrack, a.n, b->grad);
+  a.ckpt = c->adj_scratch;
+  a.gtrack = static_cast<char*>(c->adj_scratch) + n_ck * 27 * N * word;
+  const dim3 grid((unsigned)((N + kBlock - 1) / kBlock));
+  if (b->fp64) {
+    hipLaunchKernelGGL(k_adjoint<double>, grid, dim3(kBlock), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_rows<double>, dim3((unsigned)(L * 3)), dim3(256), 0, st,
+                       static_cast<const double*>(a.gtrack), a.n, b->grad);
+  } else {
+    hipLaunchKernelGGL(k_adjoint<float>, grid, dim3(kBlock), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_rows<float>, dim3((unsigned)(L * 3)), dim3(256), 0, st,
+                       static_cast<const float*>(a.gtrack), a.n, b->grad);
+  }
   HIP_TRY(hipGetLastError());
   return MKF_OK;
 }

@@ -1,7 +1,7 @@
-// mkf_adjoint_math.h -- the barrel fit step written once for two scalar types:
-// float (the forward pass and the recomputation from checkpoints) and Dual<D>,
-// a value with D forward tangents, which gives D columns of the step's
-// Jacobian per evaluation.  The arithmetic follows mkf_track_math.h (and so the
+// mkf_adjoint_math.h -- the barrel fit step written once for every model type:
+// float or double (the forward pass and the recomputation from checkpoints)
+// and Dual<T, D>, a value with D forward tangents, which gives D columns of the
+// step's Jacobian per evaluation.  The arithmetic follows mkf_track_math.h (and so the
 // reference's order of operations, cited there); only the scalar type differs.
 #pragma once
 
@@ -13,84 +13,104 @@ namespace mkf_adj {
 
 using mkf::S;
 
-template <int D>
+// A value with D forward tangents over the scalar type T (float: the FP32 model;
+// double: the same model evaluated in FP64).
+template <class T, int D>
 struct Dual {
-  float v;
-  float d[D];
+  T v;
+  T d[D];
 };
 
+// Scalar type of a model value (float, double, or a Dual's).
+template <class V> struct ScalarOf { using type = V; };
+template <class T, int D> struct ScalarOf<Dual<T, D>> { using type = T; };
+template <class V> using SOf = typename ScalarOf<V>::type;
+// Keeps a scalar operand out of template deduction, so a float literal combines
+// with a double Dual.
+template <class T> struct Id { using type = T; };
+template <class T> using NoDed = typename Id<T>::type;
+
 // ---- Dual arithmetic ----------------------------------------------------------
-template <int D> __device__ __forceinline__ Dual<D> cst(float x) {
-  Dual<D> r;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> cst(T x) {
+  Dual<T, D> r;
   r.v = x;
 #pragma unroll
-  for (int i = 0; i < D; ++i) r.d[i] = 0.f;
+  for (int i = 0; i < D; ++i) r.d[i] = T(0);
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator+(const Dual<D>& a, const Dual<D>& b) {
-  Dual<D> r; r.v = a.v + b.v;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator+(const Dual<T, D>& a, const Dual<T, D>& b) {
+  Dual<T, D> r; r.v = a.v + b.v;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = a.d[i] + b.d[i];
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator-(const Dual<D>& a, const Dual<D>& b) {
-  Dual<D> r; r.v = a.v - b.v;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator-(const Dual<T, D>& a, const Dual<T, D>& b) {
+  Dual<T, D> r; r.v = a.v - b.v;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = a.d[i] - b.d[i];
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator-(const Dual<D>& a) {
-  Dual<D> r; r.v = -a.v;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator-(const Dual<T, D>& a) {
+  Dual<T, D> r; r.v = -a.v;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = -a.d[i];
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator*(const Dual<D>& a, const Dual<D>& b) {
-  Dual<D> r; r.v = a.v * b.v;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator*(const Dual<T, D>& a, const Dual<T, D>& b) {
+  Dual<T, D> r; r.v = a.v * b.v;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator/(const Dual<D>& a, const Dual<D>& b) {
-  Dual<D> r;
-  const float ib = 1.f / b.v;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator/(const Dual<T, D>& a, const Dual<T, D>& b) {
+  Dual<T, D> r;
+  const T ib = T(1) / b.v;
   r.v = a.v / b.v;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator+(const Dual<D>& a, float b) { Dual<D> r = a; r.v = a.v + b; return r; }
-template <int D> __device__ __forceinline__ Dual<D> operator+(float a, const Dual<D>& b) { return b + a; }
-template <int D> __device__ __forceinline__ Dual<D> operator-(const Dual<D>& a, float b) { Dual<D> r = a; r.v = a.v - b; return r; }
-template <int D> __device__ __forceinline__ Dual<D> operator-(float a, const Dual<D>& b) {
-  Dual<D> r; r.v = a - b.v;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator+(const Dual<T, D>& a, NoDed<T> b) {
+  Dual<T, D> r = a; r.v = a.v + b; return r;
+}
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator+(NoDed<T> a, const Dual<T, D>& b) { return b + a; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator-(const Dual<T, D>& a, NoDed<T> b) {
+  Dual<T, D> r = a; r.v = a.v - b; return r;
+}
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator-(NoDed<T> a, const Dual<T, D>& b) {
+  Dual<T, D> r; r.v = a - b.v;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = -b.d[i];
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator*(const Dual<D>& a, float b) {
-  Dual<D> r; r.v = a.v * b;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator*(const Dual<T, D>& a, NoDed<T> b) {
+  Dual<T, D> r; r.v = a.v * b;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = a.d[i] * b;
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator*(float a, const Dual<D>& b) { return b * a; }
-template <int D> __device__ __forceinline__ Dual<D> operator/(const Dual<D>& a, float b) {
-  Dual<D> r; r.v = a.v / b;
-  const float ib = 1.f / b;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator*(NoDed<T> a, const Dual<T, D>& b) { return b * a; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator/(const Dual<T, D>& a, NoDed<T> b) {
+  Dual<T, D> r; r.v = a.v / b;
+  const T ib = T(1) / b;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = a.d[i] * ib;
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> operator/(float a, const Dual<D>& b) { return cst<D>(a) / b; }
-template <int D> __device__ __forceinline__ Dual<D>& operator+=(Dual<D>& a, const Dual<D>& b) { a = a + b; return a; }
-template <int D> __device__ __forceinline__ Dual<D>& operator-=(Dual<D>& a, const Dual<D>& b) { a = a - b; return a; }
-template <int D> __device__ __forceinline__ Dual<D>& operator-=(Dual<D>& a, float b) { a.v -= b; return a; }
-template <int D> __device__ __forceinline__ Dual<D>& operator+=(Dual<D>& a, float b) { a.v += b; return a; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D> operator/(NoDed<T> a, const Dual<T, D>& b) {
+  return cst<T, D>(a) / b;
+}
+template <class T, int D> __device__ __forceinline__ Dual<T, D>& operator+=(Dual<T, D>& a, const Dual<T, D>& b) { a = a + b; return a; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D>& operator-=(Dual<T, D>& a, const Dual<T, D>& b) { a = a - b; return a; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D>& operator-=(Dual<T, D>& a, NoDed<T> b) { a.v -= b; return a; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D>& operator+=(Dual<T, D>& a, NoDed<T> b) { a.v += b; return a; }
 
-// ---- scalar functions over both types ----------------------------------------
+// ---- scalar functions over all model types ----------------------------------
+// float: the fit's own refined hardware divide / reciprocal / root (mkf_track_math.h);
+// double: IEEE operations.
 __device__ __forceinline__ float val(float x) { return x; }
-template <int D> __device__ __forceinline__ float val(const Dual<D>& x) { return x.v; }
+__device__ __forceinline__ double val(double x) { return x; }
+template <class T, int D> __device__ __forceinline__ T val(const Dual<T, D>& x) { return x.v; }
 
 __device__ __forceinline__ float vrcp(float x) { return mkf::frcp(x); }
 __device__ __forceinline__ float vdiv(float a, float b) { return mkf::fdiv(a, b); }
@@ -99,44 +119,63 @@ __device__ __forceinline__ float vsin(float x) { return sinf(x); }
 __device__ __forceinline__ float vcos(float x) { return cosf(x); }
 __device__ __forceinline__ float vlog(float x) { return logf(x); }
 __device__ __forceinline__ float vabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double vrcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ double vdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double vsin(double x) { return sin(x); }
+__device__ __forceinline__ double vcos(double x) { return cos(x); }
+__device__ __forceinline__ double vlog(double x) { return log(x); }
+__device__ __forceinline__ double vabs(double x) { return fabs(x); }
+// plain-math versions for the value part of a Dual
+__device__ __forceinline__ float ssqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double ssqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float ssin(float x) { return sinf(x); }
+__device__ __forceinline__ double ssin(double x) { return sin(x); }
+__device__ __forceinline__ float scos(float x) { return cosf(x); }
+__device__ __forceinline__ double scos(double x) { return cos(x); }
+__device__ __forceinline__ float slog(float x) { return logf(x); }
+__device__ __forceinline__ double slog(double x) { return log(x); }
 
-template <int D> __device__ __forceinline__ Dual<D> vrcp(const Dual<D>& x) { return 1.f / x; }
-template <int D> __device__ __forceinline__ Dual<D> vdiv(const Dual<D>& a, const Dual<D>& b) { return a / b; }
-template <int D> __device__ __forceinline__ Dual<D> vsqrt(const Dual<D>& x) {
-  Dual<D> r; r.v = sqrtf(x.v);
-  const float h = r.v > 0.f ? 0.5f / r.v : 0.f;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> vrcp(const Dual<T, D>& x) { return T(1) / x; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D> vdiv(const Dual<T, D>& a, const Dual<T, D>& b) {
+  return a / b;
+}
+template <class T, int D> __device__ __forceinline__ Dual<T, D> vsqrt(const Dual<T, D>& x) {
+  Dual<T, D> r; r.v = ssqrt(x.v);
+  const T h = r.v > T(0) ? T(0.5) / r.v : T(0);
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = x.d[i] * h;
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> vsin(const Dual<D>& x) {
-  Dual<D> r; r.v = sinf(x.v);
-  const float c = cosf(x.v);
+template <class T, int D> __device__ __forceinline__ Dual<T, D> vsin(const Dual<T, D>& x) {
+  Dual<T, D> r; r.v = ssin(x.v);
+  const T c = scos(x.v);
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = x.d[i] * c;
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> vcos(const Dual<D>& x) {
-  Dual<D> r; r.v = cosf(x.v);
-  const float s = -sinf(x.v);
+template <class T, int D> __device__ __forceinline__ Dual<T, D> vcos(const Dual<T, D>& x) {
+  Dual<T, D> r; r.v = scos(x.v);
+  const T s = -ssin(x.v);
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = x.d[i] * s;
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> vlog(const Dual<D>& x) {
-  Dual<D> r; r.v = logf(x.v);
-  const float ix = 1.f / x.v;
+template <class T, int D> __device__ __forceinline__ Dual<T, D> vlog(const Dual<T, D>& x) {
+  Dual<T, D> r; r.v = slog(x.v);
+  const T ix = T(1) / x.v;
 #pragma unroll
   for (int i = 0; i < D; ++i) r.d[i] = x.d[i] * ix;
   return r;
 }
-template <int D> __device__ __forceinline__ Dual<D> vabs(const Dual<D>& x) { return x.v < 0.f ? -x : x; }
+template <class T, int D> __device__ __forceinline__ Dual<T, D> vabs(const Dual<T, D>& x) { return x.v < T(0) ? -x : x; }
 
 template <class V> __device__ __forceinline__ V vmax(const V& a, const V& b) { return val(a) >= val(b) ? a : b; }
-template <class V> struct MkC;
-template <> struct MkC<float> { static __device__ __forceinline__ float c(float x) { return x; } };
-template <int D> struct MkC<Dual<D>> { static __device__ __forceinline__ Dual<D> c(float x) { return cst<D>(x); } };
-template <class V> __device__ __forceinline__ V C(float x) { return MkC<V>::c(x); }
+template <class V> struct MkC { static __device__ __forceinline__ V c(SOf<V> x) { return x; } };
+template <class T, int D> struct MkC<Dual<T, D>> {
+  static __device__ __forceinline__ Dual<T, D> c(T x) { return cst<T, D>(x); }
+};
+template <class V> __device__ __forceinline__ V C(SOf<V> x) { return MkC<V>::c(x); }
 
 // ---- helix to radius r (barrel), approximate Jacobian -------------------------
 template <class V>
@@ -156,8 +195,9 @@ __device__ __forceinline__ bool helix_at_r_t(V (&par)[6], int chg, const V& r, f
   const V xin = par[0], yin = par[1], zin = par[2];
   const V ipt = par[3], phiin = par[4], theta = par[5];
   V r0 = vsqrt(xin * xin + yin * yin);
-  const float k = mkf::fdiv(float(chg) * 100.f, -mkf::kSol * bf);
-  const float kinv = mkf::frcp(k);
+  using T = SOf<V>;
+  const T k = vdiv(T(chg) * T(100.f), -T(mkf::kSol) * T(bf));
+  const T kinv = vrcp(k);
   const V pt = vrcp(ipt);
   V D = C<V>(0.f), cosa = C<V>(0.f), sina = C<V>(0.f), cosah = C<V>(0.f), sinah = C<V>(0.f), id = C<V>(0.f);
   V cosPorT = vcos(phiin), sinPorT = vsin(phiin);
@@ -183,7 +223,7 @@ __device__ __forceinline__ bool helix_at_r_t(V (&par)[6], int chg, const V& r, f
     cosa = 1.f - 2.f * sinah * sinah;
     sina = 2.f * sinah * cosah;
     if (i + 1 != mkf::kNiter) {
-      const V oor0 = (val(r0) > 0.f && fabsf(val(r) - val(r0)) < 0.0001f) ? vrcp(r0) : C<V>(0.f);
+      const V oor0 = (val(r0) > T(0) && vabs(val(r) - val(r0)) < T(0.0001f)) ? vrcp(r0) : C<V>(0.f);
       const V dadipt = id * kinv;
       const V dadx = -x * ipt * kinv * oor0;
       const V dady = -y * ipt * kinv * oor0;
@@ -258,9 +298,11 @@ __device__ __forceinline__ void material_t(V radL, const V& xi, float propSign, 
   const V gamma2 = vdiv(p2 + mpi2, C<V>(mpi2));
   const V gamma = vsqrt(gamma2);
   constexpr float me = 0.0005;
-  const V wmax = vdiv(2.f * me * beta2 * gamma2, 1.f + vdiv(2.f * gamma * me, C<V>(mpi)) + me * me / (mpi * mpi));
+  const V wmax = vdiv(2.f * me * beta2 * gamma2,
+                      1.f + vdiv(2.f * gamma * me, C<V>(mpi)) + SOf<V>(me) * SOf<V>(me) / (SOf<V>(mpi) * SOf<V>(mpi)));
   constexpr float I = 16.0e-9 * 10.75;
-  const V deltahalf = logf(28.816e-9f * sqrtf(2.33f * 0.498f) / I) + vlog(beta * gamma) - 0.5f;
+  using T = SOf<V>;
+  const V deltahalf = slog(T(28.816e-9f) * ssqrt(T(2.33f) * T(0.498f)) / T(I)) + vlog(beta * gamma) - 0.5f;
   const V dEdx = val(beta) < 1.f
                      ? (2.f * vdiv(xi * invCos * (0.5f * vlog(vdiv(2.f * me * beta2 * gamma2 * wmax, C<V>(I * I))) -
                                                   beta2 - deltahalf),
@@ -306,14 +348,15 @@ __device__ __forceinline__ void similarity_t(const JacRT<V>& J, V (&Cm)[21]) {
   for (int e = 0; e < 21; ++e) Cm[e] = out[e];
 }
 
-// ---- 2x2 inverse: the value as in the fit (double determinant), tangents of 1/det
+// ---- 2x2 inverse: FP32 values as in the fit (double determinant), tangents of 1/det;
+// FP64: the plain inverse.
 __device__ __forceinline__ void invert_sym2_t(float& a0, float& a1, float& a2) { mkf::invert_sym2(a0, a1, a2); }
 template <int D>
-__device__ __forceinline__ void invert_sym2_t(Dual<D>& a0, Dual<D>& a1, Dual<D>& a2) {
+__device__ __forceinline__ void invert_sym2_t(Dual<float, D>& a0, Dual<float, D>& a1, Dual<float, D>& a2) {
   float v0 = a0.v, v1 = a1.v, v2 = a2.v;
   mkf::invert_sym2(v0, v1, v2);                       // values exactly as the float path
   const float s = (float)(1.0 / ((double)a0.v * a2.v - (double)a1.v * a1.v));
-  Dual<D> n0, n1, n2;
+  Dual<float, D> n0, n1, n2;
   n0.v = v0; n1.v = v1; n2.v = v2;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -324,6 +367,14 @@ __device__ __forceinline__ void invert_sym2_t(Dual<D>& a0, Dual<D>& a1, Dual<D>&
     n2.d[i] = ds * a0.v + s * a0.d[i];
   }
   a0 = n0; a1 = n1; a2 = n2;
+}
+template <class V>
+__device__ __forceinline__ void invert_sym2_t(V& a0, V& a1, V& a2) {   // double and Dual<double>
+  const V s = vrcp(a0 * a2 - a1 * a1);
+  const V tmp = s * a2;
+  a1 = -(a1 * s);
+  a2 = s * a0;
+  a0 = tmp;
 }
 
 // ---- kalmanOperation (barrel): chi2 and update --------------------------------
@@ -373,7 +424,8 @@ __device__ __forceinline__ V kalman_t(V (&par)[6], V (&err)[21], const V (&mp)[3
 template <class V>
 __device__ __forceinline__ V align_step(V (&par)[6], V (&err)[21], int chg, const float (&hp)[3], const float (&he)[6],
                                         const V& m, const V& drphi, const V& dz, float xi_per_x0, float bfield,
-                                        int pflags, float rh_inv) {
+                                        int pflags, SOf<V> rh_inv) {
+  using T = SOf<V>;
   V mp[3];
   mp[0] = hp[0] - drphi * (hp[1] * rh_inv);
   mp[1] = hp[1] + drphi * (hp[0] * rh_inv);
@@ -386,8 +438,8 @@ __device__ __forceinline__ V align_step(V (&par)[6], V (&err)[21], int chg, cons
   const bool fail = helix_at_r_t(out, chg, r, bfield, J);
   if (!fail) {
     if (pflags & mkf::PF_MATERIAL) {
-      const float r0 = sqrtf(val(par[0]) * val(par[0]) + val(par[1]) * val(par[1]));
-      material_t(m, m * xi_per_x0, val(r) > r0 ? 1.f : -1.f, out, err);
+      const T r0 = ssqrt(val(par[0]) * val(par[0]) + val(par[1]) * val(par[1]));
+      material_t(m, m * T(xi_per_x0), val(r) > r0 ? 1.f : -1.f, out, err);
     }
     if (val(out[4]) >= mkf::kPI) out[4] -= mkf::kTwoPI;
     if (val(out[4]) < -mkf::kPI) out[4] += mkf::kTwoPI;

@@ -23,6 +23,18 @@ def _theta(th, dev):
     return torch.from_numpy(np.ascontiguousarray(th, np.float32)).to(dev)
 
 
+def test_fp64_forward_matches_fp64_oracle_model(setup, gpu_device):
+    ev, geo, th, ctx = setup
+    data = mk.AlignmentData(ctx, ev, gpu_device, fp64=True)
+    chi2, _ = data.run(_theta(th, gpu_device))
+    # the float32 theta the device sees, in the FP64 model
+    c64 = oracle.align_chi2(ev, th.astype(np.float32).astype(np.float64), n_threads=8)
+    g = chi2.cpu().numpy().astype(np.float64)
+    rel = np.abs(g - c64) / np.maximum(1.0, c64)
+    # per-track chi2 is returned as float32: agreement to its rounding
+    assert np.percentile(rel, 99) < 1e-6 and rel.max() < 1e-4, (np.percentile(rel, 99), rel.max())
+
+
 def test_forward_matches_oracle_model(setup, gpu_device):
     ev, geo, th, ctx = setup
     data = mk.AlignmentData(ctx, ev, gpu_device)
@@ -40,11 +52,16 @@ def test_forward_matches_oracle_model(setup, gpu_device):
     assert abs(g.sum() - c64.sum()) / c64.sum() < 1e-4
 
 
-def test_gradient_matches_fp64_finite_differences(setup, gpu_device):
+@pytest.mark.parametrize("fp64", [True, False])
+def test_gradient_matches_fp64_finite_differences(setup, gpu_device, fp64):
+    """FP64 adjoint: 1e-3 of the FP64 central differences (the north_star bound).  FP32 adjoint: the
+    gradient of the FP32 model, whose residuals carry the rounding of two ~1 m coordinates (~6e-6 cm
+    against ~1e-2 cm residuals); summed over tracks with cancellation that is a few 1e-3 relative."""
     ev, geo, th, ctx = setup
     th0 = th.copy()
     th0[:, 1:] += np.random.default_rng(3).normal(0, 20e-4, (th.shape[0], 2))   # off the optimum
-    data = mk.AlignmentData(ctx, ev, gpu_device)
+    th0 = th0.astype(np.float32).astype(np.float64)      # the values the device sees
+    data = mk.AlignmentData(ctx, ev, gpu_device, fp64=fp64)
     _, grad = data.run(_theta(th0, gpu_device))
     grad = grad.cpu().numpy().astype(np.float64)
     L = th.shape[0]
@@ -55,17 +72,21 @@ def test_gradient_matches_fp64_finite_differences(setup, gpu_device):
             tp = th0.copy(); tp[l, k] += h
             tm = th0.copy(); tm[l, k] -= h
             fd[l, k] = (oracle.align_chi2(ev, tp, n_threads=8).sum() - oracle.align_chi2(ev, tm, n_threads=8).sum()) / (2 * h)
+    bound = 1e-3 if fp64 else 1e-2
     for k in range(3):
         scale = np.abs(fd[:, k]).max()
         err = np.abs(grad[:, k] - fd[:, k]) / np.maximum(np.abs(fd[:, k]), 1e-2 * scale)
-        assert err.max() < 1e-3, (k, err.max(), np.argmax(err))
+        print("fp64=%s param %d: max rel err %.3g (layer %d), median %.3g" % (fp64, k, err.max(), np.argmax(err),
+                                                                              np.median(err)))
+        assert err.max() < bound, (k, err.max(), np.argmax(err))
 
 
-def test_checkpointing_does_not_change_the_result(setup, gpu_device):
+@pytest.mark.parametrize("fp64", [False, True])
+def test_checkpointing_does_not_change_the_result(setup, gpu_device, fp64):
     ev, geo, th, ctx = setup
     out = []
     for ck in (1, 5, 25):
-        data = mk.AlignmentData(ctx, ev, gpu_device, checkpoint_every=ck)
+        data = mk.AlignmentData(ctx, ev, gpu_device, checkpoint_every=ck, fp64=fp64)
         c, g = data.run(_theta(th, gpu_device))
         out.append((c.cpu().numpy(), g.cpu().numpy()))
     for c, g in out[1:]:
