@@ -69,6 +69,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join("profiles", "k_fit_plain-fit-at-scale.md")
+
+
+def profiled_traffic():
+    """HBM bytes per batched-fit launch on the plain-fit-at-scale workload, as measured by two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, each scaled by a copy-kernel calibration) and
+    recorded in the committed profile summary; None when the summary is absent."""
+    import re
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\[k_fit batched\]: (\d+)", text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -239,7 +255,9 @@ def main():
         "batched_over_plain": tps_b / tps_p,
         "parity": parity,
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+                     "traffic": None if uneven or n != 1_000_000 else profiled_traffic(),
+                     "traffic_source": None if uneven else TRAFFIC_PROFILE,
                      "kernel": "k_fit_fused", "kernel_ms": kern_b,
                      "flop_per_track_layer": FLOP_PER_TRACK_LAYER,
                      "algorithmic_GBps": alg_bytes_b / (kern_b * 1e-3) / 1e9},
