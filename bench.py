@@ -58,7 +58,8 @@ def parse():
                     help="fit (default: the metric's workload); fit-uneven: 2M tracks, barrel + 2x9 disks, "
                          "8-30 hits, 5%% missing, FP64 fallback on; find-ce / find-bh: clone-engine / best-hit "
                          "finding per event; adjoint: gradient of the fit's chi2 (secondary lines)")
-    ap.add_argument("--events", type=int, default=4, help="distinct events generated for the find workloads")
+    ap.add_argument("--events", type=int, default=50,
+                    help="events per step of the find workloads (all in one event of hits and one launch)")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration run for rocprofv3 --pmc: one 256 MiB device copy, then "
                          "--steps batched and plain fits; no timing line")
@@ -282,44 +283,40 @@ def _timed(torch, stream, fn, steps, warmup):
 
 
 def bench_find(args, mk, torch, dev):
-    """Track finding per event: 17-layer barrel (r = 4..100 cm), U(5k, 15k) seeds per event, noise hits
-    at 2 per cm^2 on every layer, 3-hit seeds; clone engine with up to 6 candidates (chi2 cut 30) or
-    best-hit (chi2 cut 15).  A step is one event: LayerOfHits build + finding, inputs in HBM."""
+    """Track finding over a batch of events: 17-layer barrel (r = 4..100 cm), U(5k, 15k) seeds per event,
+    noise hits at 2 per cm^2 on every layer, 3-hit seeds; clone engine with up to 6 candidates (chi2 cut 30)
+    or best-hit (chi2 cut 15).  All events share one event of hits (event e owns layers e*17 ..) and one
+    finder launch; a step is the whole batch: LayerOfHits build + finding, inputs in HBM."""
     ce = args.workload == "find-ce"
     geo = mk.geometry(1)
     descs = mk.barrel_layer_descs(geo)
+    L = len(descs)
     rng = np.random.default_rng(args.seed)
     events = []
     t0 = time.time()
     for e in range(args.events):
         p = mk.default_toy_params(seed=args.seed + e, simulate_material=0)
-        ev = mk.generate_find_event(geo, p, int(rng.integers(5000, 15001)), noise_per_cm2=2.0)
-        events.append(ev)
-    log("generated %d events in %.1f s; seeds %s, hits %s" % (len(events), time.time() - t0,
-        [e["n_seeds"] for e in events], [int(e["hit_offset"][-1]) for e in events]))
+        events.append(mk.generate_find_event(geo, p, int(rng.integers(5000, 15001)), noise_per_cm2=2.0))
+    cat = mk.concat_find_events(events, L)
+    log("generated %d events in %.1f s; %d seeds, %d hits" % (len(events), time.time() - t0, cat["n_seeds"],
+                                                                int(cat["hit_offset"][-1])))
     ctx = mk.Context(bfield=3.8)
-    L = len(descs)
-    eohs, batches, hits = [], [], []
-    for ev in events:
-        eoh = mk.EventOfHits(ctx, descs)
-        h = torch.from_numpy(ev["hits_flat"]).to(dev)
-        hits.append(h)
-        eohs.append(eoh)
-        batches.append(mk.CEBatch(ev, L, 3, device=dev, chi2_cut=30.0, max_cands=6) if ce else
-                       mk.FindBatch(ev, L, 3, device=dev, chi2_cut=15.0))
+    eoh = mk.EventOfHits(ctx, descs * len(events))
+    hits = torch.from_numpy(cat["hits_flat"]).to(dev)
+    batch = (mk.CEBatch(cat, L, 3, device=dev, chi2_cut=30.0, max_cands=6, layer_base=cat["layer_base"]) if ce else
+             mk.FindBatch(cat, L, 3, device=dev, chi2_cut=15.0, layer_base=cat["layer_base"]))
     stream = torch.cuda.current_stream()
 
     def step(i):
-        k = i % len(events)
-        eohs[k].load(events[k]["hit_offset"], hits[k])
+        eoh.load(cat["hit_offset"], hits)
         if ce:
-            ctx.find_tracks_clone_engine(eohs[k], batches[k])
+            ctx.find_tracks_clone_engine(eoh, batch)
         else:
-            batches[k].reset()
-            ctx.find_tracks_best_hit(eohs[k], batches[k])
+            batch.reset()
+            ctx.find_tracks_best_hit(eoh, batch)
 
     wall, per = _timed(torch, stream, step, args.steps, args.warmup)
-    seeds = sum(events[i % len(events)]["n_seeds"] for i in range(args.steps))
+    seeds = cat["n_seeds"] * args.steps
     ms = wall * 1e3 / args.steps
     cpu = None
     if not args.no_cpu:
@@ -339,10 +336,11 @@ def bench_find(args, mk, torch, dev):
            "value": seeds / wall, "unit": "seeds/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic: toy MC finding events (signal + uniform noise hits), seeds %d.." % args.seed,
-           "config": {"workload": "find per event: 17 barrel layers, U(5k,15k) seeds, noise 2/cm^2, %s" %
-                      ("up to 6 candidates, chi2 cut 30" if ce else "best hit, chi2 cut 15"),
-                      "events": len(events)},
-           "event_ms_events": float(np.mean(per)), "cpu_baseline": cpu}
+           "config": {"workload": "find, %d events per step in one launch: 17 barrel layers, U(5k,15k) seeds per "
+                                  "event, noise 2/cm^2, %s" % (len(events), "up to 6 candidates, chi2 cut 30" if ce
+                                                               else "best hit, chi2 cut 15"),
+                      "events": len(events), "seeds": cat["n_seeds"], "hits": int(cat["hit_offset"][-1])},
+           "ms_per_event": ms / len(events), "cpu_baseline": cpu}
     print(json.dumps(res), flush=True)
     ctx.close()
 

@@ -199,7 +199,8 @@ typedef struct mkf_layer_desc {
   float c_c2_sf, c_c2_0, c_c2_1, c_c2_2;
 } mkf_layer_desc;
 
-#define MKF_MAX_FIND_LAYERS 64
+#define MKF_MAX_FIND_LAYERS 64    /* layers one track is searched through */
+#define MKF_MAX_EOH_LAYERS 2048   /* layers of one mkf_event_of_hits (several events may share it) */
 #define MKF_HIT_IDX_MAX 16 /* candidates per track and layer (MPlexHitIdxMax, MkFitter.h:18) */
 
 /* Window-selection result per track (WSR_Result, TrackerInfo.h:17-31). */
@@ -210,7 +211,12 @@ typedef struct mkf_layer_desc {
 typedef struct mkf_event_of_hits mkf_event_of_hits;
 
 /* EventOfHits (HitStructures.h:280-319): one LayerOfHits per layer, in the
- * order the finding visits them.  Device storage grows on demand at load. */
+ * order the finding visits them.  Device storage grows on demand at load.
+ * Several events may share one: event e's layers are then a contiguous run
+ * of layers (e.g. e * L .. e * L + L - 1), so that one load builds every
+ * event's LayerOfHits and one finder launch serves seeds of all of them (a
+ * seed names its event's first layer in the batch's layer_base).  At most
+ * MKF_MAX_EOH_LAYERS layers, and 1024 q bins per layer. */
 int mkf_event_of_hits_create(mkf_context* ctx, int n_layers, const mkf_layer_desc* layers, mkf_event_of_hits** out);
 int mkf_event_of_hits_destroy(mkf_event_of_hits* eoh);
 
@@ -317,6 +323,9 @@ typedef struct mkf_find_batch {
   int prop_to_hit;  /* Config::finding_requires_propagation_to_hit_pos (Config.cc:52): chi2 and update after
                        propagating to each hit (kalmanPropagateAndComputeChi2 / ...AndUpdate) */
   int intra_pflags; /* Config::finding_intra_layer_pflags for that propagation */
+  int n_layers;      /* layers per event searched (hot_idx rows); 0: all layers of the event of hits */
+  const int* layer_base; /* [n] device: first layer (in the event of hits) of each seed's event, or NULL: 0.
+                            A seed whose event would run past the last layer is left untouched. */
 } mkf_find_batch;
 
 /* MkBuilder::FindTracksBestHit (MkBuilder.cc:1362-1480) over the layers of
@@ -355,6 +364,8 @@ typedef struct mkf_ce_batch {
   float* score;           /* [K][n]  getScoreCand of each candidate; -1e16 for empty slots */
   int* n_found;           /* [K][n] */
   int* hot_idx;           /* [K][n_layers][n] hit index or fake code per layer, -9 none */
+  int n_layers;           /* layers per event searched; 0: all layers of the event of hits */
+  const int* layer_base;  /* [n] device: first layer of each seed's event, or NULL: 0 (as in mkf_find_batch) */
 } mkf_ce_batch;
 
 /* MkBuilder::FindTracksCloneEngine / find_tracks_in_layers (MkBuilder.cc:1847-2060)

@@ -116,7 +116,8 @@ class MkfCEBatch(ctypes.Structure):
                 ("pflags", ctypes.c_int), ("intra_pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int),
                 ("max_cands", ctypes.c_int), ("max_holes", ctypes.c_int), ("max_consec_holes", ctypes.c_int),
                 ("n_cands", _c_void_p), ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p),
-                ("chi2", _c_void_p), ("score", _c_void_p), ("n_found", _c_void_p), ("hot_idx", _c_void_p)]
+                ("chi2", _c_void_p), ("score", _c_void_p), ("n_found", _c_void_p), ("hot_idx", _c_void_p),
+                ("n_layers", ctypes.c_int), ("layer_base", _c_void_p)]
 
 
 class MkfAdjointBatch(ctypes.Structure):
@@ -143,7 +144,7 @@ class MkfFindBatch(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p),
                 ("chi2", _c_void_p), ("hot_idx", _c_void_p), ("n_found", _c_void_p), ("first_layer", ctypes.c_int),
                 ("chi2_cut", ctypes.c_float), ("pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int),
-                ("intra_pflags", ctypes.c_int)]
+                ("intra_pflags", ctypes.c_int), ("n_layers", ctypes.c_int), ("layer_base", _c_void_p)]
 
 
 def lib_path():
@@ -311,6 +312,30 @@ def generate_find_event(geo, params, n_tracks, noise_per_cm2=2.0, seed_layers=3,
                 seed_hits=np.ascontiguousarray(sh[:, :ns]), seed_mc=smc[:ns].copy())
 
 
+def concat_find_events(events, n_layers):
+    """Several finding events as one: the layers of event e become layers e*n_layers .. of a shared event
+    of hits, the seeds are concatenated, and layer_base [n] gives each seed its event's first layer.
+    Hit indices stay layer-local, so the finders' outputs read as for the separate events."""
+    counts, hits, mc = [], [], []
+    for ev in events:
+        counts.append(np.diff(ev["hit_offset"]).astype(np.int64))
+        hits.append(ev["hits_flat"])
+        mc.append(ev["hit_mc"])
+    cnt = np.concatenate(counts)
+    out = dict(hit_offset=np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32),
+               hits_flat=np.ascontiguousarray(np.concatenate(hits), np.float32),
+               hit_mc=np.concatenate(mc).astype(np.int32),
+               n_seeds=int(sum(ev["n_seeds"] for ev in events)))
+    for k in ("seed_par", "seed_err", "seed_hits"):
+        out[k] = np.ascontiguousarray(np.concatenate([ev[k][..., :ev["n_seeds"]] for ev in events], axis=-1))
+    for k in ("seed_chg", "seed_mc"):
+        out[k] = np.concatenate([ev[k][:ev["n_seeds"]] for ev in events])
+    out["layer_base"] = np.concatenate([np.full(ev["n_seeds"], e * n_layers, np.int32)
+                                        for e, ev in enumerate(events)])
+    out["event_seed_offset"] = np.concatenate([[0], np.cumsum([ev["n_seeds"] for ev in events])]).astype(np.int64)
+    return out
+
+
 def disk_layer_descs(geo, q_bin=1.5, min_dphi=0.01, max_dphi=0.05, min_dq=1.0, max_dq=2.0, half_thickness=0.01):
     """LayerInfo + selection limits for the +z disks of a disk-only geometry (endcap layers: q = r).
 
@@ -463,11 +488,16 @@ class FindBatch:
     """Device arrays of one best-hit finding batch (seeds in, found tracks out)."""
 
     def __init__(self, ev, n_layers, first_layer, device="cuda:0", chi2_cut=15.0, pflags=0, prop_to_hit=False,
-                 intra_pflags=0):
+                 intra_pflags=0, layer_base=None):
+        """n_layers: layers per event.  layer_base [n]: each seed's first layer in a shared event of hits
+        (several events in one; see concat_find_events), or None."""
         import torch
         n = int(ev["n_seeds"])
         dev = torch.device(device)
         self.n = n
+        self.n_layers = int(n_layers)
+        self.layer_base = None if layer_base is None else \
+            torch.from_numpy(np.ascontiguousarray(layer_base, np.int32)).to(dev)
         self.par = torch.from_numpy(np.ascontiguousarray(ev["seed_par"][:, :n], np.float32)).to(dev)
         self.err = torch.from_numpy(np.ascontiguousarray(ev["seed_err"][:, :n], np.float32)).to(dev)
         self.chg = torch.from_numpy(np.ascontiguousarray(ev["seed_chg"][:n], np.int32)).to(dev)
@@ -496,6 +526,7 @@ class FindBatch:
         b.hot_idx, b.n_found = _ptr(self.hot_idx), _ptr(self.n_found)
         b.first_layer, b.chi2_cut, b.pflags = self.first_layer, self.chi2_cut, self.pflags
         b.prop_to_hit, b.intra_pflags = self.prop_to_hit, self.intra_pflags
+        b.n_layers, b.layer_base = self.n_layers, _ptr(self.layer_base)
         return b
 
     def numpy(self):
@@ -508,12 +539,14 @@ class CEBatch:
     """Device arrays of one clone-engine finding batch: seeds in, up to max_cands candidates per seed out."""
 
     def __init__(self, ev, n_layers, first_layer, device="cuda:0", chi2_cut=30.0, pflags=0, intra_pflags=0,
-                 prop_to_hit=True, max_cands=6, max_holes=4, max_consec_holes=1):
+                 prop_to_hit=True, max_cands=6, max_holes=4, max_consec_holes=1, layer_base=None):
         import torch
         n = int(ev["n_seeds"])
         dev = torch.device(device)
         K, L = int(max_cands), int(n_layers)
         self.n, self.K, self.L = n, K, L
+        self.layer_base = None if layer_base is None else \
+            torch.from_numpy(np.ascontiguousarray(layer_base, np.int32)).to(dev)
         f = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dt)).to(dev)
         self.seed_par = f(ev["seed_par"][:, :n], np.float32)
         self.seed_err = f(ev["seed_err"][:, :n], np.float32)
@@ -547,6 +580,7 @@ class CEBatch:
             self.max_consec_holes
         for k in ("n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx"):
             setattr(b, k, _ptr(getattr(self, k)))
+        b.n_layers, b.layer_base = self.L, _ptr(self.layer_base)
         return b
 
     def numpy(self):

@@ -98,7 +98,8 @@ __device__ __forceinline__ void get(const float* X, Cand& c) {
 }
 
 struct CeArgs {
-  int n, n_layers, first_layer;
+  int n, n_layers, first_layer, eoh_layers;
+  const int* __restrict__ layer_base;
   int max_cands, max_holes, max_consec, prop_to_hit, intra_pflags, inter_pflags;
   float chi2_cut, bfield;
   Material mat;
@@ -159,7 +160,9 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   const int lane = threadIdx.x;
   const int g = lane / kS, c = lane % kS;
   const int seed = blockIdx.x * kG + g;
-  const bool gvalid = seed < a.n;
+  const int base = (seed < a.n && a.layer_base) ? a.layer_base[seed] : 0;   // first layer of the seed's event
+  const bool gvalid = seed < a.n && base >= 0 && base <= a.eoh_layers - a.n_layers;
+  const LayerDev* __restrict__ lays = a.layers + (gvalid ? base : 0);
   const size_t N = (size_t)a.n;
   const int K = a.max_cands;
 
@@ -192,7 +195,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   }
 
   for (int layer = a.first_layer; layer < a.n_layers; ++layer) {
-    const LayerDev& d = a.layers[layer];
+    const LayerDev& d = lays[layer];
     const bool barrel = d.is_barrel != 0;
     // ---- (a) find_tracks_unroll_candidates
     if (c == 0 && gvalid) {
@@ -463,14 +466,19 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   if (h->ctx != c) return fail(MKF_ERR_INVALID_ARG, "event of hits belongs to another context");
   if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
   if (b->n_seeds < 0) return fail(MKF_ERR_INVALID_ARG, "n_seeds < 0");
-  if (b->first_layer < 1 || b->first_layer > h->n_layers)
-    return fail(MKF_ERR_INVALID_ARG, "first_layer %d outside [1, %d]", b->first_layer, h->n_layers);
+  const int LE = b->n_layers ? b->n_layers : h->n_layers;   // layers per event
+  if (LE < 1 || LE > h->n_layers || LE > MKF_MAX_FIND_LAYERS)
+    return fail(MKF_ERR_INVALID_ARG, "n_layers %d outside [1, min(%d, %d)]", LE, h->n_layers, MKF_MAX_FIND_LAYERS);
+  if (b->first_layer < 1 || b->first_layer > LE)
+    return fail(MKF_ERR_INVALID_ARG, "first_layer %d outside [1, %d]", b->first_layer, LE);
+  if (!b->layer_base && LE != h->n_layers)
+    return fail(MKF_ERR_INVALID_ARG, "n_layers %d below the event of hits' %d needs layer_base", LE, h->n_layers);
   if (b->max_cands < 1 || b->max_cands > MKF_CE_MAX_CANDS)
     return fail(MKF_ERR_INVALID_ARG, "max_cands must be in [1, %d]", MKF_CE_MAX_CANDS);
   if (((b->pflags | b->intra_pflags) & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
     return fail(MKF_ERR_INVALID_ARG, "material requested but the context has no material grid");
   if (b->n_seeds == 0) return MKF_OK;
-  const size_t N = b->n_seeds, K = b->max_cands, L = h->n_layers;
+  const size_t N = b->n_seeds, K = b->max_cands, L = LE;
   if (K * 21 * N * 4 >= ((size_t)1 << 31) || K * L * N >= ((size_t)1 << 31))
     return fail(MKF_ERR_INVALID_ARG, "batch too large");
   CHECK(check_dev(b->seed_par, 6 * N * 4, "seed_par"));
@@ -485,7 +493,8 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   CHECK(check_dev(b->score, K * N * 4, "score"));
   CHECK(check_dev(b->n_found, K * N * 4, "n_found"));
   CHECK(check_dev(b->hot_idx, K * L * N * 4, "hot_idx"));
-  const int node_cap = b->first_layer + 3 * kS * (h->n_layers - b->first_layer) + kS;
+  if (b->layer_base) CHECK(check_dev(b->layer_base, N * 4, "layer_base"));
+  const int node_cap = b->first_layer + 3 * kS * (LE - b->first_layer) + kS;
   const size_t need = (size_t)node_cap * N;
   hipStream_t st = (hipStream_t)stream;
   if (need > h->ce_node_cap) {
@@ -498,7 +507,9 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   }
   CeArgs a;
   a.n = b->n_seeds;
-  a.n_layers = h->n_layers;
+  a.n_layers = LE;
+  a.eoh_layers = h->n_layers;
+  a.layer_base = b->layer_base;
   a.first_layer = b->first_layer;
   a.max_cands = b->max_cands;
   a.max_holes = b->max_holes;

@@ -42,16 +42,19 @@ using namespace mkf_find;
 
 namespace {
 
+// Per-load hit offsets into the layer table, passed by value in chunks of
+// kOffChunk layers (stream-ordered, no host staging buffer to keep alive).
+constexpr int kOffChunk = 256;
 struct OffsetsArg {
-  int off[MKF_MAX_FIND_LAYERS + 1];
+  int first, count;
+  int off[kOffChunk + 1];
 };
 
-// Per-load hit offsets into the layer table (stream-ordered; no host staging).
-__global__ void k_set_offsets(OffsetsArg o, int n_layers, LayerDev* __restrict__ L) {
+__global__ void k_set_offsets(OffsetsArg o, LayerDev* __restrict__ L) {
   const int l = threadIdx.x;
-  if (l < n_layers) {
-    L[l].hit_off = o.off[l];
-    L[l].n_hits = o.off[l + 1] - o.off[l];
+  if (l < o.count) {
+    L[o.first + l].hit_off = o.off[l];
+    L[o.first + l].n_hits = o.off[l + 1] - o.off[l];
   }
 }
 
@@ -145,7 +148,8 @@ k_select(LayerDev d, int n, const float* __restrict__ par, const float* __restri
 // ---- FindTracksBestHit -------------------------------------------------------
 
 struct FindArgs {
-  int n, n_layers, first_layer;
+  int n, n_layers, first_layer, eoh_layers;
+  const int* __restrict__ layer_base;
   float chi2_cut;
   int pflags, prop_to_hit, intra_pflags;
   float bfield;
@@ -173,6 +177,9 @@ __global__ void __launch_bounds__(256) k_find_best_hit(FindArgs a) {
   const int stride = gridDim.x * blockDim.x;
   const size_t n = (size_t)a.n;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += stride) {
+    const int base = a.layer_base ? a.layer_base[t] : 0;   // first layer of the seed's event
+    if (base < 0 || base > a.eoh_layers - a.n_layers) continue;
+    const LayerDev* __restrict__ lays = a.layers + base;
     float par[6], err[21];
 #pragma unroll
     for (int e = 0; e < 6; ++e) par[e] = a.par[e * n + t];
@@ -184,7 +191,7 @@ __global__ void __launch_bounds__(256) k_find_best_hit(FindArgs a) {
     for (int l = 0; l < a.first_layer; ++l) nfound += a.hot[l * n + t] >= 0;
     int layer = a.first_layer;
     for (; layer < a.n_layers; ++layer) {
-      const LayerDev& d = a.layers[layer];
+      const LayerDev& d = lays[layer];
       float pp[6], pe[21];
 #pragma unroll
       for (int e = 0; e < 6; ++e) pp[e] = par[e];
@@ -330,8 +337,8 @@ int mkf_event_of_hits_create(mkf_context* c, int n_layers, const mkf_layer_desc*
   CHECK(set_device(c));
   if (!out || !desc) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
   *out = nullptr;
-  if (n_layers <= 0 || n_layers > MKF_MAX_FIND_LAYERS)
-    return fail(MKF_ERR_INVALID_ARG, "n_layers must be in [1, %d]", MKF_MAX_FIND_LAYERS);
+  if (n_layers <= 0 || n_layers > MKF_MAX_EOH_LAYERS)
+    return fail(MKF_ERR_INVALID_ARG, "n_layers must be in [1, %d]", MKF_MAX_EOH_LAYERS);
   std::vector<LayerDev> L(n_layers);
   int bin_off = 0;
   for (int l = 0; l < n_layers; ++l) {
@@ -367,6 +374,7 @@ int mkf_event_of_hits_create(mkf_context* c, int n_layers, const mkf_layer_desc*
   h->total_bins = bin_off;
   hipError_t e = hipMalloc(&h->d_layers, sizeof(LayerDev) * n_layers);
   if (e == hipSuccess) e = hipMemcpy(h->d_layers, L.data(), sizeof(LayerDev) * n_layers, hipMemcpyHostToDevice);
+
   if (e == hipSuccess) e = hipMalloc(&h->d_bins, sizeof(int2) * (size_t)bin_off);
   if (e != hipSuccess) {
     int rc = fail(MKF_ERR_HIP, "event-of-hits setup: %s", hipGetErrorString(e));
@@ -398,19 +406,23 @@ int mkf_event_of_hits_load(mkf_event_of_hits* h, const int* hit_offset, const fl
   for (int l = 0; l < L; ++l)
     if (hit_offset[l + 1] < hit_offset[l]) return fail(MKF_ERR_INVALID_ARG, "hit_offset decreases at layer %d", l);
   const size_t total = (size_t)hit_offset[L];
-  if (total * 9 * 4 >= ((size_t)1 << 32)) return fail(MKF_ERR_INVALID_ARG, "too many hits in one event");
+  // hit positions are int throughout (keys, ranks, bin ranges, the sort's item count)
+  if (total >= ((size_t)1 << 31) - 1) return fail(MKF_ERR_INVALID_ARG, "too many hits in one event of hits");
   if (total) CHECK(check_dev(hits, total * 9 * 4, "hits"));
   hipStream_t st = (hipStream_t)stream;
   CHECK(grow(h, total, st));
-  OffsetsArg o;
-  std::memset(&o, 0, sizeof(o));
-  for (int l = 0; l <= L; ++l) o.off[l] = hit_offset[l];
   for (int l = 0; l < L; ++l) {
     h->layers[l].hit_off = hit_offset[l];
     h->layers[l].n_hits = hit_offset[l + 1] - hit_offset[l];
   }
-  hipLaunchKernelGGL(k_set_offsets, dim3(1), dim3(MKF_MAX_FIND_LAYERS), 0, st, o, L, h->d_layers);
-  HIP_TRY(hipGetLastError());
+  for (int f = 0; f < L; f += kOffChunk) {
+    OffsetsArg o;
+    o.first = f;
+    o.count = std::min(kOffChunk, L - f);
+    for (int l = 0; l <= o.count; ++l) o.off[l] = hit_offset[f + l];
+    hipLaunchKernelGGL(k_set_offsets, dim3(1), dim3(kOffChunk), 0, st, o, h->d_layers);
+    HIP_TRY(hipGetLastError());
+  }
   const int* bins_i = reinterpret_cast<const int*>(h->d_bins);
   hipLaunchKernelGGL(k_fill_i32, dim3(std::min(blocks_for((size_t)h->total_bins * 2, 256), 4096)), dim3(256), 0, st,
                      const_cast<int*>(bins_i), (long)h->total_bins * 2, 0);
@@ -475,23 +487,31 @@ int mkf_find_tracks_best_hit(mkf_context* c, mkf_event_of_hits* h, const mkf_fin
   if (h->ctx != c) return fail(MKF_ERR_INVALID_ARG, "event of hits belongs to another context");
   if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
   if (b->n_tracks < 0) return fail(MKF_ERR_INVALID_ARG, "n_tracks < 0");
-  if (b->first_layer < 0 || b->first_layer > h->n_layers)
-    return fail(MKF_ERR_INVALID_ARG, "first_layer %d outside [0, %d]", b->first_layer, h->n_layers);
+  const int LE = b->n_layers ? b->n_layers : h->n_layers;   // layers per event
+  if (LE < 1 || LE > h->n_layers || LE > MKF_MAX_FIND_LAYERS)
+    return fail(MKF_ERR_INVALID_ARG, "n_layers %d outside [1, min(%d, %d)]", LE, h->n_layers, MKF_MAX_FIND_LAYERS);
+  if (b->first_layer < 0 || b->first_layer > LE)
+    return fail(MKF_ERR_INVALID_ARG, "first_layer %d outside [0, %d]", b->first_layer, LE);
   if (((b->pflags | (b->prop_to_hit ? b->intra_pflags : 0)) & MKF_PF_APPLY_MATERIAL) && !c->d_rl)
     return fail(MKF_ERR_INVALID_ARG, "material requested but the context has no material grid");
   if (b->n_tracks == 0) return MKF_OK;
   const size_t N = b->n_tracks;
-  if (21 * N * 4 >= ((size_t)1 << 31) || (size_t)h->n_layers * N >= ((size_t)1 << 31))
+  if (21 * N * 4 >= ((size_t)1 << 31) || (size_t)LE * N >= ((size_t)1 << 31))
     return fail(MKF_ERR_INVALID_ARG, "batch too large");
   CHECK(check_dev(b->par, 6 * N * 4, "par"));
   CHECK(check_dev(b->err, 21 * N * 4, "err"));
   CHECK(check_dev(b->chg, N * 4, "chg"));
   CHECK(check_dev(b->chi2, N * 4, "chi2"));
-  CHECK(check_dev(b->hot_idx, (size_t)h->n_layers * N * 4, "hot_idx"));
+  CHECK(check_dev(b->hot_idx, (size_t)LE * N * 4, "hot_idx"));
   CHECK(check_dev(b->n_found, N * 4, "n_found"));
+  if (b->layer_base) CHECK(check_dev(b->layer_base, N * 4, "layer_base"));
+  else if (LE != h->n_layers)
+    return fail(MKF_ERR_INVALID_ARG, "n_layers %d below the event of hits' %d needs layer_base", LE, h->n_layers);
   FindArgs a;
   a.n = b->n_tracks;
-  a.n_layers = h->n_layers;
+  a.n_layers = LE;
+  a.eoh_layers = h->n_layers;
+  a.layer_base = b->layer_base;
   a.first_layer = b->first_layer;
   a.chi2_cut = b->chi2_cut;
   a.pflags = b->pflags;

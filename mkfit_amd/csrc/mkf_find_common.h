@@ -20,8 +20,10 @@ constexpr int kNPhi = 256;                      // Config::m_nphi
 constexpr float kFPhiFine = 1024 / kTwoPI;      // HitStructures.h:153-158 (float arithmetic)
 constexpr float kPIOver2 = kPI / 2.0f;          // Config.h:79
 constexpr int kHitIdxMax = MKF_HIT_IDX_MAX;
-constexpr int kLayerShift = 26, kQShift = 11;
-constexpr int kMaxQBins = 1 << 15;
+// Sort key: layer (11 bits) | q bin (10 bits) | fine phi bin (11 bits).
+constexpr int kLayerShift = 21, kQShift = 11;
+constexpr int kMaxQBins = 1 << 10;
+static_assert(MKF_MAX_EOH_LAYERS <= (1 << (32 - kLayerShift)), "layer index must fit the sort key");
 
 struct LayerDev {
   int is_barrel, nq, hit_off, n_hits, bin_off;

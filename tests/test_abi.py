@@ -76,7 +76,9 @@ def test_struct_layouts():
     assert ctypes.sizeof(mk.MkfFinderLanes) == 8 + 9 * 8                 # n + pad, 9 pointers
     assert mk.IDX_CHI2_DTYPE.itemsize == 48
     assert ctypes.sizeof(mk.MkfLayerDesc) == 23 * 4
-    assert ctypes.sizeof(mk.MkfFindBatch) == 4 + 4 + 6 * 8 + 5 * 4 + 4   # n, pad, 6 pointers, 5 scalars, pad
-    assert ctypes.sizeof(mk.MkfCEBatch) == 8 + 4 * 8 + 8 * 4 + 8 * 8      # n + pad, 4 ptrs, 8 scalars, 8 ptrs
+    # n, pad, 6 pointers, 6 scalars (incl. n_layers), layer_base pointer
+    assert ctypes.sizeof(mk.MkfFindBatch) == 4 + 4 + 6 * 8 + 6 * 4 + 8
+    # n + pad, 4 ptrs, 8 scalars, 8 ptrs, n_layers + pad, layer_base pointer
+    assert ctypes.sizeof(mk.MkfCEBatch) == 8 + 4 * 8 + 8 * 4 + 8 * 8 + 8 + 8
     # 2 ints, 7 ptrs, 3 scalars + pad, 2 ptrs, the fp64 switch + pad
     assert ctypes.sizeof(mk.MkfAdjointBatch) == 8 + 7 * 8 + 12 + 4 + 2 * 8 + 8

@@ -1,0 +1,102 @@
+"""Several events in one event of hits and one finder launch (batch field layer_base): every event's
+results equal those of finding it alone, and the oracle's."""
+import numpy as np
+import pytest
+
+import mkfit_amd as mk
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.fixture(scope="module")
+def events():
+    geo = mk.geometry(1)
+    descs = mk.barrel_layer_descs(geo)
+    evs = [mk.generate_find_event(geo, mk.default_toy_params(seed=40 + e, simulate_material=0), 1500 + 700 * e,
+                                  noise_per_cm2=1.0 + e) for e in range(3)]
+    return evs, descs
+
+
+def _slice(out, k, lo, hi):
+    return out[k][..., lo:hi]
+
+
+def test_best_hit_events_in_one_launch(gpu_device, events):
+    import torch
+    evs, descs = events
+    L = len(descs)
+    cat = mk.concat_find_events(evs, L)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs * len(evs))
+    eoh.load(cat["hit_offset"], _t(cat["hits_flat"], gpu_device))
+    b = mk.FindBatch(cat, L, 3, device=gpu_device, chi2_cut=15.0, layer_base=cat["layer_base"])
+    ctx.find_tracks_best_hit(eoh, b)
+    torch.cuda.synchronize()
+    got = b.numpy()
+    off = cat["event_seed_offset"]
+    for e, ev in enumerate(evs):
+        one = mk.EventOfHits(ctx, descs)
+        one.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+        b1 = mk.FindBatch(ev, L, 3, device=gpu_device, chi2_cut=15.0)
+        ctx.find_tracks_best_hit(one, b1)
+        torch.cuda.synchronize()
+        ref = b1.numpy()
+        for k in ref:
+            assert np.array_equal(_slice(got, k, off[e], off[e + 1]), ref[k]), (e, k)
+        cpu = oracle.find_best_hit(ev, descs, 3, chi2_cut=15.0, bfield=3.8, n_threads=8)
+        same = np.all(ref["hot_idx"] == cpu["hot_idx"], axis=0)
+        assert same.mean() >= 0.99, (e, same.mean())
+
+
+def test_clone_engine_events_in_one_launch(gpu_device, events):
+    import torch
+    evs, descs = events
+    L = len(descs)
+    cat = mk.concat_find_events(evs, L)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs * len(evs))
+    eoh.load(cat["hit_offset"], _t(cat["hits_flat"], gpu_device))
+    b = mk.CEBatch(cat, L, 3, device=gpu_device, chi2_cut=30.0, max_cands=6, layer_base=cat["layer_base"])
+    ctx.find_tracks_clone_engine(eoh, b)
+    torch.cuda.synchronize()
+    got = b.numpy()
+    off = cat["event_seed_offset"]
+    for e, ev in enumerate(evs):
+        one = mk.EventOfHits(ctx, descs)
+        one.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+        b1 = mk.CEBatch(ev, L, 3, device=gpu_device, chi2_cut=30.0, max_cands=6)
+        ctx.find_tracks_clone_engine(one, b1)
+        torch.cuda.synchronize()
+        ref = b1.numpy()
+        for k in ref:
+            assert np.array_equal(_slice(got, k, off[e], off[e + 1]), ref[k]), (e, k)
+
+
+def test_layer_base_out_of_range_leaves_seeds_untouched(gpu_device, events):
+    import torch
+    evs, descs = events
+    L = len(descs)
+    cat = mk.concat_find_events(evs[:2], L)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs * 2)
+    eoh.load(cat["hit_offset"], _t(cat["hits_flat"], gpu_device))
+    base = cat["layer_base"].copy()
+    base[::5] = 2 * L          # one past the last event
+    base[1::7] = -L
+    b = mk.FindBatch(cat, L, 3, device=gpu_device, layer_base=base)
+    hot0 = b.hot_idx.clone()
+    ctx.find_tracks_best_hit(eoh, b)
+    torch.cuda.synchronize()
+    bad = (base < 0) | (base > L)
+    g = b.numpy()
+    assert np.array_equal(g["par"][:, bad], cat["seed_par"][:, bad])
+    assert np.array_equal(g["hot_idx"][:, bad], hot0.cpu().numpy()[:, bad])
+    assert (g["hot_idx"][3:, ~bad] >= 0).mean() > 0.5          # the others were searched
+    with pytest.raises(mk.MkfError, match="layer_base"):
+        ctx.find_tracks_best_hit(eoh, mk.FindBatch(cat, L, 3, device=gpu_device))
