@@ -7,8 +7,13 @@
 // All arithmetic is FP32 VALU (the 6x6 products are far too small for MFMA);
 // the only FP64 is the 2x2 determinant, as in the reference.
 //
-// The arithmetic order of every expression follows the reference so results
-// agree to FP32 rounding:
+// The arithmetic follows the reference's expressions so results agree to FP32
+// rounding.  Where a step is regrouped to save instructions (the sine and
+// cosine pairs, the material terms, the dDdx / dDdy increments of the helix
+// iteration), the comment at the step gives the algebra;
+// the cancelling parts (the dDdipt / dDdphi increments, the Jacobian's small
+// differences, the Kalman gain and covariance update) keep the reference's
+// term order.  Sources:
 //   helix step      mkFit/PropagationMPlex.icc:6-246
 //   to-R driver     mkFit/PropagationMPlex.cc:408-505
 //   helixAtZ        mkFit/PropagationMPlex.cc:604-712, driver :510-601
@@ -65,6 +70,11 @@ __device__ __forceinline__ float hipo(float x, float y) { return fsqrt(x * x + y
 // squares with one rounding, then the refined square root.
 __device__ __forceinline__ float fhypot(float x, float y) { return fsqrt(__builtin_fmaf(x, x, y * y)); }
 
+// ln x as the hardware base-2 logarithm (v_log_f32, 1 ulp) times ln 2, for
+// positive normal arguments away from 1 (the material terms' arguments), where
+// it stays within a few ulp of the correctly rounded value.
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+
 __device__ __forceinline__ void sincos4(float x, float& s, float& c) {   // Matrix.h:48-56
   const float x2 = x * x;
   c = 1.f - 0.5f * x2 + 0.04166667f * x2 * x2;
@@ -86,6 +96,7 @@ struct JacR {
   float j10, j11, j13, j14;
   float j20, j21, j23, j24, j25;
   float j40, j41, j43, j44;
+  float isin_t, cos_t;   // 1 / sin(theta) and cos(theta), reused by the material step
 };
 
 // Endcap Jacobian (PropagationMPlex.cc:618-624 initial pattern; rows 2 and 3 structural).
@@ -93,10 +104,20 @@ struct JacZ {
   float j02, j03, j04, j05;
   float j12, j13, j14, j15;
   float j42, j43, j45;
+  float isin_t, cos_t;   // 1 / sin(theta) and cos(theta), reused by the material step
 };
 
 // ---- helix to radius r ------------------------------------------------------
 // par in/out (out only meaningful when the return value is false = no failure).
+//
+// The iteration and the Jacobian are the reference's (PropagationMPlex.icc:6-246)
+// with some products regrouped.  Per iteration, with (pxn, pyn) the rotated
+// momentum that becomes the next (pxin, pyin), g = x pxn + y pyn and
+// w = ipt / r0 (using k * kinv = 1): k dadx = -x w and k dady = -y w, so the
+// dDdx / dDdy increments are x (1 - w g) / r0 and y (1 - w g) / r0.  The dDdipt
+// and dDdphi brackets, and the small differences in the Jacobian, keep the
+// reference's term order: their terms nearly cancel once the step has
+// converged, and the order fixes the rounding.
 __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, float bfield, bool param_b, JacR& J) {
   const float xin = par[0], yin = par[1], zin = par[2];
   const float ipt = par[3], phiin = par[4], theta = par[5];
@@ -105,11 +126,13 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
   const float k = fdiv(float(chg) * 100.f, -kSol * bf);
   const float kinv = frcp(k);
   const float pt = frcp(ipt);
+  const float iptkinv = ipt * kinv;
 
   float D = 0.f, cosa = 0.f, sina = 0.f, cosah = 0.f, sinah = 0.f, id = 0.f;
-  float cosPorT = cosf(phiin), sinPorT = sinf(phiin);
-  float pxin = cosPorT * pt;
-  float pyin = sinPorT * pt;
+  float cosP, sinP;
+  sincosf(phiin, &sinP, &cosP);
+  float pxin = cosP * pt;
+  float pyin = sinP * pt;
 
   float dDdx = r0 > 0.f ? fdiv(-xin, r0) : 0.f;
   float dDdy = r0 > 0.f ? fdiv(-yin, r0) : 0.f;
@@ -133,25 +156,23 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
     sincos4(id * ipt * kinv * 0.5f, sinah, cosah);
     cosa = 1.f - 2.f * sinah * sinah;
     sina = 2.f * sinah * cosah;
+    const float pxn = pxin * cosa - pyin * sina;
+    const float pyn = pyin * cosa + pxin * sina;
 
     const float oor0 = (r0 > 0.f && fabsf(r - r0) < 0.0001f) ? frcp(r0) : 0.f;
     // Every derivative term is scaled by oor0: with oor0 == 0 the sums keep
     // their values, so the block is skipped (a whole wavefront skips it on the
     // first iterations, before the step has converged to within 1 um).
     if (i + 1 != kNiter && oor0 != 0.f) {
-      const float dadipt = id * kinv;
-      const float dadx = -x * ipt * kinv * oor0;
-      const float dady = -y * ipt * kinv * oor0;
+      const float g = x * pxn + y * pyn;
+      const float q = oor0 * (1.f - ipt * oor0 * g);
+      dDdx -= x * q;
+      dDdy -= y * q;
       const float pxca = pxin * cosa;
       const float pxsa = pxin * sina;
       const float pyca = pyin * cosa;
       const float pysa = pyin * sina;
-      float tmp;
-      tmp = k * dadx;
-      dDdx -= (x * (1.f + tmp * (pxca - pysa)) + y * tmp * (pyca + pxsa)) * oor0;
-      tmp = k * dady;
-      dDdy -= (x * tmp * (pxca - pysa) + y * (1.f + tmp * (pyca + pxsa))) * oor0;
-      tmp = dadipt * ipt;
+      const float tmp = id * kinv * ipt;
       dDdipt -= k * (x * (pxca * tmp - pysa * tmp - pyca - pxsa + pyin) +
                      y * (pyca * tmp + pxsa * tmp - pysa + pxca - pxin)) * pt * oor0;
       dDdphi += k * (x * (pysa - pxin + pxca) - y * (pxsa - pyin + pyca)) * oor0;
@@ -159,9 +180,8 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
 
     x = x + 2.f * k * sinah * (pxin * cosah - pyin * sinah);
     y = y + 2.f * k * sinah * (pyin * cosah + pxin * sinah);
-    const float pxinold = pxin;
-    pxin = pxin * cosa - pyin * sina;
-    pyin = pyin * cosa + pxinold * sina;
+    pxin = pxn;
+    pyin = pyn;
   }
 
   const float alpha = D * ipt * kinv;
@@ -171,28 +191,39 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
   const float dadphi = dDdphi * ipt * kinv;
 
   sincos4(alpha, sina, cosa);
+  // (cosP cosa - sinP sina) and (sinP cosa + cosP sina) are cos / sin(phi + alpha);
+  // the small differences (u cosa - sina), (1 - cosa) keep the reference's form,
+  // which forms them from alpha-sized quantities.
+  const float cc = cosP * cosa - sinP * sina;
+  const float ss = sinP * cosa + cosP * sina;
+  const float kp = k * pt, kpp = kp * pt;
+  const float u = ipt * dadipt;
+  const float ucs = u * cosa - sina, omc = 1.f - cosa, us = u * sina;
 
-  J.j00 = 1.f + k * dadx * (cosPorT * cosa - sinPorT * sina) * pt;
-  J.j01 = k * dady * (cosPorT * cosa - sinPorT * sina) * pt;
-  J.j03 = k * (cosPorT * (ipt * dadipt * cosa - sina) + sinPorT * ((1.f - cosa) - ipt * dadipt * sina)) * pt * pt;
-  J.j04 = k * (cosPorT * dadphi * cosa - sinPorT * dadphi * sina - sinPorT * sina + cosPorT * cosa - cosPorT) * pt;
-  J.j10 = k * dadx * (sinPorT * cosa + cosPorT * sina) * pt;
-  J.j11 = 1.f + k * dady * (sinPorT * cosa + cosPorT * sina) * pt;
-  J.j13 = k * (sinPorT * (ipt * dadipt * cosa - sina) + cosPorT * (ipt * dadipt * sina - (1.f - cosa))) * pt * pt;
-  J.j14 = k * (sinPorT * dadphi * cosa + cosPorT * dadphi * sina + sinPorT * cosa + cosPorT * sina - sinPorT) * pt;
+  J.j00 = 1.f + k * dadx * cc * pt;
+  J.j01 = k * dady * cc * pt;
+  J.j03 = kpp * (cosP * ucs + sinP * (omc - us));
+  J.j04 = kp * (cosP * dadphi * cosa - sinP * dadphi * sina - sinP * sina + cosP * cosa - cosP);
+  J.j10 = k * dadx * ss * pt;
+  J.j11 = 1.f + k * dady * ss * pt;
+  J.j13 = kpp * (sinP * ucs + cosP * (us - omc));
+  J.j14 = kp * (sinP * dadphi * cosa + cosP * dadphi * sina + sinP * cosa + cosP * sina - sinP);
 
-  cosPorT = cosf(theta);
-  sinPorT = sinf(theta);
-  sinPorT = frcp(sinPorT);
+  float cosT, sinT;
+  sincosf(theta, &sinT, &cosT);
+  const float isinT = frcp(sinT);
+  J.isin_t = isinT;
+  J.cos_t = cosT;
+  const float kc = kp * cosT * isinT;   // k pt cot(theta)
 
   par[0] = x;
   par[1] = y;
-  par[2] = zin + k * alpha * cosPorT * pt * sinPorT;
-  J.j20 = k * cosPorT * dadx * pt * sinPorT;
-  J.j21 = k * cosPorT * dady * pt * sinPorT;
-  J.j23 = k * cosPorT * (ipt * dadipt - alpha) * pt * pt * sinPorT;
-  J.j24 = k * dadphi * cosPorT * pt * sinPorT;
-  J.j25 = -k * alpha * pt * sinPorT * sinPorT;
+  par[2] = zin + k * alpha * cosT * pt * isinT;
+  J.j20 = kc * dadx;
+  J.j21 = kc * dady;
+  J.j23 = kc * pt * (u - alpha);
+  J.j24 = kc * dadphi;
+  J.j25 = -kp * alpha * isinT * isinT;
   par[4] = phiin + alpha;
   J.j40 = dadx;
   J.j41 = dady;
@@ -208,8 +239,11 @@ __device__ __forceinline__ void helix_at_z(float (&par)[6], int chg, float zout,
   const float k = fdiv(float(chg) * 100.f, -kSol * bf);
   const float kinv = frcp(k);
   const float pt = frcp(ipt);
-  const float cosP = cosf(phiin), sinP = sinf(phiin);
-  const float cosT = cosf(theta), sinT = sinf(theta);
+  float cosP, sinP, cosT, sinT;
+  sincosf(phiin, &sinP, &cosP);
+  sincosf(theta, &sinT, &cosT);
+  J.isin_t = frcp(sinT);
+  J.cos_t = cosT;
   const float tanT = fdiv(sinT, cosT);
   const float icos2T = frcp(cosT * cosT);
   const float pxin = cosP * pt;
@@ -242,36 +276,44 @@ __device__ __forceinline__ void helix_at_z(float (&par)[6], int chg, float zout,
 }
 
 // ---- applyMaterialEffects, one lane -----------------------------------------
+// The reference's formulas (PropagationMPlex.cc:716-767) with its ten divisions
+// and three logarithms regrouped around three reciprocals and two logarithms:
+//   p = pt / sin(theta);  E^2 = p^2 + m^2;  beta^2 = p^2 / E^2;  gamma^2 = E^2 / m^2
+//   thetaMSC = 0.0136 (1 + 0.038 ln radL) / (beta p)
+//   wmax = 2 me beta^2 gamma^2 / (1 + 2 gamma me / m + me^2 / m^2)
+//   0.5 ln(2 me beta^2 gamma^2 wmax / I^2) - deltahalf
+//     = 0.5 ln(wmax) + 0.5 ln(2 me) - ln(28.816e-9 sqrt(2.33 * 0.498)) + 0.5
+// since deltahalf carries ln(beta gamma) = 0.5 ln(beta^2 gamma^2) and ln(I / I).  The terms
+// agree with the reference's to a few FP32 ulp.  sin_t / cos_t are the sine and
+// cosine of par[5], which the helix step has already formed; isin_t = 1 / sin_t.
 __device__ __forceinline__ void material_effects(float radL, float xi, float propSign, float (&par)[6],
-                                                 float (&err)[21], bool is_barrel) {
+                                                 float (&err)[21], bool is_barrel, float isin_t, float cos_t) {
   if (radL < 1e-13f) return;
-  const float theta = par[5];
-  const float pt = frcp(par[3]);
-  const float p = fdiv(pt, sinf(theta));
-  const float p2 = p * p;
   constexpr float mpi = 0.140;
   constexpr float mpi2 = mpi * mpi;
-  const float beta2 = fdiv(p2, p2 + mpi2);
+  constexpr float me = 0.0005;
+  constexpr float I = 16.0e-9 * 10.75;
+  const float pt = frcp(par[3]);
+  const float p = pt * isin_t;
+  const float p2 = p * p;
+  const float e2 = p2 + mpi2;
+  const float beta2 = p2 * frcp(e2);
   const float beta = fsqrt(beta2);
-  const float invCos = is_barrel ? fdiv(p, pt) : frcp(fabsf(cosf(theta)));
+  const float invCos = is_barrel ? isin_t : frcp(fabsf(cos_t));
   radL = radL * invCos;
   if (radL < 1e-13f) return;
-  const float thetaMSC = fdiv(0.0136f * (1.f + 0.038f * logf(radL)), beta * p);
+  const float thetaMSC = 0.0136f * (1.f + 0.038f * flog(radL)) * frcp(beta * p);
   const float thetaMSC2 = thetaMSC * thetaMSC * radL;
   err[S(4, 4)] += thetaMSC2;
   err[S(5, 5)] += thetaMSC2;
-  const float gamma2 = fdiv(p2 + mpi2, mpi2);
-  const float gamma = fsqrt(gamma2);
-  constexpr float me = 0.0005;
-  const float wmax = fdiv(2.f * me * beta2 * gamma2, 1.f + fdiv(2.f * gamma * me, mpi) + me * me / (mpi * mpi));
-  constexpr float I = 16.0e-9 * 10.75;
-  const float deltahalf = logf(28.816e-9f * sqrtf(2.33f * 0.498f) / I) + logf(beta * gamma) - 0.5f;  // first term folds to a constant
-  const float dEdx = beta < 1.f
-                         ? (2.f * fdiv(xi * invCos * (0.5f * logf(fdiv(2.f * me * beta2 * gamma2 * wmax, I * I)) - beta2 - deltahalf), beta2))
-                         : 0.f;
-  const float dP = fdiv(propSign * dEdx, beta);
+  const float gamma = fsqrt(e2 * (1.f / mpi2));
+  const float wmax = (2.f * me / mpi2) * p2 * frcp(1.f + gamma * (2.f * me / mpi) + me * me / (mpi * mpi));
+  // 0.5 ln(2 me) - ln(28.816e-9 sqrt(2.33 * 0.498)) + 0.5 (the ln I terms cancel)
+  constexpr float kLogTerm = 14.3341009f;
+  const float dEdx = beta < 1.f ? 2.f * xi * invCos * (0.5f * flog(wmax) + kLogTerm - beta2) * frcp(beta2) : 0.f;
+  const float dP = propSign * dEdx * frcp(beta);
   par[3] = fdiv(p, fmaxf(p + dP, 0.001f) * pt);
-  err[S(3, 3)] += fdiv(dP * dP, p2 * pt * pt);
+  err[S(3, 3)] += dP * dP * frcp(p2 * pt * pt);
 }
 
 __device__ __forceinline__ void lookup_material(const Material& m, float zq, float rq, float& rl, float& xi) {
@@ -371,7 +413,7 @@ __device__ __forceinline__ bool propagate_to_r(float (&par)[6], float (&err)[21]
     if (!no_mat) lookup_material(mat, out[2], r, rl, xi);
     const float r0 = hipo(par[0], par[1]);
     const float propSign = r > r0 ? 1.f : -1.f;
-    material_effects(rl, xi, propSign, out, err, true);
+    material_effects(rl, xi, propSign, out, err, true, J.isin_t, J.cos_t);
   }
   squash_phi(out[4]);
   similarity_r(J, err);
@@ -390,13 +432,16 @@ __device__ __forceinline__ void propagate_to_z(float (&par)[6], float (&err)[21]
     float rl = 0.f, xi = 0.f;
     if (!no_mat) lookup_material(mat, zout, fhypot(par[0], par[1]), rl, xi);
     const float propSign = fabsf(zout) > fabsf(zin) ? 1.f : -1.f;
-    material_effects(rl, xi, propSign, par, err, false);
+    material_effects(rl, xi, propSign, par, err, false, J.isin_t, J.cos_t);
   }
   squash_phi(par[4]);
   similarity_z(J, err);
 }
 
 // ---- 2x2 inverse with a double determinant ----------------------------------
+// As the reference (MatriplexSym.h:427-444): the products of two floats are
+// exact in double, so a0 a2 - a1^2 is rounded once; a strong update can cancel
+// it to a few ulp of its terms, and this keeps the reference's value there.
 __device__ __forceinline__ void invert_sym2(float& a0, float& a1, float& a2) {
   const double det = (double)a0 * a2 - (double)a1 * a1;
   // 1/det to well below float resolution: hardware FP64 reciprocal refined by
@@ -411,7 +456,10 @@ __device__ __forceinline__ void invert_sym2(float& a0, float& a1, float& a2) {
 }
 
 // ---- kalmanOperation (barrel), one lane --------------------------------------
-// ps state in par/err is replaced by the updated state when `update`.
+// ps state in par/err is replaced by the updated state when `update`.  The
+// gain and the covariance update keep the reference's products and their order
+// (KalmanHTG, KalmanGain, KHMult, KHC): a strong update cancels most of C, and
+// the order fixes the rounding of what remains.
 template <bool CHI2, bool UPDATE>
 __device__ __forceinline__ float kalman_barrel(float (&par)[6], float (&err)[21], const float (&mp)[3],
                                                const float (&me)[6]) {
