@@ -62,27 +62,37 @@ __global__ void k_fill_i32(int* __restrict__ p, long n, int v) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// Hit::phi / Hit::z / Hit::r (Hit.h:64-80) and the sort key of every hit.
-__global__ void __launch_bounds__(256)
-k_hit_keys(int total, int n_layers, const float* __restrict__ hits, const LayerDev* __restrict__ L,
-           uint32_t* __restrict__ keys, int* __restrict__ vals, float* __restrict__ phi_o, float* __restrict__ q_o) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  int lo = 0, hi = n_layers - 1;   // layer owning hit i: the last one starting at or before it
+// Hit::phi and Hit::z (barrel) / Hit::r (endcap) of a hit (Hit.h:64-80).  Not
+// inlined: the key kernel and the fill kernel run this one piece of machine
+// code, so the phi and q stored with a sorted hit are bit for bit the values
+// its sort key was formed from.
+__device__ __noinline__ float2 hit_phi_q(float x, float y, float z, int is_barrel) {
+  return make_float2(atan2f(y, x), is_barrel ? z : sqrtf(x * x + y * y));
+}
+
+// Layer owning event hit i: the last one whose hit range starts at or before it.
+__device__ __forceinline__ int layer_of_hit(int i, int n_layers, const LayerDev* __restrict__ L) {
+  int lo = 0, hi = n_layers - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (L[mid].hit_off <= i) lo = mid; else hi = mid - 1;
   }
+  return lo;
+}
+
+// The sort key of every hit.
+__global__ void __launch_bounds__(256)
+k_hit_keys(int total, int n_layers, const float* __restrict__ hits, const LayerDev* __restrict__ L,
+           uint32_t* __restrict__ keys, int* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int lo = layer_of_hit(i, n_layers, L);
   const LayerDev& d = L[lo];
-  const float x = hits[(size_t)i * 9 + 0], y = hits[(size_t)i * 9 + 1], z = hits[(size_t)i * 9 + 2];
-  const float phi = atan2f(y, x);
-  const float q = d.is_barrel ? z : sqrtf(x * x + y * y);
-  const int qb = q_bin_checked(d, q);
-  const int pf = phi_bin_fine(phi);
+  const float2 pq = hit_phi_q(hits[(size_t)i * 9 + 0], hits[(size_t)i * 9 + 1], hits[(size_t)i * 9 + 2], d.is_barrel);
+  const int qb = q_bin_checked(d, pq.y);
+  const int pf = phi_bin_fine(pq.x);
   keys[i] = ((uint32_t)lo << kLayerShift) | ((uint32_t)qb << kQShift) | (uint32_t)pf;
   vals[i] = i;
-  phi_o[i] = phi;
-  q_o[i] = q;
 }
 
 // Sorted copies and bin ranges (HitStructures.cc:130-183).  The fine phi bin
@@ -92,9 +102,9 @@ k_hit_keys(int total, int n_layers, const float* __restrict__ hits, const LayerD
 // the later run wins here as well.
 __global__ void __launch_bounds__(256)
 k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* __restrict__ vals_s,
-           const LayerDev* __restrict__ L, const float* __restrict__ hits, const float* __restrict__ phi_o,
-           const float* __restrict__ q_o, int* __restrict__ ranks, int* __restrict__ inv, int2* __restrict__ bins,
-           float* __restrict__ shits, float* __restrict__ phis, float* __restrict__ qs) {
+           const LayerDev* __restrict__ L, const float* __restrict__ hits, int* __restrict__ ranks,
+           int* __restrict__ inv, int2* __restrict__ bins, float* __restrict__ shits, float* __restrict__ phis,
+           float* __restrict__ qs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   const uint32_t key = keys_s[i];
@@ -104,10 +114,14 @@ k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* 
     const int j = vals_s[i];
     ranks[i] = j - d.hit_off;
     inv[j] = i;
-    phis[i] = phi_o[j];
-    qs[i] = q_o[j];
+    float h[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) shits[(size_t)i * 9 + e] = hits[(size_t)j * 9 + e];
+    for (int e = 0; e < 9; ++e) h[e] = hits[(size_t)j * 9 + e];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) shits[(size_t)i * 9 + e] = h[e];
+    const float2 pq = hit_phi_q(h[0], h[1], h[2], d.is_barrel);
+    phis[i] = pq.x;
+    qs[i] = pq.y;
   }
   const bool fine1024 = (key & 0x400u) != 0;
   if ((pass == 1) != fine1024) return;
@@ -289,13 +303,13 @@ __global__ void __launch_bounds__(256) k_find_best_hit(FindArgs a) {
 namespace {
 
 void free_hit_arrays(mkf_event_of_hits* h) {
-  void* ps[] = {h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s, h->d_phi_o, h->d_q_o,
+  void* ps[] = {h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s,
                 h->d_phis, h->d_qs,     h->d_shits, h->d_ranks, h->d_inv,   h->d_temp};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   h->d_keys = h->d_keys_s = nullptr;
   h->d_vals = h->d_vals_s = h->d_ranks = h->d_inv = nullptr;
-  h->d_phi_o = h->d_q_o = h->d_phis = h->d_qs = h->d_shits = nullptr;
+  h->d_phis = h->d_qs = h->d_shits = nullptr;
   h->d_temp = nullptr;
   h->temp_bytes = 0;
   h->cap = 0;
@@ -311,8 +325,6 @@ int grow(mkf_event_of_hits* h, size_t total, hipStream_t st) {
   HIP_TRY(hipMalloc(&h->d_keys_s, cap * 4));
   HIP_TRY(hipMalloc(&h->d_vals, cap * 4));
   HIP_TRY(hipMalloc(&h->d_vals_s, cap * 4));
-  HIP_TRY(hipMalloc(&h->d_phi_o, cap * 4));
-  HIP_TRY(hipMalloc(&h->d_q_o, cap * 4));
   HIP_TRY(hipMalloc(&h->d_phis, cap * 4));
   HIP_TRY(hipMalloc(&h->d_qs, cap * 4));
   HIP_TRY(hipMalloc(&h->d_ranks, cap * 4));
@@ -430,14 +442,14 @@ int mkf_event_of_hits_load(mkf_event_of_hits* h, const int* hit_offset, const fl
   if (total) {
     const int nt = (int)total;
     hipLaunchKernelGGL(k_hit_keys, dim3(blocks_for(total, 256)), dim3(256), 0, st, nt, L, hits, h->d_layers,
-                       h->d_keys, h->d_vals, h->d_phi_o, h->d_q_o);
+                       h->d_keys, h->d_vals);
     HIP_TRY(hipGetLastError());
     size_t tb = h->temp_bytes;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(h->d_temp, tb, h->d_keys, h->d_keys_s, h->d_vals, h->d_vals_s, nt, 0,
                                                32, st));
     for (int pass = 0; pass < 2; ++pass) {
       hipLaunchKernelGGL(k_loh_fill, dim3(blocks_for(total, 256)), dim3(256), 0, st, nt, pass, h->d_keys_s,
-                         h->d_vals_s, h->d_layers, hits, h->d_phi_o, h->d_q_o, h->d_ranks, h->d_inv, h->d_bins, h->d_shits,
+                         h->d_vals_s, h->d_layers, hits, h->d_ranks, h->d_inv, h->d_bins, h->d_shits,
                          h->d_phis, h->d_qs);
       HIP_TRY(hipGetLastError());
     }

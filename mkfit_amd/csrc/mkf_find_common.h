@@ -221,8 +221,6 @@ struct mkf_event_of_hits {
   uint32_t* d_keys_s = nullptr;
   int* d_vals = nullptr;
   int* d_vals_s = nullptr;
-  float* d_phi_o = nullptr;
-  float* d_q_o = nullptr;
   float* d_phis = nullptr;
   float* d_qs = nullptr;
   float* d_shits = nullptr;
