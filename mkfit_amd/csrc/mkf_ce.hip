@@ -145,7 +145,7 @@ __device__ __forceinline__ void propagate(const LayerDev& d, float (&p)[6], floa
 
 __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   __shared__ float X[kLanes * kW];          // candidate state exchange
-  __shared__ float4 E[kLanes][kHitIdxMax];  // accepted-hit records {score, chi2_hit, sorted pos, hit_cnt}
+  __shared__ float4 E[kLanes][kS];          // kept accepted-hit records {score, chi2_hit, sorted pos, hit_cnt}
   __shared__ int ecnt[kLanes];
   __shared__ float fscore[kLanes];          // invalid-hit record score
   __shared__ int fcode[kLanes];             // its hit code, 0 = none
@@ -258,7 +258,28 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
           if (c2 < max_c2 && strip_q_compatible(barrel, pe, q, mp, me)) {
             const float tot = st.chi2 + c2;
             const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot, pt, true);
-            E[lane][ne++] = make_float4(sc, c2, __int_as_float(gi), __int_as_float(hc));
+            const float4 rec = make_float4(sc, c2, __int_as_float(gi), __int_as_float(hc));
+            // Keep only this lane's best K records, in hit order.  The seed's
+            // top K (by score, earlier generation first on ties) never takes a
+            // record that K records of its own lane beat, so the merge below
+            // sees every record it can select.  Worst kept = lowest score, the
+            // latest of equals; the new record comes after all kept ones, so it
+            // replaces the worst only with a strictly higher score.
+            if (ne < K) {
+              E[lane][ne++] = rec;
+            } else {
+              int wk = 0;
+              float ws = E[lane][0].x;
+              for (int k = 1; k < K; ++k)
+                if (!(E[lane][k].x > ws)) {
+                  wk = k;
+                  ws = E[lane][k].x;
+                }
+              if (sc > ws) {
+                for (int k = wk; k < K - 1; ++k) E[lane][k] = E[lane][k + 1];
+                E[lane][K - 1] = rec;
+              }
+            }
           }
           ++hc;
         });

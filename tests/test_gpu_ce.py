@@ -78,7 +78,9 @@ def _true_fraction(hot, ev, first_layer):
     return (truth == ev["seed_mc"][None, :]).sum() / h.size
 
 
-@pytest.mark.parametrize("K,noise", [(6, 2.0), (1, 2.0), (8, 3.0)])
+# K = 1 and the 8/cm^2 noise give windows with more accepted hits than K: the kernel keeps
+# only each candidate's best K records, which must not change the seed's selection.
+@pytest.mark.parametrize("K,noise", [(6, 2.0), (1, 2.0), (8, 3.0), (4, 8.0)])
 def test_ce_matches_oracle(gpu_device, K, noise):
     ev, descs = _event(100 + K, 4000, noise)
     g = _run(ev, descs, gpu_device, K)
