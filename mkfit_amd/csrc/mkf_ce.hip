@@ -145,7 +145,11 @@ __device__ __forceinline__ void propagate(const LayerDev& d, float (&p)[6], floa
 
 __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   __shared__ float X[kLanes * kW];          // candidate state exchange
-  __shared__ float4 E[kLanes][kS];          // kept accepted-hit records {score, chi2_hit, sorted pos, hit_cnt}
+  __shared__ float2 E[kLanes][kS];          // kept accepted-hit records {score, hit_cnt (int bits)}
+  __shared__ int Gh[kLanes][kHitIdxMax];    // sorted positions of each lane's window hits, in scan order
+  __shared__ float Gc[kLanes][kHitIdxMax];  // their chi2 if accepted, else -1
+  __shared__ float Gm[kLanes];              // each lane's dynamic chi2 cut
+  __shared__ int Gn[kLanes + 1];            // exclusive prefix of the window hit counts; Gn[kLanes] = total
   __shared__ int ecnt[kLanes];
   __shared__ float fscore[kLanes];          // invalid-hit record score
   __shared__ int fcode[kLanes];             // its hit code, 0 = none
@@ -222,6 +226,18 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     __syncthreads();
     if (c == 0) s_merge[g] = finding && s_any[g];
     if (c == 0 && finding && !s_any[g]) s_state[g] = 2;
+    // FindCandidatesCloneEngine (MkFinder.cc:1114-1284) in three phases, so the
+    // costly part -- the propagation to each window hit and its chi2 -- runs
+    // on a compacted list of (lane, hit) pairs with every lane of the
+    // wavefront busy, however unevenly the hits fall into the lanes' windows:
+    //   (a1) each lane propagates to the layer, builds its window and lists
+    //        the window's hits (at most 16, in scan order);
+    //   (a2) a wavefront prefix sum of the list lengths numbers all pairs, and
+    //        lane i evaluates pairs i, i + 64, ... for their owners;
+    //   (a3) each lane turns its accepted hits into records, in scan order.
+    int nh = 0;
+    float pt_l = 0.f;
+    bool search = false;
     if (active) {
       float pp[6], pe[21];
 #pragma unroll
@@ -240,56 +256,88 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       } else if (w.wsr == MKF_WSR_OUTSIDE) {
         xkind[lane] = 1;
       } else {
-        // FindCandidatesCloneEngine (MkFinder.cc:1114-1284)
-        const float max_c2 = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
-        const float pt = fabsf(1.0f / pp[3]);
-        int hc = 0, ne = 0;
-        scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) {
-          float mp[3], me[6];
-          load_hit9(a.shits + (size_t)gi * 9, mp, me);
-          float q[6], qe[21];
+        search = true;
+        Gm[lane] = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
+        pt_l = fabsf(1.0f / pp[3]);
+        nh = scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) { Gh[lane][nh++] = gi; });
+        float* xs = &X[lane * kW];                       // X is free until the end of (a)
 #pragma unroll
-          for (int e = 0; e < 6; ++e) q[e] = pp[e];
+        for (int e = 0; e < 6; ++e) xs[e] = pp[e];
 #pragma unroll
-          for (int e = 0; e < 21; ++e) qe[e] = pe[e];
-          if (a.prop_to_hit) propagate(d, q, qe, st.chg, barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a);
-          const float c2 = fabsf(barrel ? kalman_barrel<true, false>(q, qe, mp, me)
-                                        : kalman_endcap<true, false>(q, qe, mp, me));
-          if (c2 < max_c2 && strip_q_compatible(barrel, pe, q, mp, me)) {
-            const float tot = st.chi2 + c2;
-            const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot, pt, true);
-            const float4 rec = make_float4(sc, c2, __int_as_float(gi), __int_as_float(hc));
-            // Keep only this lane's best K records, in hit order.  The seed's
-            // top K (by score, earlier generation first on ties) never takes a
-            // record that K records of its own lane beat, so the merge below
-            // sees every record it can select.  Worst kept = lowest score, the
-            // latest of equals; the new record comes after all kept ones, so it
-            // replaces the worst only with a strictly higher score.
-            if (ne < K) {
-              E[lane][ne++] = rec;
-            } else {
-              int wk = 0;
-              float ws = E[lane][0].x;
-              for (int k = 1; k < K; ++k)
-                if (!(E[lane][k].x > ws)) {
-                  wk = k;
-                  ws = E[lane][k].x;
-                }
-              if (sc > ws) {
-                for (int k = wk; k < K - 1; ++k) E[lane][k] = E[lane][k + 1];
-                E[lane][K - 1] = rec;
-              }
-            }
-          }
-          ++hc;
-        });
-        ecnt[lane] = ne;
+        for (int e = 0; e < 21; ++e) xs[6 + e] = pe[e];
+        xs[27] = __int_as_float(st.chg);
         int fake = (st.ninside + st.ntail < a.max_holes && st.ntail < a.max_consec) ? -1 : -2;
         if (w.wsr == MKF_WSR_EDGE) fake = -3;
         const int nt = fake == -1 ? st.ntail + 1 : st.ntail;
         fcode[lane] = fake;
-        fscore[lane] = score_calc(st.nfound, nt, 0, st.ninside, st.chi2 < 0 ? 0.f : st.chi2, pt, true);
+        fscore[lane] = score_calc(st.nfound, nt, 0, st.ninside, st.chi2 < 0 ? 0.f : st.chi2, pt_l, true);
       }
+    }
+    // (a2) inclusive prefix sum of nh over the wavefront (the workgroup is one wavefront)
+    int incl = nh;
+#pragma unroll
+    for (int o = 1; o < kLanes; o <<= 1) {
+      const int v = __shfl_up(incl, o, kLanes);
+      if (lane >= o) incl += v;
+    }
+    Gn[lane] = incl - nh;
+    if (lane == kLanes - 1) Gn[kLanes] = incl;
+    __syncthreads();
+    const int total = Gn[kLanes];
+    for (int j = lane; j < total; j += kLanes) {
+      int lo = 0, hi = kLanes - 1;                       // owner: the last lane whose range starts at or before j
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (Gn[mid] <= j) lo = mid; else hi = mid - 1;
+      }
+      const int o = lo, k = j - Gn[lo];
+      const float* xs = &X[o * kW];
+      float mp[3], me[6];
+      load_hit9(a.shits + (size_t)Gh[o][k] * 9, mp, me);
+      float q[6], qe[21], pe[21];
+#pragma unroll
+      for (int e = 0; e < 6; ++e) q[e] = xs[e];
+#pragma unroll
+      for (int e = 0; e < 21; ++e) qe[e] = pe[e] = xs[6 + e];
+      if (a.prop_to_hit)
+        propagate(d, q, qe, __float_as_int(xs[27]), barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a);
+      const float c2 = fabsf(barrel ? kalman_barrel<true, false>(q, qe, mp, me)
+                                    : kalman_endcap<true, false>(q, qe, mp, me));
+      Gc[o][k] = (c2 < Gm[o] && strip_q_compatible(barrel, pe, q, mp, me)) ? c2 : -1.f;
+    }
+    __syncthreads();
+    // (a3) records of the accepted hits, in scan order
+    if (search) {
+      int ne = 0;
+      for (int hc = 0; hc < nh; ++hc) {
+        const float c2 = Gc[lane][hc];
+        if (!(c2 >= 0.f)) continue;
+        const float tot = st.chi2 + c2;
+        const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot, pt_l, true);
+        const float2 rec = make_float2(sc, __int_as_float(hc));
+        // Keep only this lane's best K records, in hit order.  The seed's
+        // top K (by score, earlier generation first on ties) never takes a
+        // record that K records of its own lane beat, so the merge below
+        // sees every record it can select.  Worst kept = lowest score, the
+        // latest of equals; the new record comes after all kept ones, so it
+        // replaces the worst only with a strictly higher score.
+        if (ne < K) {
+          E[lane][ne++] = rec;
+        } else {
+          int wk = 0;
+          float ws = E[lane][0].x;
+          for (int kk = 1; kk < K; ++kk)
+            if (!(E[lane][kk].x > ws)) {
+              wk = kk;
+              ws = E[lane][kk].x;
+            }
+          if (sc > ws) {
+            for (int kk = wk; kk < K - 1; ++kk) E[lane][kk] = E[lane][kk + 1];
+            E[lane][K - 1] = rec;
+          }
+        }
+      }
+      ecnt[lane] = ne;
     }
     put(&X[lane * kW], st);
     __syncthreads();
@@ -318,9 +366,9 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         for (int s = 0; s < kS; ++s) {
           const int l = g * kS + s;
           if (ptr[s] < ecnt[l]) {
-            const float4 r = E[l][ptr[s]];
-            if (__float_as_int(r.w) == hc) {
-              insert(r.x, 2, s, __float_as_int(r.z), r.y);
+            const float2 r = E[l][ptr[s]];
+            if (__float_as_int(r.y) == hc) {
+              insert(r.x, 2, s, Gh[l][hc], Gc[l][hc]);
               ++ptr[s];
             }
           }
