@@ -174,6 +174,8 @@ typedef struct mkf_adjoint_batch {
   float* grad;            /* [n_layers][3] out: d(sum over tracks of chi2)/d theta */
   int fp64;               /* 0: the FP32 fit model (as mkf_fit_batched computes it); 1: the same model in FP64,
                              forward and reverse (checkpoints twice the size) */
+  int tangent;            /* 0: each step's derivative by the hand-written reverse sweep; 1: from forward
+                             tangents, 30 input directions (a cross-check, several times slower) */
 } mkf_adjoint_batch;
 
 int mkf_fit_adjoint(mkf_context* ctx, const mkf_adjoint_batch* batch, void* stream);

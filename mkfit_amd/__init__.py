@@ -124,7 +124,7 @@ class MkfAdjointBatch(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("n_layers", ctypes.c_int)] + [
         (k, _c_void_p) for k in ("seed_par", "seed_err", "seed_chg", "hit_pos", "hit_err", "n_hits", "theta")] + [
         ("pflags", ctypes.c_int), ("xi_per_x0", ctypes.c_float), ("checkpoint_every", ctypes.c_int),
-        ("chi2", _c_void_p), ("grad", _c_void_p), ("fp64", ctypes.c_int)]
+        ("chi2", _c_void_p), ("grad", _c_void_p), ("fp64", ctypes.c_int), ("tangent", ctypes.c_int)]
 
 
 class MkfFinderLanes(ctypes.Structure):
@@ -595,7 +595,7 @@ class AlignmentData:
     """
 
     def __init__(self, ctx, ev, device="cuda:0", pflags=PF_APPLY_MATERIAL, xi_per_x0=2.2e-3, checkpoint_every=5,
-                 fp64=False):
+                 fp64=False, tangent=False):
         import torch
         dev = torch.device(device)
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
@@ -608,6 +608,7 @@ class AlignmentData:
         self.grad = torch.zeros((self.n_layers, 3), dtype=torch.float32, device=dev)
         self.pflags, self.xi_per_x0, self.checkpoint_every = int(pflags), float(xi_per_x0), int(checkpoint_every)
         self.fp64 = int(bool(fp64))
+        self.tangent = int(bool(tangent))   # step derivatives from forward tangents (cross-check) instead of reverse
 
     def run(self, theta, stream=None):
         """Per-track chi2 [n] and d(sum chi2)/d theta [n_layers, 3] at theta (device float32 [n_layers, 3])."""
@@ -618,7 +619,7 @@ class AlignmentData:
         b.hit_pos, b.hit_err, b.n_hits, b.theta = _ptr(self.hit_pos), _ptr(self.hit_err), _ptr(self.n_hits), _ptr(th)
         b.pflags, b.xi_per_x0, b.checkpoint_every = self.pflags, self.xi_per_x0, self.checkpoint_every
         b.chi2, b.grad = _ptr(self.chi2), _ptr(self.grad)
-        b.fp64 = self.fp64
+        b.fp64, b.tangent = self.fp64, self.tangent
         _check(lib().mkf_fit_adjoint(self.ctx.h, ctypes.byref(b), self.ctx._stream(stream)))
         return self.chi2.clone(), self.grad.clone()
 

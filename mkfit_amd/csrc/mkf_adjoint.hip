@@ -11,8 +11,10 @@
 // vector-Jacobian product
 //     v = lambda_out^T d(state_out)/d(in) + d(chi2_l)/d(in),
 //     in = (state_in, theta_l)  (30 directions)
-// is evaluated from forward tangents, Dual<D> carrying D input directions per
-// pass.  v[0:27] is the new lambda, v[27:30] the track's gradient for layer l.
+// is evaluated by the hand-written reverse sweep through the step
+// (mkf_adjoint_rev.h) -- or, as a cross-check, from forward tangents, Dual<D>
+// carrying D input directions per pass (10 passes).  v[0:27] is the new
+// lambda, v[27:30] the track's gradient for layer l.
 // Per-track gradient rows go to HBM and a second kernel sums them over tracks
 // in double, in a fixed order (deterministic result).
 #include <hip/hip_runtime.h>
@@ -21,6 +23,7 @@
 
 #include "../../include/mkfit_amd.h"
 #include "mkf_adjoint_math.h"
+#include "mkf_adjoint_rev.h"
 #include "mkf_internal.h"
 
 using namespace mkf;
@@ -70,9 +73,12 @@ __device__ __forceinline__ void load_hit(const AdjArgs& a, int l, int t, float (
   rh_inv = 1.0 / __builtin_sqrt(x * x + y * y);
 }
 
-template <class T>
-__global__ void __launch_bounds__(kBlock) k_adjoint(AdjArgs a) {
-  __shared__ T vbuf[kBlock][kDirs + 1];
+// TAN: the step's product from forward tangents instead of the reverse sweep.
+// The FP32 reverse sweep fits two waves per SIMD (256 VGPRs).
+template <class T, bool TAN>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && !TAN) ? 2 : 1, 8)))
+k_adjoint(AdjArgs a) {
+  __shared__ T vbuf[TAN ? kBlock : 1][kDirs + 1];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n) return;
   const size_t n = (size_t)a.n;
@@ -108,7 +114,6 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(AdjArgs a) {
   T lam[27];
 #pragma unroll
   for (int e = 0; e < 27; ++e) lam[e] = T(0);
-  T* vb = vbuf[threadIdx.x];
   for (int l = nh - 1; l >= 0; --l) {
     const int c0 = (l / a.ck) * a.ck;
     const T* ck = ckpt + (size_t)(l / a.ck) * 27 * n + t;
@@ -127,43 +132,56 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(AdjArgs a) {
     T rhi;
     load_hit(a, l, t, hp, he, rhi);
     const T th0 = a.theta[3 * l], th1 = a.theta[3 * l + 1], th2 = a.theta[3 * l + 2];
-    constexpr int TD = kTanD<T>;
-    for (int d0 = 0; d0 < kDirs; d0 += TD) {
-      using DV = Dual<T, TD>;
-      DV dp[6], de[21], m, dr, dz;
+    if constexpr (!TAN) {
+      T gp[6], ge[21], gth[3];
+      (void)align_step_vjp<T>(par, err, chg, hp, he, th0, th1, th2, a.xi_per_x0, a.bfield, a.pflags, rhi,
+                              *reinterpret_cast<const T(*)[6]>(lam), *reinterpret_cast<const T(*)[21]>(lam + 6), gp,
+                              ge, gth);
 #pragma unroll
-      for (int e = 0; e < 6; ++e) {
-        dp[e].v = par[e];
+      for (int e = 0; e < 6; ++e) lam[e] = gp[e];
 #pragma unroll
-        for (int k = 0; k < TD; ++k) dp[e].d[k] = (e == d0 + k) ? T(1) : T(0);
+      for (int e = 0; e < 21; ++e) lam[6 + e] = ge[e];
+      for (int q = 0; q < 3; ++q) gtrack[((size_t)l * 3 + q) * n + t] = gth[q];
+    } else {
+      T* vb = vbuf[threadIdx.x];
+      constexpr int TD = kTanD<T>;
+      for (int d0 = 0; d0 < kDirs; d0 += TD) {
+        using DV = Dual<T, TD>;
+        DV dp[6], de[21], m, dr, dz;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+          dp[e].v = par[e];
+#pragma unroll
+          for (int k = 0; k < TD; ++k) dp[e].d[k] = (e == d0 + k) ? T(1) : T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < 21; ++e) {
+          de[e].v = err[e];
+#pragma unroll
+          for (int k = 0; k < TD; ++k) de[e].d[k] = (6 + e == d0 + k) ? T(1) : T(0);
+        }
+        m.v = th0; dr.v = th1; dz.v = th2;
+#pragma unroll
+        for (int k = 0; k < TD; ++k) {
+          m.d[k] = (27 == d0 + k) ? T(1) : T(0);
+          dr.d[k] = (28 == d0 + k) ? T(1) : T(0);
+          dz.d[k] = (29 == d0 + k) ? T(1) : T(0);
+        }
+        const DV c2 = align_step<DV>(dp, de, chg, hp, he, m, dr, dz, a.xi_per_x0, a.bfield, a.pflags, rhi);
+#pragma unroll
+        for (int k = 0; k < TD; ++k) {
+          T v = c2.d[k];
+#pragma unroll
+          for (int e = 0; e < 6; ++e) v += lam[e] * dp[e].d[k];
+#pragma unroll
+          for (int e = 0; e < 21; ++e) v += lam[6 + e] * de[e].d[k];
+          if (d0 + k < kDirs) vb[d0 + k] = v;
+        }
       }
 #pragma unroll
-      for (int e = 0; e < 21; ++e) {
-        de[e].v = err[e];
-#pragma unroll
-        for (int k = 0; k < TD; ++k) de[e].d[k] = (6 + e == d0 + k) ? T(1) : T(0);
-      }
-      m.v = th0; dr.v = th1; dz.v = th2;
-#pragma unroll
-      for (int k = 0; k < TD; ++k) {
-        m.d[k] = (27 == d0 + k) ? T(1) : T(0);
-        dr.d[k] = (28 == d0 + k) ? T(1) : T(0);
-        dz.d[k] = (29 == d0 + k) ? T(1) : T(0);
-      }
-      const DV c2 = align_step<DV>(dp, de, chg, hp, he, m, dr, dz, a.xi_per_x0, a.bfield, a.pflags, rhi);
-#pragma unroll
-      for (int k = 0; k < TD; ++k) {
-        T v = c2.d[k];
-#pragma unroll
-        for (int e = 0; e < 6; ++e) v += lam[e] * dp[e].d[k];
-#pragma unroll
-        for (int e = 0; e < 21; ++e) v += lam[6 + e] * de[e].d[k];
-        if (d0 + k < kDirs) vb[d0 + k] = v;
-      }
+      for (int e = 0; e < 27; ++e) lam[e] = vb[e];
+      for (int q = 0; q < 3; ++q) gtrack[((size_t)l * 3 + q) * n + t] = vb[27 + q];
     }
-#pragma unroll
-    for (int e = 0; e < 27; ++e) lam[e] = vb[e];
-    for (int q = 0; q < 3; ++q) gtrack[((size_t)l * 3 + q) * n + t] = vb[27 + q];
   }
 }
 
@@ -238,12 +256,18 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   a.gtrack = static_cast<char*>(c->adj_scratch) + n_ck * 27 * N * word;
   const dim3 grid((unsigned)((N + kBlock - 1) / kBlock));
   if (b->fp64) {
-    hipLaunchKernelGGL(k_adjoint<double>, grid, dim3(kBlock), 0, st, a);
+    if (b->tangent)
+      hipLaunchKernelGGL((k_adjoint<double, true>), grid, dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_adjoint<double, false>), grid, dim3(kBlock), 0, st, a);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_sum_rows<double>, dim3((unsigned)(L * 3)), dim3(256), 0, st,
                        static_cast<const double*>(a.gtrack), a.n, b->grad);
   } else {
-    hipLaunchKernelGGL(k_adjoint<float>, grid, dim3(kBlock), 0, st, a);
+    if (b->tangent)
+      hipLaunchKernelGGL((k_adjoint<float, true>), grid, dim3(kBlock), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_adjoint<float, false>), grid, dim3(kBlock), 0, st, a);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_sum_rows<float>, dim3((unsigned)(L * 3)), dim3(256), 0, st,
                        static_cast<const float*>(a.gtrack), a.n, b->grad);

@@ -1,8 +1,10 @@
 // mkf_adjoint_math.h -- the barrel fit step written once for every model type:
 // float or double (the forward pass and the recomputation from checkpoints)
 // and Dual<T, D>, a value with D forward tangents, which gives D columns of the
-// step's Jacobian per evaluation.  The arithmetic follows mkf_track_math.h (and so the
-// reference's order of operations, cited there); only the scalar type differs.
+// step's Jacobian per evaluation (the check of the reverse-mode derivative in
+// mkf_adjoint_rev.h).  The arithmetic follows the reference's order of
+// operations as cited in mkf_track_math.h (the material step and the 2x2
+// determinant in their reference form); only the scalar type differs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -183,6 +185,14 @@ struct JacRT {
   V j00, j01, j03, j04, j10, j11, j13, j14, j20, j21, j23, j24, j25, j40, j41, j43, j44;
 };
 
+// Forward values of the helix step that its reverse needs: the position and
+// momentum entering each iteration and the accumulated path and derivatives.
+template <class V>
+struct HelixTape {
+  V x[mkf::kNiter], y[mkf::kNiter], px[mkf::kNiter], py[mkf::kNiter];
+  V D, dDdx, dDdy, dDdipt, dDdphi;
+};
+
 template <class V>
 __device__ __forceinline__ void sincos4_t(const V& x, V& s, V& c) {   // Matrix.h:48-56
   const V x2 = x * x;
@@ -191,7 +201,8 @@ __device__ __forceinline__ void sincos4_t(const V& x, V& s, V& c) {   // Matrix.
 }
 
 template <class V>
-__device__ __forceinline__ bool helix_at_r_t(V (&par)[6], int chg, const V& r, float bf, JacRT<V>& J) {
+__device__ __forceinline__ bool helix_at_r_t(V (&par)[6], int chg, const V& r, float bf, JacRT<V>& J,
+                                             HelixTape<V>* tape = nullptr) {
   const V xin = par[0], yin = par[1], zin = par[2];
   const V ipt = par[3], phiin = par[4], theta = par[5];
   V r0 = vsqrt(xin * xin + yin * yin);
@@ -210,6 +221,12 @@ __device__ __forceinline__ bool helix_at_r_t(V (&par)[6], int chg, const V& r, f
   V x = xin, y = yin;
 #pragma unroll
   for (int i = 0; i < mkf::kNiter; ++i) {
+    if (tape) {
+      tape->x[i] = x;
+      tape->y[i] = y;
+      tape->px[i] = pxin;
+      tape->py[i] = pyin;
+    }
     r0 = vsqrt(x * x + y * y);
     const V oodotp = vdiv(r0 * pt, pxin * x + pyin * y);
     if (val(oodotp) > 5.0f || val(oodotp) < 0) {
@@ -243,6 +260,13 @@ __device__ __forceinline__ bool helix_at_r_t(V (&par)[6], int chg, const V& r, f
     const V pxinold = pxin;
     pxin = pxin * cosa - pyin * sina;
     pyin = pyin * cosa + pxinold * sina;
+  }
+  if (tape) {
+    tape->D = D;
+    tape->dDdx = dDdx;
+    tape->dDdy = dDdy;
+    tape->dDdipt = dDdipt;
+    tape->dDdphi = dDdphi;
   }
   const V alpha = D * ipt * kinv;
   const V dadx = dDdx * ipt * kinv;

@@ -53,6 +53,23 @@ def test_forward_matches_oracle_model(setup, gpu_device):
 
 
 @pytest.mark.parametrize("fp64", [True, False])
+def test_reverse_sweep_matches_tangent_form(setup, gpu_device, fp64):
+    """The step derivative by the hand-written reverse sweep against the same product from 30 forward
+    tangents: in FP64 both are the exact derivative of the same model (agreement to rounding); in
+    FP32 each is that derivative evaluated in float with its own operation order."""
+    ev, geo, th, ctx = setup
+    rev = mk.AlignmentData(ctx, ev, gpu_device, fp64=fp64)
+    tan = mk.AlignmentData(ctx, ev, gpu_device, fp64=fp64, tangent=True)
+    c_r, g_r = rev.run(_theta(th, gpu_device))
+    c_t, g_t = tan.run(_theta(th, gpu_device))
+    assert np.array_equal(c_r.cpu().numpy(), c_t.cpu().numpy())       # the same forward pass
+    g_r, g_t = g_r.cpu().numpy().astype(np.float64), g_t.cpu().numpy().astype(np.float64)
+    scale = np.abs(g_t).max(axis=0)                                   # per parameter kind
+    d = np.abs(g_r - g_t) / scale
+    assert d.max() < (1e-5 if fp64 else 2e-3), d.max(axis=0)
+
+
+@pytest.mark.parametrize("fp64", [True, False])
 def test_gradient_matches_fp64_finite_differences(setup, gpu_device, fp64):
     """FP64 adjoint: 1e-3 of the FP64 central differences (the north_star bound).  FP32 adjoint: the
     gradient of the FP32 model, whose residuals carry the rounding of two ~1 m coordinates (~6e-6 cm
