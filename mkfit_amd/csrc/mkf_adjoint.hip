@@ -185,20 +185,35 @@ k_adjoint(AdjArgs a) {
   }
 }
 
-// Sum over tracks of each parameter's row, in double; one block per parameter.
+// Sum over tracks of each parameter's row, in double and in a fixed order:
+// block (row, part) sums its fixed slice of the row into partial[row][part],
+// then one thread per row adds its kSumParts partials in order.  The result
+// does not depend on timing.
+constexpr int kSumParts = 64;
 template <class T>
-__global__ void __launch_bounds__(256) k_sum_rows(const T* __restrict__ rows, int n, float* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_sum_parts(const T* __restrict__ rows, int n, double* __restrict__ partial) {
   __shared__ double part[256];
+  const int chunk = (n + kSumParts - 1) / kSumParts;
+  const int lo = blockIdx.y * chunk, hi = min(n, lo + chunk);
   const T* r = rows + (size_t)blockIdx.x * n;
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += (double)r[i];
+  for (int i = lo + threadIdx.x; i < hi; i += 256) s += (double)r[i];
   part[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = (float)part[0];
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.x * kSumParts + blockIdx.y] = part[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_rows(const double* __restrict__ partial, int n_rows, float* __restrict__ out) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n_rows) return;
+  double s = 0.0;
+  if (partial)
+    for (int p = 0; p < kSumParts; ++p) s += partial[(size_t)row * kSumParts + p];
+  out[row] = (float)s;
 }
 
 }  // namespace
@@ -216,8 +231,8 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   CHECK(check_dev(b->grad, L * 3 * 4, "grad"));
   hipStream_t st = (hipStream_t)stream;
   if (N == 0) {
-    hipLaunchKernelGGL(k_sum_rows<float>, dim3((unsigned)(L * 3)), dim3(256), 0, st, (const float*)nullptr, 0,
-                       b->grad);
+    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((L * 3 + 255) / 256)), dim3(256), 0, st, (const double*)nullptr,
+                       (int)(L * 3), b->grad);
     HIP_TRY(hipGetLastError());
     return MKF_OK;
   }
@@ -231,7 +246,9 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   CHECK(check_dev(b->chi2, N * 4, "chi2"));
   const size_t n_ck = (L + b->checkpoint_every - 1) / b->checkpoint_every;
   const size_t word = b->fp64 ? 8 : 4;
-  const size_t need = (n_ck * 27 + L * 3) * N * word;
+  // scratch: the sums' partials (256-byte aligned size), checkpoints, per-track gradient rows
+  const size_t sum_bytes = (L * 3 * kSumParts * sizeof(double) + 255) / 256 * 256;
+  const size_t need = sum_bytes + (n_ck * 27 + L * 3) * N * word;
   std::lock_guard<std::mutex> lk(c->mu);
   if (need > c->adj_bytes) {
     HIP_TRY(hipDeviceSynchronize());
@@ -252,8 +269,11 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   a.hit_pos = b->hit_pos; a.hit_err = b->hit_err; a.n_hits = b->n_hits;
   a.theta = b->theta;
   a.chi2 = b->chi2;
-  a.ckpt = c->adj_scratch;
-  a.gtrack = static_cast<char*>(c->adj_scratch) + n_ck * 27 * N * word;
+  double* partial = static_cast<double*>(c->adj_scratch);
+  a.ckpt = static_cast<char*>(c->adj_scratch) + sum_bytes;
+  a.gtrack = static_cast<char*>(a.ckpt) + n_ck * 27 * N * word;
+  const dim3 sum_grid((unsigned)(L * 3), (unsigned)kSumParts);
+  const dim3 fin_grid((unsigned)((L * 3 + 255) / 256));
   const dim3 grid((unsigned)((N + kBlock - 1) / kBlock));
   if (b->fp64) {
     if (b->tangent)
@@ -261,17 +281,19 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
     else
       hipLaunchKernelGGL((k_adjoint<double, false>), grid, dim3(kBlock), 0, st, a);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_rows<double>, dim3((unsigned)(L * 3)), dim3(256), 0, st,
-                       static_cast<const double*>(a.gtrack), a.n, b->grad);
+    hipLaunchKernelGGL(k_sum_parts<double>, sum_grid, dim3(256), 0, st, static_cast<const double*>(a.gtrack), a.n,
+                       partial);
   } else {
     if (b->tangent)
       hipLaunchKernelGGL((k_adjoint<float, true>), grid, dim3(kBlock), 0, st, a);
     else
       hipLaunchKernelGGL((k_adjoint<float, false>), grid, dim3(kBlock), 0, st, a);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_rows<float>, dim3((unsigned)(L * 3)), dim3(256), 0, st,
-                       static_cast<const float*>(a.gtrack), a.n, b->grad);
+    hipLaunchKernelGGL(k_sum_parts<float>, sum_grid, dim3(256), 0, st, static_cast<const float*>(a.gtrack), a.n,
+                       partial);
   }
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_rows, fin_grid, dim3(256), 0, st, (const double*)partial, (int)(L * 3), b->grad);
   HIP_TRY(hipGetLastError());
   return MKF_OK;
 }
