@@ -1,7 +1,8 @@
 // mkf_ce.hip -- clone-engine track finding on gfx950.
 //
-// One 64-lane workgroup carries 8 seeds through every layer: lane g*8 + c holds
-// candidate c of seed g (at most 8 candidates per seed) with its state in VGPRs.
+// One 64-lane workgroup carries 8 or 10 seeds through every layer: lane
+// g*KS + c holds candidate c of seed g (KS = 8 slots per seed, or 6 when at most
+// 6 candidates are kept) with its state in VGPRs.
 // Per layer:
 //   (a) lanes propagate their candidate to the layer, select hits and compute
 //       each hit's chi2 (after propagating to the hit, as the reference does
@@ -34,9 +35,8 @@ using namespace mkf_find;
 
 namespace {
 
-constexpr int kS = 8;                 // candidate slots per seed
-constexpr int kG = 8;                 // seeds per workgroup
-constexpr int kLanes = kS * kG;       // 64: one wavefront
+constexpr int kSMax = 8;              // candidate slots per seed, at most
+constexpr int kLanes = 64;            // one wavefront per workgroup
 constexpr int kW = 37;                // LDS words per exchanged candidate (36 used, odd stride)
 
 struct Cand {
@@ -143,7 +143,14 @@ __device__ __forceinline__ void propagate(const LayerDev& d, float (&p)[6], floa
     propagate_to_z(p, e, chg, target, pflags, false, a.bfield, a.mat);
 }
 
+// KS candidate slots per seed (max_cands <= KS): KS = 6 packs 10 seeds into the
+// wavefront's 64 lanes (4 idle), KS = 8 packs 8.  Lanes past the last whole
+// seed group (g == kG) take no part.
+template <int KS>
 __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
+  constexpr int kS = KS;
+  constexpr int kG = kLanes / KS;             // seeds per workgroup
+  constexpr int kGA = kG + 1;                 // seed rows in LDS: one more for the idle lanes
   __shared__ float X[kLanes * kW];          // candidate state exchange
   __shared__ float2 E[kLanes][kS];          // kept accepted-hit records {score, hit_cnt (int bits)}
   __shared__ int Gh[kLanes][kHitIdxMax];    // sorted positions of each lane's window hits, in scan order
@@ -154,18 +161,19 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   __shared__ float fscore[kLanes];          // invalid-hit record score
   __shared__ int fcode[kLanes];             // its hit code, 0 = none
   __shared__ int xkind[kLanes];             // 1 = held back unchanged, 2 = stopped replica
-  __shared__ int4 D[kG][kS];                // next candidates {kind, parent slot, value, node}
-  __shared__ float Dsc[kG][kS], Dc2[kG][kS];
-  __shared__ float BS[kG][kW];              // best short candidate
-  __shared__ int s_bs[kG], s_state[kG], s_ncand[kG], s_nnodes[kG], s_any[kG], s_nnew[kG], s_merge[kG];
-  __shared__ int O[kG][kS + 1];             // final order (kS = best short)
-  __shared__ int s_nout[kG];
+  __shared__ int4 D[kGA][kS];               // next candidates {kind, parent slot, value, node}
+  __shared__ float Dsc[kGA][kS], Dc2[kGA][kS];
+  __shared__ float BS[kGA][kW];             // best short candidate
+  __shared__ int s_bs[kGA], s_state[kGA], s_ncand[kGA], s_nnodes[kGA], s_any[kGA], s_nnew[kGA], s_merge[kGA];
+  __shared__ int O[kGA][kS + 1];            // final order (kS = best short)
+  __shared__ int s_nout[kGA];
 
   const int lane = threadIdx.x;
   const int g = lane / kS, c = lane % kS;
   const int seed = blockIdx.x * kG + g;
-  const int base = (seed < a.n && a.layer_base) ? a.layer_base[seed] : 0;   // first layer of the seed's event
-  const bool gvalid = seed < a.n && base >= 0 && base <= a.eoh_layers - a.n_layers;
+  const bool gin = g < kG && seed < a.n;
+  const int base = (gin && a.layer_base) ? a.layer_base[seed] : 0;   // first layer of the seed's event
+  const bool gvalid = gin && base >= 0 && base <= a.eoh_layers - a.n_layers;
   const LayerDev* __restrict__ lays = a.layers + (gvalid ? base : 0);
   const size_t N = (size_t)a.n;
   const int K = a.max_cands;
@@ -563,7 +571,7 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   CHECK(check_dev(b->n_found, K * N * 4, "n_found"));
   CHECK(check_dev(b->hot_idx, K * L * N * 4, "hot_idx"));
   if (b->layer_base) CHECK(check_dev(b->layer_base, N * 4, "layer_base"));
-  const int node_cap = b->first_layer + 3 * kS * (LE - b->first_layer) + kS;
+  const int node_cap = b->first_layer + 3 * kSMax * (LE - b->first_layer) + kSMax;
   const size_t need = (size_t)node_cap * N;
   hipStream_t st = (hipStream_t)stream;
   if (need > h->ce_node_cap) {
@@ -596,8 +604,13 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   a.node_cap = node_cap;
   a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
   a.ranks = h->d_ranks;
-  const int grid = (int)((N + kG - 1) / kG);
-  hipLaunchKernelGGL(k_find_clone_engine, dim3(grid), dim3(kLanes), 0, st, a);
+  if (b->max_cands <= 6) {
+    const int grid = (int)((N + kLanes / 6 - 1) / (kLanes / 6));
+    hipLaunchKernelGGL(k_find_clone_engine<6>, dim3(grid), dim3(kLanes), 0, st, a);
+  } else {
+    const int grid = (int)((N + kLanes / 8 - 1) / (kLanes / 8));
+    hipLaunchKernelGGL(k_find_clone_engine<8>, dim3(grid), dim3(kLanes), 0, st, a);
+  }
   HIP_TRY(hipGetLastError());
   return MKF_OK;
 }
