@@ -70,9 +70,9 @@ __device__ __noinline__ float2 hit_phi_q(float x, float y, float z, int is_barre
   return make_float2(atan2f(y, x), is_barrel ? z : sqrtf(x * x + y * y));
 }
 
-// Layer owning event hit i: the last one whose hit range starts at or before it.
-__device__ __forceinline__ int layer_of_hit(int i, int n_layers, const LayerDev* __restrict__ L) {
-  int lo = 0, hi = n_layers - 1;
+// Layer owning event hit i: the last one in [lo, hi] whose hit range starts at
+// or before it.
+__device__ __forceinline__ int layer_of_hit(int i, int lo, int hi, const LayerDev* __restrict__ L) {
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (L[mid].hit_off <= i) lo = mid; else hi = mid - 1;
@@ -80,13 +80,21 @@ __device__ __forceinline__ int layer_of_hit(int i, int n_layers, const LayerDev*
   return lo;
 }
 
-// The sort key of every hit.
+// The sort key of every hit.  The block's first and last hits are located
+// over all layers once; every thread then searches only the layers between.
 __global__ void __launch_bounds__(256)
 k_hit_keys(int total, int n_layers, const float* __restrict__ hits, const LayerDev* __restrict__ L,
            uint32_t* __restrict__ keys, int* __restrict__ vals) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int span[2];
+  const int i0 = blockIdx.x * blockDim.x;
+  if (threadIdx.x < 2) {
+    const int e = threadIdx.x == 0 ? i0 : min(i0 + (int)blockDim.x, total) - 1;
+    span[threadIdx.x] = layer_of_hit(e, 0, n_layers - 1, L);
+  }
+  __syncthreads();
+  const int i = i0 + threadIdx.x;
   if (i >= total) return;
-  const int lo = layer_of_hit(i, n_layers, L);
+  const int lo = layer_of_hit(i, span[0], span[1], L);
   const LayerDev& d = L[lo];
   const float2 pq = hit_phi_q(hits[(size_t)i * 9 + 0], hits[(size_t)i * 9 + 1], hits[(size_t)i * 9 + 2], d.is_barrel);
   const int qb = q_bin_checked(d, pq.y);
