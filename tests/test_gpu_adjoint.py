@@ -71,7 +71,7 @@ def test_reverse_sweep_matches_tangent_form(setup, gpu_device, fp64):
 
 @pytest.mark.parametrize("fp64", [True, False])
 def test_gradient_matches_fp64_finite_differences(setup, gpu_device, fp64):
-    """FP64 adjoint: 1e-3 of the FP64 central differences (the north_star bound).  FP32 adjoint: the
+    """FP64 adjoint: 1e-3 of the FP64 central differences (the required bound).  FP32 adjoint: the
     gradient of the FP32 model, whose residuals carry the rounding of two ~1 m coordinates (~6e-6 cm
     against ~1e-2 cm residuals); summed over tracks with cancellation that is a few 1e-3 relative."""
     ev, geo, th, ctx = setup
