@@ -38,6 +38,15 @@ namespace {
 constexpr int kSMax = 8;              // candidate slots per seed, at most
 constexpr int kLanes = 64;            // one wavefront per workgroup
 constexpr int kW = 37;                // LDS words per exchanged candidate (36 used, odd stride)
+// Words of a lane's X row during (a), beside the propagated state in words 0-27:
+// chi2 cut, exclusive prefix of the window hit counts, layer hit offset and
+// barrel flag of the lane's layer.  Word 36, which put / get leave alone, holds
+// the lane's hold-back kind from (a) to the merge.
+constexpr int kXCut = 28, kXPre = 29, kXOff = 30, kXBar = 31, kXKind = 36;
+// A kept accepted-hit record is {hit_cnt << kPosBits | position in the layer}.
+constexpr int kPosBits = 28;
+constexpr int kPosMask = (1 << kPosBits) - 1;
+static_assert(kHitIdxMax <= 16, "hit_cnt must fit above the position bits");
 
 struct Cand {
   float par[6], err[21];
@@ -135,9 +144,9 @@ __device__ __forceinline__ int append_node(const CeArgs& a, int s, int* cnt, int
   return k;
 }
 
-__device__ __forceinline__ void propagate(const LayerDev& d, float (&p)[6], float (&e)[21], int chg, float target,
+__device__ __forceinline__ void propagate(bool barrel, float (&p)[6], float (&e)[21], int chg, float target,
                                           int pflags, const CeArgs& a) {
-  if (d.is_barrel)
+  if (barrel)
     propagate_to_r(p, e, chg, target, pflags, false, a.bfield, a.mat);
   else
     propagate_to_z(p, e, chg, target, pflags, false, a.bfield, a.mat);
@@ -151,17 +160,16 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   constexpr int kS = KS;
   constexpr int kG = kLanes / KS;             // seeds per workgroup
   constexpr int kGA = kG + 1;                 // seed rows in LDS: one more for the idle lanes
-  __shared__ float X[kLanes * kW];          // candidate state exchange
-  __shared__ float2 E[kLanes][kS];          // kept accepted-hit records {score, hit_cnt (int bits)}
-  __shared__ int Gh[kLanes][kHitIdxMax];    // sorted positions of each lane's window hits, in scan order
-  __shared__ float Gc[kLanes][kHitIdxMax];  // their chi2 if accepted, else -1
-  __shared__ float Gm[kLanes];              // each lane's dynamic chi2 cut
-  __shared__ int Gn[kLanes + 1];            // exclusive prefix of the window hit counts; Gn[kLanes] = total
-  __shared__ int ecnt[kLanes];
-  __shared__ float fscore[kLanes];          // invalid-hit record score
-  __shared__ int fcode[kLanes];             // its hit code, 0 = none
-  __shared__ int xkind[kLanes];             // 1 = held back unchanged, 2 = stopped replica
-  __shared__ int4 D[kGA][kS];               // next candidates {kind, parent slot, value, node}
+  // 22.9 KB (kS = 6) / 23.2 KB (kS = 8): seven workgroups per CU.
+  __shared__ float X[kLanes * kW];          // candidate state exchange (and the kX* words in (a))
+  __shared__ int Gh[kLanes][kHitIdxMax];    // each lane's window hits (positions in the layer), in scan
+                                            // order; from (a3) on its kept records, first ecnt
+  __shared__ float Gc[kLanes][kHitIdxMax];  // their chi2 if accepted, else -1; then the kept records'
+  __shared__ float Es[kLanes][kS];          // the kept records' scores
+  __shared__ unsigned char ecnt[kLanes];    // kept records
+  __shared__ signed char fcode[kLanes];     // invalid-hit record's hit code, 0 = none
+  __shared__ float fscore[kLanes];          // its score
+  __shared__ int2 D[kGA][kS];               // next candidates {kind | parent slot << 2, value}
   __shared__ float Dsc[kGA][kS], Dc2[kGA][kS];
   __shared__ float BS[kGA][kW];             // best short candidate
   __shared__ int s_bs[kGA], s_state[kGA], s_ncand[kGA], s_nnodes[kGA], s_any[kGA], s_nnew[kGA], s_merge[kGA];
@@ -216,7 +224,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     }
     ecnt[lane] = 0;
     fcode[lane] = 0;
-    xkind[lane] = 0;
+    X[lane * kW + kXKind] = __int_as_float(0);      // 1 = held back unchanged, 2 = stopped replica
     __syncthreads();
     const bool finding = gvalid && s_state[g] == 1;
     const bool valid = finding && c < s_ncand[g];
@@ -252,7 +260,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       for (int e = 0; e < 6; ++e) pp[e] = st.par[e];
 #pragma unroll
       for (int e = 0; e < 21; ++e) pe[e] = st.err[e];
-      propagate(d, pp, pe, st.chg, d.propagate_to, a.inter_pflags, a);
+      propagate(barrel, pp, pe, st.chg, d.propagate_to, a.inter_pflags, a);
       Window w;
       hit_window(d, pp, pe, w);
       // find_tracks_handle_missed_layers (MkBuilder.cc:1568-1630)
@@ -260,20 +268,22 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         const int k = append_node(a, seed, &s_nnodes[g], -2, layer, st.last);
         if (k >= 0) st.last = k;
         add_hit_counts(st, -2, 0.f);
-        xkind[lane] = 2;
+        X[lane * kW + kXKind] = __int_as_float(2);
       } else if (w.wsr == MKF_WSR_OUTSIDE) {
-        xkind[lane] = 1;
+        X[lane * kW + kXKind] = __int_as_float(1);
       } else {
         search = true;
-        Gm[lane] = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
         pt_l = fabsf(1.0f / pp[3]);
-        nh = scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) { Gh[lane][nh++] = gi; });
-        float* xs = &X[lane * kW];                       // X is free until the end of (a)
+        nh = scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) { Gh[lane][nh++] = gi - d.hit_off; });
+        float* xs = &X[lane * kW];                       // words 0-35 are free until the end of (a)
 #pragma unroll
         for (int e = 0; e < 6; ++e) xs[e] = pp[e];
 #pragma unroll
         for (int e = 0; e < 21; ++e) xs[6 + e] = pe[e];
         xs[27] = __int_as_float(st.chg);
+        xs[kXCut] = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
+        xs[kXOff] = __int_as_float(d.hit_off);
+        xs[kXBar] = __int_as_float(barrel ? 1 : 0);
         int fake = (st.ninside + st.ntail < a.max_holes && st.ntail < a.max_consec) ? -1 : -2;
         if (w.wsr == MKF_WSR_EDGE) fake = -3;
         const int nt = fake == -1 ? st.ntail + 1 : st.ntail;
@@ -288,30 +298,31 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       const int v = __shfl_up(incl, o, kLanes);
       if (lane >= o) incl += v;
     }
-    Gn[lane] = incl - nh;
-    if (lane == kLanes - 1) Gn[kLanes] = incl;
+    X[lane * kW + kXPre] = __int_as_float(incl - nh);
+    const int total = __shfl(incl, kLanes - 1, kLanes);
     __syncthreads();
-    const int total = Gn[kLanes];
     for (int j = lane; j < total; j += kLanes) {
       int lo = 0, hi = kLanes - 1;                       // owner: the last lane whose range starts at or before j
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (Gn[mid] <= j) lo = mid; else hi = mid - 1;
+        if (__float_as_int(X[mid * kW + kXPre]) <= j) lo = mid; else hi = mid - 1;
       }
-      const int o = lo, k = j - Gn[lo];
-      const float* xs = &X[o * kW];
+      const float* xs = &X[lo * kW];
+      const int o = lo, k = j - __float_as_int(xs[kXPre]);
+      // The owner's layer: in a batch of events, lanes of other seeds hold other events' layers.
+      const bool ob = __float_as_int(xs[kXBar]) != 0;
       float mp[3], me[6];
-      load_hit9(a.shits + (size_t)Gh[o][k] * 9, mp, me);
+      load_hit9(a.shits + (size_t)(__float_as_int(xs[kXOff]) + Gh[o][k]) * 9, mp, me);
       float q[6], qe[21], pe[21];
 #pragma unroll
       for (int e = 0; e < 6; ++e) q[e] = xs[e];
 #pragma unroll
       for (int e = 0; e < 21; ++e) qe[e] = pe[e] = xs[6 + e];
       if (a.prop_to_hit)
-        propagate(d, q, qe, __float_as_int(xs[27]), barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a);
-      const float c2 = fabsf(barrel ? kalman_barrel<true, false>(q, qe, mp, me)
-                                    : kalman_endcap<true, false>(q, qe, mp, me));
-      Gc[o][k] = (c2 < Gm[o] && strip_q_compatible(barrel, pe, q, mp, me)) ? c2 : -1.f;
+        propagate(ob, q, qe, __float_as_int(xs[27]), ob ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a);
+      const float c2 = fabsf(ob ? kalman_barrel<true, false>(q, qe, mp, me)
+                                : kalman_endcap<true, false>(q, qe, mp, me));
+      Gc[o][k] = (c2 < xs[kXCut] && strip_q_compatible(ob, pe, q, mp, me)) ? c2 : -1.f;
     }
     __syncthreads();
     // (a3) records of the accepted hits, in scan order
@@ -322,26 +333,37 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         if (!(c2 >= 0.f)) continue;
         const float tot = st.chi2 + c2;
         const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot, pt_l, true);
-        const float2 rec = make_float2(sc, __int_as_float(hc));
+        const int rec = hc << kPosBits | Gh[lane][hc];
         // Keep only this lane's best K records, in hit order.  The seed's
         // top K (by score, earlier generation first on ties) never takes a
         // record that K records of its own lane beat, so the merge below
         // sees every record it can select.  Worst kept = lowest score, the
         // latest of equals; the new record comes after all kept ones, so it
-        // replaces the worst only with a strictly higher score.
+        // replaces the worst only with a strictly higher score.  The kept
+        // records go to the front of Gh / Gc in place: slot ne <= hc, and
+        // every slot past hc is still unread.
         if (ne < K) {
-          E[lane][ne++] = rec;
+          Es[lane][ne] = sc;
+          Gh[lane][ne] = rec;
+          Gc[lane][ne] = c2;
+          ++ne;
         } else {
           int wk = 0;
-          float ws = E[lane][0].x;
+          float ws = Es[lane][0];
           for (int kk = 1; kk < K; ++kk)
-            if (!(E[lane][kk].x > ws)) {
+            if (!(Es[lane][kk] > ws)) {
               wk = kk;
-              ws = E[lane][kk].x;
+              ws = Es[lane][kk];
             }
           if (sc > ws) {
-            for (int kk = wk; kk < K - 1; ++kk) E[lane][kk] = E[lane][kk + 1];
-            E[lane][K - 1] = rec;
+            for (int kk = wk; kk < K - 1; ++kk) {
+              Es[lane][kk] = Es[lane][kk + 1];
+              Gh[lane][kk] = Gh[lane][kk + 1];
+              Gc[lane][kk] = Gc[lane][kk + 1];
+            }
+            Es[lane][K - 1] = sc;
+            Gh[lane][K - 1] = rec;
+            Gc[lane][K - 1] = c2;
           }
         }
       }
@@ -374,9 +396,9 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         for (int s = 0; s < kS; ++s) {
           const int l = g * kS + s;
           if (ptr[s] < ecnt[l]) {
-            const float2 r = E[l][ptr[s]];
-            if (__float_as_int(r.y) == hc) {
-              insert(r.x, 2, s, Gh[l][hc], Gc[l][hc]);
+            const int r = Gh[l][ptr[s]];
+            if ((int)((unsigned)r >> kPosBits) == hc) {
+              insert(Es[l][ptr[s]], 2, s, d.hit_off + (r & kPosMask), Gc[l][ptr[s]]);
               ++ptr[s];
             }
           }
@@ -388,10 +410,10 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       int np = 0;
       int xs[kS], nx = 0;
       for (int s = 0; s < kS; ++s)
-        if (xkind[g * kS + s]) xs[nx++] = s;
+        if (__float_as_int(X[(g * kS + s) * kW + kXKind])) xs[nx++] = s;
       if (nent == 0) {
         for (int i = 0; i < nx; ++i) {
-          D[g][np] = make_int4(1, xs[i], 0, -1);
+          D[g][np] = make_int2(1 | xs[i] << 2, 0);
           Dsc[g][np] = X[(g * kS + xs[i]) * kW + 29];
           ++np;
         }
@@ -412,18 +434,18 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
             continue;
           }
           while (xi < nx && X[(g * kS + xs[xi]) * kW + 29] > tsc[ih] && np < K) {
-            D[g][np] = make_int4(1, xs[xi], 0, -1);
+            D[g][np] = make_int2(1 | xs[xi] << 2, 0);
             Dsc[g][np] = X[(g * kS + xs[xi]) * kW + 29];
             ++np; ++xi;
           }
           if (np >= K) break;
-          D[g][np] = make_int4(tkind[ih], tslot[ih], tval[ih], 0);
+          D[g][np] = make_int2(tkind[ih] | tslot[ih] << 2, tval[ih]);
           Dsc[g][np] = tsc[ih];
           Dc2[g][np] = tc2[ih];
           ++np;
         }
         while (xi < nx && np < K) {
-          D[g][np] = make_int4(1, xs[xi], 0, -1);
+          D[g][np] = make_int2(1 | xs[xi] << 2, 0);
           Dsc[g][np] = X[(g * kS + xs[xi]) * kW + 29];
           ++np; ++xi;
         }
@@ -434,19 +456,20 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     // ---- (c) new candidates; UpdateWithLastHit for those that took a hit
     if (s_merge[g] && gvalid) {
       if (c < s_nnew[g]) {
-        const int4 dd = D[g][c];
-        get(&X[(g * kS + dd.y) * kW], st);
-        if (dd.x != 1) {
-          const int hit = dd.x == 2 ? a.ranks[dd.z] : dd.z;
+        const int2 dd = D[g][c];
+        const int kind = dd.x & 3, slot = dd.x >> 2, val = dd.y;
+        get(&X[(g * kS + slot) * kW], st);
+        if (kind != 1) {
+          const int hit = kind == 2 ? a.ranks[val] : val;
           const int k = append_node(a, seed, &s_nnodes[g], hit, layer, st.last);
           if (k >= 0) st.last = k;
           add_hit_counts(st, hit, Dc2[g][c]);
           st.score = Dsc[g][c];
-          if (dd.x == 2) {
+          if (kind == 2) {
             float mp[3], me[6];
-            load_hit9(a.shits + (size_t)dd.z * 9, mp, me);
+            load_hit9(a.shits + (size_t)val * 9, mp, me);
             if (a.prop_to_hit)
-              propagate(d, st.par, st.err, st.chg, barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.inter_pflags, a);
+              propagate(barrel, st.par, st.err, st.chg, barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.inter_pflags, a);
             if (barrel)
               kalman_barrel<false, true>(st.par, st.err, mp, me);
             else
@@ -543,6 +566,8 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   if (h->ctx != c) return fail(MKF_ERR_INVALID_ARG, "event of hits belongs to another context");
   if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
   if (b->n_seeds < 0) return fail(MKF_ERR_INVALID_ARG, "n_seeds < 0");
+  for (const LayerDev& ld : h->layers)                      // hit records carry 28-bit layer positions
+    if (ld.n_hits > kPosMask) return fail(MKF_ERR_INVALID_ARG, "a layer holds %d hits, above %d", ld.n_hits, kPosMask);
   const int LE = b->n_layers ? b->n_layers : h->n_layers;   // layers per event
   if (LE < 1 || LE > h->n_layers || LE > MKF_MAX_FIND_LAYERS)
     return fail(MKF_ERR_INVALID_ARG, "n_layers %d outside [1, min(%d, %d)]", LE, h->n_layers, MKF_MAX_FIND_LAYERS);
