@@ -46,6 +46,7 @@ constexpr int kXCut = 28, kXPre = 29, kXOff = 30, kXBar = 31, kXKind = 36;
 // A kept accepted-hit record is {hit_cnt << kPosBits | position in the layer}.
 constexpr int kPosBits = 28;
 constexpr int kPosMask = (1 << kPosBits) - 1;
+constexpr int kBsW = 40;              // words per seed of the best short candidate store (36 used)
 static_assert(kHitIdxMax <= 16, "hit_cnt must fit above the position bits");
 
 struct Cand {
@@ -121,6 +122,7 @@ struct CeArgs {
   float* __restrict__ err;
   int* __restrict__ chg;
   float* __restrict__ chi2;
+  float* __restrict__ bs;         // per seed kBsW words: the best short candidate
   float* __restrict__ score;
   int* __restrict__ n_found;
   int* __restrict__ hot;
@@ -160,7 +162,8 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   constexpr int kS = KS;
   constexpr int kG = kLanes / KS;             // seeds per workgroup
   constexpr int kGA = kG + 1;                 // seed rows in LDS: one more for the idle lanes
-  // 22.9 KB (kS = 6) / 23.2 KB (kS = 8): seven workgroups per CU.
+  // 20.0 KB (kS = 6): eight workgroups per CU, as many as the VGPRs allow;
+  // 20.6 KB (kS = 8): seven.
   __shared__ float X[kLanes * kW];          // candidate state exchange (and the kX* words in (a))
   __shared__ int Gh[kLanes][kHitIdxMax];    // each lane's window hits (positions in the layer), in scan
                                             // order; from (a3) on its kept records, first ecnt
@@ -169,12 +172,11 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   __shared__ unsigned char ecnt[kLanes];    // kept records
   __shared__ signed char fcode[kLanes];     // invalid-hit record's hit code, 0 = none
   __shared__ float fscore[kLanes];          // its score
-  __shared__ int2 D[kGA][kS];               // next candidates {kind | parent slot << 2, value}
-  __shared__ float Dsc[kGA][kS], Dc2[kGA][kS];
-  __shared__ float BS[kGA][kW];             // best short candidate
-  __shared__ int s_bs[kGA], s_state[kGA], s_ncand[kGA], s_nnodes[kGA], s_any[kGA], s_nnew[kGA], s_merge[kGA];
-  __shared__ int O[kGA][kS + 1];            // final order (kS = best short)
-  __shared__ int s_nout[kGA];
+  __shared__ int D[kGA][kS];                // next candidates: kind | parent slot << 2 | record << 5
+  __shared__ float s_bsc[kGA];              // score of the best short candidate (the candidate: a.bs)
+  __shared__ int s_nnodes[kGA];
+  __shared__ unsigned char s_bs[kGA], s_state[kGA], s_ncand[kGA], s_any[kGA], s_nnew[kGA], s_merge[kGA];
+  __shared__ unsigned char s_nout[kGA];
 
   const int lane = threadIdx.x;
   const int g = lane / kS, c = lane % kS;
@@ -192,7 +194,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     s_ncand[g] = gvalid ? 1 : 0;
     s_nnodes[g] = 0;
     s_bs[g] = 0;
-    BS[g][29] = kScoreWorst;
+    s_bsc[g] = kScoreWorst;
   }
   __syncthreads();
   if (gvalid && c == 0) {                                    // CombCandidate::ImportSeed
@@ -374,19 +376,18 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     // ---- (b) CandCloner::ProcessSeedRange, one lane per seed
     if (c == 0 && s_merge[g]) {
       float tsc[kS];
-      int tkind[kS], tslot[kS], tval[kS];
-      float tc2[kS];
+      int tkind[kS], tslot[kS], tval[kS];         // tval: record of the slot's lane (kind 2), hit code (3)
       int nt = 0, nent = 0;
-      auto insert = [&](float sc, int kind, int slot, int val, float cc2) {
+      auto insert = [&](float sc, int kind, int slot, int val) {
         ++nent;
         if (nt == K && !(sc > tsc[K - 1])) return;
         int pos = nt < K ? nt : K - 1;
         while (pos > 0 && tsc[pos - 1] < sc) {
           tsc[pos] = tsc[pos - 1]; tkind[pos] = tkind[pos - 1]; tslot[pos] = tslot[pos - 1];
-          tval[pos] = tval[pos - 1]; tc2[pos] = tc2[pos - 1];
+          tval[pos] = tval[pos - 1];
           --pos;
         }
-        tsc[pos] = sc; tkind[pos] = kind; tslot[pos] = slot; tval[pos] = val; tc2[pos] = cc2;
+        tsc[pos] = sc; tkind[pos] = kind; tslot[pos] = slot; tval[pos] = val;
         if (nt < K) ++nt;
       };
       int ptr[kS];
@@ -398,14 +399,14 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
           if (ptr[s] < ecnt[l]) {
             const int r = Gh[l][ptr[s]];
             if ((int)((unsigned)r >> kPosBits) == hc) {
-              insert(Es[l][ptr[s]], 2, s, d.hit_off + (r & kPosMask), Gc[l][ptr[s]]);
+              insert(Es[l][ptr[s]], 2, s, ptr[s]);
               ++ptr[s];
             }
           }
         }
       for (int s = 0; s < kS; ++s) {
         const int l = g * kS + s;
-        if (fcode[l]) insert(fscore[l], 3, s, fcode[l], 0.f);
+        if (fcode[l]) insert(fscore[l], 3, s, fcode[l]);
       }
       int np = 0;
       int xs[kS], nx = 0;
@@ -413,40 +414,36 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         if (__float_as_int(X[(g * kS + s) * kW + kXKind])) xs[nx++] = s;
       if (nent == 0) {
         for (int i = 0; i < nx; ++i) {
-          D[g][np] = make_int2(1 | xs[i] << 2, 0);
-          Dsc[g][np] = X[(g * kS + xs[i]) * kW + 29];
+          D[g][np] = 1 | xs[i] << 2;
           ++np;
         }
       } else {
         int xi = 0;
         for (int ih = 0; ih < nt; ++ih) {
           if (tkind[ih] == 3 && tval[ih] == -2) {
-            if (tsc[ih] > BS[g][29]) {                      // best short candidate
+            if (tsc[ih] > s_bsc[g]) {                       // best short candidate
               Cand b;
               get(&X[(g * kS + tslot[ih]) * kW], b);
               const int k = append_node(a, seed, &s_nnodes[g], -2, layer, b.last);
               if (k >= 0) b.last = k;
               add_hit_counts(b, -2, 0.f);
               b.score = tsc[ih];
-              put(BS[g], b);
+              put(a.bs + (size_t)seed * kBsW, b);
+              s_bsc[g] = b.score;
               s_bs[g] = 1;
             }
             continue;
           }
           while (xi < nx && X[(g * kS + xs[xi]) * kW + 29] > tsc[ih] && np < K) {
-            D[g][np] = make_int2(1 | xs[xi] << 2, 0);
-            Dsc[g][np] = X[(g * kS + xs[xi]) * kW + 29];
+            D[g][np] = 1 | xs[xi] << 2;
             ++np; ++xi;
           }
           if (np >= K) break;
-          D[g][np] = make_int2(tkind[ih] | tslot[ih] << 2, tval[ih]);
-          Dsc[g][np] = tsc[ih];
-          Dc2[g][np] = tc2[ih];
+          D[g][np] = tkind[ih] | tslot[ih] << 2 | (tkind[ih] == 2 ? tval[ih] << 5 : 0);
           ++np;
         }
         while (xi < nx && np < K) {
-          D[g][np] = make_int2(1 | xs[xi] << 2, 0);
-          Dsc[g][np] = X[(g * kS + xs[xi]) * kW + 29];
+          D[g][np] = 1 | xs[xi] << 2;
           ++np; ++xi;
         }
       }
@@ -456,15 +453,17 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     // ---- (c) new candidates; UpdateWithLastHit for those that took a hit
     if (s_merge[g] && gvalid) {
       if (c < s_nnew[g]) {
-        const int2 dd = D[g][c];
-        const int kind = dd.x & 3, slot = dd.x >> 2, val = dd.y;
-        get(&X[(g * kS + slot) * kW], st);
+        const int dd = D[g][c];
+        const int kind = dd & 3, l = g * kS + ((dd >> 2) & 7), ri = dd >> 5;
+        get(&X[l * kW], st);
         if (kind != 1) {
+          // kind 2: the slot lane's kept record ri; kind 3: its invalid-hit record
+          const int val = kind == 2 ? d.hit_off + (Gh[l][ri] & kPosMask) : fcode[l];
           const int hit = kind == 2 ? a.ranks[val] : val;
           const int k = append_node(a, seed, &s_nnodes[g], hit, layer, st.last);
           if (k >= 0) st.last = k;
-          add_hit_counts(st, hit, Dc2[g][c]);
-          st.score = Dsc[g][c];
+          add_hit_counts(st, hit, kind == 2 ? Gc[l][ri] : 0.f);
+          st.score = kind == 2 ? Es[l][ri] : fscore[l];
           if (kind == 2) {
             float mp[3], me[6];
             load_hit9(a.shits + (size_t)val * 9, mp, me);
@@ -492,6 +491,8 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   if (valid) st.score = cand_score(st);
   put(&X[lane * kW], st);
   __syncthreads();
+  int (*O)[kS + 1] = reinterpret_cast<int (*)[kS + 1]>(&Gh[0][0]);   // final order (kS = best short); Gh is free
+  static_assert(kGA * (kS + 1) <= kLanes * kHitIdxMax, "the final order must fit in Gh");
   if (c == 0 && gvalid) {
     const int nc = s_ncand[g];
     int ord[kS + 1];
@@ -510,9 +511,9 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       // The best short candidate's score is recomputed only alongside the
       // candidates' (HitStructures.cc:515-521); alone it keeps its finding score.
       Cand b;
-      get(BS[g], b);
+      get(a.bs + (size_t)seed * kBsW, b);
       if (m > 0) b.score = cand_score(b);
-      put(BS[g], b);
+      put(a.bs + (size_t)seed * kBsW, b);
       if (m > 0) {
         if (b.score > X[(g * kS + ord[m - 1]) * kW + 29]) {
           int ci = 0;
@@ -541,7 +542,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     return;
   }
   Cand o;
-  get(O[g][c] == kS ? BS[g] : &X[(g * kS + O[g][c]) * kW], o);
+  get(O[g][c] == kS ? a.bs + (size_t)seed * kBsW : &X[(g * kS + O[g][c]) * kW], o);
 #pragma unroll
   for (int e = 0; e < 6; ++e) a.par[((size_t)c * 6 + e) * N + seed] = o.par[e];
 #pragma unroll
@@ -597,7 +598,7 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   CHECK(check_dev(b->hot_idx, K * L * N * 4, "hot_idx"));
   if (b->layer_base) CHECK(check_dev(b->layer_base, N * 4, "layer_base"));
   const int node_cap = b->first_layer + 3 * kSMax * (LE - b->first_layer) + kSMax;
-  const size_t need = (size_t)node_cap * N;
+  const size_t need = ((size_t)node_cap + kBsW / 4) * N;    // nodes, then the best short candidates
   hipStream_t st = (hipStream_t)stream;
   if (need > h->ce_node_cap) {
     HIP_TRY(hipStreamSynchronize(st));
@@ -627,6 +628,7 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   a.n_found = b->n_found; a.hot = b->hot_idx;
   a.nodes = reinterpret_cast<int4*>(h->d_ce_nodes);
   a.node_cap = node_cap;
+  a.bs = reinterpret_cast<float*>(a.nodes + (size_t)node_cap * N);
   a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
   a.ranks = h->d_ranks;
   if (b->max_cands <= 6) {
