@@ -390,20 +390,37 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         tsc[pos] = sc; tkind[pos] = kind; tslot[pos] = slot; tval[pos] = val;
         if (nt < K) ++nt;
       };
-      int ptr[kS];
+      // Generation order: hit_cnt, then lane.  Each slot's records are in
+      // increasing hit_cnt, so the records are merged by always taking the
+      // slot whose next record has the lowest hit_cnt (the lowest slot among
+      // equals); a slot with none left reads kHitIdxMax.
+      int ptr[kS], cur[kS];
 #pragma unroll
-      for (int s = 0; s < kS; ++s) ptr[s] = 0;
-      for (int hc = 0; hc < kHitIdxMax; ++hc)                // generation order: hit_cnt, then lane
-        for (int s = 0; s < kS; ++s) {
-          const int l = g * kS + s;
-          if (ptr[s] < ecnt[l]) {
-            const int r = Gh[l][ptr[s]];
-            if ((int)((unsigned)r >> kPosBits) == hc) {
-              insert(Es[l][ptr[s]], 2, s, ptr[s]);
-              ++ptr[s];
-            }
+      for (int s = 0; s < kS; ++s) {
+        const int l = g * kS + s;
+        ptr[s] = 0;
+        cur[s] = ecnt[l] > 0 ? (int)((unsigned)Gh[l][0] >> kPosBits) : kHitIdxMax;
+      }
+      while (true) {
+        int sb = 0, hb = cur[0], pb = ptr[0];
+#pragma unroll
+        for (int s = 1; s < kS; ++s)
+          if (cur[s] < hb) {
+            hb = cur[s];
+            sb = s;
+            pb = ptr[s];
           }
-        }
+        if (hb >= kHitIdxMax) break;
+        const int l = g * kS + sb;
+        insert(Es[l][pb], 2, sb, pb);
+        const int nx = pb + 1 < ecnt[l] ? (int)((unsigned)Gh[l][pb + 1] >> kPosBits) : kHitIdxMax;
+#pragma unroll
+        for (int s = 0; s < kS; ++s)
+          if (s == sb) {
+            ptr[s] = pb + 1;
+            cur[s] = nx;
+          }
+      }
       for (int s = 0; s < kS; ++s) {
         const int l = g * kS + s;
         if (fcode[l]) insert(fscore[l], 3, s, fcode[l]);
