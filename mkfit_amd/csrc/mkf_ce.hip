@@ -375,19 +375,32 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     __syncthreads();
     // ---- (b) CandCloner::ProcessSeedRange, one lane per seed
     if (c == 0 && s_merge[g]) {
+      // The seed's best K so far, by score; a new entry goes after the kept
+      // ones of equal score.  Entry info: kind | slot << 2 | v << 5 with v the
+      // record in the slot lane's list (kind 2) or minus the hit code (kind 3).
+      // Every array index is a compile-time constant, so the lists stay in
+      // registers.
       float tsc[kS];
-      int tkind[kS], tslot[kS], tval[kS];         // tval: record of the slot's lane (kind 2), hit code (3)
+      int ti[kS];
       int nt = 0, nent = 0;
-      auto insert = [&](float sc, int kind, int slot, int val) {
+      auto insert = [&](float sc, int info) {
         ++nent;
-        if (nt == K && !(sc > tsc[K - 1])) return;
-        int pos = nt < K ? nt : K - 1;
-        while (pos > 0 && tsc[pos - 1] < sc) {
-          tsc[pos] = tsc[pos - 1]; tkind[pos] = tkind[pos - 1]; tslot[pos] = tslot[pos - 1];
-          tval[pos] = tval[pos - 1];
-          --pos;
-        }
-        tsc[pos] = sc; tkind[pos] = kind; tslot[pos] = slot; tval[pos] = val;
+        int pos = 0;                                  // kept entries with score >= sc
+#pragma unroll
+        for (int q = 0; q < kS; ++q) pos += (q < nt && !(tsc[q] < sc)) ? 1 : 0;
+        if (pos >= K) return;
+#pragma unroll
+        for (int q = kS - 1; q > 0; --q)
+          if (q > pos) {
+            tsc[q] = tsc[q - 1];
+            ti[q] = ti[q - 1];
+          }
+#pragma unroll
+        for (int q = 0; q < kS; ++q)
+          if (q == pos) {
+            tsc[q] = sc;
+            ti[q] = info;
+          }
         if (nt < K) ++nt;
       };
       // Generation order: hit_cnt, then lane.  Each slot's records are in
@@ -412,7 +425,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
           }
         if (hb >= kHitIdxMax) break;
         const int l = g * kS + sb;
-        insert(Es[l][pb], 2, sb, pb);
+        insert(Es[l][pb], 2 | sb << 2 | pb << 5);
         const int nx = pb + 1 < ecnt[l] ? (int)((unsigned)Gh[l][pb + 1] >> kPosBits) : kHitIdxMax;
 #pragma unroll
         for (int s = 0; s < kS; ++s)
@@ -423,45 +436,54 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       }
       for (int s = 0; s < kS; ++s) {
         const int l = g * kS + s;
-        if (fcode[l]) insert(fscore[l], 3, s, fcode[l]);
+        if (fcode[l]) insert(fscore[l], 3 | s << 2 | -fcode[l] << 5);
       }
       int np = 0;
-      int xs[kS], nx = 0;
+      unsigned xm = 0;                                // held-back slots, lowest first
       for (int s = 0; s < kS; ++s)
-        if (__float_as_int(X[(g * kS + s) * kW + kXKind])) xs[nx++] = s;
+        if (__float_as_int(X[(g * kS + s) * kW + kXKind])) xm |= 1u << s;
       if (nent == 0) {
-        for (int i = 0; i < nx; ++i) {
-          D[g][np] = 1 | xs[i] << 2;
+        for (; xm; xm &= xm - 1) {
+          D[g][np] = 1 | (__ffs(xm) - 1) << 2;
           ++np;
         }
       } else {
-        int xi = 0;
         for (int ih = 0; ih < nt; ++ih) {
-          if (tkind[ih] == 3 && tval[ih] == -2) {
-            if (tsc[ih] > s_bsc[g]) {                       // best short candidate
+          float sc = tsc[0];
+          int info = ti[0];
+#pragma unroll
+          for (int q = 1; q < kS; ++q)
+            if (q == ih) {
+              sc = tsc[q];
+              info = ti[q];
+            }
+          const int kind = info & 3, slot = (info >> 2) & 7;
+          if (kind == 3 && (info >> 5) == 2) {        // hit code -2: a short candidate
+            if (sc > s_bsc[g]) {                      // best short candidate
               Cand b;
-              get(&X[(g * kS + tslot[ih]) * kW], b);
+              get(&X[(g * kS + slot) * kW], b);
               const int k = append_node(a, seed, &s_nnodes[g], -2, layer, b.last);
               if (k >= 0) b.last = k;
               add_hit_counts(b, -2, 0.f);
-              b.score = tsc[ih];
+              b.score = sc;
               put(a.bs + (size_t)seed * kBsW, b);
               s_bsc[g] = b.score;
               s_bs[g] = 1;
             }
             continue;
           }
-          while (xi < nx && X[(g * kS + xs[xi]) * kW + 29] > tsc[ih] && np < K) {
-            D[g][np] = 1 | xs[xi] << 2;
-            ++np; ++xi;
+          while (xm && X[(g * kS + __ffs(xm) - 1) * kW + 29] > sc && np < K) {
+            D[g][np] = 1 | (__ffs(xm) - 1) << 2;
+            ++np;
+            xm &= xm - 1;
           }
           if (np >= K) break;
-          D[g][np] = tkind[ih] | tslot[ih] << 2 | (tkind[ih] == 2 ? tval[ih] << 5 : 0);
+          D[g][np] = kind == 2 ? info : (info & 31);
           ++np;
         }
-        while (xi < nx && np < K) {
-          D[g][np] = 1 | xs[xi] << 2;
-          ++np; ++xi;
+        for (; xm && np < K; xm &= xm - 1) {
+          D[g][np] = 1 | (__ffs(xm) - 1) << 2;
+          ++np;
         }
       }
       s_nnew[g] = np;
