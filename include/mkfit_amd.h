@@ -379,7 +379,8 @@ typedef struct mkf_ce_batch {
  * their new hit (UpdateWithLastHit, MkFinder.cc:1290-1308); the final order is
  * CombCandidate::MergeCandsAndBestShortOne (HitStructures.cc:511-552).  Hit
  * cluster charge and overlap (second-module) hits are not modelled: hits carry
- * no charge or module id. */
+ * no charge or module id.  Returns MKF_ERR_INVALID_ARG if a layer of `eoh`
+ * holds 2^28 hits or more (hit records carry 28-bit positions in the layer). */
 int mkf_find_tracks_clone_engine(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_ce_batch* batch, void* stream);
 
 #ifdef __cplusplus
