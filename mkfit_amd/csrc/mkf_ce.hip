@@ -41,8 +41,8 @@ constexpr int kW = 37;                // LDS words per exchanged candidate (36 u
 // Words of a lane's X row during (a), beside the propagated state in words 0-27:
 // chi2 cut, exclusive prefix of the window hit counts, layer hit offset and
 // barrel flag of the lane's layer.  Word 36, which put / get leave alone, holds
-// the lane's hold-back kind from (a) to the merge.
-constexpr int kXCut = 28, kXPre = 29, kXOff = 30, kXBar = 31, kXKind = 36;
+// the score of the lane's invalid-hit record from (a) to the new candidates (c).
+constexpr int kXCut = 28, kXPre = 29, kXOff = 30, kXBar = 31, kXScore = 36;
 // A kept accepted-hit record is {hit_cnt << kPosBits | position in the layer}.
 constexpr int kPosBits = 28;
 constexpr int kPosMask = (1 << kPosBits) - 1;
@@ -162,8 +162,8 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   constexpr int kS = KS;
   constexpr int kG = kLanes / KS;             // seeds per workgroup
   constexpr int kGA = kG + 1;                 // seed rows in LDS: one more for the idle lanes
-  // 20.0 KB (kS = 6): eight workgroups per CU, as many as the VGPRs allow;
-  // 20.6 KB (kS = 8): seven.
+  // 19.9 KB (kS = 6) / 20.4 KB (kS = 8): eight workgroups per CU, as many as
+  // the VGPRs allow.
   __shared__ float X[kLanes * kW];          // candidate state exchange (and the kX* words in (a))
   __shared__ int Gh[kLanes][kHitIdxMax];    // each lane's window hits (positions in the layer), in scan
                                             // order; from (a3) on its kept records, first ecnt
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   __shared__ float Es[kLanes][kS];          // the kept records' scores
   __shared__ unsigned char ecnt[kLanes];    // kept records
   __shared__ signed char fcode[kLanes];     // invalid-hit record's hit code, 0 = none
-  __shared__ float fscore[kLanes];          // its score
+  __shared__ unsigned char xkind[kLanes];   // 1 = held back unchanged, 2 = stopped replica
   __shared__ int D[kGA][kS];                // next candidates: kind | parent slot << 2 | record << 5
   __shared__ float s_bsc[kGA];              // score of the best short candidate (the candidate: a.bs)
   __shared__ int s_nnodes[kGA];
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     }
     ecnt[lane] = 0;
     fcode[lane] = 0;
-    X[lane * kW + kXKind] = __int_as_float(0);      // 1 = held back unchanged, 2 = stopped replica
+    xkind[lane] = 0;
     __syncthreads();
     const bool finding = gvalid && s_state[g] == 1;
     const bool valid = finding && c < s_ncand[g];
@@ -270,9 +270,9 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         const int k = append_node(a, seed, &s_nnodes[g], -2, layer, st.last);
         if (k >= 0) st.last = k;
         add_hit_counts(st, -2, 0.f);
-        X[lane * kW + kXKind] = __int_as_float(2);
+        xkind[lane] = 2;
       } else if (w.wsr == MKF_WSR_OUTSIDE) {
-        X[lane * kW + kXKind] = __int_as_float(1);
+        xkind[lane] = 1;
       } else {
         search = true;
         pt_l = fabsf(1.0f / pp[3]);
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         if (w.wsr == MKF_WSR_EDGE) fake = -3;
         const int nt = fake == -1 ? st.ntail + 1 : st.ntail;
         fcode[lane] = fake;
-        fscore[lane] = score_calc(st.nfound, nt, 0, st.ninside, st.chi2 < 0 ? 0.f : st.chi2, pt_l, true);
+        X[lane * kW + kXScore] = score_calc(st.nfound, nt, 0, st.ninside, st.chi2 < 0 ? 0.f : st.chi2, pt_l, true);
       }
     }
     // (a2) inclusive prefix sum of nh over the wavefront (the workgroup is one wavefront)
@@ -436,12 +436,12 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       }
       for (int s = 0; s < kS; ++s) {
         const int l = g * kS + s;
-        if (fcode[l]) insert(fscore[l], 3 | s << 2 | -fcode[l] << 5);
+        if (fcode[l]) insert(X[l * kW + kXScore], 3 | s << 2 | -fcode[l] << 5);
       }
       int np = 0;
       unsigned xm = 0;                                // held-back slots, lowest first
       for (int s = 0; s < kS; ++s)
-        if (__float_as_int(X[(g * kS + s) * kW + kXKind])) xm |= 1u << s;
+        if (xkind[g * kS + s]) xm |= 1u << s;
       if (nent == 0) {
         for (; xm; xm &= xm - 1) {
           D[g][np] = 1 | (__ffs(xm) - 1) << 2;
@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
           const int k = append_node(a, seed, &s_nnodes[g], hit, layer, st.last);
           if (k >= 0) st.last = k;
           add_hit_counts(st, hit, kind == 2 ? Gc[l][ri] : 0.f);
-          st.score = kind == 2 ? Es[l][ri] : fscore[l];
+          st.score = kind == 2 ? Es[l][ri] : X[l * kW + kXScore];
           if (kind == 2) {
             float mp[3], me[6];
             load_hit9(a.shits + (size_t)val * 9, mp, me);
