@@ -238,6 +238,6 @@ struct mkf_event_of_hits {
   void* d_temp = nullptr;
   size_t temp_bytes = 0;
   bool loaded = false;
-  void* d_ce_nodes = nullptr;     // clone-engine HitOnTrack node store
-  size_t ce_node_cap = 0;         // in nodes (16 B each)
+  void* d_ce_nodes = nullptr;     // clone-engine HitOnTrack node store, then its best short candidates
+  size_t ce_node_cap = 0;         // in 16-byte units
 };
