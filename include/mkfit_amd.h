@@ -367,7 +367,9 @@ typedef struct mkf_ce_batch {
   int* n_found;           /* [K][n] */
   int* hot_idx;           /* [K][n_layers][n] hit index or fake code per layer, -9 none */
   int n_layers;           /* layers per event searched; 0: all layers of the event of hits */
-  const int* layer_base;  /* [n] device: first layer of each seed's event, or NULL: 0 (as in mkf_find_batch) */
+  const int* layer_base;  /* [n] device: first layer of each seed's event, or NULL: 0.  A seed whose event would
+                             run past the last layer gets no candidates: n_cands 0, hot_idx -9, score -1e16,
+                             n_found 0 (its par/err/chg/chi2 slots are not written) */
 } mkf_ce_batch;
 
 /* MkBuilder::FindTracksCloneEngine / find_tracks_in_layers (MkBuilder.cc:1847-2060)

@@ -4,7 +4,10 @@
 // truncated or foreign file gives an error instead of a wild read.
 #include "../../include/mkfit_event_file.h"
 
+#include <algorithm>
+#include <climits>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -110,6 +113,11 @@ int read_tracks(Cursor& c, TrackSet& s) {
   s.n = n;
   s.par.resize(6 * (size_t)n); s.err.resize(21 * (size_t)n); s.chi2.resize(n); s.score.resize(n);
   s.chg.resize(n); s.label.resize(n); s.n_found.resize(n); s.status.resize(n); s.hot_offset.resize(n + 1, 0);
+  // HitOnTrack offsets are summed in 64 bits: each track may claim up to 32768
+  // entries, so an int sum over many tracks could wrap.  The total must fit an
+  // int and the bytes left in the event.
+  int64_t hot_sum = 0;
+  const int64_t hot_cap = std::min<int64_t>(INT_MAX, (int64_t)(c.left / kHotBytes));
   for (int t = 0; t < n; ++t) {
     const unsigned char* r = rec + (size_t)t * kTrackBytes;
     for (int e = 0; e < 6; ++e) s.par[(size_t)e * n + t] = get<float>(r + kTsPar + 4 * e);
@@ -122,7 +130,9 @@ int read_tracks(Cursor& c, TrackSet& s) {
     s.n_found[t] = get<int16_t>(r + kTrNFound);
     s.status[t] = get<uint32_t>(r + kTrStatus);
     s.label[t] = get<int32_t>(r + kTrLabel);
-    s.hot_offset[t + 1] = s.hot_offset[t] + n_hots;
+    hot_sum += n_hots;
+    if (hot_sum > hot_cap) return fail("track %d: HitOnTrack count beyond the bytes left in the event", t);
+    s.hot_offset[t + 1] = (int)hot_sum;
   }
   const int total = s.hot_offset[n];
   if ((long)data_size != 8L + (long)n * kTrackBytes + (long)total * kHotBytes)

@@ -250,6 +250,7 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   const size_t sum_bytes = (L * 3 * kSumParts * sizeof(double) + 255) / 256 * 256;
   const size_t need = sum_bytes + (n_ck * 27 + L * 3) * N * word;
   std::lock_guard<std::mutex> lk(c->mu);
+  CHECK(scratch_acquire(c->adj_done, st));
   if (need > c->adj_bytes) {
     HIP_TRY(hipDeviceSynchronize());
     if (c->adj_scratch) HIP_TRY(hipFree(c->adj_scratch));
@@ -295,5 +296,5 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_sum_rows, fin_grid, dim3(256), 0, st, (const double*)partial, (int)(L * 3), b->grad);
   HIP_TRY(hipGetLastError());
-  return MKF_OK;
+  return scratch_release(c->adj_done, st);
 }

@@ -569,9 +569,12 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     for (int i = 0; i < m; ++i) O[g][i] = ord[i];
     s_nout[g] = m;
     a.n_cands[seed] = m;
+  } else if (c == 0 && gin) {      // layer_base outside the event of hits: no candidates
+    s_nout[g] = 0;
+    a.n_cands[seed] = 0;
   }
   __syncthreads();
-  if (!gvalid || c >= K) return;
+  if (!gin || c >= K) return;
   const int L = a.n_layers;
   int* hot = a.hot + ((size_t)c * L) * N + seed;
   for (int l = 0; l < L; ++l) hot[(size_t)l * N] = -9;
@@ -639,6 +642,8 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   const int node_cap = b->first_layer + 3 * kSMax * (LE - b->first_layer) + kSMax;
   const size_t need = ((size_t)node_cap + kBsW / 4) * N;    // nodes, then the best short candidates
   hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(h->ce_mu);
+  CHECK(scratch_acquire(h->ce_done, st));
   if (need > h->ce_node_cap) {
     HIP_TRY(hipStreamSynchronize(st));
     if (h->d_ce_nodes) HIP_TRY(hipFree(h->d_ce_nodes));
@@ -678,5 +683,5 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
     hipLaunchKernelGGL(k_find_clone_engine<8>, dim3(grid), dim3(kLanes), 0, st, a);
   }
   HIP_TRY(hipGetLastError());
-  return MKF_OK;
+  return scratch_release(h->ce_done, st);
 }

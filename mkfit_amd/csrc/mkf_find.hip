@@ -19,7 +19,7 @@
 // Every layer's rows start at its hit offset, so a lane addresses a layer
 // with one scalar base and no per-layer allocations.
 //
-// Sort key: layer << 26 | q bin << 11 | fine phi bin.  Within one layer this is
+// Sort key: layer << 21 | q bin << 11 | fine phi bin.  Within one layer this is
 // the order of the reference's key fine_phi + (q_bin << 16)
 // (HitStructures.cc:114-128) because the fine phi bin is at most 1024; the
 // layer bits keep layers apart so one sort over the event serves all layers,
@@ -440,6 +440,7 @@ int mkf_event_of_hits_destroy(mkf_event_of_hits* h) {
   (void)hipDeviceSynchronize();
   free_hit_arrays(h);
   if (h->d_ce_nodes) (void)hipFree(h->d_ce_nodes);
+  if (h->ce_done) (void)hipEventDestroy(h->ce_done);
   if (h->d_layers) (void)hipFree(h->d_layers);
   if (h->d_bins) (void)hipFree(h->d_bins);
   delete h;

@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mkfit_amd.h"
@@ -240,4 +241,9 @@ struct mkf_event_of_hits {
   bool loaded = false;
   void* d_ce_nodes = nullptr;     // clone-engine HitOnTrack node store, then its best short candidates
   size_t ce_node_cap = 0;         // in 16-byte units
+  // Clone-engine calls on one event of hits share d_ce_nodes: the mutex
+  // serialises the host side (growing the store), and ce_done, recorded after
+  // each call's kernel, makes the next call's stream wait for it.
+  std::mutex ce_mu;
+  hipEvent_t ce_done = nullptr;
 };

@@ -503,9 +503,11 @@ int mkf_context_create(const mkf_config* cfg, mkf_context** out) {
 int mkf_context_destroy(mkf_context* c) {
   if (!c) return MKF_OK;
   (void)hipSetDevice(c->device);
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second.exec);
   if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
   if (c->adj_scratch) (void)hipFree(c->adj_scratch);
+  if (c->adj_done) (void)hipEventDestroy(c->adj_done);
   if (c->d_rl) (void)hipFree(c->d_rl);
   if (c->d_xi) (void)hipFree(c->d_xi);
   delete c;
@@ -624,6 +626,17 @@ int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
   key.pca_pflags = a.pca_pflags;
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
+    if (c->graphs.size() >= mkf_context::kMaxGraphs) {
+      // Evict the least recently launched graph; a launch of it may still be
+      // in flight, so the device drains first (rare: only callers that cycle
+      // through more than kMaxGraphs distinct batches get here).
+      auto lru = c->graphs.begin();
+      for (auto j = c->graphs.begin(); j != c->graphs.end(); ++j)
+        if (j->second.last_use < lru->second.last_use) lru = j;
+      HIP_TRY(hipDeviceSynchronize());
+      HIP_TRY(hipGraphExecDestroy(lru->second.exec));
+      c->graphs.erase(lru);
+    }
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal));
@@ -634,9 +647,10 @@ int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
     e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(MKF_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    it = c->graphs.emplace(key, ge).first;
+    it = c->graphs.emplace(key, mkf_context::GraphEntry{ge, 0}).first;
   }
-  HIP_TRY(hipGraphLaunch(it->second, (hipStream_t)stream));
+  it->second.last_use = ++c->graph_tick;
+  HIP_TRY(hipGraphLaunch(it->second.exec, (hipStream_t)stream));
   return MKF_OK;
 }
 

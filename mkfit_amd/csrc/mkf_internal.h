@@ -40,10 +40,23 @@ struct mkf_context {
   float* d_xi = nullptr;
   mkf_launch_shape shape{256, 8, 0};
   hipStream_t capture_stream = nullptr;
-  std::map<GraphKey, hipGraphExec_t> graphs;
+  // Instantiated batched-fit graphs by batch (pointers, sizes, flags, shape),
+  // at most kMaxGraphs; the least recently launched one is destroyed to make
+  // room for a new one.
+  struct GraphEntry {
+    hipGraphExec_t exec;
+    unsigned long long last_use;
+  };
+  static constexpr size_t kMaxGraphs = 16;
+  std::map<GraphKey, GraphEntry> graphs;
+  unsigned long long graph_tick = 0;
   std::mutex mu;
   void* adj_scratch = nullptr;   // adjoint checkpoints and per-track gradient rows
   size_t adj_bytes = 0;
+  // Recorded on the stream of the last mkf_fit_adjoint call after its last
+  // kernel that touches adj_scratch; the next call's stream waits on it, so
+  // calls on different streams take the scratch one after the other.
+  hipEvent_t adj_done = nullptr;
 };
 
 namespace mkf_detail {
@@ -104,6 +117,20 @@ inline int set_device(mkf_context* c) {
 }
 
 inline mkf::Material mat_of(const mkf_context* c) { return mkf::Material{c->d_rl, c->d_xi}; }
+
+// Orders a call's use of a shared scratch buffer after the previous call's:
+// `st` waits for `done` (when a previous call recorded it).  The caller
+// records `done` on `st` after its last kernel that uses the buffer
+// (scratch_release).  Both run under the owner's mutex.
+inline int scratch_acquire(hipEvent_t& done, hipStream_t st) {
+  if (!done) HIP_TRY(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+  else HIP_TRY(hipStreamWaitEvent(st, done, 0));
+  return MKF_OK;
+}
+inline int scratch_release(hipEvent_t done, hipStream_t st) {
+  HIP_TRY(hipEventRecord(done, st));
+  return MKF_OK;
+}
 
 }  // namespace mkf_detail
 

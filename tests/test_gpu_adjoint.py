@@ -153,3 +153,27 @@ def test_gradient_descent_recovers_offsets(gpu_device):
     s_z = weak_mode_residual(np.zeros(L), th_true[:, 2], radii, "z")
     assert r_rphi.std() < 0.1 * s_rphi.std(), (r_rphi.std(), s_rphi.std())
     assert r_z.std() < 0.3 * s_z.std(), (r_z.std(), s_z.std())
+
+
+def test_two_streams_share_the_scratch_in_order(setup, gpu_device):
+    """Two gradient calls of one context on two streams, queued back to back without a host wait: the
+    second call's stream waits for the first call's kernels before it reuses the context's checkpoint
+    and gradient-row scratch, so both results equal single-stream runs."""
+    import torch
+    ev, geo, th, ctx = setup
+    a = mk.AlignmentData(ctx, ev, gpu_device)
+    half = {k: np.ascontiguousarray(v[..., :2000]) for k, v in ev.items()}
+    b = mk.AlignmentData(ctx, half, gpu_device)
+    th_a, th_b = _theta(th, gpu_device), _theta(th * 1.01, gpu_device)
+    ref_a = [x.cpu().numpy() for x in a.run(th_a)]
+    ref_b = [x.cpu().numpy() for x in b.run(th_b)]
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(3):
+        with torch.cuda.stream(s1):        # run() launches and copies its results on the current stream
+            out_a = a.run(th_a)
+        with torch.cuda.stream(s2):
+            out_b = b.run(th_b)
+        torch.cuda.synchronize()
+        assert np.array_equal(out_a[0].cpu().numpy(), ref_a[0]) and np.array_equal(out_a[1].cpu().numpy(), ref_a[1])
+        assert np.array_equal(out_b[0].cpu().numpy(), ref_b[0]) and np.array_equal(out_b[1].cpu().numpy(), ref_b[1])

@@ -26,13 +26,13 @@ def _event(seed, n, noise, pt_min=0.5):
     return mk.generate_find_event(geo, p, n, noise_per_cm2=noise), mk.barrel_layer_descs(geo)
 
 
-def _run(ev, descs, dev, K, first_layer=3, chi2_cut=30.0):
+def _run(ev, descs, dev, K, first_layer=3, chi2_cut=30.0, prop_to_hit=True):
     import torch
     ctx = mk.Context(bfield=3.8)
     eoh = mk.EventOfHits(ctx, descs)
     hits = _t(ev["hits_flat"], dev) if len(ev["hits_flat"]) else torch.zeros((1, 9), device=dev)
     eoh.load(ev["hit_offset"], hits)
-    b = mk.CEBatch(ev, len(descs), first_layer, device=dev, chi2_cut=chi2_cut, max_cands=K)
+    b = mk.CEBatch(ev, len(descs), first_layer, device=dev, chi2_cut=chi2_cut, max_cands=K, prop_to_hit=prop_to_hit)
     ctx.find_tracks_clone_engine(eoh, b)
     torch.cuda.synchronize()
     return b.numpy()
@@ -89,6 +89,16 @@ def test_ce_matches_oracle(gpu_device, K, noise):
     # cut, whose chi2 (squared residuals of nearly equal coordinates) carries ~1e-4 FP32 noise.
     _compare(g, c, name="ce_K%d" % K, score_p99=1e-4 if noise < 5 else 3e-4)
     assert abs(_true_fraction(g["hot_idx"][0], ev, 3) - _true_fraction(c["hot_idx"][0], ev, 3)) < 3e-3
+
+
+@pytest.mark.parametrize("K", [1, 6])
+def test_ce_without_propagation_to_hit(gpu_device, K):
+    """The toy geometry's setting (finding_requires_propagation_to_hit_pos = false, Geoms/CylCowWLids.cc:225):
+    chi2 and the update use the state at the layer, not one propagated to each hit."""
+    ev, descs = _event(140 + K, 3000, 2.0)
+    g = _run(ev, descs, gpu_device, K, prop_to_hit=False)
+    c = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=K, prop_to_hit=False, n_threads=8)
+    _compare(g, c, name="ce_noprop_K%d" % K)
 
 
 def test_ce_noise_free(gpu_device):

@@ -151,3 +151,29 @@ def test_layer_base_out_of_range_leaves_seeds_untouched(gpu_device, events):
     assert (g["hot_idx"][3:, ~bad] >= 0).mean() > 0.5          # the others were searched
     with pytest.raises(mk.MkfError, match="layer_base"):
         ctx.find_tracks_best_hit(eoh, mk.FindBatch(cat, L, 3, device=gpu_device))
+
+
+def test_clone_engine_layer_base_out_of_range_gives_no_candidates(gpu_device, events):
+    import torch
+    evs, descs = events
+    L = len(descs)
+    cat = mk.concat_find_events(evs[:2], L)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs * 2)
+    eoh.load(cat["hit_offset"], _t(cat["hits_flat"], gpu_device))
+    base = cat["layer_base"].copy()
+    base[::5] = 2 * L
+    base[1::7] = -L
+    b = mk.CEBatch(cat, L, 3, device=gpu_device, max_cands=6, layer_base=base)
+    b.n_cands.fill_(77)
+    b.hot_idx.fill_(123)
+    b.score.fill_(5.0)
+    b.n_found.fill_(9)
+    ctx.find_tracks_clone_engine(eoh, b)
+    torch.cuda.synchronize()
+    bad = (base < 0) | (base > L)
+    g = b.numpy()
+    assert np.all(g["n_cands"][bad] == 0)
+    assert np.all(g["hot_idx"][:, :, bad] == -9)
+    assert np.all(g["score"][:, bad] == np.float32(-1e16)) and np.all(g["n_found"][:, bad] == 0)
+    assert np.all(g["n_cands"][~bad] >= 1)
