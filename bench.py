@@ -1,19 +1,27 @@
 """Throughput benchmark: Kalman track fit on one MI355X.
 
-Workload (the plain-fit-at-scale configuration): 1,000,000 seeded toy-MC tracks
-through 25 barrel layers r = 3..110 cm (|z| <= 120 cm), uniform Bz = 3.8 T,
-x/X0 = 0.02 per layer with multiple scattering and energy loss; FP32 state.
-A "step" is one complete fit of the batch (every track through all of its
-hits), inputs resident in HBM, output written out of place.
+Headline workload (batched-fit-uneven, the batched kernel's own configuration
+and the largest single-GPU one): 2,000,000 seeded toy-MC tracks in one batch,
+25 barrel layers r = 3..110 cm plus 2 x 9 endcap disks |z| = 30..270 cm,
+|eta| < 2.5, 8-30 hits per track, 5 % of layers missing, uniform Bz = 3.8 T,
+x/X0 = 0.02 per layer with multiple scattering and energy loss, chi2
+accumulated, FP64 covariance fallback on; FP32 state.  A "step" is one
+complete fit of the batch (every track through all of its hits), inputs
+resident in HBM, output written out of place.
 
 Two forms of the same fit are timed on the same inputs:
-  plain    -- mkf_fit_plain, one kernel launch per layer (state round-trips HBM)
-  batched  -- mkf_fit_batched, one persistent fused kernel from a cached hipGraph
+  plain    -- mkf_fit_plain, one kernel launch per hit slot (state round-trips HBM)
+  batched  -- mkf_fit_batched: the tracks' order within sort tiles, then one
+              fused kernel over all slots, both replayed from a cached hipGraph
 `value` is the batched throughput; the plain one and their ratio ride along.
+The plain-fit-at-scale workload (1,000,000 tracks, 25 barrel layers, one hit
+per layer) rides along in the same JSON line under "fit_at_scale" (skip it
+with --no-secondary; --workload fit makes it the headline instead).
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 import argparse
+import copy
 import json
 import os
 import sys
@@ -26,7 +34,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tracks fitted/sec on 1 MI355X, batched vs plain kernel, at ≤1e-5 rel param err"
 # Algorithmic FP32 operations of one barrel track-layer step (propagate with
-# material + chi2 + update), counted on the oracle restatement by
+# material + chi2 + update) and of one endcap step (propagate to z with
+# material + endcap chi2 + update), counted on the oracle restatement by
 # oracle/flopcount.cc (+,-,*,/ = 1; sqrt, sin, cos, log = 1).
 FLOP_PER_TRACK_LAYER = 1394
 FLOP_PER_ENDCAP_STEP = 413 + 168
@@ -47,17 +56,20 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n-tracks", type=int, default=1_000_000)
+    ap.add_argument("--n-tracks", type=int, default=None,
+                    help="tracks of the fit workloads (default 2,000,000 uneven, 1,000,000 at scale)")
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle checker")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the plain-fit-at-scale line")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--shape", type=str, default=None,
-                    help="block,blocks_per_cu,variant for the batched kernel (skips the autotuner)")
-    ap.add_argument("--workload", default="fit", choices=["fit", "fit-uneven", "find-ce", "find-bh", "adjoint"],
-                    help="fit (default: the metric's workload); fit-uneven: 2M tracks, barrel + 2x9 disks, "
-                         "8-30 hits, 5%% missing, FP64 fallback on; find-ce / find-bh: clone-engine / best-hit "
-                         "finding per event; adjoint: gradient of the fit's chi2 (secondary lines)")
+                    help="block,blocks_per_cu,variant[,sort_tile] for the batched kernel (skips the autotuner)")
+    ap.add_argument("--workload", default="fit-uneven", choices=["fit-uneven", "fit", "find-ce", "find-bh", "adjoint"],
+                    help="fit-uneven (default, the headline): 2M tracks, barrel + 2x9 disks, 8-30 hits, 5%% "
+                         "missing, FP64 fallback on; fit: plain-fit-at-scale, 1M tracks, 25 barrel layers; "
+                         "find-ce / find-bh: clone-engine / best-hit finding per event; adjoint: gradient of the "
+                         "fit's chi2")
     ap.add_argument("--events", type=int, default=50,
                     help="events per step of the find workloads (all in one event of hits and one launch)")
     ap.add_argument("--probe", action="store_true",
@@ -70,20 +82,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join("profiles", "k_fit_plain-fit-at-scale.md")
+# Committed rocprofv3 summaries holding the copy-calibrated FETCH_SIZE + WRITE_SIZE of one batched launch.
+TRAFFIC_PROFILES = {
+    True: (os.path.join("profiles", "k_fit_batched-fit-uneven.md"), "k_fit batched uneven"),
+    False: (os.path.join("profiles", "k_fit_plain-fit-at-scale.md"), "k_fit batched"),
+}
 
 
-def profiled_traffic():
-    """HBM bytes per batched-fit launch on the plain-fit-at-scale workload, as measured by two
-    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, each scaled by a copy-kernel calibration) and
-    recorded in the committed profile summary; None when the summary is absent."""
+def profiled_traffic(uneven):
+    """HBM bytes per batched-fit launch, as measured by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, each
+    scaled by a copy-kernel calibration) and recorded in the committed profile summary; None when absent."""
     import re
+    path, label = TRAFFIC_PROFILES[uneven]
     try:
-        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+        text = open(os.path.join(ROOT, path)).read()
     except OSError:
-        return None
-    m = re.search(r"^traffic_bytes_per_launch\[k_fit batched\]: (\d+)", text, re.M)
-    return int(m.group(1)) if m else None
+        return None, path
+    m = re.search(r"^traffic_bytes_per_launch\[%s\]: (\d+)" % re.escape(label), text, re.M)
+    return (int(m.group(1)) if m else None), path
 
 
 def main():
@@ -103,16 +119,29 @@ def main():
     if args.workload == "adjoint":
         return bench_adjoint(args, mk, torch, dev)
     uneven = args.workload == "fit-uneven"
+    res = bench_fit(args, mk, torch, dev, uneven)
+    if res is None:  # --probe
+        return
+    if uneven and not args.no_secondary:
+        a2 = copy.copy(args)
+        a2.n_tracks, a2.shape = None, None
+        sec = bench_fit(a2, mk, torch, dev, False)
+        res["fit_at_scale"] = {k: sec[k] for k in ("value", "unit", "ms_per_step", "config", "plain",
+                                                    "batched_over_plain", "parity", "roofline", "cpu_baseline")}
+    print(json.dumps(res), flush=True)
+
+
+def bench_fit(args, mk, torch, dev, uneven):
     geo = mk.geometry(3 if uneven else 2)
     rl, xi = mk.material_grid(geo)
     if uneven:
         params = mk.default_toy_params(seed=args.seed, bfield=3.8, eta_max=2.5, hits_min=8, hits_max=30,
                                        missing_frac=0.05)
-        n = 2_000_000 if args.n_tracks == 1_000_000 else args.n_tracks
+        n = args.n_tracks or 2_000_000
         n_slots = 30
     else:
         params = mk.default_toy_params(seed=args.seed, bfield=3.8)
-        n = args.n_tracks
+        n = args.n_tracks or 1_000_000
         n_slots = geo.n_barrel
     t0 = time.time()
     ev = mk.generate(geo, params, n, n_slots)
@@ -125,13 +154,13 @@ def main():
     n_track_layers = int(ev["n_hits"].sum())
     stream = torch.cuda.current_stream()
 
-    shape, tuned_ms = None, None
+    tuned_ms = None
     if args.shape:
-        b_, p_, v_ = (int(x) for x in args.shape.split(","))
-        ctx.set_batched_shape(b_, p_, v_)
+        ctx.set_batched_shape(*(int(x) for x in args.shape.split(",")))
     elif not args.no_autotune:
+        t0 = time.time()
         shape, tuned_ms = ctx.autotune_batched(batch, reps=2)
-        log("autotuned batched shape %s: %.3f ms" % (shape, tuned_ms))
+        log("autotuned batched shape %s: %.3f ms (%.1f s)" % (shape, tuned_ms, time.time() - t0))
     shape = ctx.get_batched_shape()
 
     if args.probe:
@@ -148,7 +177,7 @@ def main():
         log("probe: copy %d bytes; %d batched fits and 1 plain fit of %d tracks, %d track-layers, shape %s" %
             (src.numel() * 4, args.steps, n, n_track_layers, shape))
         ctx.close()
-        return
+        return None
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):
@@ -171,46 +200,45 @@ def main():
     ms_p = wall_p * 1e3 / args.steps
     tps_b = n / (ms_b * 1e-3)
     tps_p = n / (ms_p * 1e-3)
-    kern_b = float(np.mean(per_b))  # one fused kernel per step, timed with events on its stream
-    log("plain %.3f ms/step (%.3g tracks/s); batched %.3f ms/step (%.3g tracks/s), kernel %.3f ms" %
-        (ms_p, tps_p, ms_b, tps_b, kern_b))
+    graph_ms = float(np.mean(per_b))  # one graph per step (ordering + fused kernel), HIP events on its stream
+    order_ms = 0.0
+    if shape[3] > 0:
+        # the ordering kernel on its own (same kernel the graph replays), so the fused kernel's share is known
+        order = torch.empty(n, dtype=torch.int32, device=dev)
+        _, per_o = timed(lambda: ctx.fit_order(batch, shape[3], order), args.steps, args.warmup)
+        order_ms = float(np.mean(per_o))
+    kern_b = graph_ms - order_ms
+    log("plain %.3f ms/step (%.3g tracks/s); batched %.3f ms/step (%.3g tracks/s): graph %.3f ms = ordering %.3f "
+        "+ fused kernel %.3f; batched/plain %.2f" % (ms_p, tps_p, ms_b, tps_b, graph_ms, order_ms, kern_b,
+                                                     tps_b / tps_p))
 
     # The two GPU forms must agree bit for bit on the measured output.
-    out_b = [batch.par.cpu().numpy(), batch.err.cpu().numpy(), batch.chi2.cpu().numpy()]
-    out_p = [plain_out.par.cpu().numpy(), plain_out.err.cpu().numpy(), plain_out.chi2.cpu().numpy()]
-    identical = all(np.array_equal(a, b) for a, b in zip(out_b, out_p))
+    out_b = [batch.par.cpu().numpy(), batch.err.cpu().numpy(), batch.chi2.cpu().numpy(), batch.chg.cpu().numpy()]
+    out_p = [plain_out.par.cpu().numpy(), plain_out.err.cpu().numpy(), plain_out.chi2.cpu().numpy(),
+             plain_out.chg.cpu().numpy()]
+    identical = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(out_b, out_p))
     parity = {"batched_equals_plain": identical}
 
     cpu = None
     if not args.no_cpu:
         # CPU baseline leg: the oracle restatement, timed on the host, and (as the checker) a
-        # sample of the measured GPU output against it.
+        # sample of the measured GPU output against it, track by track.
         import oracle
+        from oracle.parity import describe, flagged_parity
         ncpu = min(16, os.cpu_count() or 1)
         orc = oracle.Oracle(bfield=params.bfield, rl_grid=rl, xi_grid=xi)
         rng = np.random.default_rng(0)
         idx = np.sort(rng.choice(n, min(n, 20000), replace=False))
         sub = {k: np.ascontiguousarray(v[..., idx]) for k, v in ev.items()}
-        opar, _, _, _ = orc.fit(sub, pflags=pflags, fit_flags=fflags, n_threads=ncpu)
-        opar64, _, _, _ = orc.fit(sub, pflags=pflags, fit_flags=fflags, dtype=np.float64, n_threads=ncpu)
-        fl = np.array([1.0, 1.0, 1.0, 0.1, 1.0, 1.0])[:, None]
-
-        def rel_err(a, b):
-            d = np.abs(a.astype(np.float64) - b)
-            d[4] = np.minimum(d[4], np.abs(d[4] - 2 * np.pi))
-            return (d / np.maximum(np.abs(b), fl)).max(axis=0)
-
-        rel = rel_err(out_b[0][:, idx], opar)
-        frac_ok = float(np.mean(rel <= 1e-5))
-        # the FP32 noise floor: how far the CPU's own FP32 fit is from its FP64 fit, beside the GPU's
-        frac_cpu64 = float(np.mean(rel_err(opar, opar64) <= 1e-5))
-        frac_gpu64 = float(np.mean(rel_err(out_b[0][:, idx], opar64) <= 1e-5))
-        parity.update({"sample_tracks": int(len(idx)), "max_rel_par_err_vs_cpu": float(rel.max()),
-                       "frac_tracks_within_1e-5": frac_ok,
-                       "frac_within_1e-5_of_fp64": {"gpu_fp32": frac_gpu64, "cpu_fp32": frac_cpu64}})
-        log("parity: batched==plain %s; sample max rel par err %.3g, fraction of tracks <= 1e-5: %.5f; "
-            "within 1e-5 of the FP64 fit: GPU %.5f, CPU FP32 %.5f" % (identical, rel.max(), frac_ok, frac_gpu64,
-                                                                      frac_cpu64))
+        o32 = orc.fit(sub, pflags=pflags, fit_flags=fflags, n_threads=ncpu)
+        o64 = orc.fit(sub, pflags=pflags, fit_flags=fflags, dtype=np.float64, n_threads=ncpu)
+        r = flagged_parity((out_b[0][:, idx], out_b[1][:, idx], out_b[2][idx]), (o32[0], o32[1], o32[3]),
+                           (o64[0], o64[1], o64[3]))
+        r.pop("unexplained_idx")
+        parity.update({"sample_tracks": int(len(idx)), "per_track": r,
+                       "max_rel_par_err_vs_cpu": r["all_max"]["par"],
+                       "frac_tracks_within_1e-5": r["frac_tracks_within_tol"]})
+        log("parity: batched==plain %s; %s" % (identical, describe(r)))
         best = 1e30
         for _ in range(args.cpu_reps):
             t = time.perf_counter()
@@ -223,12 +251,16 @@ def main():
         log("cpu baseline %.3g tracks/s (%.2f s per pass)" % (n / best, best))
 
     flops = FLOP_PER_TRACK_LAYER * n_track_layers
+    n_endcap = 0
     if uneven:   # endcap steps: propagation to z + endcap update (oracle/flopcount.cc)
         valid = np.arange(n_slots)[:, None] < ev["n_hits"][None, :]
         n_endcap = int(np.sum(valid & ((ev["hit_flags"] & 1) != 0)))
         flops = FLOP_PER_TRACK_LAYER * (n_track_layers - n_endcap) + FLOP_PER_ENDCAP_STEP * n_endcap
     achieved_tflops = flops / (kern_b * 1e-3) / 1e12
     alg_bytes_b = n_track_layers * HIT_BYTES + n * TRACK_BYTES
+    traffic, traffic_src = profiled_traffic(uneven)
+    if n != (2_000_000 if uneven else 1_000_000):
+        traffic = None  # the profile holds the default size only
     res = {
         "metric": METRIC,
         "value": tps_b,
@@ -248,24 +280,25 @@ def main():
                          "accumulated, FP64 covariance fallback" % n) if uneven else
                         ("plain-fit-at-scale: %d tracks, 25 barrel layers r=3-110 cm, |z|<=120 cm, Bz=3.8 T, "
                          "x/X0=0.02 per layer, material on, chi2 accumulated" % n),
-            "n_tracks": n, "layers": n_slots, "track_layers": n_track_layers,
-            "batched_shape": {"block": shape[0], "blocks_per_cu": shape[1], "variant": shape[2]},
+            "n_tracks": n, "layers": n_slots, "track_layers": n_track_layers, "endcap_steps": n_endcap,
+            "batched_shape": {"block": shape[0], "blocks_per_cu": shape[1], "variant": shape[2],
+                              "sort_tile": shape[3]},
         },
         "plain": {"value": tps_p, "ms_per_step": ms_p, "unit": "tracks/s",
                   "achieved_GBps_algorithmic": n_track_layers * PLAIN_BYTES / (ms_p * 1e-3) / 1e9},
         "batched_over_plain": tps_b / tps_p,
         "parity": parity,
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-                     "traffic": None if uneven or n != 1_000_000 else profiled_traffic(),
-                     "traffic_source": None if uneven else TRAFFIC_PROFILE,
-                     "kernel": "k_fit_fused", "kernel_ms": kern_b,
-                     "flop_per_track_layer": FLOP_PER_TRACK_LAYER,
+                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": "k_fit (fused)", "kernel_ms": kern_b, "graph_ms": graph_ms,
+                     "ordering_ms": order_ms,
+                     "flop_per_track_layer": FLOP_PER_TRACK_LAYER, "flop_per_endcap_step": FLOP_PER_ENDCAP_STEP,
+                     "algorithmic_bytes": alg_bytes_b,
                      "algorithmic_GBps": alg_bytes_b / (kern_b * 1e-3) / 1e9},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(res), flush=True)
     ctx.close()
+    return res
 
 
 def _timed(torch, stream, fn, steps, warmup):

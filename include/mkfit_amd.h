@@ -129,9 +129,22 @@ int mkf_fit_batched(mkf_context* ctx, const mkf_fit_batch* batch, void* stream);
 /* Launch shape used by mkf_fit_batched (set by the autotuner or by hand). */
 typedef struct mkf_launch_shape {
   int block;          /* threads per block: 64, 128, 256 or 512 */
-  int blocks_per_cu;  /* persistent grid = blocks_per_cu * CU count */
+  int blocks_per_cu;  /* persistent grid = blocks_per_cu * CU count; 0: one thread per track */
   int variant;        /* kernel variant index (occupancy / register bound) */
+  int sort_tile;      /* 0, or 1024 / 4096 / 16384: before the fit, the tracks of each tile of
+                         sort_tile consecutive tracks are ordered by (hit count, endcap pattern of
+                         their slots), so a wavefront runs tracks of one length and one branch per
+                         slot (the reference groups its blocks the same way: one is_barrel pattern
+                         per block, MkFitter.cc:473-503).  Results do not depend on it. */
 } mkf_launch_shape;
+
+/* The track order mkf_fit_batched follows with a sort tile: order[i] is the
+ * track fitted at position i (a permutation of [0, n_tracks) that stays within
+ * each tile).  No counterpart in the reference, whose drivers group tracks
+ * into blocks of one layer signature on the host (MkBuilder.cc:594-625,
+ * MkFitter.cc:473-503); exposed for callers that fit the same batch in
+ * several passes and for timing the ordering on its own. */
+int mkf_fit_order(mkf_context* ctx, const mkf_fit_batch* batch, int sort_tile, int* order, void* stream);
 
 int mkf_set_batched_shape(mkf_context* ctx, const mkf_launch_shape* shape);
 int mkf_get_batched_shape(mkf_context* ctx, mkf_launch_shape* shape);

@@ -48,7 +48,8 @@ class MkfFitBatch(ctypes.Structure):
 
 
 class MkfLaunchShape(ctypes.Structure):
-    _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int), ("variant", ctypes.c_int)]
+    _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int), ("variant", ctypes.c_int),
+                ("sort_tile", ctypes.c_int)]
 
 
 class MkfGeometry(ctypes.Structure):
@@ -91,6 +92,7 @@ ABI_SYMBOLS = [
     "mkf_kalman_operation", "mkf_kalman_operation_endcap",
     "mkf_fit_plain", "mkf_fit_batched",
     "mkf_set_batched_shape", "mkf_get_batched_shape", "mkf_autotune_batched", "mkf_num_batched_variants",
+    "mkf_fit_order",
     "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
@@ -170,6 +172,7 @@ def lib():
         L.mkf_fit_plain.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p]
         L.mkf_fit_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p]
         L.mkf_set_batched_shape.argtypes = [_c_void_p, ctypes.POINTER(MkfLaunchShape)]
+        L.mkf_fit_order.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), ctypes.c_int, _c_void_p, _c_void_p]
         L.mkf_get_batched_shape.argtypes = [_c_void_p, ctypes.POINTER(MkfLaunchShape)]
         L.mkf_event_of_hits_create.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(MkfLayerDesc),
                                                ctypes.POINTER(_c_void_p)]
@@ -712,14 +715,20 @@ class Context:
         _check(lib().mkf_update_with_last_hit(self.h, eoh.h, int(layer), ctypes.byref(st), _ptr(hit_idx),
                                               int(bool(prop_to_hit)), int(pflags), self._stream(stream)))
 
-    def set_batched_shape(self, block, blocks_per_cu, variant):
-        s = MkfLaunchShape(int(block), int(blocks_per_cu), int(variant))
+    def fit_order(self, batch, sort_tile, order, stream=None):
+        """Writes the batched fit's track order for `sort_tile` into `order` (int32 device tensor, n_tracks)."""
+        _check(lib().mkf_fit_order(self.h, ctypes.byref(batch.c_struct()), int(sort_tile), _ptr(order),
+                                   self._stream(stream)))
+
+    def set_batched_shape(self, block, blocks_per_cu, variant, sort_tile=0):
+        s = MkfLaunchShape(int(block), int(blocks_per_cu), int(variant), int(sort_tile))
         _check(lib().mkf_set_batched_shape(self.h, ctypes.byref(s)))
 
     def get_batched_shape(self):
+        """(block, blocks_per_cu, variant, sort_tile)"""
         s = MkfLaunchShape()
         _check(lib().mkf_get_batched_shape(self.h, ctypes.byref(s)))
-        return (s.block, s.blocks_per_cu, s.variant)
+        return (s.block, s.blocks_per_cu, s.variant, s.sort_tile)
 
     def autotune_batched(self, batch, reps=3):
         import torch
@@ -729,7 +738,7 @@ class Context:
         torch.cuda.synchronize(self.device)
         _check(lib().mkf_autotune_batched(self.h, ctypes.byref(batch.c_struct()), _ptr(scratch), int(reps),
                                           ctypes.byref(s), ctypes.byref(ms)))
-        return (s.block, s.blocks_per_cu, s.variant), ms.value
+        return (s.block, s.blocks_per_cu, s.variant, s.sort_tile), ms.value
 
 
 class FitBatch:

@@ -152,15 +152,6 @@ __device__ __forceinline__ int scan_window(const LayerDev& d, const Window& w, c
   return cnt;
 }
 
-// Workgroups are dealt to the 8 XCDs in turn.  This renumbers block b so that
-// XCD x walks the x-th contiguous eighth of [0, gridDim.x): XCD x takes
-// gridDim.x / 8 + (x < gridDim.x % 8) blocks, so the map is a bijection.  Work
-// ordered by layer or event then shares one L2 per XCD instead of all eight.
-__device__ __forceinline__ int xcd_major_block() {
-  const int nb = gridDim.x, r8 = nb % 8, q8 = nb / 8, x = blockIdx.x % 8;
-  return x * q8 + min(x, r8) + blockIdx.x / 8;
-}
-
 // getHitSelDynamicChi2Cut (MkFinder.cc:202-220)
 __device__ __forceinline__ float dyn_chi2_cut(const LayerDev& d, float cut_min, float invpt, float theta_in) {
   const float theta = fabsf(theta_in - kPIOver2);

@@ -127,6 +127,9 @@ struct FitArgs {
   int pflags, fit_flags, pca_pflags;
   float bfield;
   Material mat;
+  // Batched form with a sort tile: position i of the launch fits track
+  // order[i] (k_fit_order); NULL: position i fits track i.
+  const int* __restrict__ order;
 };
 
 // One hit slot of one track: the body of the reference's per-layer loop
@@ -338,6 +341,9 @@ __device__ __forceinline__ void store_state(const FitArgs& a, int t, const float
 // PF: software-pipeline the hit loads (slot s+1 in flight while slot s
 // computes) at the cost of 9 more live VGPRs; the loads do not change the
 // arithmetic, so both settings give the same bits.
+// With a track order (sort tiles) the blocks walk the positions in XCD-major
+// order, so the wavefronts of one tile run on one XCD and the cache lines of
+// the tile's hits, which its wavefronts share, are fetched into one L2.
 template <int MINW, int FORM, bool FB, bool PF = true>
 __global__ void __attribute__((amdgpu_flat_work_group_size(64, 512), amdgpu_waves_per_eu(MINW, 8)))
 k_fit(FitArgs a, int s_begin, int s_end) {
@@ -346,7 +352,9 @@ k_fit(FitArgs a, int s_begin, int s_end) {
   const FitBufs b = make_bufs(a);
   // The launch that ends at the last slot also takes every track to its PCA.
   const bool to_pca = (a.fit_flags & MKF_FIT_PROP_TO_PCA) && s_end >= a.n_slots;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += stride) {
+  const int blk = a.order ? xcd_major_block() : (int)blockIdx.x;
+  for (int i = blk * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+    const int t = a.order ? a.order[i] : i;
     const uint32_t tb = (uint32_t)t * 4u;  // lane byte offset in a 4-byte element array
     const int s1 = min(min(bld_i(b.n_hits, tb, 0), a.n_slots), s_end);
     if (s_begin >= s1 && !from_seed && !to_pca) continue;
@@ -370,6 +378,64 @@ k_fit(FitArgs a, int s_begin, int s_end) {
     }
     if (to_pca) propagate_to_pca(par, err, chg, a);
     store_state(a, t, par, err, chg, chi2);
+  }
+}
+
+// Track order of the batched fit.  One block per tile of TILE consecutive
+// tracks: each track gets the key (hit count, endcap flags of its slots,
+// position in the tile) -- slot s of the flags in bit s mod 40 -- and the
+// block sorts the tile's keys in LDS (bitonic network, 1024 threads), then
+// writes the tracks in key order.  Tracks of one length and one barrel/endcap
+// pattern then share wavefronts, so a wavefront's slot loop lasts as long as
+// its tracks and takes one branch per slot; the permutation stays inside the
+// tile, so the lines of a tile's hits are shared by its own wavefronts.  The
+// fit of each track is the same whatever its position, so the order changes
+// no result.
+template <int TILE>
+__global__ void __launch_bounds__(1024) k_fit_order(const int* __restrict__ n_hits,
+                                                     const unsigned char* __restrict__ hit_flags, int n, int n_slots,
+                                                     int* __restrict__ order) {
+  static_assert(TILE >= 1024 && (TILE & (TILE - 1)) == 0 && TILE <= 16384, "tile: power of two, 1024..16384");
+  __shared__ uint64_t key[TILE];
+  const int base = blockIdx.x * TILE;
+  const int m = min(TILE, n - base);
+  for (int j = threadIdx.x; j < TILE; j += 1024) {
+    uint64_t k = ~0ull;  // padding sorts last
+    if (j < m) {
+      const int t = base + j;
+      const int nh = min(max(n_hits[t], 0), n_slots);
+      uint64_t mask = 0;
+      if (hit_flags)
+        for (int s = 0; s < nh; ++s)
+          if (hit_flags[(size_t)s * n + t] & MKF_HIT_ENDCAP) mask |= 1ull << (s % 40);
+      k = ((uint64_t)min(nh, 1023) << 54) | (mask << 14) | (uint64_t)j;
+    }
+    key[j] = k;
+  }
+  for (int size = 2; size <= TILE; size <<= 1)
+    for (int half = size >> 1; half > 0; half >>= 1) {
+      __syncthreads();
+      for (int p = threadIdx.x; p < TILE / 2; p += 1024) {
+        const int lo = 2 * half * (p / half) + (p % half), hi = lo + half;
+        const uint64_t x = key[lo], y = key[hi];
+        const bool ascending = (lo & size) == 0;
+        if ((x > y) == ascending) {
+          key[lo] = y;
+          key[hi] = x;
+        }
+      }
+    }
+  __syncthreads();
+  for (int j = threadIdx.x; j < m; j += 1024) order[base + j] = base + (int)(key[j] & 0x3fff);
+}
+
+using OrderKernel = void (*)(const int*, const unsigned char*, int, int, int*);
+static int order_kernel(int tile, OrderKernel* k) {
+  switch (tile) {
+    case 1024: *k = k_fit_order<1024>; return MKF_OK;
+    case 4096: *k = k_fit_order<4096>; return MKF_OK;
+    case 16384: *k = k_fit_order<16384>; return MKF_OK;
+    default: return fail(MKF_ERR_INVALID_ARG, "sort_tile must be 0, 1024, 4096 or 16384");
   }
 }
 
@@ -418,6 +484,7 @@ int validate_batch(const mkf_fit_batch* b, FitArgs& a, const mkf_context* c) {
   a.pca_pflags = b->pca_pflags;
   a.bfield = c->bfield;
   a.mat = mat_of(c);
+  a.order = nullptr;
   if (n == 0) return MKF_OK;
   CHECK(check_dev(b->par, 6 * n * 4, "par"));
   CHECK(check_dev(b->err, 21 * n * 4, "err"));
@@ -443,8 +510,20 @@ int validate_batch(const mkf_fit_batch* b, FitArgs& a, const mkf_context* c) {
 
 int grid_for(int n, int block) { return (n + block - 1) / block; }
 
-int launch_fused(mkf_context* c, const FitArgs& a, const mkf_launch_shape& sh, hipStream_t st) {
+// The batched fit: with a sort tile, k_fit_order writes the track order to
+// `order` (n ints of device memory) and the fused kernel follows it.
+int launch_fused(mkf_context* c, FitArgs a, const mkf_launch_shape& sh, int* order, hipStream_t st) {
   if (a.n == 0) return MKF_OK;
+  a.order = nullptr;
+  if (sh.sort_tile > 0) {
+    OrderKernel ok;
+    CHECK(order_kernel(sh.sort_tile, &ok));
+    if (!order) return fail(MKF_ERR_INVALID_ARG, "no track-order buffer");
+    hipLaunchKernelGGL(ok, dim3((a.n + sh.sort_tile - 1) / sh.sort_tile), dim3(1024), 0, st, a.n_hits, a.hit_flags,
+                       a.n, a.n_slots, order);
+    HIP_TRY(hipGetLastError());
+    a.order = order;
+  }
   const int block = sh.block;
   int grid = sh.blocks_per_cu > 0 ? sh.blocks_per_cu * c->num_cus : grid_for(a.n, block);
   grid = std::max(1, std::min(grid, grid_for(a.n, block)));
@@ -456,7 +535,24 @@ int launch_fused(mkf_context* c, const FitArgs& a, const mkf_launch_shape& sh, h
 
 bool shape_ok(const mkf_launch_shape& s) {
   return (s.block == 64 || s.block == 128 || s.block == 256 || s.block == 512) && s.blocks_per_cu >= 0 &&
-         s.blocks_per_cu <= 64 && s.variant >= 0 && s.variant < kNumFusedVariants;
+         s.blocks_per_cu <= 64 && s.variant >= 0 && s.variant < kNumFusedVariants &&
+         (s.sort_tile == 0 || s.sort_tile == 1024 || s.sort_tile == 4096 || s.sort_tile == 16384);
+}
+
+// Grows the context's track-order buffer to n ints.  Cached graphs hold the
+// old address, so they are dropped (after the device drains) when it moves.
+// Called under c->mu.
+int ensure_fit_order(mkf_context* c, int n) {
+  if ((size_t)n <= c->fit_order_cap) return MKF_OK;
+  HIP_TRY(hipDeviceSynchronize());
+  for (auto& kv : c->graphs) HIP_TRY(hipGraphExecDestroy(kv.second.exec));
+  c->graphs.clear();
+  if (c->fit_order) HIP_TRY(hipFree(c->fit_order));
+  c->fit_order = nullptr;
+  c->fit_order_cap = 0;
+  HIP_TRY(hipMalloc(&c->fit_order, (size_t)n * sizeof(int)));
+  c->fit_order_cap = (size_t)n;
+  return MKF_OK;
 }
 
 }  // namespace
@@ -508,6 +604,8 @@ int mkf_context_destroy(mkf_context* c) {
   if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
   if (c->adj_scratch) (void)hipFree(c->adj_scratch);
   if (c->adj_done) (void)hipEventDestroy(c->adj_done);
+  if (c->fit_order) (void)hipFree(c->fit_order);
+  if (c->fit_done) (void)hipEventDestroy(c->fit_done);
   if (c->d_rl) (void)hipFree(c->d_rl);
   if (c->d_xi) (void)hipFree(c->d_xi);
   delete c;
@@ -617,6 +715,8 @@ int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
   CHECK(validate_batch(b, a, c));
   if (a.n == 0) return MKF_OK;
   std::lock_guard<std::mutex> lk(c->mu);
+  const bool sorted = c->shape.sort_tile > 0;
+  if (sorted) CHECK(ensure_fit_order(c, a.n));
   GraphKey key;
   const void* ptrs[12] = {b->par,    b->err,       b->chg,      b->chi2,     b->hit_pos,  b->hit_err,
                           b->n_hits, b->hit_flags, b->seed_par, b->seed_err, b->seed_chg, nullptr};
@@ -624,6 +724,7 @@ int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
   key.n = a.n; key.slots = a.n_slots; key.pflags = a.pflags; key.fflags = a.fit_flags;
   key.block = c->shape.block; key.bpc = c->shape.blocks_per_cu; key.variant = c->shape.variant;
   key.pca_pflags = a.pca_pflags;
+  key.tile = c->shape.sort_tile;
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     if (c->graphs.size() >= mkf_context::kMaxGraphs) {
@@ -640,7 +741,7 @@ int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal));
-    int rc = launch_fused(c, a, c->shape, c->capture_stream);
+    int rc = launch_fused(c, a, c->shape, c->fit_order, c->capture_stream);
     hipError_t e = hipStreamEndCapture(c->capture_stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return fail(MKF_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -650,7 +751,25 @@ int mkf_fit_batched(mkf_context* c, const mkf_fit_batch* b, void* stream) {
     it = c->graphs.emplace(key, mkf_context::GraphEntry{ge, 0}).first;
   }
   it->second.last_use = ++c->graph_tick;
+  // The track-order buffer is shared by every call on this context: a call on
+  // another stream waits for the previous call's graph to finish with it.
+  if (sorted) CHECK(scratch_acquire(c->fit_done, (hipStream_t)stream));
   HIP_TRY(hipGraphLaunch(it->second.exec, (hipStream_t)stream));
+  if (sorted) CHECK(scratch_release(c->fit_done, (hipStream_t)stream));
+  return MKF_OK;
+}
+
+int mkf_fit_order(mkf_context* c, const mkf_fit_batch* b, int sort_tile, int* order, void* stream) {
+  CHECK(set_device(c));
+  FitArgs a;
+  CHECK(validate_batch(b, a, c));
+  if (a.n == 0) return MKF_OK;
+  OrderKernel ok;
+  CHECK(order_kernel(sort_tile, &ok));
+  CHECK(check_dev(order, (size_t)a.n * sizeof(int), "order"));
+  hipLaunchKernelGGL(ok, dim3((a.n + sort_tile - 1) / sort_tile), dim3(1024), 0, (hipStream_t)stream, a.n_hits,
+                     a.hit_flags, a.n, a.n_slots, order);
+  HIP_TRY(hipGetLastError());
   return MKF_OK;
 }
 
@@ -703,19 +822,27 @@ int mkf_autotune_batched(mkf_context* c, const mkf_fit_batch* b, float* scratch,
   HIP_TRY(hipEventCreate(&e1));
   const int blocks[] = {64, 128, 256, 512};
   const int bpcs[] = {0, 1, 2, 4, 8, 16};
+  const int tiles[] = {0, 1024, 4096, 16384};
   mkf_launch_shape bs = c->shape;
   float bt = 1e30f;
   int rc = MKF_OK;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    rc = ensure_fit_order(c, a.n);
+  }
+  int* order = c->fit_order;
+  if (!rc) rc = scratch_acquire(c->fit_done, st);
+  for (int tile : tiles)
   for (int v = 0; v < kNumFusedVariants && !rc; ++v)
     for (int bl : blocks)
       for (int bp : bpcs) {
         if (rc) break;
-        mkf_launch_shape sh{bl, bp, v};
+        mkf_launch_shape sh{bl, bp, v, tile};
         float tot = 0.f;
         for (int r = 0; r < reps + 1 && !rc; ++r) {  // first run is a warm-up
           if (save(true) != hipSuccess) { rc = fail(MKF_ERR_HIP, "autotune restore failed"); break; }
           (void)hipEventRecord(e0, st);
-          rc = launch_fused(c, a, sh, st);
+          rc = launch_fused(c, a, sh, order, st);
           (void)hipEventRecord(e1, st);
           if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(MKF_ERR_HIP, "autotune sync failed"); break; }
           float ms = 0.f;
@@ -725,6 +852,7 @@ int mkf_autotune_batched(mkf_context* c, const mkf_fit_batch* b, float* scratch,
         if (!rc && tot / reps < bt) { bt = tot / reps; bs = sh; }
       }
   if (!rc && save(true) != hipSuccess) rc = fail(MKF_ERR_HIP, "autotune final restore failed");
+  if (c->fit_done) (void)scratch_release(c->fit_done, st);
   (void)hipStreamSynchronize(st);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

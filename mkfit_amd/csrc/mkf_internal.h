@@ -23,12 +23,12 @@
 
 struct GraphKey {
   const void* ptrs[12];
-  int n, slots, pflags, fflags, block, bpc, variant, pca_pflags;
+  int n, slots, pflags, fflags, block, bpc, variant, pca_pflags, tile;
   bool operator<(const GraphKey& o) const {
     int c = std::memcmp(ptrs, o.ptrs, sizeof(ptrs));
     if (c) return c < 0;
-    return std::tie(n, slots, pflags, fflags, block, bpc, variant, pca_pflags) <
-           std::tie(o.n, o.slots, o.pflags, o.fflags, o.block, o.bpc, o.variant, o.pca_pflags);
+    return std::tie(n, slots, pflags, fflags, block, bpc, variant, pca_pflags, tile) <
+           std::tie(o.n, o.slots, o.pflags, o.fflags, o.block, o.bpc, o.variant, o.pca_pflags, o.tile);
   }
 };
 
@@ -38,7 +38,7 @@ struct mkf_context {
   float bfield = 3.8112f;
   float* d_rl = nullptr;
   float* d_xi = nullptr;
-  mkf_launch_shape shape{256, 8, 0};
+  mkf_launch_shape shape{256, 0, 0, 0};
   hipStream_t capture_stream = nullptr;
   // Instantiated batched-fit graphs by batch (pointers, sizes, flags, shape),
   // at most kMaxGraphs; the least recently launched one is destroyed to make
@@ -57,6 +57,12 @@ struct mkf_context {
   // kernel that touches adj_scratch; the next call's stream waits on it, so
   // calls on different streams take the scratch one after the other.
   hipEvent_t adj_done = nullptr;
+  // Track order of the batched fit (sort tiles), n_tracks ints; the graphs
+  // capture its address, so growing it drops every cached graph.  fit_done
+  // orders its users across streams as adj_done does for adj_scratch.
+  int* fit_order = nullptr;
+  size_t fit_order_cap = 0;
+  hipEvent_t fit_done = nullptr;
 };
 
 namespace mkf_detail {
@@ -155,6 +161,16 @@ __device__ __forceinline__ int bld_i(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint
 }
 __device__ __forceinline__ unsigned bld_u8(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
   return (unsigned)__builtin_amdgcn_raw_buffer_load_b8(r, (int)vo, (int)so, 0);
+}
+
+// Workgroups are dealt to the 8 XCDs in turn.  This renumbers block b so that
+// XCD x walks the x-th contiguous eighth of [0, gridDim.x): XCD x takes
+// gridDim.x / 8 + (x < gridDim.x % 8) blocks, so the map is a bijection.  Work
+// ordered by layer, event or sort tile then shares one L2 per XCD instead of
+// all eight.
+__device__ __forceinline__ int xcd_major_block() {
+  const int nb = gridDim.x, r8 = nb % 8, q8 = nb / 8, x = blockIdx.x % 8;
+  return x * q8 + min(x, r8) + blockIdx.x / 8;
 }
 
 }  // namespace mkf

@@ -10,7 +10,8 @@ import pytest
 
 import mkfit_amd as mk
 import oracle
-from _common import (TOL_CHI2, TOL_COV, TOL_PAR, assert_parity, chi2_rel, cov_rel, event, par_rel, summary)
+from _common import (TOL_CHI2, TOL_COV, TOL_PAR, assert_parity, assert_track_parity, chi2_rel, cov_rel, event,
+                     par_rel, summary)
 
 pytestmark = pytest.mark.gpu
 
@@ -159,18 +160,18 @@ def test_fit_parity(gpu_device, case, batched):
     o64 = orc.fit(ev, pflags=pflags, fit_flags=fflags, dtype=np.float64, n_threads=8)
     ctx = mk.Context(p.bfield, rl, xi)
     g = _fit_gpu(ctx, ev, gpu_device, pflags, fflags, batched)
-    # Charge flips of ill-conditioned tracks (chi2/ndf > 100 in FP64) differ
-    # between FP32 and FP64 on the CPU too; every other track must agree.
-    good = o64[3] / (2 * np.maximum(ev["n_hits"], 1)) < 100
-    assert (g[2][good] == o[2][good]).all(), "charges differ on well-conditioned tracks"
-    assert np.mean(g[2] != o[2]) < 1e-3
     print()
-    # 17-30 layers of FP32 updates: ~99.7% of entries within tolerance (see assert_parity)
+    # Every track: unflagged ones within tolerance (or nearer the FP64 fit than
+    # the FP32 oracle), flagged ones counted and bounded (assert_track_parity).
+    r = assert_track_parity("fit", (g[0], g[1], g[3]), (o[0], o[1], o[3]), (o64[0], o64[1], o64[3]))
+    # Charge flips of FP32-conditioning tracks differ between FP32 and FP64 on
+    # the CPU too; every unflagged track must keep the oracle's charge.
+    from oracle.parity import flagged_tracks
+    fl = flagged_tracks((o[0], o[1], o[3]), (o64[0], o64[1], o64[3]))
+    assert (g[2][~fl] == o[2][~fl]).all(), "charges differ on unflagged tracks"
+    # the entry-level statistics of the earlier criterion, kept as a summary
     assert_parity("fit", g[0], o[0], o64[0], "par", TOL_PAR, frac=0.99)
     assert_parity("fit", g[1], o[1], o64[1], "cov", TOL_COV, frac=0.99)
-    # chi2 sums squared residuals, each the difference of two nearly equal
-    # coordinates: FP32 carries ~1e-4 relative error per hit there.
-    assert_parity("fit", g[3], o[3], o64[3], "chi2", TOL_CHI2, frac=0.95)
 
 
 def test_plain_equals_batched(gpu_device):
@@ -193,7 +194,8 @@ def test_fit_ragged_sizes(gpu_device, n):
     o64 = orc.fit(sub, pflags=mk.PF_APPLY_MATERIAL, dtype=np.float64)
     for batched in (False, True):
         g = _fit_gpu(ctx, sub, gpu_device, mk.PF_APPLY_MATERIAL, mk.FIT_CHI2, batched)
-        assert_parity("ragged", g[0], o[0], o64[0], "par", TOL_PAR, frac=0.95 if n > 1 else 0.0)
+        assert_track_parity("ragged n=%d" % n, (g[0], g[1], g[3]), (o[0], o[1], o[3]), (o64[0], o64[1], o64[3]),
+                            max_flagged=1.0 if n < 100 else 0.12)
 
 
 def test_fit_zero_hits_and_empty(gpu_device):
@@ -253,8 +255,9 @@ def test_all_batched_variants_identical(gpu_device, fallback):
     ff = mk.FIT_CHI2 | mk.FIT_CHARGE_FLIP | (mk.FIT_FP64_FALLBACK if fallback else 0)
     ref = None
     for v in range(mk.lib().mkf_num_batched_variants()):
-        for block, bpc in ((256, 0), (64, 4), (512, 1)):
-            ctx.set_batched_shape(block, bpc, v)
+        for block, bpc, tile in ((256, 0, 0), (64, 4, 0), (512, 1, 0), (64, 0, 1024), (256, 0, 4096),
+                                 (128, 2, 16384)):
+            ctx.set_batched_shape(block, bpc, v, tile)
             g = _fit_gpu(ctx, ev, gpu_device, mk.PF_APPLY_MATERIAL, ff, True)
             if ref is None:
                 ref = g
@@ -262,8 +265,8 @@ def test_all_batched_variants_identical(gpu_device, fallback):
                 ndiff = int(np.sum(~((x == y) | (np.isnan(x) & np.isnan(y)))))
                 if ndiff:
                     bad = np.where(~((x == y) | (np.isnan(x) & np.isnan(y))))
-                    print("variant %d shape (%d,%d): %s differs in %d entries, max abs %.3g, tracks %s" % (
-                        v, block, bpc, name, ndiff, np.nanmax(np.abs(x.astype(np.float64) - y)),
+                    print("variant %d shape (%d,%d,%d): %s differs in %d entries, max abs %.3g, tracks %s" % (
+                        v, block, bpc, tile, name, ndiff, np.nanmax(np.abs(x.astype(np.float64) - y)),
                         np.unique(bad[-1])[:10]))
                 assert ndiff == 0
 
@@ -312,5 +315,5 @@ def test_full_size_properties(gpu_device):
     orc = oracle.Oracle(p.bfield, rl, xi)
     o = orc.fit(sub, pflags=mk.PF_APPLY_MATERIAL, n_threads=8)
     o64 = orc.fit(sub, pflags=mk.PF_APPLY_MATERIAL, dtype=np.float64, n_threads=8)
-    assert_parity("full-size sample", par[:, idx], o[0], o64[0], "par", TOL_PAR, frac=0.99)
-    assert_parity("full-size sample", err[:, idx], o[1], o64[1], "cov", TOL_COV, frac=0.99)
+    assert_track_parity("full-size sample", (par[:, idx], err[:, idx], chi2[idx]), (o[0], o[1], o[3]),
+                        (o64[0], o64[1], o64[3]))
