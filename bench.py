@@ -224,19 +224,18 @@ def bench_fit(args, mk, torch, dev, uneven):
         # CPU baseline leg: the oracle restatement, timed on the host, and (as the checker) a
         # sample of the measured GPU output against it, track by track.
         import oracle
-        from oracle.parity import describe, flagged_parity
+        from oracle.parity import describe, fit_reference, flagged_parity
         ncpu = min(16, os.cpu_count() or 1)
         orc = oracle.Oracle(bfield=params.bfield, rl_grid=rl, xi_grid=xi)
         rng = np.random.default_rng(0)
         idx = np.sort(rng.choice(n, min(n, 20000), replace=False))
         sub = {k: np.ascontiguousarray(v[..., idx]) for k, v in ev.items()}
-        o32 = orc.fit(sub, pflags=pflags, fit_flags=fflags, n_threads=ncpu)
-        o64 = orc.fit(sub, pflags=pflags, fit_flags=fflags, dtype=np.float64, n_threads=ncpu)
+        o32, o64, chk = fit_reference(params.bfield, rl, xi, sub, pflags, fflags, n_threads=ncpu)
         r = flagged_parity((out_b[0][:, idx], out_b[1][:, idx], out_b[2][idx]), (o32[0], o32[1], o32[3]),
-                           (o64[0], o64[1], o64[3]))
-        r.pop("unexplained_idx")
+                           (o64[0], o64[1], o64[3]), chk)
         parity.update({"sample_tracks": int(len(idx)), "per_track": r,
                        "max_rel_par_err_vs_cpu": r["all_max"]["par"],
+                       "max_rel_par_err_vs_cpu_unflagged": r["unflagged_max"]["par"],
                        "frac_tracks_within_1e-5": r["frac_tracks_within_tol"]})
         log("parity: batched==plain %s; %s" % (identical, describe(r)))
         best = 1e30

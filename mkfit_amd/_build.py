@@ -63,7 +63,7 @@ def build_shim_demo():
 
 def build_oracle():
     odir = os.path.join(ROOT, "oracle")
-    _run(["make", "-s", "-C", odir, "-B", "all"])
+    _run(["make", "-s", "-j4", "-C", odir, "-B", "all"])
 
 
 def build_all():

@@ -43,9 +43,29 @@ def _cpu_has_avx512():
         return False
 
 
-def default_lib_path():
-    name = "liborc_avx512.so" if _cpu_has_avx512() else "liborc_avx2.so"
-    return os.path.join(_HERE, name)
+_OVERRIDE = []
+
+
+def default_lib_path(contract=True):
+    """The ISA-matched build; contract=False selects the build without FMA contraction (-ffp-contract=off)."""
+    if _OVERRIDE and contract:
+        return _OVERRIDE[-1]
+    name = "liborc_avx512" if _cpu_has_avx512() else "liborc_avx2"
+    return os.path.join(_HERE, name + ("" if contract else "_nofma") + ".so")
+
+
+class use_lib:
+    """Context manager: every oracle call inside uses the library at `path` (the mutation tests' builds)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        _OVERRIDE.append(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        _OVERRIDE.pop()
 
 
 def lib(path=None):
@@ -75,8 +95,8 @@ def _p(a):
 class Oracle:
     """Holds the B field and material grid (Config:: globals of the reference)."""
 
-    def __init__(self, bfield=3.8112, rl_grid=None, xi_grid=None, lib_path=None):
-        self.L = lib(lib_path)
+    def __init__(self, bfield=3.8112, rl_grid=None, xi_grid=None, lib_path=None, contract=True):
+        self.L = lib(lib_path or default_lib_path(contract))
         self._rl = None if rl_grid is None else np.ascontiguousarray(rl_grid, np.float32)
         self._xi = None if xi_grid is None else np.ascontiguousarray(xi_grid, np.float32)
         self.ctx = OrcCtx(float(bfield), _p(self._rl), _p(self._xi))
@@ -136,8 +156,12 @@ class Oracle:
                                                _p(oe), _p(opar), _p(c2))
         return oe, opar, c2
 
-    def fit(self, ev, pflags=0, fit_flags=FIT_CHI2, dtype=np.float32, n_threads=1, with_flags=True, pca_pflags=0):
-        """Whole fit of a toy-MC event dict; returns (par, err, chg, chi2)."""
+    def fit(self, ev, pflags=0, fit_flags=FIT_CHI2, dtype=np.float32, n_threads=1, with_flags=True, pca_pflags=0,
+            ties=False, tie_shift=0.0):
+        """Whole fit of a toy-MC event dict; returns (par, err, chg, chi2), and with ties=True also [3][n] margins
+        of each track's steps to their thresholds: the helix step's |r - r0| test and its |oodotp - 5| failure
+        test, and material bin edges (the first and third in float spacings).  tie_shift moves the helix step's
+        1e-4 cm threshold by that many float spacings of r (FP32 only)."""
         sfx, T = self._dt(dtype)
         n = ev["seed_par"].shape[1]
         S = ev["hit_pos"].shape[0]
@@ -148,9 +172,22 @@ class Oracle:
         hf = np.ascontiguousarray(ev["hit_flags"], np.uint8) if with_flags else None
         b = _Batch32(n, S, _p(par), _p(err), _p(chg), _p(chi2), _p(hp), _p(he), _p(nh), _p(hf), int(pflags),
                      int(fit_flags), int(pca_pflags))
-        rc = getattr(self.L, "orc_fit_" + sfx)(ctypes.byref(self.ctx), ctypes.byref(b), int(n_threads))
+        if ties:
+            if sfx != "f32":
+                raise ValueError("threshold margins are recorded by the FP32 fit")
+            tie = np.zeros((3, n), np.float32)
+            rc = self.L.orc_fit_ties_f32(ctypes.byref(self.ctx), ctypes.byref(b), int(n_threads), _p(tie))
+        elif tie_shift:
+            if sfx != "f32":
+                raise ValueError("the threshold shift applies to the FP32 fit")
+            rc = self.L.orc_fit_tie_shift_f32(ctypes.byref(self.ctx), ctypes.byref(b), int(n_threads),
+                                              ctypes.c_float(tie_shift))
+        else:
+            rc = getattr(self.L, "orc_fit_" + sfx)(ctypes.byref(self.ctx), ctypes.byref(b), int(n_threads))
         if rc != 0:
             raise RuntimeError("oracle fit failed (%d)" % rc)
+        if ties:
+            return par, err, chg, chi2, tie
         return par, err, chg, chi2
 
 

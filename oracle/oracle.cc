@@ -254,8 +254,16 @@ void joseph_fp64(const T* psErr, const T* psPar, const T* msErr, const T* msPar,
 // per-lane pattern: both operations run where a block mixes them, each lane
 // keeps its own) and the fake-hit rule for missing layers.
 template <class T, class B>
-void fit_block(const Ctx& ctx, const B& bt, int t0, int cnt) {
+void fit_block(const Ctx& ctx_in, const B& bt, int t0, int cnt, float* tie_out = nullptr) {
   const int n = bt.n_tracks;
+  Ctx ctx = ctx_in;
+  alignas(64) float tie_r[NN], tie_dot[NN], tie_mat[NN];
+  if (tie_out) {
+    for (int l = 0; l < NN; ++l) tie_r[l] = tie_dot[l] = tie_mat[l] = 1e30f;
+    ctx.tie_r = tie_r;
+    ctx.tie_dot = tie_dot;
+    ctx.tie_mat = tie_mat;
+  }
   alignas(64) T err[21 * NN], par[6 * NN], chi2[NN];
   alignas(64) T errP[21 * NN], parP[6 * NN], errU[21 * NN], parU[6 * NN], c2[NN];
   alignas(64) T errE[21 * NN], parE[6 * NN], c2E[NN], errPE[21 * NN], parPE[6 * NN];
@@ -335,15 +343,21 @@ void fit_block(const Ctx& ctx, const B& bt, int t0, int cnt) {
   unpack(bt.par, par, 6, n, t0, cnt);
   unpack(bt.chi2, chi2, 1, n, t0, cnt);
   unpack(bt.chg, chg, 1, n, t0, cnt);
+  if (tie_out) {
+    unpack(tie_out, tie_r, 1, n, t0, cnt);
+    unpack(tie_out + n, tie_dot, 1, n, t0, cnt);
+    unpack(tie_out + 2 * (size_t)n, tie_mat, 1, n, t0, cnt);
+  }
 }
 
 template <class T, class B>
-int fit_batch(const orc_ctx* c, const B* b, int n_threads) {
+int fit_batch(const orc_ctx* c, const B* b, int n_threads, float* tie_out = nullptr, float tie_shift = 0.f) {
   if (!b || b->n_tracks < 0 || b->n_slots < 0) return 1;
-  const Ctx ctx = make_ctx(c);
+  Ctx ctx = make_ctx(c);
+  ctx.tie_shift = tie_shift;
   for_each_block(b->n_tracks, n_threads, [&](int blk) {
     const int t0 = blk * NN, cnt = std::min(NN, b->n_tracks - t0);
-    fit_block<T>(ctx, *b, t0, cnt);
+    fit_block<T>(ctx, *b, t0, cnt, tie_out);
   });
   return 0;
 }
@@ -396,5 +410,15 @@ int orc_kalman_op_f64(int op, int ec, int n, const double* pe, const double* pp,
 }
 int orc_fit_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int nt) { return fit_batch<float>(c, b, nt); }
 int orc_fit_f64(const orc_ctx* c, const orc_fit_batch_f64* b, int nt) { return fit_batch<double>(c, b, nt); }
+// The FP32 fit, also returning per track how close its steps came to their thresholds: ties [3][n_tracks] =
+// Ctx::tie_r, Ctx::tie_dot, Ctx::tie_mat (1e30 where a track never met one).
+int orc_fit_ties_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int nt, float* ties) {
+  return ties ? fit_batch<float>(c, b, nt, ties) : 1;
+}
+// The FP32 fit with the helix step's 1e-4 cm threshold moved by tie_shift float spacings of r: the result on
+// the other side of every threshold tie within that distance (Ctx::tie_shift).
+int orc_fit_tie_shift_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int nt, float tie_shift) {
+  return fit_batch<float>(c, b, nt, nullptr, tie_shift);
+}
 
 }  // extern "C"

@@ -91,6 +91,8 @@ int orc_kalman_op_f64(int op, int endcap, int n, const double* ps_err, const dou
 /* Whole fit, one thread per chunk of two NN blocks (the reference's TBB grain). */
 int orc_fit_f32(const orc_ctx* ctx, const orc_fit_batch_f32* b, int n_threads);
 int orc_fit_f64(const orc_ctx* ctx, const orc_fit_batch_f64* b, int n_threads);
+int orc_fit_ties_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int n_threads, float* ties);
+int orc_fit_tie_shift_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int n_threads, float tie_shift);
 
 /* ---- Fit as a function of per-layer material and alignment (oracle_align.cc) */
 

@@ -60,6 +60,19 @@ struct Ctx {
   float bfield;            // uniform field in T (Config::Bfield)
   const float* rl_grid;    // [kNBinsZME][kNBinsRME] radiation lengths (Config::RlgridME)
   const float* xi_grid;    // [kNBinsZME][kNBinsRME] (Config::XigridME)
+  // Optional per-lane records [NN] of how close the helix step came to its two
+  // thresholds: min over iterations of | |r - r0| - 1e-4 | in units of the
+  // float spacing at r (the switch of the derivative terms) and of
+  // |oodotp - 5| (the failure test).  A checker uses them to tell threshold
+  // ties from rounding noise.
+  float* tie_r = nullptr;
+  float* tie_dot = nullptr;
+  // min over lookups of the distance, in float spacings of the coordinate, of
+  // the material lookup point to a bin edge across which the grid changes
+  float* tie_mat = nullptr;
+  // Moves the helix step's 1e-4 cm threshold by this many float spacings of r
+  // (0 = the reference's test): the checker's "other side of the tie" result.
+  float tie_shift = 0.f;
 };
 
 // Symmetric packed index, MatriplexSym.h:21.
@@ -220,6 +233,7 @@ inline void helix_at_r(const T* inPar, const int* inChg, const T* msRad, T* outP
     for (int i = 0; i < kNiter; ++i) {
       r0 = hipo(outPar[0 * NN + n], outPar[1 * NN + n]);
       const T oodotp = r0 * pt / (pxin * outPar[0 * NN + n] + pyin * outPar[1 * NN + n]);
+      if (ctx.tie_dot) ctx.tie_dot[n] = std::min(ctx.tie_dot[n], float(std::abs(oodotp - T(5.0f))));
       if (oodotp > T(5.0f) || oodotp < T(0)) {
         id = 0;
         failFlag[n] = 1;
@@ -235,7 +249,13 @@ inline void helix_at_r(const T* inPar, const int* inChg, const T* msRad, T* outP
       if (i + 1 != kNiter) {
         const T x = outPar[0 * NN + n];
         const T y = outPar[1 * NN + n];
-        const T oor0 = (r0 > T(0) && std::abs(r - r0) < T(0.0001f)) ? T(1) / r0 : T(0);
+        T thr = T(0.0001f);
+        if (ctx.tie_r || ctx.tie_shift != 0.f) {
+          const float rf = float(r), ulp = std::nextafter(rf, 2 * rf + 1) - rf;
+          if (ctx.tie_r) ctx.tie_r[n] = std::min(ctx.tie_r[n], float(std::abs(std::abs(r - r0) - thr)) / ulp);
+          thr += T(ctx.tie_shift * ulp);
+        }
+        const T oor0 = (r0 > T(0) && std::abs(r - r0) < thr) ? T(1) / r0 : T(0);
         const T dadipt = id * kinv;
         const T dadx = -x * ipt * kinv * oor0;
         const T dady = -y * ipt * kinv * oor0;
@@ -432,6 +452,25 @@ template <class T> inline void lookup_material(const Ctx& ctx, T zq, T rq, T& rl
   xi = ok ? T(ctx.xi_grid[zb * kNBinsRME + rb]) : T(0);
 }
 
+// Distance of a lookup point to the nearest bin edge across which the material grid changes, in float
+// spacings of the coordinate crossing it (for Ctx::tie_mat).
+inline float material_edge_margin(const Ctx& ctx, float zq, float rq) {
+  if (!ctx.rl_grid) return 1e30f;
+  auto val = [&](int zb, int rb) -> float {
+    return (zb >= 0 && zb < kNBinsZME && rb >= 0 && rb < kNBinsRME) ? ctx.rl_grid[zb * kNBinsRME + rb] : 0.f;
+  };
+  const float zf = std::abs(zq), rf = rq;
+  const int zb = zbin_me(zq), rb = rbin_me(rq);
+  float m = 1e30f;
+  const float xz = zf * kNBinsZME / kRangeZME, xr = rf * kNBinsRME / kRangeRME;
+  const int ez = (int)std::lround(xz), er = (int)std::lround(xr);
+  if (val(ez - 1, rb) != val(ez, rb))
+    m = std::min(m, std::abs(xz - ez) / ((std::nextafter(zf, 2 * zf + 1) - zf) * kNBinsZME / kRangeZME));
+  if (val(zb, er - 1) != val(zb, er))
+    m = std::min(m, std::abs(xr - er) / ((std::nextafter(rf, 2 * rf + 1) - rf) * kNBinsRME / kRangeRME));
+  return m;
+}
+
 // ---- propagateHelixToRMPlex (PropagationMPlex.cc:408-505) -------------------
 template <class T>
 // matRl / matXi, when given, replace the grid lookup with per-lane material
@@ -456,6 +495,8 @@ inline void propagate_helix_to_r(const T* inErr, const T* inPar, const int* inCh
           hitsXi[n] = matXi[n];
         } else {
           lookup_material(ctx, outPar[2 * NN + n], msRad[n], hitsRl[n], hitsXi[n]);
+          if (ctx.tie_mat)
+            ctx.tie_mat[n] = std::min(ctx.tie_mat[n], material_edge_margin(ctx, float(outPar[2 * NN + n]), float(msRad[n])));
         }
       }
       const T r0 = hipo(inPar[0 * NN + n], inPar[1 * NN + n]);
@@ -495,8 +536,11 @@ inline void propagate_helix_to_z(const T* inErr, const T* inPar, const int* inCh
     for (int n = 0; n < NN; ++n) {
       hitsRl[n] = 0; hitsXi[n] = 0; propSign[n] = 1;
       if (n >= N_proc) continue;
-      if (!(noMatEff && noMatEff[n]))
-        lookup_material(ctx, msZ[n], T(std::hypot(outPar[0 * NN + n], outPar[1 * NN + n])), hitsRl[n], hitsXi[n]);
+      if (!(noMatEff && noMatEff[n])) {
+        const T rq = T(std::hypot(outPar[0 * NN + n], outPar[1 * NN + n]));
+        lookup_material(ctx, msZ[n], rq, hitsRl[n], hitsXi[n]);
+        if (ctx.tie_mat) ctx.tie_mat[n] = std::min(ctx.tie_mat[n], material_edge_margin(ctx, float(msZ[n]), float(rq)));
+      }
       const T zout = msZ[n];
       const T zin = inPar[2 * NN + n];
       propSign[n] = (std::abs(zout) > std::abs(zin) ? T(1) : T(-1));

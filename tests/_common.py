@@ -125,25 +125,20 @@ def assert_parity(name, gpu, cpu32, cpu64, kind, tol, frac=0.999):
     return line
 
 
-def assert_track_parity(name, gpu, cpu32, cpu64, max_flagged=0.12):
-    """Per-track parity of a fit (oracle/parity.py): gpu/cpu32/cpu64 = (par, err, chi2 or None).
-
-    Every unflagged track passes element by element (within tolerance of the
-    FP32 oracle, or no further from the FP64 fit than the FP32 oracle); the
-    flagged tracks -- those whose FP32 oracle result is itself beyond a
-    tolerance of the FP64 oracle -- are at most `max_flagged` of the sample,
-    and on them the GPU is no further from the FP64 fit than three times the
-    FP32 oracle's largest distance (plus the tolerance).  Returns the result
-    dict."""
+def assert_track_parity(name, gpu, cpu32, cpu64, chk, max_flagged=None):
+    """Per-track parity of a fit (oracle/parity.py): gpu/cpu32/cpu64 = (par, err, chi2 or None), chk the
+    oracle's per-track checker data (fit_reference).  Unflagged tracks are within the tolerance of the FP32
+    oracle in every element; flagged tracks (rounding alone moves the reference result by more than a third
+    of the tolerance) are within four times that spread; tracks with a threshold tie may instead match the
+    oracle with the tie taken the other way; at most 0.02 % of the tracks (2 in a small sample) may exceed
+    their bound, by at most 2x, each listed.  Returns the result dict."""
     from oracle.parity import describe, flagged_parity
-    r = flagged_parity(gpu, cpu32, cpu64)
+    r = flagged_parity(gpu, cpu32, cpu64, chk)
     line = "%s: %s" % (name, describe(r))
     print(line)
-    assert r["n_unexplained"] == 0, (line, r["unexplained_idx"])
-    assert r["n_flagged"] <= max_flagged * r["n"], line
-    fg, fc = r["flagged_vs_fp64_max"]["gpu"], r["flagged_vs_fp64_max"]["cpu32"]
-    for k, t in (("par", TOL_PAR), ("cov", TOL_COV), ("chi2", TOL_CHI2)):
-        assert fg[k] <= 3 * fc[k] + t, (k, fg, fc, line)
+    assert r["ok"], (line, r["beyond"])
+    if max_flagged is not None:
+        assert r["n_flagged"] <= max_flagged * r["n"], line
     return r
 
 

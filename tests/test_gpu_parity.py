@@ -10,6 +10,7 @@ import pytest
 
 import mkfit_amd as mk
 import oracle
+from oracle.parity import fit_reference
 from _common import (TOL_CHI2, TOL_COV, TOL_PAR, assert_parity, assert_track_parity, chi2_rel, cov_rel, event,
                      par_rel, summary)
 
@@ -155,23 +156,20 @@ FIT_CASES = [
 def test_fit_parity(gpu_device, case, batched):
     preset, n, seed, pflags, fflags, kw = FIT_CASES[case]
     geo, p, ev, rl, xi = event(preset, n, seed, **kw)
-    orc = oracle.Oracle(p.bfield, rl, xi)
-    o = orc.fit(ev, pflags=pflags, fit_flags=fflags, n_threads=8)
-    o64 = orc.fit(ev, pflags=pflags, fit_flags=fflags, dtype=np.float64, n_threads=8)
+    o, o64, chk = fit_reference(p.bfield, rl, xi, ev, pflags, fflags)
     ctx = mk.Context(p.bfield, rl, xi)
     g = _fit_gpu(ctx, ev, gpu_device, pflags, fflags, batched)
     print()
-    # Every track: unflagged ones within tolerance (or nearer the FP64 fit than
-    # the FP32 oracle), flagged ones counted and bounded (assert_track_parity).
-    r = assert_track_parity("fit", (g[0], g[1], g[3]), (o[0], o[1], o[3]), (o64[0], o64[1], o64[3]))
-    # Charge flips of FP32-conditioning tracks differ between FP32 and FP64 on
-    # the CPU too; every unflagged track must keep the oracle's charge.
-    from oracle.parity import flagged_tracks
-    fl = flagged_tracks((o[0], o[1], o[3]), (o64[0], o64[1], o64[3]))
-    assert (g[2][~fl] == o[2][~fl]).all(), "charges differ on unflagged tracks"
-    # the entry-level statistics of the earlier criterion, kept as a summary
-    assert_parity("fit", g[0], o[0], o64[0], "par", TOL_PAR, frac=0.99)
-    assert_parity("fit", g[1], o[1], o64[1], "cov", TOL_COV, frac=0.99)
+    # Every track: unflagged ones within tolerance of the FP32 oracle, flagged
+    # ones (rounding-sensitive, or with a threshold tie) within their bound
+    # (assert_track_parity).
+    r = assert_track_parity("fit", (g[0], g[1], g[3]), (o[0], o[1], o[3]), (o64[0], o64[1], o64[3]), chk)
+    # Charge flips of rounding-sensitive tracks can differ between FP32
+    # roundings; every unflagged track must keep the oracle's charge.
+    from oracle.parity import flag_mask
+    unflagged = ~flag_mask(chk)
+    assert (g[2][unflagged] == o[2][unflagged]).all(), "charges differ on unflagged tracks"
+    assert r["n_flagged"] < n
 
 
 def test_plain_equals_batched(gpu_device):
@@ -188,14 +186,12 @@ def test_plain_equals_batched(gpu_device):
 def test_fit_ragged_sizes(gpu_device, n):
     geo, p, ev, rl, xi = event(1, 1000, 41)
     sub = {k: np.ascontiguousarray(v[..., :n]) for k, v in ev.items()}
-    orc = oracle.Oracle(p.bfield, rl, xi)
-    o = orc.fit(sub, pflags=mk.PF_APPLY_MATERIAL)
+    o, o64, chk = fit_reference(p.bfield, rl, xi, sub, mk.PF_APPLY_MATERIAL, mk.FIT_CHI2)
     ctx = mk.Context(p.bfield, rl, xi)
-    o64 = orc.fit(sub, pflags=mk.PF_APPLY_MATERIAL, dtype=np.float64)
     for batched in (False, True):
         g = _fit_gpu(ctx, sub, gpu_device, mk.PF_APPLY_MATERIAL, mk.FIT_CHI2, batched)
         assert_track_parity("ragged n=%d" % n, (g[0], g[1], g[3]), (o[0], o[1], o[3]), (o64[0], o64[1], o64[3]),
-                            max_flagged=1.0 if n < 100 else 0.12)
+                            chk)
 
 
 def test_fit_zero_hits_and_empty(gpu_device):
@@ -312,8 +308,6 @@ def test_full_size_properties(gpu_device):
     # a seeded sample through the oracle
     idx = np.random.default_rng(0).choice(len(nh), 20000, replace=False)
     sub = {k: np.ascontiguousarray(v[..., idx]) for k, v in ev.items()}
-    orc = oracle.Oracle(p.bfield, rl, xi)
-    o = orc.fit(sub, pflags=mk.PF_APPLY_MATERIAL, n_threads=8)
-    o64 = orc.fit(sub, pflags=mk.PF_APPLY_MATERIAL, dtype=np.float64, n_threads=8)
+    o, o64, chk = fit_reference(p.bfield, rl, xi, sub, mk.PF_APPLY_MATERIAL, mk.FIT_CHI2)
     assert_track_parity("full-size sample", (par[:, idx], err[:, idx], chi2[idx]), (o[0], o[1], o[3]),
-                        (o64[0], o64[1], o64[3]))
+                        (o64[0], o64[1], o64[3]), chk)

@@ -3,8 +3,8 @@ import numpy as np
 import pytest
 
 import mkfit_amd as mk
-import oracle
-from _common import TOL_COV, TOL_PAR, assert_parity, par_rel
+from oracle.parity import fit_reference
+from _common import assert_track_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -21,12 +21,9 @@ def test_pca_parity(gpu_device, form):
                                pca_pflags=0)
     (ctx.fit_plain if form == "plain" else ctx.fit_batched)(b)
     torch.cuda.synchronize()
-    O = oracle.Oracle(bfield=3.8, rl_grid=rl, xi_grid=xi)
-    par, err, chg, chi2 = O.fit(ev, pflags=2, fit_flags=1 | 8, n_threads=8, pca_pflags=0)
-    p64, e64, _, _ = O.fit(ev, pflags=2, fit_flags=1 | 8, dtype=np.float64, n_threads=8, pca_pflags=0)
     # The step to the PCA extrapolates the fitted state over up to a metre, so FP32 rounding of the
-    # angles (~1e-7 rad) moves the position there by ~1e-5 cm: the FP32 oracle itself is that far from
-    # FP64.  The GPU must be as close to FP32 and FP64 as the FP32 oracle is to FP64.
-    floor = float(np.mean(par_rel(par, p64) <= TOL_PAR))
-    assert_parity("pca", b.par.cpu().numpy(), par, p64, "par", TOL_PAR, frac=floor - 0.05)
-    assert_parity("pca", b.err.cpu().numpy(), err, e64, "cov", TOL_COV, frac=0.95)
+    # angles (~1e-7 rad) moves the position there by ~1e-5 cm: most tracks are rounding-sensitive, and
+    # the per-track criterion holds each to its own measured spread.
+    o, o64, chk = fit_reference(3.8, rl, xi, ev, mk.PF_APPLY_MATERIAL, ff, pca_pflags=0)
+    g = (b.par.cpu().numpy(), b.err.cpu().numpy(), b.chi2.cpu().numpy())
+    assert_track_parity("pca", g, (o[0], o[1], o[3]), (o64[0], o64[1], o64[3]), chk)

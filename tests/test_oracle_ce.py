@@ -240,7 +240,7 @@ def test_ce_matches_composition(K):
                 assert got["n_found"][i, s] == c.nfound and got["chg"][i, s] == c.chg
                 np.testing.assert_allclose(got["par"][i, :, s], c.par, rtol=2e-4, atol=2e-5)
                 np.testing.assert_allclose(got["score"][i, s], c.score, rtol=1e-5, atol=1e-3)
-    assert same >= 0.99 * ev["n_seeds"], same
+    assert same == ev["n_seeds"], same
 
 
 def test_ce_invariants():

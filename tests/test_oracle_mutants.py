@@ -1,10 +1,14 @@
 """The oracle pins have teeth: each planted mistake must make some pin fail.
 
 Every mutant is a one-place textual change to a copy of the oracle source (a
-dropped term, a flipped sign, a transposed operand, a wrong index), compiled
-into a temporary library and run through every pin of tests/pins.py.  A
-mutant that passes all pins would mean that part of the restatement is not
-pinned; the test fails and names it.
+dropped term, a flipped sign, a transposed operand, a wrong index, a changed
+constant or comparison), compiled into a temporary library and run through the
+pins.  Fit mutants (oracle_core.h) run through every pin of tests/pins.py;
+finding mutants (oracle_find_core.h, oracle_find.cc, oracle_ce.cc) through the
+finding pins of test_oracle_find_pins.py, test_oracle_find.py and
+test_oracle_ce.py, cheapest first, until one fails.  A mutant that passes all
+pins would mean that part of the restatement is not pinned; the test fails and
+names it.
 """
 import concurrent.futures
 import os
@@ -48,18 +52,93 @@ MUTANTS = {
 }
 
 
-def _build(name, old, new, workdir):
+# Finding restatement: (file, original text, mutated text).  Scores and their constants (Config.h:355-359,
+# Track.h:648-670), invalid-hit codes and counts (MkFinder.cc:1237-1270), hit records (:1210-1225), the
+# candidate bookkeeping (CandCloner.cc:37-191, HitStructures.h:873-889), the best short candidate and the
+# final merge (HitStructures.cc:511-552), the stop rules (MkBuilder.cc:1521-1533, 1568-1630), best-hit
+# selection and fake codes (MkFinder.cc:691-832) and the window / bin arithmetic (MkFinder.cc:227-684,
+# HitStructures.cc:77-183).
+FIND_MUTANTS = {
+    "score_valid_bonus": ("oracle_ce.cc", "kValidHitBonus = 4,", "kValidHitBonus = 5,"),
+    "score_valid_slope": ("oracle_ce.cc", "kValidHitSlope = 0.2f", "kValidHitSlope = 0.25f"),
+    "score_hole_penalty": ("oracle_ce.cc", "kMissingHitPenalty = 8,", "kMissingHitPenalty = 7,"),
+    "score_tail_penalty": ("oracle_ce.cc", "kTailMissingHitPenalty = 3;", "kTailMissingHitPenalty = 2;"),
+    "score_lowpt_find_penalty": ("oracle_ce.cc", "penalty *= in_find ? 1.7f : 1.5f;",
+                                 "penalty *= in_find ? 1.5f : 1.5f;"),
+    "score_lowpt_final_penalty": ("oracle_ce.cc", "penalty *= in_find ? 1.7f : 1.5f;",
+                                  "penalty *= in_find ? 1.7f : 1.7f;"),
+    "score_lowpt_bonus": ("oracle_ce.cc", "bonus * (in_find ? 0.9f : 1.0f)", "bonus * (in_find ? 1.0f : 1.0f)"),
+    "score_max_bonus": ("oracle_ce.cc", "const float maxBonus = 8.0;", "const float maxBonus = 9.0;"),
+    "score_lowpt_threshold": ("oracle_ce.cc", "if (pt < 0.9) {", "if (pt < 0.8) {"),
+    "score_negative_chi2": ("oracle_ce.cc", "  if (c < 0) c = 0.f;\n", ""),
+    "count_inside_holes": ("oracle_ce.cc", "      c.ninside += c.ntail;\n", ""),
+    "count_tail_any_fake": ("oracle_ce.cc", "if (idx == -1) ++c.ntail;", "if (idx < 0) ++c.ntail;"),
+    "fake_consec_le": ("oracle_ce.cc", "lc[i].ntail < P.max_consec_holes) ? -1 : -2;",
+                       "lc[i].ntail <= P.max_consec_holes) ? -1 : -2;"),
+    "fake_holes_inside_only": ("oracle_ce.cc", "(lc[i].ninside + lc[i].ntail < P.max_holes",
+                               "(lc[i].ninside < P.max_holes"),
+    "fake_edge_code": ("oracle_ce.cc", "if (wsr[i] == WSR_Edge) fake = -3;", "if (wsr[i] == WSR_Edge) fake = -1;"),
+    "fake_tailholes": ("oracle_ce.cc", "e.ntailholes = fake == -1 ? lc[i].ntail + 1 : lc[i].ntail;",
+                       "e.ntailholes = lc[i].ntail + 1;"),
+    "hit_record_holes": ("oracle_ce.cc", "e.nholes = lc[i].ninside + lc[i].ntail;", "e.nholes = lc[i].ninside;"),
+    "ce_chi2_cut": ("oracle_ce.cc", "if (!(c2 < max_c2)) continue;", "if (!(c2 < 0.5f * max_c2)) continue;"),
+    "best_short_compare": ("oracle_ce.cc", "if (h.score > cc.best_short.score) cc.best_short = tc;",
+                           "if (h.score < cc.best_short.score) cc.best_short = tc;"),
+    "short_kept_live": ("oracle_ce.cc", "if (h.score > cc.best_short.score) cc.best_short = tc;\n          continue;",
+                        "if (h.score > cc.best_short.score) cc.best_short = tc;"),
+    "extras_at_end": ("oracle_ce.cc", "while (xi < extras.size() && extras[xi].score > tc.score",
+                      "while (false && xi < extras.size() && extras[xi].score > tc.score"),
+    "top_k_one_short": ("oracle_ce.cc", "const int num_hits = std::min((int)entries.size(), max_cands);",
+                        "const int num_hits = std::min((int)entries.size(), std::max(1, max_cands - 1));"),
+    "merge_no_pop": ("oracle_ce.cc", "if ((int)cc.cands.size() >= max_cands) cc.cands.pop_back();", ""),
+    "merge_no_rescore": ("oracle_ce.cc", "    for (auto& c : cc.cands) c.score = score_cand(c);\n", ""),
+    "apex_pt": ("oracle_ce.cc", "if (c.pT() < 1.2) {", "if (c.pT() < 0.6) {"),
+    "apex_no_dphi_test": ("oracle_ce.cc", "625.f && dphi > 1.371f && dphi < 4.512f", "625.f"),
+    "barrel_miss_not_stopped": ("oracle_ce.cc", "cc.add_hit(extras.back(), -2, layer, 0);", ""),
+    "update_no_charge_flip": ("oracle_ce.cc", "      chg[n] = -chg[n];\n", ""),
+    "bh_chi2_le": ("oracle_find.cc", "if (c2 < minChi2[it]) {", "if (c2 <= minChi2[it]) {"),
+    "bh_fake_inside_only": ("oracle_find.cc", "else if (B.ninside[it] + B.ntail[it])", "else if (B.ninside[it])"),
+    "bh_edge_code": ("oracle_find.cc", "fake_idx = -3;", "fake_idx = -1;"),
+    "bh_chi2_not_summed": ("oracle_find.cc", "      B.chi2[it] += minChi2[it];\n", ""),
+    "bh_count_inside": ("oracle_find.cc", "      B.ninside[it] += B.ntail[it];\n      B.ntail[it] = 0;",
+                        "      B.ntail[it] = 0;"),
+    "bh_fake_hit_error": ("oracle_find.cc", "const float f[6] = {666, 0, 666, 0, 0, 666};",
+                          "const float f[6] = {66, 0, 66, 0, 0, 66};"),
+    "bh_stop_code": ("oracle_find.cc", "B.hot_idx[std::min(B.nhits[i], kMaxTrkHits) - 1][i] == -2",
+                     "B.hot_idx[std::min(B.nhits[i], kMaxTrkHits) - 1][i] == -3"),
+    "bh_barrel_miss": ("oracle_find.cc", "if (x * x + y * y < r_min_sqr) {", "if (x * x + y * y < 0.8f * r_min_sqr) {"),
+    "cdist_no_wrap": ("oracle_find_core.h", "return a > kPI ? kTwoPI - a : a;", "return a;"),
+    "nsigma_phi": ("oracle_find_core.h", "const float nSigmaPhi = 3, nSigmaZ = 3", "const float nSigmaPhi = 2.5, nSigmaZ = 3"),
+    "phi_bin_end": ("oracle_find_core.h", "pb2v[it] = phi_bin(phi + dphi) + 1;", "pb2v[it] = phi_bin(phi + dphi);"),
+    "q_bin_end": ("oracle_find_core.h", "qb2v[it] = q_bin_checked(L.b, q + dq) + 1;",
+                  "qb2v[it] = q_bin_checked(L.b, q + dq);"),
+    "window_phi_cut": ("oracle_find_core.h", "if (ddphi >= dphi) continue;", "if (ddphi > 1.1f * dphi) continue;"),
+    "loh_key_shift": ("oracle_find_core.h", "(q_bin_checked(L.b, q[i]) << 16)", "(q_bin_checked(L.b, q[i]) << 15)"),
+    "edge_correction": ("oracle_find_core.h",
+                        "const float edgeCorr = std::abs(0.5f * (d.rout - d.rin) / std::tan(par[5 * NN + it]));",
+                        "const float edgeCorr = 0.f;"),
+    "wsr_inside_margin": ("oracle_find_core.h", "if (z < d.zmax - dz && z > d.zmin + dz) return WSR_Inside;",
+                          "if (z < d.zmax && z > d.zmin) return WSR_Inside;"),
+}
+
+ORACLE_FILES = ("oracle.cc", "oracle_find.cc", "oracle_ce.cc", "oracle_align.cc", "oracle.h", "oracle_core.h",
+                "oracle_find_core.h")
+
+
+def _build(name, old, new, workdir, target="oracle_core.h"):
     d = os.path.join(workdir, name)
     os.makedirs(d)
-    for f in ("oracle.cc", "oracle.h", "oracle_core.h"):
+    for f in ORACLE_FILES:
         shutil.copy(os.path.join(ORACLE_DIR, f), d)
-    p = os.path.join(d, "oracle_core.h")
+    p = os.path.join(d, target)
     src = open(p).read()
     assert src.count(old) == 1, "%s: pattern matches %d times" % (name, src.count(old))
     open(p, "w").write(src.replace(old, new))
     so = os.path.join(d, "liborc_mut.so")
-    r = subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-pthread", "-DORC_NN=8", "-o", so,
-                        os.path.join(d, "oracle.cc")], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    srcs = [os.path.join(d, "oracle.cc")] if target == "oracle_core.h" else \
+        [os.path.join(d, f) for f in ORACLE_FILES if f.endswith(".cc")]
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-pthread", "-DORC_NN=8", "-o", so] + srcs,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0, r.stdout
     return so
 
@@ -89,3 +168,47 @@ def test_mutant_is_caught(mutant_libs, name):
         except Exception as e:  # AssertionError, or NaN-driven numpy errors
             caught.append("%s: %s" % (pin.__name__, type(e).__name__))
     assert caught, "mutant %s passes every pin" % name
+
+
+@pytest.fixture(scope="module")
+def find_mutant_libs():
+    work = tempfile.mkdtemp(prefix="orc_fmut_")
+    try:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+            futs = {k: ex.submit(_build, k, a, b, work, f) for k, (f, a, b) in FIND_MUTANTS.items()}
+            yield {k: f.result() for k, f in futs.items()}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def _find_pins():
+    import test_oracle_ce as ce
+    import test_oracle_find as bh
+    import test_oracle_find_pins as fp
+    return [
+        ("invalid_hit_records", lambda: [fp.test_invalid_hit_record_known_answers(c)
+                                         for c in range(len(fp.INVALID_HIT_CASES))]),
+        ("outside_lane", fp.test_outside_lane_gets_no_record),
+        ("hit_records", fp.test_hit_record_counts_and_score),
+        ("loh_stable_sort", bh.test_loh_matches_stable_sort),
+        ("select_brute_force", bh.test_select_hit_indices_matches_brute_force),
+        ("duplicated_hits", fp.test_duplicated_hits_resolve_to_first_copy),
+        ("ce_greedy", fp.test_clone_engine_k1_is_greedy),
+        ("ce_all_paths_4_1", lambda: fp.test_clone_engine_equals_all_paths((4, 1))),
+        ("ce_all_paths_1_1", lambda: fp.test_clone_engine_equals_all_paths((1, 1))),
+        ("ce_all_paths_3_2", lambda: fp.test_clone_engine_equals_all_paths((3, 2))),
+        ("bh_loop_composition", lambda: bh.test_find_loop_matches_composition(bh.make_find_event())),
+        ("ce_composition_k6", lambda: ce.test_ce_matches_composition(6)),
+    ]
+
+
+@pytest.mark.parametrize("name", sorted(FIND_MUTANTS))
+def test_find_mutant_is_caught(find_mutant_libs, name):
+    so = find_mutant_libs[name]
+    with oracle.use_lib(so):
+        for pin_name, pin in _find_pins():
+            try:
+                pin()
+            except Exception as e:  # AssertionError, or numpy errors on the mutant's output
+                return
+    pytest.fail("mutant %s passes every finding pin" % name)
