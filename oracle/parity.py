@@ -17,8 +17,8 @@ changes of rounding move the reference algorithm's result on it:
   also counting the FP32 oracle's own distance to the FP64 oracle,
 
 with the variants: the restatement built without FMA contraction, and
-twenty-three runs on seeds whose parameters are moved by one ulp (random sign),
-alternating the FMA and no-FMA builds.  A track is *flagged* when
+twenty-three runs on inputs whose seed parameters and hit coordinates are
+moved by one ulp (random sign), alternating the FMA and no-FMA builds.  A track is *flagged* when
 FLAG_FACTOR x spread > 1, i.e. when rounding alone moves the reference
 result on it by more than a third of the tolerance.
 
@@ -36,9 +36,13 @@ bin edges).  A track with a decision within TIE_ULPS spacings (or
 the bound, either the oracle's or the oracle's with the |r - r0| threshold
 moved by TIE_ULPS spacings either way (every such tie taken the other way).
 
-Criteria (all element-wise within a track, errors in units of the tolerance):
-  unflagged tracks: within the tolerance of the FP32 oracle, every element;
-  flagged tracks:   within max(1, BOUND_FACTOR x spread);
+Parameters, covariance and chi2 are flagged separately (each group of a
+track has its own spread), since chi2 is far more sensitive to rounding.
+
+Criteria (element-wise within each group of a track, errors in units of the
+tolerance):
+  unflagged group:  within the tolerance of the FP32 oracle, every element;
+  flagged group:    within max(1, BOUND_FACTOR x spread);
   tie-flagged:      as above, against the oracle or against one of the
                     results with the ties taken the other way.
 The spread is a sampled estimate of a heavy-tailed distribution (chi2, whose
@@ -110,6 +114,13 @@ def _elem_errors(gpu, ref):
     return out
 
 
+def group_error(a, ref):
+    """[3, n]: per track, the largest element error in units of its tolerance in each group (0 parameters,
+    1 covariance, 2 chi2)."""
+    e = _elem_errors(a, ref)
+    return np.stack([(e[k] / TOL[k]).max(axis=0) for k in range(3)])
+
+
 def track_error(a, ref):
     """Per track: the largest element error in units of its tolerance, and the group (0 par, 1 cov, 2 chi2)
     and element index where it occurs."""
@@ -123,13 +134,20 @@ def track_error(a, ref):
     return worst, grp, elem
 
 
+def _ulp_moved(a, rng):
+    a = np.asarray(a, np.float32)
+    up = rng.integers(0, 2, a.shape).astype(bool)
+    return np.where(up, np.nextafter(a, np.float32(np.inf)), np.nextafter(a, np.float32(-np.inf))).astype(np.float32)
+
+
 def perturbed_seeds(ev, seed):
-    """A copy of a toy event with every seed parameter moved by one ulp, in a random direction."""
+    """A copy of a toy event with every seed parameter and every hit coordinate moved by one ulp, in a random
+    direction (the hits' rounding is what moves chi2: a residual of ~1 m coordinates against ~100 um errors)."""
     rng = np.random.default_rng(seed)
-    sp = np.asarray(ev["seed_par"], np.float32)
-    up = rng.integers(0, 2, sp.shape).astype(bool)
-    moved = np.where(up, np.nextafter(sp, np.float32(np.inf)), np.nextafter(sp, np.float32(-np.inf)))
-    return dict(ev, seed_par=moved.astype(np.float32))
+    out = dict(ev, seed_par=_ulp_moved(ev["seed_par"], rng))
+    if "hit_pos" in ev:
+        out["hit_pos"] = _ulp_moved(ev["hit_pos"], rng)
+    return out
 
 
 def rounding_variants(fit32, fit32_nofma, ev, n_perturbed=N_PERTURBED):
@@ -143,84 +161,90 @@ def rounding_variants(fit32, fit32_nofma, ev, n_perturbed=N_PERTURBED):
 
 
 def track_spread(cpu32, cpu64, variants):
-    """Per track: the largest element distance, in tolerance units, of any rounding variant from the FP32
-    oracle, and of the FP32 oracle from the FP64 oracle."""
-    s = track_error(cpu32, cpu64)[0]
+    """[3, n]: per track and group (parameters, covariance, chi2), the largest element distance, in tolerance
+    units, of any rounding variant from the FP32 oracle, and of the FP32 oracle from the FP64 oracle."""
+    s = group_error(cpu32, cpu64)
     for v in variants:
-        s = np.maximum(s, track_error(v, cpu32)[0])
+        s = np.maximum(s, group_error(v, cpu32))
     return s
 
 
+def _ties(chk, n):
+    if chk.get("tie") is None:
+        return np.zeros(n, bool)
+    m = chk["tie"]
+    return (m[0] < TIE_ULPS) | (m[1] < TIE_DOT) | (m[2] < TIE_ULPS)
+
+
 def flag_mask(chk):
-    """Boolean [n]: flagged tracks (rounding-sensitive or with a threshold tie)."""
-    flagged = FLAG_FACTOR * chk["spread"] > 1
-    if chk.get("tie") is not None:
-        m = chk["tie"]
-        flagged |= (m[0] < TIE_ULPS) | (m[1] < TIE_DOT) | (m[2] < TIE_ULPS)
-    return flagged
+    """Boolean [3, n]: per group (parameters, covariance, chi2), the flagged tracks (rounding-sensitive in that
+    group, or with a threshold tie)."""
+    spread = chk["spread"]
+    return (FLAG_FACTOR * spread > 1) | _ties(chk, spread.shape[1])[None, :]
 
 
 def flagged_parity(gpu, cpu32, cpu64, chk):
-    """gpu, cpu32, cpu64: (par [6,n], err [21,n], chi2 [n] or None); chk from fit_reference: the per-track
-    rounding spread, threshold margins and the results with the ties taken the other way.
+    """gpu, cpu32, cpu64: (par [6,n], err [21,n], chi2 [n] or None); chk from fit_reference: the per-track,
+    per-group rounding spread, threshold margins and the results with the ties taken the other way.
 
-    Returns counts (flagged, tie-flagged, unflagged within / beyond tolerance, flagged within / beyond their
-    bound, ties explained by the other side), the largest errors over unflagged and over all tracks, and the
-    tracks beyond their bound with their errors.  `ok` holds the pass verdict of the criteria in the module
-    docstring."""
+    Each group of a track (parameters, covariance, chi2) is flagged on its own.  Returns counts (per group:
+    flagged, unflagged within / beyond tolerance; per track: threshold ties, ties explained by the other side,
+    tracks beyond their bound), the largest errors over unflagged and over all tracks, and the tracks beyond
+    their bound with their errors.  `ok` holds the pass verdict of the criteria in the module docstring."""
     spread = chk["spread"]
     n = int(np.asarray(cpu32[0]).shape[1])
-    t, grp, elem = track_error(gpu, cpu32)
-    tie = np.zeros(n, bool)
-    if chk.get("tie") is not None:
-        m = chk["tie"]
-        tie = (m[0] < TIE_ULPS) | (m[1] < TIE_DOT) | (m[2] < TIE_ULPS)
+    g = group_error(gpu, cpu32)
+    tie = _ties(chk, n)
     rflag = FLAG_FACTOR * spread > 1
-    flagged = rflag | tie
+    flagged = rflag | tie[None, :]
     bound = np.where(rflag, np.maximum(1.0, BOUND_FACTOR * spread), 1.0)
-    near = t <= bound
+    near = (g <= bound).all(axis=0)
     other_side = np.zeros(n, bool)
     for alt in chk.get("alt", []):
-        other_side |= tie & ~near & (track_error(gpu, alt)[0] <= bound)
+        other_side |= tie & ~near & (group_error(gpu, alt) <= bound).all(axis=0)
     beyond = ~near & ~other_side
-    unf_beyond = ~flagged & beyond
-    fl_beyond = flagged & beyond
+    ratio = (g / bound).max(axis=0)
+    worst_grp = (g / bound).argmax(axis=0)
+    t, grp, elem = track_error(gpu, cpu32)
     e = _elem_errors(gpu, cpu32)
     names = ("par", "cov", "chi2")
     mx = lambda k, m: float(e[k].max(axis=0)[m].max()) if m.any() else 0.0
-    lst = [{"track": int(i), "err_over_tol": float(t[i]), "bound": float(bound[i]), "flagged": bool(flagged[i]),
-            "tie": bool(tie[i]), "where": "%s[%d]" % (names[grp[i]], elem[i])} for i in np.nonzero(beyond)[0][:20]]
-    ok = beyond.sum() <= max(2, MAX_BEYOND * n) and bool(np.all(t[beyond] <= 2 * bound[beyond]))
-    return {
+    lst = [{"track": int(i), "over_bound": float(ratio[i]), "group": names[worst_grp[i]],
+            "err_over_tol": float(g[worst_grp[i], i]), "bound": float(bound[worst_grp[i], i]), "tie": bool(tie[i]),
+            "where": "%s[%d]" % (names[grp[i]], elem[i])} for i in np.nonzero(beyond)[0][:20]]
+    ok = beyond.sum() <= max(2, MAX_BEYOND * n) and bool(np.all(ratio[beyond] <= 2))
+    res = {
         "n": n,
-        "n_flagged": int(flagged.sum()),
         "n_tie_flagged": int(tie.sum()),
-        "n_unflagged_within_tol": int(np.sum(~flagged & ~beyond)),
-        "n_unflagged_beyond_tol": int(unf_beyond.sum()),
-        "n_flagged_within_bound": int(np.sum(flagged & near)),
         "n_ties_taken_other_way": int(other_side.sum()),
-        "n_flagged_beyond_bound": int(fl_beyond.sum()),
-        "unflagged_max": {nm: mx(k, ~flagged) for k, nm in enumerate(names)},
+        "n_beyond_bound": int(beyond.sum()),
         "all_max": {nm: mx(k, np.ones(n, bool)) for k, nm in enumerate(names)},
         "frac_tracks_within_tol": float(np.mean(t <= 1.0)),
-        "flag_rule": "flagged: legal FP32 rounding changes (no-FMA build, one-ulp seeds, FP64) move the oracle's "
-                     "result by more than tol/%g, or a threshold decision of the track lies within %g float "
-                     "spacings; unflagged within tol, flagged within max(tol, %g x that spread) of the oracle or, "
-                     "for ties, of the oracle with the ties taken the other way" % (FLAG_FACTOR, TIE_ULPS,
-                                                                                     BOUND_FACTOR),
+        "flag_rule": "per group (parameters, covariance, chi2): flagged when legal FP32 rounding changes (no-FMA "
+                     "build, one-ulp seeds and hits, FP64) move the oracle's result by more than tol/%g, or when a "
+                     "threshold decision of the track lies within %g float spacings; unflagged within tol, flagged "
+                     "within max(tol, %g x that spread) of the oracle or, for ties, of the oracle with the ties "
+                     "taken the other way" % (FLAG_FACTOR, TIE_ULPS, BOUND_FACTOR),
         "beyond": lst,
         "ok": bool(ok),
     }
+    for k, nm in enumerate(names):
+        res[nm] = {"n_flagged": int(flagged[k].sum()),
+                   "n_unflagged_beyond_tol": int(np.sum(~flagged[k] & (g[k] > 1))),
+                   "unflagged_max": mx(k, ~flagged[k])}
+    res["unflagged_max"] = {nm: res[nm]["unflagged_max"] for nm in names}
+    return res
 
 
 def describe(res):
-    return ("%d tracks: %d flagged (%d threshold ties); unflagged: %d within tolerance, %d beyond; flagged: %d "
-            "within their bound, %d ties taken the other way, %d beyond; unflagged max par %.3g cov %.3g chi2 %.3g; "
-            "all-track max par %.3g" % (
-                res["n"], res["n_flagged"], res["n_tie_flagged"], res["n_unflagged_within_tol"],
-                res["n_unflagged_beyond_tol"], res["n_flagged_within_bound"], res["n_ties_taken_other_way"],
-                res["n_flagged_beyond_bound"], res["unflagged_max"]["par"], res["unflagged_max"]["cov"],
-                res["unflagged_max"]["chi2"], res["all_max"]["par"]))
+    return ("%d tracks, flagged par/cov/chi2 %d/%d/%d (%d threshold ties, %d taken the other way); unflagged "
+            "max err par %.3g cov %.3g chi2 %.3g (unflagged beyond tol %d/%d/%d); %d tracks beyond their bound; "
+            "tracks within tol everywhere %.4f; all-track max par %.3g" % (
+                res["n"], res["par"]["n_flagged"], res["cov"]["n_flagged"], res["chi2"]["n_flagged"],
+                res["n_tie_flagged"], res["n_ties_taken_other_way"], res["par"]["unflagged_max"],
+                res["cov"]["unflagged_max"], res["chi2"]["unflagged_max"], res["par"]["n_unflagged_beyond_tol"],
+                res["cov"]["n_unflagged_beyond_tol"], res["chi2"]["n_unflagged_beyond_tol"], res["n_beyond_bound"],
+                res["frac_tracks_within_tol"], res["all_max"]["par"]))
 
 
 def fit_reference(bfield, rl_grid, xi_grid, ev, pflags, fit_flags, n_threads=8, pca_pflags=0):

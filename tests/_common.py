@@ -138,7 +138,7 @@ def assert_track_parity(name, gpu, cpu32, cpu64, chk, max_flagged=None):
     print(line)
     assert r["ok"], (line, r["beyond"])
     if max_flagged is not None:
-        assert r["n_flagged"] <= max_flagged * r["n"], line
+        assert r["par"]["n_flagged"] <= max_flagged * r["n"], line
     return r
 
 

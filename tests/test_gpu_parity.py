@@ -167,9 +167,9 @@ def test_fit_parity(gpu_device, case, batched):
     # Charge flips of rounding-sensitive tracks can differ between FP32
     # roundings; every unflagged track must keep the oracle's charge.
     from oracle.parity import flag_mask
-    unflagged = ~flag_mask(chk)
-    assert (g[2][unflagged] == o[2][unflagged]).all(), "charges differ on unflagged tracks"
-    assert r["n_flagged"] < n
+    unflagged = ~flag_mask(chk)[0]
+    assert (g[2][unflagged] == o[2][unflagged]).all(), "charges differ on tracks with unflagged parameters"
+    assert r["par"]["n_flagged"] < n
 
 
 def test_plain_equals_batched(gpu_device):
