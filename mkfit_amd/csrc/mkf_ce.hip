@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
     const bool valid = finding && c < s_ncand[g];
     bool active = valid && st.last_hit != -2;
     if (active && fabsf(1.f / st.par[3]) < 1.2f) {          // near the apex of a low-pT helix
-      const float dphi = fabsf(atan2f(st.par[1], st.par[0]) - st.par[4]);
+      const float dphi = fabsf(atan2_cr(st.par[1], st.par[0]) - st.par[4]);
       if (st.par[0] * st.par[0] + st.par[1] * st.par[1] > 625.f && dphi > 1.371f && dphi < 4.512f) {
         const int k = append_node(a, seed, &s_nnodes[g], -2, layer, st.last);
         if (k >= 0) st.last = k;

@@ -67,7 +67,7 @@ __global__ void k_fill_i32(int* __restrict__ p, long n, int v) {
 // code, so the phi and q stored with a sorted hit are bit for bit the values
 // its sort key was formed from.
 __device__ __noinline__ float2 hit_phi_q(float x, float y, float z, int is_barrel) {
-  return make_float2(atan2f(y, x), is_barrel ? z : sqrtf(x * x + y * y));
+  return make_float2(atan2_cr(y, x), is_barrel ? z : sqrtf(x * x + y * y));
 }
 
 // Layer owning event hit i: the last one in [lo, hi] whose hit range starts at

@@ -54,6 +54,11 @@ __device__ __forceinline__ int q_bin_checked(const LayerDev& d, float q) {   // 
   const int b = (int)((q - d.qmin) * d.fq);
   return min(max(b, 0), d.nq - 1);
 }
+// atan2 of float coordinates, correctly rounded to float (as the host's atan2f of the reference): the
+// double-precision result rounded once.  Phi decides fine-bin keys and window bins, where the 1-ulp
+// differences of a faster float atan2 change which bin a hit or a window edge falls in.
+__device__ __forceinline__ float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
+
 __device__ __forceinline__ int phi_bin_fine(float phi) { return (int)floorf(kFPhiFine * (phi + kPI)); }
 __device__ __forceinline__ int phi_bin(float phi) { return phi_bin_fine(phi) >> 2; }
 __device__ __forceinline__ float cdist(float a) { return a > kPI ? kTwoPI - a : a; }   // Config.h:441
@@ -99,7 +104,7 @@ __device__ __forceinline__ void hit_window(const LayerDev& d, const float (&par)
   const float dphidx = -y / r2, dphidy = x / r2;
   const float dphi2 = dphidx * dphidx * err[S(0, 0)] + dphidy * dphidy * err[S(1, 1)] +
                       2 * dphidx * dphidy * err[S(0, 1)];
-  const float phi = atan2f(y, x);
+  const float phi = atan2_cr(y, x);
   float dphi = fmaxf(3.f * sqrtf(fabsf(dphi2)), min_dphi);
   float q, dq;
   if (d.is_barrel) {

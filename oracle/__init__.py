@@ -54,6 +54,21 @@ def default_lib_path(contract=True):
     return os.path.join(_HERE, name + ("" if contract else "_nofma") + ".so")
 
 
+class tie_shift:
+    """Context manager: every oracle call inside moves the helix step's 1e-4 cm threshold by `shift` float
+    spacings of r (the result with the threshold ties taken the other way; checker only)."""
+
+    def __init__(self, shift):
+        self.shift = float(shift)
+
+    def __enter__(self):
+        lib().orc_set_tie_shift(ctypes.c_float(self.shift))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_tie_shift(ctypes.c_float(0.0))
+
+
 class use_lib:
     """Context manager: every oracle call inside uses the library at `path` (the mutation tests' builds)."""
 
@@ -213,7 +228,7 @@ class _FindEvent(ctypes.Structure):
 class _FindTracks(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("par", ctypes.c_void_p), ("err", ctypes.c_void_p),
                 ("chg", ctypes.c_void_p), ("chi2", ctypes.c_void_p), ("hot_idx", ctypes.c_void_p),
-                ("n_found", ctypes.c_void_p)]
+                ("n_found", ctypes.c_void_p), ("margins", ctypes.c_void_p)]
 
 
 def layer_array(descs):
@@ -250,8 +265,9 @@ def select_hit_indices(desc, hits, par, err):
 
 
 def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n_threads=1, prop_to_hit=False,
-                  intra_pflags=0):
-    """Best-hit finding over a toy event dict (see mkfit_amd.generate_find_event)."""
+                  intra_pflags=0, margins=False):
+    """Best-hit finding over a toy event dict (see mkfit_amd.generate_find_event).  margins=True adds the
+    checker's decision margins [3][n_layers][n] (window edge cm, bin-edge fraction, relative chi2 gap)."""
     L = lib()
     layers = layer_array(descs)
     n = ev["n_seeds"]
@@ -268,12 +284,16 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
     fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags,
                     int(bool(prop_to_hit)), int(intra_pflags))
-    ft = _FindTracks(n, _p(par), _p(err), _p(chg), _p(chi2), _p(hot), _p(nfound))
+    mg = np.full((3, nl, n), 1e30, np.float32) if margins else None
+    ft = _FindTracks(n, _p(par), _p(err), _p(chg), _p(chi2), _p(hot), _p(nfound), _p(mg))
     ctx = OrcCtx(float(bfield), None, None)
     rc = L.orc_find_best_hit(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(ft), int(n_threads))
     if rc:
         raise RuntimeError("orc_find_best_hit failed (%d)" % rc)
-    return dict(par=par, err=err, chg=chg, chi2=chi2, hot_idx=hot, n_found=nfound)
+    out = dict(par=par, err=err, chg=chg, chi2=chi2, hot_idx=hot, n_found=nfound)
+    if margins:
+        out["margins"] = mg
+    return out
 
 
 class _CEParams(ctypes.Structure):
@@ -284,12 +304,14 @@ class _CEParams(ctypes.Structure):
 class _CETracks(ctypes.Structure):
     _fields_ = [("n_seeds", ctypes.c_int)] + [(k, ctypes.c_void_p) for k in (
         "seed_par", "seed_err", "seed_chg", "seed_hits", "n_cands", "par", "err", "chg", "chi2", "score", "n_found",
-        "hot_idx")]
+        "hot_idx", "margins", "merge_gap")]
 
 
 def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.8, max_cands=6, max_holes=4,
-                      max_consec_holes=1, prop_to_hit=True, intra_pflags=0, n_threads=1):
-    """Clone-engine finding over a toy event dict; per seed up to max_cands candidates, best first."""
+                      max_consec_holes=1, prop_to_hit=True, intra_pflags=0, n_threads=1, margins=False):
+    """Clone-engine finding over a toy event dict; per seed up to max_cands candidates, best first.  margins=True
+    adds the checker's decision margins [4][n_layers][n] (window edge cm, bin-edge fraction, chi2 vs cut,
+    score gap) and merge_gap [n]."""
     L = lib()
     layers = layer_array(descs)
     n = ev["n_seeds"]
@@ -310,8 +332,11 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
     fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags, 0, 0)
     pr = _CEParams(K, int(max_holes), int(max_consec_holes), int(bool(prop_to_hit)), int(intra_pflags))
-    tr = _CETracks(n, _p(sp), _p(se), _p(sc), _p(sh), *[_p(out[k]) for k in (
-        "n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx")])
+    if margins:
+        out["margins"] = np.full((4, nl, n), 1e30, np.float32)
+        out["merge_gap"] = np.full(n, 1e30, np.float32)
+    tr = _CETracks(n, _p(sp), _p(se), _p(sc), _p(sh), *[_p(out.get(k)) for k in (
+        "n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx", "margins", "merge_gap")])
     ctx = OrcCtx(float(bfield), None, None)
     rc = L.orc_find_clone_engine(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr),
                                  ctypes.byref(tr), int(n_threads))

@@ -354,7 +354,7 @@ template <class T, class B>
 int fit_batch(const orc_ctx* c, const B* b, int n_threads, float* tie_out = nullptr, float tie_shift = 0.f) {
   if (!b || b->n_tracks < 0 || b->n_slots < 0) return 1;
   Ctx ctx = make_ctx(c);
-  ctx.tie_shift = tie_shift;
+  if (tie_shift != 0.f) ctx.tie_shift = tie_shift;
   for_each_block(b->n_tracks, n_threads, [&](int blk) {
     const int t0 = blk * NN, cnt = std::min(NN, b->n_tracks - t0);
     fit_block<T>(ctx, *b, t0, cnt, tie_out);
@@ -419,6 +419,12 @@ int orc_fit_ties_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int nt, float
 // the other side of every threshold tie within that distance (Ctx::tie_shift).
 int orc_fit_tie_shift_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int nt, float tie_shift) {
   return fit_batch<float>(c, b, nt, nullptr, tie_shift);
+}
+// Sets the threshold shift of every later call (fits and finders; 0 restores the reference's test).  Not
+// thread-safe: the checker sets it between calls.
+int orc_set_tie_shift(float tie_shift) {
+  g_tie_shift = tie_shift;
+  return 0;
 }
 
 }  // extern "C"

@@ -93,6 +93,7 @@ int orc_fit_f32(const orc_ctx* ctx, const orc_fit_batch_f32* b, int n_threads);
 int orc_fit_f64(const orc_ctx* ctx, const orc_fit_batch_f64* b, int n_threads);
 int orc_fit_ties_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int n_threads, float* ties);
 int orc_fit_tie_shift_f32(const orc_ctx* c, const orc_fit_batch_f32* b, int n_threads, float tie_shift);
+int orc_set_tie_shift(float tie_shift);
 
 /* ---- Fit as a function of per-layer material and alignment (oracle_align.cc) */
 
@@ -147,6 +148,10 @@ typedef struct orc_find_tracks {
   int* hot_idx;   /* [n_layers][n] in: seed hit indices on layers < first_layer; out: hit index per layer,
                      -1/-2/-3 fake-hit codes (index-desc.txt), -9 layer not reached */
   int* n_found;   /* [n] out */
+  float* margins; /* optional out [3][n_layers][n], the checker's decision margins per layer searched: nearest
+                     window edge to a hit of the visited bins (cm), nearest bin edge to a window limit or a
+                     visited hit's phi (fraction of a bin), smallest relative chi2 gap (best vs other, any vs
+                     cut); 1e30 where the layer was not searched */
 } orc_find_tracks;
 
 int orc_loh_geometry(const orc_layer_desc* d, int* nq, float* qmin, float* qmax, float* fq);
@@ -210,6 +215,11 @@ typedef struct orc_ce_tracks {
   float* score;          /* [K][n] */
   int* n_found;          /* [K][n] */
   int* hot_idx;          /* [K][n_layers][n]: hit index per layer, fake codes, -9 none */
+  float* margins;        /* optional out [4][n_layers][n], the checker's decision margins per layer: window
+                            edge (cm), bin edge (bin fraction), chi2 against the cut (relative), smallest
+                            gap of the scores the selection compares relative to their chi2 sums; 1e30
+                            where none */
+  float* merge_gap;      /* [n] with margins: the same for the final merge */
 } orc_ce_tracks;
 
 int orc_find_clone_engine(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, orc_ce_tracks* t,

@@ -227,7 +227,9 @@ struct LaneCounts {
 void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const float* hits, int n_hits,
                            const float* parP, const float* errP, const int* chg, const LaneCounts* lc,
                            const int* trk, const int* xs, const int* xa, const int* wsr, int n_proc,
-                           std::vector<IdxChi2>& entries) {
+                           std::vector<IdxChi2>& entries, float* c2m = nullptr) {
+  if (c2m)
+    for (int i = 0; i < NN; ++i) c2m[i] = 1e30f;
   const orc_find_event& ev = *E.ev;
   const orc_ce_params& P = *E.p;
   const bool barrel = d.is_barrel != 0;
@@ -252,6 +254,7 @@ void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const floa
       const float max_c2 = dyn_chi2_cut(d, ev.chi2_cut, parP[3 * NN + i], parP[5 * NN + i]);
       if (!ok[i]) continue;
       const float c2 = std::abs(chi2[i]);
+      if (c2m) c2m[i] = std::min(c2m[i], std::abs(c2 - max_c2) / std::max(max_c2, 1.f));
       if (!(c2 < max_c2)) continue;
       if (!strip_q_compatible(barrel, errP, propPar, msErr, msPar, i)) continue;
       IdxChi2 e;
@@ -289,8 +292,17 @@ void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const floa
   }
 }
 
-// One seed through every layer of the plan.
-void find_seed(const CEEvent& E, Comb& cc, int layer0) {
+// Gap of two scores relative to the larger of their chi2 sums (for the checker's margins): scores differ
+// between FP32 implementations by their chi2 sums' rounding.
+inline float score_gap(float a, float ca, float b, float cb) {
+  return std::abs(a - b) / std::max(1.f, std::max(std::abs(ca), std::abs(cb)));
+}
+
+// One seed through every layer of the plan.  With `mg` (checker only, [4][n_layers], preset to 1e30), also the
+// seed's decision margins per layer: window edge (cm), bin edge (bin fraction), chi2 against the cut
+// (relative), and the smallest relative gap between scores that the candidate selection compares; `mgap`
+// gets that of the final merge.
+void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, float* mgap = nullptr) {
   const orc_find_event& ev = *E.ev;
   const orc_ce_params& P = *E.p;
   const int max_cands = P.max_cands;
@@ -334,7 +346,8 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0) {
       else
         propagate_helix_to_z(B.err, B.par, B.chg, rad, B.errP, B.parP, n_proc, ev.pflags, nullptr, E.ctx);
       int xs[NN], xa[kHitIdxMax * NN], wsr[NN];
-      select_hit_indices(d, E.loh[layer], B.parP, B.errP, n_proc, xs, xa, wsr);
+      SelMargins sm[NN];
+      select_hit_indices(d, E.loh[layer], B.parP, B.errP, n_proc, xs, xa, wsr, mg ? sm : nullptr);
       // find_tracks_handle_missed_layers
       for (int i = 0; i < n_proc; ++i) {
         const Cand& c = cc.cands[idx[i]];
@@ -356,8 +369,18 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0) {
         lc[i] = {c.chi2, c.nfound, c.noverlap, c.ninside, c.ntail};
         trk[i] = idx[i];
       }
+      float c2m[NN];
       find_candidates_block(E, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B.parP, B.errP, B.chg, lc,
-                            trk, xs, xa, wsr, n_proc, entries);
+                            trk, xs, xa, wsr, n_proc, entries, mg ? c2m : nullptr);
+      if (mg) {
+        const int L = ev.n_layers;
+        for (int i = 0; i < n_proc; ++i) {
+          if (wsr[i] == WSR_Outside) continue;
+          mg[0 * L + layer] = std::min(mg[0 * L + layer], sm[i].win);
+          mg[1 * L + layer] = std::min(mg[1 * L + layer], sm[i].bin);
+          mg[2 * L + layer] = std::min(mg[2 * L + layer], c2m[i]);
+        }
+      }
     }
     // CandCloner::ProcessSeedRange
     std::vector<Cand> cv;
@@ -366,6 +389,16 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0) {
       std::stable_sort(entries.begin(), entries.end(),
                        [](const IdxChi2& a, const IdxChi2& b) { return a.score > b.score; });
       const int num_hits = std::min((int)entries.size(), max_cands);
+      if (mg) {   // gaps the selection compares: neighbours up to the cut at K, extras and the best short one
+        float& g = mg[3 * ev.n_layers + layer];
+        for (int i = 0; i + 1 < (int)entries.size() && i < num_hits; ++i)
+          g = std::min(g, score_gap(entries[i].score, entries[i].chi2, entries[i + 1].score, entries[i + 1].chi2));
+        for (int i = 0; i < num_hits; ++i) {
+          for (const Cand& x : extras) g = std::min(g, score_gap(x.score, x.chi2, entries[i].score, entries[i].chi2));
+          if (entries[i].hitIdx == -2 && cc.best_short.valid)
+            g = std::min(g, score_gap(entries[i].score, entries[i].chi2, cc.best_short.score, cc.best_short.chi2));
+        }
+      }
       size_t xi = 0;
       int n_pushed = 0;
       for (int ih = 0; ih < num_hits; ++ih) {
@@ -423,6 +456,14 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0) {
     for (auto& c : cc.cands) c.score = score_cand(c);
     if (best_short) best_short->score = score_cand(*best_short);
     std::stable_sort(cc.cands.begin(), cc.cands.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
+    if (mgap) {
+      for (size_t i = 0; i + 1 < cc.cands.size(); ++i)
+        *mgap = std::min(*mgap, score_gap(cc.cands[i].score, cc.cands[i].chi2, cc.cands[i + 1].score,
+                                          cc.cands[i + 1].chi2));
+      if (best_short)
+        for (const Cand& c : cc.cands)
+          *mgap = std::min(*mgap, score_gap(c.score, c.chi2, best_short->score, best_short->chi2));
+    }
     if (best_short && best_short->score > cc.cands.back().score) {
       size_t ci = 0;
       while (cc.cands[ci].score > best_short->score) ++ci;
@@ -469,7 +510,14 @@ extern "C" int orc_find_clone_engine(const orc_ctx* c, const orc_find_event* ev,
       cc.cands.push_back(seed);                         // CombCandidate::ImportSeed
       cc.pickup_layer = ev->first_layer - 1;
       cc.best_short.score = kScoreWorst;
-      find_seed(E, cc, ev->first_layer);
+      std::vector<float> mg(tr->margins ? 4 * (size_t)L : 0, 1e30f);
+      float mgap = 1e30f;
+      find_seed(E, cc, ev->first_layer, tr->margins ? mg.data() : nullptr, tr->margins ? &mgap : nullptr);
+      if (tr->margins) {
+        for (int k = 0; k < 4; ++k)
+          for (int l = 0; l < L; ++l) tr->margins[((size_t)k * L + l) * n + s] = mg[(size_t)k * L + l];
+        tr->merge_gap[s] = mgap;
+      }
       const int nc = std::min((int)cc.cands.size(), K);
       tr->n_cands[s] = nc;
       for (int ic = 0; ic < K; ++ic) {

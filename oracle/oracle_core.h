@@ -56,6 +56,9 @@ constexpr int   kNBinsRME = 120;
 enum { PF_use_param_b_field = 0x1, PF_apply_material = 0x2 };   // Config.h:17-23
 enum { KFO_Calculate_Chi2 = 1, KFO_Update_Params = 2 };          // KalmanUtilsMPlex.h:11-15
 
+// Default of Ctx::tie_shift for every context created afterwards (orc_set_tie_shift; checker only).
+inline float g_tie_shift = 0.f;
+
 struct Ctx {
   float bfield;            // uniform field in T (Config::Bfield)
   const float* rl_grid;    // [kNBinsZME][kNBinsRME] radiation lengths (Config::RlgridME)
@@ -72,7 +75,7 @@ struct Ctx {
   float* tie_mat = nullptr;
   // Moves the helix step's 1e-4 cm threshold by this many float spacings of r
   // (0 = the reference's test): the checker's "other side of the tie" result.
-  float tie_shift = 0.f;
+  float tie_shift = g_tie_shift;
 };
 
 // Symmetric packed index, MatriplexSym.h:21.

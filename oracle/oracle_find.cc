@@ -74,9 +74,13 @@ struct EventLayers {
   Ctx ctx;
 };
 
-// AddBestHit (MkFinder.cc:691-832) for one block, propagated state in parP/errP.
+// AddBestHit (MkFinder.cc:691-832) for one block, propagated state in parP/errP.  With `c2m` (checker only),
+// also each lane's chi2 decision margin: the smallest relative gap between the best hit's chi2 and another
+// window hit's, or between any window hit's chi2 and the cut.
 void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock& B, const float* parP,
-                  const float* errP, const int* xsize, const int* xarr, const int* wsr, int n_proc) {
+                  const float* errP, const int* xsize, const int* xarr, const int* wsr, int n_proc,
+                  float* c2m = nullptr) {
+  float c2all[kHitIdxMax][NN];
   // With prop_to_hit (Config::finding_requires_propagation_to_hit_pos) the chi2 and the update are
   // taken after propagating the layer state to each hit's radius (barrel) or z (endcap) with the
   // intra-layer flags: kalmanPropagateAndComputeChi2 / kalmanPropagateAndUpdate
@@ -132,12 +136,26 @@ void add_best_hit(const EventLayers& E, int layer, float chi2_cut_min, FindBlock
     for (int it = 0; it < n_proc; ++it)
       if (valid(hc, it)) {
         const float c2 = std::abs(outChi2[it]);
+        c2all[hc][it] = c2;
         if (c2 < minChi2[it]) {
           minChi2[it] = c2;
           bestHit[it] = xarr[hc * NN + it];
         }
       }
   }
+  if (c2m)
+    for (int it = 0; it < n_proc; ++it) {
+      const float cut = dyn_chi2_cut(d, chi2_cut_min, parP[3 * NN + it], parP[5 * NN + it]);
+      float m = 1e30f;
+      for (int hc = 0; hc < std::min(xsize[it], kHitIdxMax); ++hc) {
+        if (!valid(hc, it)) continue;
+        const float c2 = c2all[hc][it];
+        m = std::min(m, std::abs(c2 - cut) / std::max(cut, 1.f));
+        if (bestHit[it] >= 0 && xarr[hc * NN + it] != bestHit[it])
+          m = std::min(m, std::abs(c2 - minChi2[it]) / std::max(minChi2[it], 1.f));
+      }
+      c2m[it] = m;
+    }
   for (int it = 0; it < n_proc; ++it) {
     auto fake = [&]() {
       const float f[6] = {666, 0, 666, 0, 0, 666};
@@ -294,7 +312,8 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
           propagate_helix_to_r(B.err, B.par, B.chg, rad, errP, parP, cnt, ev->pflags, nullptr, fail, ctx);
         else
           propagate_helix_to_z(B.err, B.par, B.chg, rad, errP, parP, cnt, ev->pflags, nullptr, ctx);
-        select_hit_indices(d, E.loh[layer], parP, errP, cnt, xs, xa, wsr);
+        SelMargins mg[NN];
+        select_hit_indices(d, E.loh[layer], parP, errP, cnt, xs, xa, wsr, tr->margins ? mg : nullptr);
         if (d.is_barrel) {                                        // MkBuilder.cc:1431-1455
           const float r_min_sqr = d.rin * d.rin;
           for (int i = 0; i < cnt; ++i) {
@@ -316,7 +335,16 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
             }
           }
         }
-        add_best_hit(E, layer, ev->chi2_cut, B, parP, errP, xs, xa, wsr, cnt);
+        float c2m[NN];
+        add_best_hit(E, layer, ev->chi2_cut, B, parP, errP, xs, xa, wsr, cnt, tr->margins ? c2m : nullptr);
+        if (tr->margins)
+          for (int i = 0; i < cnt; ++i) {
+            const size_t L = ev->n_layers, t = (size_t)t0 + i;
+            const bool live = wsr[i] != WSR_Outside;
+            tr->margins[(0 * L + layer) * n + t] = live ? mg[i].win : 1e30f;
+            tr->margins[(1 * L + layer) * n + t] = live ? mg[i].bin : 1e30f;
+            tr->margins[(2 * L + layer) * n + t] = live ? c2m[i] : 1e30f;
+          }
         for (int i = 0; i < cnt; ++i)                             // MkBuilder.cc:1465-1473
           if (!B.stopped[i] && B.nhits[i] > 0 && B.hot_idx[std::min(B.nhits[i], kMaxTrkHits) - 1][i] == -2) {
             B.stopped[i] = 1;

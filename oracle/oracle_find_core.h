@@ -140,9 +140,19 @@ inline float dyn_chi2_cut(const orc_layer_desc& d, float chi2_cut_min, float inv
   return chi2_cut_min;
 }
 
-// SelectHitIndices for NN lanes of propagated state (MkFinder.cc:227-684).
+// Decision margins of one lane's SelectHitIndices (for the checker, not the reference): how close any hit of
+// the visited bins came to a window edge (cm; the phi edge as arc length at the layer), and how close a window
+// limit or a visited hit's phi came to a bin edge it could cross (fraction of a q or coarse phi bin; the fine
+// phi edges count when the list reached its 16-hit cap, since they order the hits within a bin).
+struct SelMargins {
+  float win = 1e30f, bin = 1e30f;
+};
+inline float bin_edge_margin(float x) { return std::abs(x - std::round(x)); }
+
+// SelectHitIndices for NN lanes of propagated state (MkFinder.cc:227-684).  With `mg`, also the lanes'
+// decision margins.
 inline void select_hit_indices(const orc_layer_desc& d, const LoH& L, const float* par, const float* err, int n_proc,
-                        int* xsize, int* xarr /*[16][NN]*/, int* wsr) {
+                        int* xsize, int* xarr /*[16][NN]*/, int* wsr, SelMargins* mg = nullptr) {
   float dqv[NN], dphiv[NN], qv[NN], phiv[NN];
   int qb1v[NN], qb2v[NN], pb1v[NN], pb2v[NN];
   const float nSigmaPhi = 3, nSigmaZ = 3, nSigmaR = 3;
@@ -192,6 +202,31 @@ inline void select_hit_indices(const orc_layer_desc& d, const LoH& L, const floa
       continue;
     }
     const float q = qv[it], phi = phiv[it], dphi = dphiv[it], dq = dqv[it];
+    if (mg) {
+      const float r = std::sqrt(par[0 * NN + it] * par[0 * NN + it] + par[1 * NN + it] * par[1 * NN + it]);
+      SelMargins& m = mg[it];
+      m.bin = std::min({bin_edge_margin((q - dq - L.b.qmin) * L.b.fq), bin_edge_margin((q + dq - L.b.qmin) * L.b.fq),
+                        bin_edge_margin(kFPhiFine * (phi - dphi + kPI) / 4),
+                        bin_edge_margin(kFPhiFine * (phi + dphi + kPI) / 4)});
+      for (int qi = qb1v[it]; qi < qb2v[it]; ++qi)
+        for (int pi = pb1v[it]; pi < pb2v[it]; ++pi) {
+          const size_t bi = (size_t)qi * kNPhi + (pi & kPhiMask);
+          for (int hi = L.first[bi]; hi < L.end[bi]; ++hi) {
+            const float ddq = std::abs(q - L.qs[hi]);
+            const float ddphi = cdist(std::abs(phi - L.phis[hi]));
+            m.win = std::min({m.win, std::abs(ddq - dq), std::abs(ddphi - dphi) * r});
+            m.bin = std::min(m.bin, bin_edge_margin(kFPhiFine * (L.phis[hi] + kPI) / 4));   // coarse edge
+          }
+          // A hit at phi = +pi has fine bin 1024: it sorts after the row's last bin but its bin index wraps to
+          // 0, so it rewrites bin 0's range (SuckInHits, HitStructures.cc:146-171).  A hit within rounding of
+          // +pi in the row's last bin therefore decides what bin 0 holds.
+          if ((pi & kPhiMask) == 0) {
+            const size_t bl = (size_t)qi * kNPhi + kPhiMask;
+            for (int hi = L.first[bl]; hi < L.end[bl]; ++hi)
+              m.bin = std::min(m.bin, std::abs(kFPhiFine * (L.phis[hi] + kPI) - 1024.f) / 4);
+          }
+        }
+    }
     for (int qi = qb1v[it]; qi < qb2v[it]; ++qi)
       for (int pi = pb1v[it]; pi < pb2v[it]; ++pi) {
         const int pb = pi & kPhiMask;
@@ -205,6 +240,15 @@ inline void select_hit_indices(const orc_layer_desc& d, const LoH& L, const floa
           xarr[xsize[it]++ * NN + it] = L.ranks[hi];
         }
       }
+    if (mg && xsize[it] >= kHitIdxMax) {
+      // a capped list also depends on the order within a bin (the fine phi bin): any fine edge matters
+      for (int qi = qb1v[it]; qi < qb2v[it]; ++qi)
+        for (int pi = pb1v[it]; pi < pb2v[it]; ++pi) {
+          const size_t bi = (size_t)qi * kNPhi + (pi & kPhiMask);
+          for (int hi = L.first[bi]; hi < L.end[bi]; ++hi)
+            mg[it].bin = std::min(mg[it].bin, bin_edge_margin(kFPhiFine * (L.phis[hi] + kPI)) / 4);
+        }
+    }
   }
 }
 

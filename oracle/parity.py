@@ -34,7 +34,8 @@ its threshold (the |r - r0| test, the helix step's failure test, material
 bin edges).  A track with a decision within TIE_ULPS spacings (or
 |oodotp - 5| < TIE_DOT) is *tie-flagged*, and its result may equal, within
 the bound, either the oracle's or the oracle's with the |r - r0| threshold
-moved by TIE_ULPS spacings either way (every such tie taken the other way).
+moved by 1, 2 or 4 spacings either way (TIE_SHIFTS: every tie within that
+distance taken the other way).
 
 Parameters, covariance and chi2 are flagged separately (each group of a
 track has its own spread), since chi2 is far more sensitive to rounding.
@@ -71,6 +72,9 @@ MAX_BEYOND = 2e-4
 N_PERTURBED = 23
 TIE_ULPS = 4.0
 TIE_DOT = 1e-4
+# threshold shifts (float spacings) of the results with ties taken the other way: one shift flips every tie
+# within its distance in one direction, so nearer ties are also flipped alone
+TIE_SHIFTS = (1.0, -1.0, 2.0, -2.0, 4.0, -4.0)
 
 
 def par_rel(a, b):
@@ -141,12 +145,17 @@ def _ulp_moved(a, rng):
 
 
 def perturbed_seeds(ev, seed):
-    """A copy of a toy event with every seed parameter and every hit coordinate moved by one ulp, in a random
-    direction (the hits' rounding is what moves chi2: a residual of ~1 m coordinates against ~100 um errors)."""
+    """A copy of a toy event (fit or finding) with every seed parameter and every hit coordinate moved by one
+    ulp, in a random direction (the hits' rounding is what moves chi2: a residual of ~1 m coordinates against
+    ~100 um errors)."""
     rng = np.random.default_rng(seed)
     out = dict(ev, seed_par=_ulp_moved(ev["seed_par"], rng))
     if "hit_pos" in ev:
         out["hit_pos"] = _ulp_moved(ev["hit_pos"], rng)
+    if "hits_flat" in ev and len(ev["hits_flat"]):          # a finding event: the hit positions
+        h = np.array(ev["hits_flat"], np.float32)
+        h[:, :3] = _ulp_moved(h[:, :3], rng)
+        out["hits_flat"] = h
     return out
 
 
@@ -199,9 +208,12 @@ def flagged_parity(gpu, cpu32, cpu64, chk):
     flagged = rflag | tie[None, :]
     bound = np.where(rflag, np.maximum(1.0, BOUND_FACTOR * spread), 1.0)
     near = (g <= bound).all(axis=0)
+    # without recorded margins ("tie_all"), any track may match a result with its ties taken the other way:
+    # such a result equals the oracle's wherever no threshold decision was close
+    may_flip = np.ones(n, bool) if chk.get("tie_all") else tie
     other_side = np.zeros(n, bool)
     for alt in chk.get("alt", []):
-        other_side |= tie & ~near & (group_error(gpu, alt) <= bound).all(axis=0)
+        other_side |= may_flip & ~near & (group_error(gpu, alt) <= bound).all(axis=0)
     beyond = ~near & ~other_side
     ratio = (g / bound).max(axis=0)
     worst_grp = (g / bound).argmax(axis=0)
@@ -260,7 +272,7 @@ def fit_reference(bfield, rl_grid, xi_grid, ev, pflags, fit_flags, n_threads=8, 
     var = rounding_variants(lambda e: o32.fit(e, **kw), lambda e: onf.fit(e, **kw), ev)
     spread = track_spread((c32[0], c32[1], c32[3]), (c64[0], c64[1], c64[3]), var)
     alt = []
-    for shift in (TIE_ULPS, -TIE_ULPS):
+    for shift in TIE_SHIFTS:
         a = o32.fit(ev, tie_shift=shift, **kw)
         alt.append((a[0], a[1], a[3]))
     return c32, c64, {"spread": spread, "tie": tie, "alt": alt}

@@ -90,3 +90,215 @@ def pad_seed_hits(ev, first_layer):
         return ev
     pad = np.full((first_layer - sh.shape[0], sh.shape[1]), -1, np.int32)
     return dict(ev, seed_hits=np.ascontiguousarray(np.vstack([sh, pad]), np.int32))
+
+
+# Decision-margin thresholds for explaining a GPU / oracle difference in finding.  The GPU's propagated
+# states differ from the oracle's by FP32 rounding (positions ~1e-6 relative, chi2 ~1e-5): a decision whose
+# margin lies below these can go either way (chi2: the states differ after several updates, so chi2 values
+# of one hit move by up to ~2e-3 of themselves between FP32 implementations).
+WIN_EPS = 1e-4      # cm: a hit this close to a window edge
+BIN_EPS = 1e-4      # bin fraction: a window limit or a hit's phi this close to a bin edge
+CHI2_EPS = 5e-3     # relative: two chi2 values, or a chi2 and the cut, this close
+SCORE_EPS = 5e-4    # two candidate scores this close, relative to the larger of their chi2 sums
+
+
+def _classes(m, l, t, ce):
+    out = []
+    if m[0, l, t] < WIN_EPS:
+        out.append("window")
+    if m[1, l, t] < BIN_EPS:
+        out.append("bin")
+    if m[2, l, t] < CHI2_EPS:
+        out.append("chi2")
+    if ce and m[3, l, t] < SCORE_EPS:
+        out.append("score")
+    return out
+
+
+def explain_best_hit(gpu_hot, cpu_hot, margins, first_layer):
+    """Per seed whose hit list differs ([n_layers][n] each), the first differing layer and the decision there
+    that the oracle took within a margin of rounding (window edge, bin edge, chi2 tie or cut).  Returns
+    (n_differing, {class: count}, unexplained [(seed, layer)])."""
+    diff = np.nonzero(np.any(gpu_hot != cpu_hot, axis=0))[0]
+    counts, bad = {}, []
+    for t in diff:
+        l = int(np.nonzero(gpu_hot[:, t] != cpu_hot[:, t])[0][0])
+        cls = _classes(margins, l, t, False) if l >= first_layer else []
+        if not cls:
+            bad.append((int(t), l))
+        for c in cls[:1]:
+            counts[c] = counts.get(c, 0) + 1
+    return len(diff), counts, bad
+
+
+def explain_clone_engine(gpu, cpu, first_layer):
+    """As explain_best_hit for the clone engine: gpu / cpu dicts with n_cands and hot_idx [K][L][n], cpu with the
+    oracle's margins [4][L][n] and merge_gap [n].  A seed differs when its candidate lists (in order) differ.
+    Same set in another order: explained by a tie of the final merge.  Otherwise the first layer where the
+    candidate sets differ (a hit-list prefix of one side missing on the other) must hold a decision within
+    rounding (window, bin, chi2, score)."""
+    n = cpu["n_cands"].shape[0]
+    m = cpu["margins"]
+    counts, bad, n_diff = {}, [], 0
+    for t in range(n):
+        ng, nc = int(gpu["n_cands"][t]), int(cpu["n_cands"][t])
+        rg = [tuple(gpu["hot_idx"][i, :, t]) for i in range(ng)]
+        rc = [tuple(cpu["hot_idx"][i, :, t]) for i in range(nc)]
+        if rg == rc:
+            continue
+        n_diff += 1
+        if sorted(rg) == sorted(rc):
+            # the order comes from the final merge's sort, whose equal scores keep the order of the last
+            # layer's selection: a tie in either
+            if cpu["merge_gap"][t] < SCORE_EPS or np.any(m[3, :, t] < SCORE_EPS):
+                counts["merge"] = counts.get("merge", 0) + 1
+            else:
+                bad.append((t, -1))
+            continue
+        L = cpu["hot_idx"].shape[1]
+        first = L
+        for l in range(first_layer, L + 1):
+            if {r[:l] for r in rg} != {r[:l] for r in rc}:
+                first = l - 1
+                break
+        # the candidate lists are kept K at a time, so a path that split off at `first` can be dropped by
+        # the selection of any later layer: the decision within rounding may lie at `first` or after it
+        cls = []
+        for l in range(max(first, first_layer), L):
+            cls = _classes(m, l, t, True)
+            if cls:
+                break
+        if not cls:
+            bad.append((t, first))
+        for c in cls[:1]:
+            counts[c] = counts.get(c, 0) + 1
+    return n_diff, counts, bad
+
+
+def _state_spread(cpu, variants):
+    """[3, n] per seed and group (parameters, covariance, chi2): the largest distance, in tolerance units, of a
+    rounding variant's final state from the oracle's, over the variants that found the same hit list."""
+    from oracle.parity import group_error
+    s = np.zeros((3, cpu["hot_idx"].shape[-1]))
+    for v in variants:
+        same = np.all(v["hot_idx"] == cpu["hot_idx"], axis=0)
+        g = group_error((v["par"], v["err"], v["chi2"]), (cpu["par"], cpu["err"], cpu["chi2"]))
+        s = np.maximum(s, np.where(same[None, :], g, 0.0))
+    return s
+
+
+def _variants(run, ev, n_variants, seed0):
+    """The finder under legal changes of rounding: the no-FMA build, then one-ulp seed perturbations
+    alternating the builds."""
+    from oracle.parity import perturbed_seeds
+    out = []
+    for i in range(n_variants):
+        e = perturbed_seeds(ev, seed0 + i) if i else ev
+        if i % 2 == 0:
+            with oracle.use_lib(oracle.default_lib_path(contract=False)):
+                out.append(run(e))
+        else:
+            out.append(run(e))
+    return out
+
+
+def _alternatives(run, ev):
+    """The finder with the helix step's threshold ties taken the other way (oracle/parity.py): the threshold
+    moved by 1, 2, 4 and 16 float spacings either way (a finder's states pass more helix steps than a fit's,
+    and no per-track margins are recorded for them)."""
+    from oracle.parity import TIE_SHIFTS
+    out = []
+    for s in TIE_SHIFTS + (16.0, -16.0):
+        with oracle.tie_shift(s):
+            out.append(run(ev))
+    return out
+
+
+def best_hit_reference(ev, descs, first_layer, n_variants=16, **kw):
+    """The oracle's best-hit result with its decision margins, and the checker data of its final states: the
+    per-seed rounding spread and the results with threshold ties taken the other way (alt)."""
+    cpu = oracle.find_best_hit(ev, descs, first_layer, margins=True, **kw)
+    run = lambda e: oracle.find_best_hit(e, descs, first_layer, **kw)
+    spread = _state_spread(cpu, _variants(run, ev, n_variants, 500))
+    alt = []
+    for a in _alternatives(run, ev):
+        same = np.all(a["hot_idx"] == cpu["hot_idx"], axis=0)
+        alt.append(tuple(np.where(same, a[k], cpu[k]) for k in ("par", "err", "chi2")))
+    return cpu, {"spread": spread, "alt": alt}
+
+
+def assert_find_parity(name, gpu, cpu, spread, first_layer):
+    """Best-hit finding: every seed's hit list equals the oracle's, or its first difference is a decision the
+    oracle took within rounding (explain_best_hit); seeds with equal lists have final states within tolerance
+    of the oracle's, or within four times their rounding spread where it reaches a third of it (at most
+    0.02 % of the seeds beyond, by at most 2x: oracle/parity.py).  Returns (same mask, summary line)."""
+    from oracle.parity import describe, flagged_parity
+    n_diff, counts, bad = explain_best_hit(gpu["hot_idx"], cpu["hot_idx"], cpu["margins"], first_layer)
+    same = np.all(gpu["hot_idx"] == cpu["hot_idx"], axis=0)
+    sel = lambda x: x[..., same]
+    chk = {"spread": spread["spread"][:, same], "tie": None, "alt": [tuple(sel(x) for x in a) for a in spread["alt"]],
+           "tie_all": True}
+    r = flagged_parity((gpu["par"][:, same], gpu["err"][:, same], gpu["chi2"][same]),
+                       (cpu["par"][:, same], cpu["err"][:, same], cpu["chi2"][same]),
+                       (cpu["par"][:, same], cpu["err"][:, same], cpu["chi2"][same]), chk)
+    line = "%s: %d seeds, %d differ (%s), unexplained %s; equal lists: %s" % (
+        name, len(same), n_diff, counts, bad[:10], describe(r))
+    print(line)
+    assert not bad, line
+    assert r["ok"], (line, r["beyond"])
+    assert np.array_equal(gpu["n_found"][same], cpu["n_found"][same])
+    assert np.array_equal(gpu["chg"][same], cpu["chg"][same])
+    return same, line
+
+
+def clone_engine_reference(ev, descs, first_layer, n_variants=16, **kw):
+    """The oracle's clone-engine result with its decision margins, and the checker data of its final candidate
+    states: per candidate slot and seed the rounding spread [K][3][n] (variants as in best_hit_reference; a
+    variant counts for a seed when its candidate lists equal the oracle's) and the results with threshold
+    ties taken the other way (alt)."""
+    from oracle.parity import group_error
+    cpu = oracle.find_clone_engine(ev, descs, first_layer, margins=True, **kw)
+    K, n = cpu["par"].shape[0], cpu["n_cands"].shape[0]
+    run = lambda e: oracle.find_clone_engine(e, descs, first_layer, **kw)
+    same_as = lambda v: (v["n_cands"] == cpu["n_cands"]) & np.all(v["hot_idx"] == cpu["hot_idx"], axis=(0, 1))
+    spread = np.zeros((K, 3, n))
+    for v in _variants(run, ev, n_variants, 700):
+        same = same_as(v)
+        for k in range(K):
+            g = group_error((v["par"][k], v["err"][k], v["chi2"][k]), (cpu["par"][k], cpu["err"][k], cpu["chi2"][k]))
+            spread[k] = np.maximum(spread[k], np.where((same & (cpu["n_cands"] > k))[None, :], g, 0.0))
+    alt = []
+    for a in _alternatives(run, ev):
+        # per candidate: the alternative's candidate in the same slot with the same hit list
+        row = np.all(a["hot_idx"] == cpu["hot_idx"], axis=1) & (a["n_cands"] > np.arange(K)[:, None])
+        alt.append({"par": np.where(row[:, None, :], a["par"], cpu["par"]),
+                    "err": np.where(row[:, None, :], a["err"], cpu["err"]),
+                    "chi2": np.where(row, a["chi2"], cpu["chi2"])})
+    return cpu, {"spread": spread, "alt": alt}
+
+
+def assert_ce_parity(name, gpu, cpu, spread, first_layer):
+    """Clone-engine finding: every seed's candidate lists equal the oracle's, or their difference is explained
+    by a decision within rounding (explain_clone_engine); the candidates of seeds with equal lists have final
+    states within tolerance, or within four times their rounding spread where it reaches a third of it (at
+    most 0.02 % beyond, by at most 2x: oracle/parity.py).  Returns (same mask, summary line)."""
+    from oracle.parity import describe, flagged_parity
+    n_diff, counts, bad = explain_clone_engine(gpu, cpu, first_layer)
+    same = (gpu["n_cands"] == cpu["n_cands"]) & np.all(gpu["hot_idx"] == cpu["hot_idx"], axis=(0, 1))
+    K = cpu["par"].shape[0]
+    cols = [(k, np.nonzero(same & (cpu["n_cands"] > k))[0]) for k in range(K)]
+    cat = lambda d, key: np.concatenate([d[key][k][..., idx] for k, idx in cols], axis=-1)
+    g = (cat(gpu, "par"), cat(gpu, "err"), cat(gpu, "chi2"))
+    c = (cat(cpu, "par"), cat(cpu, "err"), cat(cpu, "chi2"))
+    sp = np.concatenate([spread["spread"][k][:, idx] for k, idx in cols], axis=-1)
+    alt = [(cat(a, "par"), cat(a, "err"), cat(a, "chi2")) for a in spread["alt"]]
+    r = flagged_parity(g, c, c, {"spread": sp, "tie": None, "alt": alt, "tie_all": True})
+    line = "%s: %d seeds, %d differ (%s), unexplained %s; candidates of equal seeds: %s" % (
+        name, len(same), n_diff, counts, bad[:10], describe(r))
+    print(line)
+    assert not bad, line
+    assert r["ok"], (line, r["beyond"])
+    for k, idx in cols:
+        assert np.array_equal(gpu["n_found"][k, idx], cpu["n_found"][k, idx])
+        assert np.array_equal(gpu["chg"][k, idx], cpu["chg"][k, idx])
+    return same, line

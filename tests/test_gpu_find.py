@@ -11,7 +11,8 @@ import pytest
 import mkfit_amd as mk
 import oracle
 from _common import RawDeviceBuffer, par_rel
-from _find_common import F_PHI_FINE, _brute_select, _keys, _random_hits, edge_explained, pad_seed_hits
+from _find_common import (F_PHI_FINE, _brute_select, _keys, _random_hits, assert_find_parity, best_hit_reference,
+                          edge_explained, pad_seed_hits)
 
 pytestmark = pytest.mark.gpu
 
@@ -148,17 +149,11 @@ def _run_find(ctx, eoh, ev, n_layers, first_layer, dev, chi2_cut=15.0):
     return b, b.numpy()
 
 
-def _compare_find(gpu, cpu, min_same=0.99):
-    same = np.all(gpu["hot_idx"] == cpu["hot_idx"], axis=0)
-    assert same.mean() >= min_same, (same.mean(), np.nonzero(~same)[0][:8])
-    assert np.array_equal(gpu["n_found"][same], cpu["n_found"][same])
-    assert np.array_equal(gpu["chg"][same], cpu["chg"][same])
-    if same.any():
-        pr = par_rel(gpu["par"][:, same], cpu["par"][:, same])
-        assert np.percentile(pr, 99) < 1e-4 and pr.max() < 1e-2, (np.percentile(pr, 99), pr.max())
-        c2 = np.abs(gpu["chi2"][same] - cpu["chi2"][same]) / np.maximum(1.0, np.abs(cpu["chi2"][same]))
-        assert np.percentile(c2, 99) < 1e-3, np.percentile(c2, 99)
-    return same
+def _compare_find(gpu, ev, descs, first_layer=3, **kw):
+    """The GPU's best-hit result against the oracle's, seed by seed (assert_find_parity); returns the oracle's."""
+    cpu, spread = best_hit_reference(ev, descs, first_layer, n_threads=8, bfield=3.8, **kw)
+    assert_find_parity("best hit", gpu, cpu, spread, first_layer)
+    return cpu
 
 
 def _true_fraction(res, ev, first_layer):
@@ -172,8 +167,7 @@ def _true_fraction(res, ev, first_layer):
 def test_find_best_hit_matches_oracle(setup, gpu_device):
     ev, descs, ctx, eoh = setup
     _, gpu = _run_find(ctx, eoh, ev, len(descs), 3, gpu_device)
-    cpu = oracle.find_best_hit(ev, descs, first_layer=3, chi2_cut=15.0, bfield=3.8, n_threads=8)
-    _compare_find(gpu, cpu)
+    cpu = _compare_find(gpu, ev, descs, chi2_cut=15.0)
     assert abs(_true_fraction(gpu, ev, 3) - _true_fraction(cpu, ev, 3)) < 2e-3
     assert _true_fraction(gpu, ev, 3) > 0.8
 
@@ -184,8 +178,7 @@ def test_find_best_hit_propagate_to_hit(setup, gpu_device):
     b = mk.FindBatch(ev, len(descs), 3, device=gpu_device, chi2_cut=15.0, prop_to_hit=True)
     ctx.find_tracks_best_hit(eoh, b)
     torch.cuda.synchronize()
-    cpu = oracle.find_best_hit(ev, descs, first_layer=3, chi2_cut=15.0, bfield=3.8, n_threads=8, prop_to_hit=True)
-    _compare_find(b.numpy(), cpu)
+    _compare_find(b.numpy(), ev, descs, chi2_cut=15.0, prop_to_hit=True)
 
 
 def test_find_noise_free_exact_path(gpu_device):
@@ -194,8 +187,7 @@ def test_find_noise_free_exact_path(gpu_device):
     eoh = mk.EventOfHits(ctx, descs)
     eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
     _, gpu = _run_find(ctx, eoh, ev, len(descs), 3, gpu_device)
-    cpu = oracle.find_best_hit(ev, descs, first_layer=3, bfield=3.8, n_threads=8)
-    _compare_find(gpu, cpu, min_same=0.998)
+    _compare_find(gpu, ev, descs)
     assert _true_fraction(gpu, ev, 3) > 0.97
 
 
@@ -234,8 +226,7 @@ def test_find_edge_cases(setup, gpu_device):
         ex.load(evx["hit_offset"], _t(evx["hits_flat"], gpu_device) if len(evx["hits_flat"]) else
                 torch.zeros((1, 9), device=gpu_device))
         _, g = _run_find(ctx, ex, evx, L, 3, gpu_device)
-        c = oracle.find_best_hit(evx, descs, first_layer=3, bfield=3.8, n_threads=8)
-        _compare_find(g, c)
+        _compare_find(g, evx, descs)
         assert np.all(g["hot_idx"][5:7] < 0)
 
 
@@ -271,6 +262,5 @@ def test_find_on_endcap_disks(gpu_device):
         r_g, b_g, _, _, _ = eoh.layer_views(l)
         assert np.mean(r_g == r_c) > 0.999
     _, gpu = _run_find(ctx, eoh, ev, len(descs), 3, gpu_device)
-    cpu = oracle.find_best_hit(ev, descs, first_layer=3, chi2_cut=15.0, bfield=3.8, n_threads=8)
-    _compare_find(gpu, cpu)
+    _compare_find(gpu, ev, descs, chi2_cut=15.0)
 

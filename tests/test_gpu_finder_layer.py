@@ -3,9 +3,10 @@ mkf_update_with_last_hit) against the oracle's, on identical inputs, and of a be
 driven from the host through the per-layer entry points against the oracle's FindTracksBestHit.
 
 Integer results (hit codes, record lists) must agree exactly except where a float32 rounding
-difference decides a comparison (two chi2 equal to rounding, a chi2 on the cut); those lanes are
-counted and bounded.  Floating-point results: parameters 1e-5 and covariances 1e-4 relative at the
-99th percentile, as for the fit.
+difference decides a comparison (two chi2 equal to rounding, a chi2 on the cut); each such lane is
+checked against the oracle's chi2 values.  Floating-point results: parameters 1e-5 and covariances
+1e-4 relative at the 99th percentile (single steps from identical inputs); the host-driven layer loop
+is held to the whole-loop criterion (tests/_find_common.py::assert_find_parity).
 """
 import numpy as np
 import pytest
@@ -13,6 +14,7 @@ import pytest
 import mkfit_amd as mk
 import oracle
 from _common import RawDeviceBuffer, cov_rel, par_rel
+from _find_common import CHI2_EPS, assert_find_parity, best_hit_reference
 from test_oracle_finder_layer import _lanes_at
 
 pytestmark = pytest.mark.gpu
@@ -49,6 +51,26 @@ def _gpu_lanes(ln, dev):
                           n_tail_minus_one=_t(ln["n_tail_minus_one"], dev) if "n_found" in ln else None)
 
 
+def _explain_chi2_choice(ev, descs, layer, ln, xs, xa, lanes, g_code, c_code, cut):
+    """Each lane in `lanes` chose differently on the GPU and in the oracle from identical inputs: the oracle's
+    chi2 of the two choices must agree within rounding (CHI2_EPS), or one of them lie at the cut."""
+    if len(lanes) == 0:
+        return
+    O = oracle.Oracle(bfield=3.8)
+    hl = _layer_hits(ev, layer)
+    for t in lanes:
+        c2 = {}
+        for h in xa[:max(xs[t], 0), t]:
+            _, _, v = O.kalman_operation(1, ln["prop_err"][:, t:t + 1], ln["prop_par"][:, t:t + 1],
+                                         hl[h, 3:].reshape(6, 1).copy(), hl[h, :3].reshape(3, 1).copy())
+            c2[int(h)] = abs(float(v[0]))
+        pick = [c2[int(x)] for x in (g_code[t], c_code[t]) if x >= 0]
+        near_cut = any(abs(v - cut) / cut < CHI2_EPS for v in pick)
+        tie = len(pick) == 2 and abs(pick[0] - pick[1]) / max(1.0, min(pick)) < CHI2_EPS
+        assert near_cut or tie, (t, g_code[t], c_code[t], c2)
+    assert len(lanes) <= len(xs) // 500
+
+
 def _state_close(gp, ge, cp, ce):
     pr = par_rel(gp, cp)
     cr = cov_rel(ge, ce)
@@ -71,7 +93,8 @@ def test_add_best_hit_matches_oracle(setup, gpu_device, layer):
     gn = g.numpy()
     gc = code.cpu().numpy()
     same = gc == cpu["hit_code"]
-    assert same.mean() >= 0.998, (same.mean(), np.nonzero(~same)[0][:8])
+    # lanes whose chosen hit differs: the two hits' chi2 (the oracle's, same inputs) tie, or one sits at the cut
+    _explain_chi2_choice(ev, descs, layer, ln, xs, xa, np.nonzero(~same)[0], gc, cpu["hit_code"], 15.0)
     for k in ("chg", "n_found", "n_inside_minus_one", "n_tail_minus_one"):
         assert np.array_equal(gn[k][same], cpu[k][same]), k
     _state_close(gn["par"][:, same], gn["err"][:, same], cpu["par"][:, same], cpu["err"][:, same])
@@ -105,7 +128,15 @@ def test_find_candidates_matches_oracle(setup, gpu_device, layer):
         for k in ints:
             if not np.array_equal(ent_g[t, :ne_g[t]][k], ent_c[t, :ne_c[t]][k]):
                 same[t] = False
-    assert same.mean() >= 0.998, (same.mean(), np.nonzero(~same)[0][:8])
+    # lanes whose records differ: a hit's chi2 at the cut (the oracle's records with no cut list every chi2)
+    diff = np.nonzero(~same)[0]
+    if len(diff):
+        ne_all, ent_all = oracle.find_candidates_ce(ev, descs, layer, ln, xs, xa, w, chi2_cut=1e30, max_holes=3,
+                                                    max_consec_holes=1, bfield=3.8)
+        for t in diff:
+            c2 = ent_all[t, :ne_all[t]]["chi2_hit"][ent_all[t, :ne_all[t]]["hit_idx"] >= 0]
+            assert np.any(np.abs(c2 - 30.0) / 30.0 < CHI2_EPS), (t, c2)
+    assert len(diff) <= n // 500
     m = np.zeros(ent_c.shape, bool)
     for t in np.nonzero(same)[0]:
         m[t, :ne_c[t]] = True
@@ -188,15 +219,10 @@ def test_best_hit_loop_through_layer_steps(setup, gpu_device):
     out_par[:, active], out_err[:, active] = cur_par[:, active], cur_err[:, active]
     out_chg[active], out_chi2[active], out_nf[active] = g.chg[active], g.chi2[active], g.n_found[active]
     torch.cuda.synchronize()
-    gpu = dict(par=out_par.cpu().numpy(), chg=out_chg.cpu().numpy(), chi2=out_chi2.cpu().numpy(), hot_idx=hot,
-               n_found=out_nf.cpu().numpy())
-    cpu = oracle.find_best_hit(ev, descs, first_layer=first, chi2_cut=15.0, bfield=3.8, n_threads=8)
-    same = np.all(gpu["hot_idx"] == cpu["hot_idx"], axis=0)
-    assert same.mean() >= 0.99, (same.mean(), np.nonzero(~same)[0][:8])
-    assert np.array_equal(gpu["n_found"][same], cpu["n_found"][same])
-    assert np.array_equal(gpu["chg"][same], cpu["chg"][same])
-    pr = par_rel(gpu["par"][:, same], cpu["par"][:, same])
-    assert np.percentile(pr, 99) < 1e-4 and pr.max() < 1e-2, (np.percentile(pr, 99), pr.max())
+    gpu = dict(par=out_par.cpu().numpy(), err=out_err.cpu().numpy(), chg=out_chg.cpu().numpy(),
+               chi2=out_chi2.cpu().numpy(), hot_idx=hot, n_found=out_nf.cpu().numpy())
+    cpu, chk = best_hit_reference(ev, descs, first, chi2_cut=15.0, bfield=3.8, n_threads=8)
+    assert_find_parity("best-hit layer loop", gpu, cpu, chk, first)
 
 
 def test_layer_steps_reject_bad_arguments(setup, gpu_device):
