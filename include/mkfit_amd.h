@@ -193,6 +193,16 @@ typedef struct mkf_adjoint_batch {
 
 int mkf_fit_adjoint(mkf_context* ctx, const mkf_adjoint_batch* batch, void* stream);
 
+/* mkf_fit_adjoint without any device allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight fit-and-descend loops, harnesses that forbid
+ * hipMalloc.  Same kernels and results as mkf_fit_adjoint; the checkpoints, gradient rows and sum partials
+ * live in a fixed 64 MiB device array that each GPU holds for the life of the process, so a batch may need at
+ * most 64 MiB: 64*3*8*n_layers (256-aligned) + (ceil(n_layers/checkpoint_every)*27 + 3*n_layers) * n_tracks *
+ * (fp64 ? 8 : 4) bytes, e.g. 79,000 tracks on 25 layers every 5 in FP32; larger batches return
+ * MKF_ERR_INVALID_ARG.  Every call on one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other contexts, other threads) race and must be ordered by the caller. */
+int mkf_fit_adjoint_noalloc(mkf_context* ctx, const mkf_adjoint_batch* batch, void* stream);
+
 /* ---- Best-hit track finding -------------------------------------------------
  *
  * Hits are the reference's Hit (Hit.h:165-183) as 9 floats: x, y, z, then the

@@ -96,7 +96,7 @@ ABI_SYMBOLS = [
     "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
-    "mkf_fit_adjoint",
+    "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc",
 ]
 CE_MAX_CANDS = 8
 
@@ -613,8 +613,9 @@ class AlignmentData:
         self.fp64 = int(bool(fp64))
         self.tangent = int(bool(tangent))   # step derivatives from forward tangents (cross-check) instead of reverse
 
-    def run(self, theta, stream=None):
-        """Per-track chi2 [n] and d(sum chi2)/d theta [n_layers, 3] at theta (device float32 [n_layers, 3])."""
+    def run(self, theta, stream=None, noalloc=False):
+        """Per-track chi2 [n] and d(sum chi2)/d theta [n_layers, 3] at theta (device float32 [n_layers, 3]).
+        noalloc: through mkf_fit_adjoint_noalloc (fixed device scratch, no allocation; at most 64 MiB of it)."""
         th = theta.detach().to(self.grad.device, dtype=self.grad.dtype).contiguous()
         b = MkfAdjointBatch()
         b.n_tracks, b.n_layers = self.n, self.n_layers
@@ -623,7 +624,8 @@ class AlignmentData:
         b.pflags, b.xi_per_x0, b.checkpoint_every = self.pflags, self.xi_per_x0, self.checkpoint_every
         b.chi2, b.grad = _ptr(self.chi2), _ptr(self.grad)
         b.fp64, b.tangent = self.fp64, self.tangent
-        _check(lib().mkf_fit_adjoint(self.ctx.h, ctypes.byref(b), self.ctx._stream(stream)))
+        f = lib().mkf_fit_adjoint_noalloc if noalloc else lib().mkf_fit_adjoint
+        _check(f(self.ctx.h, ctypes.byref(b), self.ctx._stream(stream)))
         return self.chi2.clone(), self.grad.clone()
 
 

@@ -218,8 +218,18 @@ __global__ void __launch_bounds__(256) k_sum_rows(const double* __restrict__ par
 
 }  // namespace
 
-extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void* stream) {
-  CHECK(set_device(c));
+// Device memory of mkf_fit_adjoint_noalloc: the same layout as the context's adjoint scratch (sum partials,
+// checkpoints, per-track gradient rows), in a fixed array each device holds for the life of the process.  Host
+// code uses the address hipGetSymbolAddress returns for it on each device (mkf_context::adj_static), never the
+// array's name.  Every call on one device shares it; calls that overlap on one device race.
+constexpr size_t kAdjNoallocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kAdjNoallocBytes];
+
+namespace {
+
+// Checks the batch; returns through `need` the scratch bytes the call takes and through `sum_bytes` the
+// 256-byte aligned size of the sums' partials at its front.
+int adjoint_check(const mkf_adjoint_batch* b, size_t& need, size_t& sum_bytes) {
   if (!b) return fail(MKF_ERR_INVALID_ARG, "batch is NULL");
   if (b->n_tracks < 0 || b->n_layers <= 0 || b->n_layers > 1024)
     return fail(MKF_ERR_INVALID_ARG, "need n_tracks >= 0 and 1 <= n_layers <= 1024");
@@ -229,13 +239,9 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   const size_t N = b->n_tracks, L = b->n_layers;
   CHECK(check_dev(b->theta, L * 3 * 4, "theta"));
   CHECK(check_dev(b->grad, L * 3 * 4, "grad"));
-  hipStream_t st = (hipStream_t)stream;
-  if (N == 0) {
-    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((L * 3 + 255) / 256)), dim3(256), 0, st, (const double*)nullptr,
-                       (int)(L * 3), b->grad);
-    HIP_TRY(hipGetLastError());
-    return MKF_OK;
-  }
+  need = 0;
+  sum_bytes = 0;
+  if (N == 0) return MKF_OK;
   if (L * 6 * N * 4 >= ((size_t)1 << 32)) return fail(MKF_ERR_INVALID_ARG, "batch too large");
   CHECK(check_dev(b->seed_par, 6 * N * 4, "seed_par"));
   CHECK(check_dev(b->seed_err, 21 * N * 4, "seed_err"));
@@ -247,18 +253,23 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   const size_t n_ck = (L + b->checkpoint_every - 1) / b->checkpoint_every;
   const size_t word = b->fp64 ? 8 : 4;
   // scratch: the sums' partials (256-byte aligned size), checkpoints, per-track gradient rows
-  const size_t sum_bytes = (L * 3 * kSumParts * sizeof(double) + 255) / 256 * 256;
-  const size_t need = sum_bytes + (n_ck * 27 + L * 3) * N * word;
-  std::lock_guard<std::mutex> lk(c->mu);
-  CHECK(scratch_acquire(c->adj_done, st));
-  if (need > c->adj_bytes) {
-    HIP_TRY(hipDeviceSynchronize());
-    if (c->adj_scratch) HIP_TRY(hipFree(c->adj_scratch));
-    c->adj_scratch = nullptr;
-    c->adj_bytes = 0;
-    HIP_TRY(hipMalloc(&c->adj_scratch, need));
-    c->adj_bytes = need;
+  sum_bytes = (L * 3 * kSumParts * sizeof(double) + 255) / 256 * 256;
+  need = sum_bytes + (n_ck * 27 + L * 3) * N * word;
+  return MKF_OK;
+}
+
+// The forward fit, reverse sweep and sums of a checked batch, with `scratch` holding at least the bytes
+// adjoint_check asked for.  Every byte of the scratch that a kernel reads is written earlier in the same call.
+int adjoint_launch(const mkf_context* c, const mkf_adjoint_batch* b, void* scr, size_t sum_bytes, hipStream_t st) {
+  const size_t N = b->n_tracks, L = b->n_layers;
+  if (N == 0) {
+    hipLaunchKernelGGL(k_sum_rows, dim3((unsigned)((L * 3 + 255) / 256)), dim3(256), 0, st, (const double*)nullptr,
+                       (int)(L * 3), b->grad);
+    HIP_TRY(hipGetLastError());
+    return MKF_OK;
   }
+  const size_t n_ck = (L + b->checkpoint_every - 1) / b->checkpoint_every;
+  const size_t word = b->fp64 ? 8 : 4;
   AdjArgs a;
   a.n = b->n_tracks;
   a.n_layers = b->n_layers;
@@ -270,8 +281,8 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   a.hit_pos = b->hit_pos; a.hit_err = b->hit_err; a.n_hits = b->n_hits;
   a.theta = b->theta;
   a.chi2 = b->chi2;
-  double* partial = static_cast<double*>(c->adj_scratch);
-  a.ckpt = static_cast<char*>(c->adj_scratch) + sum_bytes;
+  double* partial = static_cast<double*>(scr);
+  a.ckpt = static_cast<char*>(scr) + sum_bytes;
   a.gtrack = static_cast<char*>(a.ckpt) + n_ck * 27 * N * word;
   const dim3 sum_grid((unsigned)(L * 3), (unsigned)kSumParts);
   const dim3 fin_grid((unsigned)((L * 3 + 255) / 256));
@@ -296,5 +307,54 @@ extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void*
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_sum_rows, fin_grid, dim3(256), 0, st, (const double*)partial, (int)(L * 3), b->grad);
   HIP_TRY(hipGetLastError());
+  return MKF_OK;
+}
+
+}  // namespace
+
+namespace mkf_detail {
+
+// Called by mkf_context_create with the context's device current: the device address of this device's copy
+// of `scratch`.
+int adjoint_noalloc_setup(mkf_context* c) {
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return fail(MKF_ERR_HIP, "no-allocation scratch: %s", hipGetErrorString(e));
+  c->adj_static = p;
+  c->adj_static_bytes = kAdjNoallocBytes;
+  return MKF_OK;
+}
+
+}  // namespace mkf_detail
+
+extern "C" int mkf_fit_adjoint(mkf_context* c, const mkf_adjoint_batch* b, void* stream) {
+  CHECK(set_device(c));
+  size_t need = 0, sum_bytes = 0;
+  CHECK(adjoint_check(b, need, sum_bytes));
+  hipStream_t st = (hipStream_t)stream;
+  if (b->n_tracks == 0) return adjoint_launch(c, b, nullptr, 0, st);
+  std::lock_guard<std::mutex> lk(c->mu);
+  CHECK(scratch_acquire(c->adj_done, st));
+  if (need > c->adj_bytes) {
+    HIP_TRY(hipDeviceSynchronize());
+    if (c->adj_scratch) HIP_TRY(hipFree(c->adj_scratch));
+    c->adj_scratch = nullptr;
+    c->adj_bytes = 0;
+    HIP_TRY(hipMalloc(&c->adj_scratch, need));
+    c->adj_bytes = need;
+  }
+  CHECK(adjoint_launch(c, b, c->adj_scratch, sum_bytes, st));
   return scratch_release(c->adj_done, st);
+}
+
+extern "C" int mkf_fit_adjoint_noalloc(mkf_context* c, const mkf_adjoint_batch* b, void* stream) {
+  CHECK(set_device(c));
+  size_t need = 0, sum_bytes = 0;
+  CHECK(adjoint_check(b, need, sum_bytes));
+  if (!c->adj_static) return fail(MKF_ERR_HIP, "no-allocation scratch not set up for this context");
+  if (need > c->adj_static_bytes)
+    return fail(MKF_ERR_INVALID_ARG,
+                "batch needs %zu bytes of adjoint scratch; mkf_fit_adjoint_noalloc has %zu (fewer tracks, or a "
+                "larger checkpoint_every)", need, c->adj_static_bytes);
+  return adjoint_launch(c, b, c->adj_static, sum_bytes, (hipStream_t)stream);
 }

@@ -592,6 +592,10 @@ int mkf_context_create(const mkf_config* cfg, mkf_context** out) {
     mkf_context_destroy(c);
     return rc;
   }
+  if (int rc = adjoint_noalloc_setup(c)) {
+    mkf_context_destroy(c);
+    return rc;
+  }
   *out = c;
   return MKF_OK;
 }

@@ -63,6 +63,10 @@ struct mkf_context {
   int* fit_order = nullptr;
   size_t fit_order_cap = 0;
   hipEvent_t fit_done = nullptr;
+  // This device's fixed scratch array of mkf_fit_adjoint_noalloc (device address from hipGetSymbolAddress at
+  // context creation); shared by every context on the device.
+  void* adj_static = nullptr;
+  size_t adj_static_bytes = 0;
 };
 
 namespace mkf_detail {
@@ -137,6 +141,9 @@ inline int scratch_release(hipEvent_t done, hipStream_t st) {
   HIP_TRY(hipEventRecord(done, st));
   return MKF_OK;
 }
+
+// Sets mkf_context::adj_static for the context's device (mkf_adjoint.hip); the device must be current.
+int adjoint_noalloc_setup(mkf_context* c);
 
 }  // namespace mkf_detail
 
