@@ -408,6 +408,49 @@ typedef struct mkf_ce_batch {
  * holds 2^28 hits or more (hit records carry 28-bit positions in the layer). */
 int mkf_find_tracks_clone_engine(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_ce_batch* batch, void* stream);
 
+/* ---- Backward fit of found candidates -------------------------------------- */
+
+#define MKF_BKFIT_PCA           1 /* Config::includePCA: after the last layer, propagate to the PCA with pca_pflags
+                                     (BkFitPropTracksToPCA) and output that state (BkFitOutputTracks' Par[iP]) */
+#define MKF_BKFIT_FAKE_MATERIAL 2 /* fake hits take material: MkFinder::BkFitFitTracksBH (MkFinder.cc:1455-1570),
+                                     the best-hit driver's form; without it MkFinder::BkFitFitTracks */
+
+typedef struct mkf_bkfit_batch {
+  int n_cands;
+  const float* in_par;   /* [6][n]  the candidate's state after finding (a clone-engine slot: mkf_ce_batch par) */
+  const float* in_err;   /* [21][n] scaled by 100 before the fit (BkFitInputTracks) */
+  const int* in_chg;     /* [n] */
+  const int* hot_idx;    /* [n_layers][n] the candidate's HitOnTrack per plan layer: hit index within the layer,
+                            a fake-hit code < 0 (skipped), or -9 (no node); an index past the layer's hits counts
+                            as a fake-hit code */
+  int n_layers;          /* plan layers per candidate (hot_idx rows), 1..MKF_MAX_FIND_LAYERS */
+  int last_layer;        /* SteeringParams::m_bkw_fit_last: the lowest plan layer visited (0: all) */
+  const int* layer_base; /* [n] device: first layer of each candidate's event in eoh, or NULL: 0.  A candidate
+                            whose plan would run past the last layer of eoh is skipped: none of its outputs is written */
+  int group;             /* candidates fitted as one block, 1, 2, 4, ..., 64: the reference's NN-wide MkFinder
+                            (MkBuilder::fit_cands, blocks of NN from the first candidate).  A block visits every
+                            plan layer on which one of its candidates has a hit; the others of the block take a
+                            fake hit {666,0,666,0,0,666} at their position there (MkFinder.cc:1635-1647) */
+  int pflags;            /* Config::backward_fit_pflags */
+  int pca_pflags;        /* Config::pca_prop_pflags */
+  int flags;             /* MKF_BKFIT_* */
+  float* par;            /* [6][n] out (may alias in_par) */
+  float* err;            /* [21][n] out (may alias in_err) */
+  int* chg;              /* [n] out (may alias in_chg) */
+  float* chi2;           /* [n] out: sum of the update chi2 of every visited layer, fake hits included */
+  float* score;          /* [n] or NULL; out: getScoreCand (HitStructures.h:574-585) with the HitOnTrack counts of
+                            hot_idx (TrackCand::addHitIdx) and the fitted pT and chi2; left as it was where chi2 is
+                            NaN */
+} mkf_bkfit_batch;
+
+/* MkBuilder::BackwardFit / fit_cands (MkBuilder.cc:2124-2253), or BackwardFitBH / fit_cands_BH (:2069-2122)
+ * with MKF_BKFIT_FAKE_MATERIAL, over one candidate per seed: BkFitInputTracks (covariance x100, chi2 0;
+ * MkFinder.cc:1368-1399), BkFitFitTracks over the plan layers in reverse (the layer's HitOnTrack hit, or a
+ * fake hit; propagation to the hit's r or z with pflags, Kalman update, charge flip; MkFinder.cc:1573-1742),
+ * optionally the PCA step (:1746-1749), BkFitOutputTracks (:1425-1446).  One GPU lane per candidate; the
+ * lanes of a block share their per-layer decision through a wavefront ballot. */
+int mkf_backward_fit(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_bkfit_batch* batch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

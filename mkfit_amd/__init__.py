@@ -96,7 +96,7 @@ ABI_SYMBOLS = [
     "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
-    "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc",
+    "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc", "mkf_backward_fit",
 ]
 CE_MAX_CANDS = 8
 
@@ -120,6 +120,17 @@ class MkfCEBatch(ctypes.Structure):
                 ("n_cands", _c_void_p), ("par", _c_void_p), ("err", _c_void_p), ("chg", _c_void_p),
                 ("chi2", _c_void_p), ("score", _c_void_p), ("n_found", _c_void_p), ("hot_idx", _c_void_p),
                 ("n_layers", ctypes.c_int), ("layer_base", _c_void_p)]
+
+
+BKFIT_PCA, BKFIT_FAKE_MATERIAL = 1, 2
+
+
+class MkfBkFitBatch(ctypes.Structure):
+    _fields_ = [("n_cands", ctypes.c_int), ("in_par", _c_void_p), ("in_err", _c_void_p), ("in_chg", _c_void_p),
+                ("hot_idx", _c_void_p), ("n_layers", ctypes.c_int), ("last_layer", ctypes.c_int),
+                ("layer_base", _c_void_p), ("group", ctypes.c_int), ("pflags", ctypes.c_int),
+                ("pca_pflags", ctypes.c_int), ("flags", ctypes.c_int), ("par", _c_void_p), ("err", _c_void_p),
+                ("chg", _c_void_p), ("chi2", _c_void_p), ("score", _c_void_p)]
 
 
 class MkfAdjointBatch(ctypes.Structure):
@@ -615,6 +626,50 @@ class CEBatch:
                 for k in ("n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx")}
 
 
+class BkFitBatch:
+    """Device arrays of a backward fit of found candidates (mkf_backward_fit): the candidates' state and
+    HitOnTrack lists in, the fitted state, chi2 and score out.  `cand` holds device tensors or numpy arrays par
+    [6][n], err [21][n], chg [n], hot_idx [n_layers][n] and optionally score [n] -- e.g. slot 0 of a CEBatch."""
+
+    def __init__(self, cand, device="cuda:0", group=1, pflags=0, pca_pflags=0, include_pca=True,
+                 fake_material=False, last_layer=0, layer_base=None):
+        import torch
+        dev = torch.device(device)
+
+        def f(a, dt):
+            if isinstance(a, torch.Tensor):
+                return a.to(device=dev, dtype=dt).contiguous()
+            return torch.from_numpy(np.ascontiguousarray(a, {torch.float32: np.float32,
+                                                            torch.int32: np.int32}[dt])).to(dev)
+        self.in_par = f(cand["par"], torch.float32)
+        self.in_err = f(cand["err"], torch.float32)
+        self.in_chg = f(cand["chg"], torch.int32)
+        self.hot_idx = f(cand["hot_idx"], torch.int32)
+        self.L, self.n = int(self.hot_idx.shape[0]), int(self.hot_idx.shape[1])
+        self.layer_base = None if layer_base is None else f(layer_base, torch.int32)
+        # outputs start as the inputs: a candidate the fit does not write (outside its event) reads as given
+        self.par = self.in_par.clone()
+        self.err = self.in_err.clone()
+        self.chg = self.in_chg.clone()
+        self.chi2 = torch.zeros(self.n, dtype=torch.float32, device=dev)
+        self.score = f(cand["score"], torch.float32).clone() if "score" in cand else \
+            torch.zeros(self.n, dtype=torch.float32, device=dev)
+        self.group, self.pflags, self.pca_pflags, self.last_layer = int(group), int(pflags), int(pca_pflags), \
+            int(last_layer)
+        self.flags = (BKFIT_PCA if include_pca else 0) | (BKFIT_FAKE_MATERIAL if fake_material else 0)
+
+    def c_struct(self):
+        b = MkfBkFitBatch()
+        b.n_cands, b.n_layers, b.last_layer, b.group = self.n, self.L, self.last_layer, self.group
+        b.pflags, b.pca_pflags, b.flags = self.pflags, self.pca_pflags, self.flags
+        for k in ("in_par", "in_err", "in_chg", "hot_idx", "layer_base", "par", "err", "chg", "chi2", "score"):
+            setattr(b, k, _ptr(getattr(self, k)))
+        return b
+
+    def numpy(self):
+        return {k: getattr(self, k).cpu().numpy() for k in ("par", "err", "chg", "chi2", "score")}
+
+
 class AlignmentData:
     """Device copy of a fit batch for the gradient with respect to per-layer material and alignment.
 
@@ -719,6 +774,10 @@ class Context:
     def find_tracks_clone_engine(self, eoh, batch, stream=None):
         _check(lib().mkf_find_tracks_clone_engine(self.h, eoh.h, ctypes.byref(batch.c_struct()),
                                                   self._stream(stream)))
+
+    def backward_fit(self, eoh, batch, stream=None):
+        """mkf_backward_fit of a BkFitBatch over the layers of `eoh`."""
+        _check(lib().mkf_backward_fit(self.h, eoh.h, ctypes.byref(batch.c_struct()), self._stream(stream)))
 
     def add_best_hit(self, eoh, layer, lanes, xsize, xarr, wsr, hit_code, chi2_cut=15.0, prop_to_hit=False,
                      intra_pflags=0, stream=None):

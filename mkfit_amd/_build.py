@@ -32,7 +32,8 @@ def build_hip():
     out = os.path.join(PKG, "libmkfit_amd.so")
     srcs = [os.path.join(CSRC, "mkf_fit.hip"), os.path.join(CSRC, "mkf_find.hip"),
             os.path.join(CSRC, "mkf_ce.hip"), os.path.join(CSRC, "mkf_finder_layer.hip"),
-            os.path.join(CSRC, "mkf_adjoint.hip"), os.path.join(CSRC, "event_file.cc")]
+            os.path.join(CSRC, "mkf_adjoint.hip"), os.path.join(CSRC, "mkf_bkfit.hip"),
+            os.path.join(CSRC, "event_file.cc")]
     _run([HIPCC] + HIP_FLAGS + ["-o", out + ".tmp"] + srcs)
     os.replace(out + ".tmp", out)
     return out

@@ -180,8 +180,8 @@ class Oracle:
         sfx, T = self._dt(dtype)
         n = ev["seed_par"].shape[1]
         S = ev["hit_pos"].shape[0]
-        par = np.array(ev["seed_par"], T); err = np.array(ev["seed_err"], T)
-        chg = np.array(ev["chg"], np.int32); chi2 = np.zeros(n, T)
+        par = np.array(ev["seed_par"], T, order="C"); err = np.array(ev["seed_err"], T, order="C")
+        chg = np.array(ev["chg"], np.int32, order="C"); chi2 = np.zeros(n, T)
         hp = np.ascontiguousarray(ev["hit_pos"], np.float32); he = np.ascontiguousarray(ev["hit_err"], np.float32)
         nh = np.ascontiguousarray(ev["n_hits"], np.int32)
         hf = np.ascontiguousarray(ev["hit_flags"], np.uint8) if with_flags else None
@@ -271,8 +271,9 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     L = lib()
     layers = layer_array(descs)
     n = ev["n_seeds"]
-    par = np.array(ev["seed_par"][:, :n], np.float32); err = np.array(ev["seed_err"][:, :n], np.float32)
-    chg = np.array(ev["seed_chg"][:n], np.int32); chi2 = np.zeros(n, np.float32)
+    par = np.array(ev["seed_par"][:, :n], np.float32, order="C")
+    err = np.array(ev["seed_err"][:, :n], np.float32, order="C")
+    chg = np.array(ev["seed_chg"][:n], np.int32, order="C"); chi2 = np.zeros(n, np.float32)
     nl = len(descs)
     if ev["seed_hits"].shape[0] < first_layer:
         raise ValueError("seed_hits has %d rows; first_layer %d needs one per layer before it"
@@ -365,7 +366,7 @@ def _lanes(lanes):
                par=np.zeros((6, n), np.float32), err=np.zeros((21, n), np.float32))
     for k in LANE_KEYS:
         dt = np.float32 if k == "chi2" else np.int32
-        out[k] = np.array(lanes.get(k, np.zeros(n, dt)), dt)
+        out[k] = np.array(lanes.get(k, np.zeros(n, dt)), dt, order="C")
     st = _FinderLanes(n, *[_p(out[k]) for k in ("prop_par", "prop_err", "par", "err") + LANE_KEYS])
     return out, st
 
@@ -447,4 +448,58 @@ def align_chi2(ev, theta, pflags=2, xi_per_x0=2.2e-3, dtype=np.float64, bfield=3
     rc = f(ctypes.byref(OrcCtx(float(bfield), None, None)), ctypes.byref(b), _p(th), _p(out), int(n_threads))
     if rc:
         raise RuntimeError("orc_align_chi2 failed (%d)" % rc)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Backward fit of found candidates (oracle_bkfit.cc)
+# ---------------------------------------------------------------------------
+
+class _BkParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int) for k in ("group", "pflags", "pca_pflags", "include_pca", "fake_material",
+                                            "last_layer")]
+
+
+class _BkTracks(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("n_layers", ctypes.c_int)] + [(k, ctypes.c_void_p) for k in (
+        "hot_idx", "layer_base", "par", "err", "chg", "chi2", "score")]
+
+
+def backward_fit(ev, descs, cand, group=1, pflags=0, pca_pflags=0, include_pca=True, fake_material=False,
+                 last_layer=0, bfield=3.8, rl_grid=None, xi_grid=None, layer_base=None, dtype=np.float32,
+                 n_threads=1, ties=False, tie_shift=0.0):
+    """MkBuilder::BackwardFit over one candidate per seed: cand holds par [6][n], err [21][n], chg [n], hot_idx
+    [n_layers][n] and optionally score [n] (a clone-engine result's slot 0, or a best-hit result).  Returns
+    dict(par, err, chg, chi2, score) of the fitted candidates, plus tie [3][n] with ties=True (FP32 only)."""
+    T = np.float64 if np.dtype(dtype) == np.float64 else np.float32
+    hot = np.ascontiguousarray(cand["hot_idx"], np.int32)
+    nl, n = hot.shape
+    out = dict(par=np.array(cand["par"], T, order="C"), err=np.array(cand["err"], T, order="C"),
+               chg=np.array(cand["chg"], np.int32, order="C"), chi2=np.zeros(n, T),
+               score=np.array(cand["score"], T, order="C") if "score" in cand else np.zeros(n, T))
+    lb = None if layer_base is None else np.ascontiguousarray(layer_base, np.int32)
+    layers = layer_array(descs)
+    off = np.ascontiguousarray(ev["hit_offset"], np.int32)
+    hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
+    fe = _FindEvent(len(descs), ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), 0, 0.0, 0, 0, 0)
+    pr = _BkParams(int(group), int(pflags), int(pca_pflags), int(bool(include_pca)), int(bool(fake_material)),
+                   int(last_layer))
+    tr = _BkTracks(n, nl, _p(hot), _p(lb), *[_p(out[k]) for k in ("par", "err", "chg", "chi2", "score")])
+    rl = None if rl_grid is None else np.ascontiguousarray(rl_grid, np.float32)
+    xi = None if xi_grid is None else np.ascontiguousarray(xi_grid, np.float32)
+    ctx = OrcCtx(float(bfield), _p(rl), _p(xi))
+    L = lib()
+    if T == np.float64:
+        if ties or tie_shift:
+            raise ValueError("threshold margins and shifts apply to the FP32 fit")
+        rc = L.orc_backward_fit_f64(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr), ctypes.byref(tr),
+                                    int(n_threads))
+    else:
+        tie = np.zeros((3, n), np.float32) if ties else None
+        rc = L.orc_backward_fit_f32(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr), ctypes.byref(tr),
+                                    int(n_threads), _p(tie), ctypes.c_float(tie_shift))
+        if ties:
+            out["tie"] = tie
+    if rc:
+        raise RuntimeError("orc_backward_fit failed (%d)" % rc)
     return out

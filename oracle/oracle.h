@@ -235,6 +235,47 @@ int orc_find_candidates_ce(const orc_ctx* ctx, const orc_find_event* ev, const o
 int orc_update_with_last_hit(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, int layer,
                              const orc_finder_lanes* lanes, const int* hit_idx);
 
+/* ---- Backward fit of found candidates (oracle_bkfit.cc) ------------------- */
+
+typedef struct orc_bkfit_params {
+  int group;         /* candidates fitted as one block (the reference's NN-wide MkFinder), >= 1 */
+  int pflags;        /* Config::backward_fit_pflags */
+  int pca_pflags;    /* Config::pca_prop_pflags */
+  int include_pca;   /* Config::includePCA: propagate to the PCA and output that state */
+  int fake_material; /* 0: BkFitFitTracks (fake hits without material); 1: BkFitFitTracksBH (with) */
+  int last_layer;    /* SteeringParams::m_bkw_fit_last: the lowest plan index visited */
+} orc_bkfit_params;
+
+typedef struct orc_bkfit_tracks_f32 {
+  int n, n_layers;
+  const int* hot_idx;    /* [n_layers][n] HitOnTrack index per plan layer (hit index in the layer, fake code < 0,
+                            -9 no node); indices past the layer's hits count as fake codes */
+  const int* layer_base; /* [n] first layer of each candidate's event in ev, or NULL: 0; candidates whose plan
+                            runs past ev's layers are left untouched */
+  float* par;            /* [6][n]  in: the candidate's state; out: fitted (at the PCA with include_pca) */
+  float* err;            /* [21][n] */
+  int* chg;              /* [n] */
+  float* chi2;           /* [n] out */
+  float* score;          /* [n] or NULL: out getScoreCand unless chi2 is NaN */
+} orc_bkfit_tracks_f32;
+
+typedef struct orc_bkfit_tracks_f64 {
+  int n, n_layers;
+  const int* hot_idx;
+  const int* layer_base;
+  double* par;
+  double* err;
+  int* chg;
+  double* chi2;
+  double* score;
+} orc_bkfit_tracks_f64;
+
+/* ties: NULL or [3][n] out, the threshold margins of orc_fit_ties_f32; tie_shift: as orc_fit_tie_shift_f32. */
+int orc_backward_fit_f32(const orc_ctx* ctx, const orc_find_event* ev, const orc_bkfit_params* p,
+                         const orc_bkfit_tracks_f32* t, int n_threads, float* ties, float tie_shift);
+int orc_backward_fit_f64(const orc_ctx* ctx, const orc_find_event* ev, const orc_bkfit_params* p,
+                         const orc_bkfit_tracks_f64* t, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

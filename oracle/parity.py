@@ -276,3 +276,34 @@ def fit_reference(bfield, rl_grid, xi_grid, ev, pflags, fit_flags, n_threads=8, 
         a = o32.fit(ev, tie_shift=shift, **kw)
         alt.append((a[0], a[1], a[3]))
     return c32, c64, {"spread": spread, "tie": tie, "alt": alt}
+
+
+def bkfit_reference(ev, descs, cand, n_perturbed=N_PERTURBED, **kw):
+    """The oracle side of a backward-fit comparison (oracle.backward_fit over the candidates `cand` of a
+    finding event): FP32 and FP64 results (par, err, chg, chi2, score) and the checker data {spread, tie, alt},
+    with the rounding variants of fit_reference -- the no-FMA build, and one-ulp moves of the candidate
+    parameters and hit coordinates alternating the two builds."""
+    import oracle
+    r = oracle.backward_fit(ev, descs, cand, ties=True, **kw)
+    c32 = (r["par"], r["err"], r["chg"], r["chi2"], r["score"])
+    r64 = oracle.backward_fit(ev, descs, cand, dtype=np.float64, **kw)
+    c64 = (r64["par"], r64["err"], r64["chg"], r64["chi2"], r64["score"])
+    nofma = oracle.default_lib_path(contract=False)
+
+    def run(e, c, no_fma):
+        if no_fma:
+            with oracle.use_lib(nofma):
+                o = oracle.backward_fit(e, descs, c, **kw)
+        else:
+            o = oracle.backward_fit(e, descs, c, **kw)
+        return o["par"], o["err"], o["chi2"]
+    var = [run(ev, cand, True)]
+    for i in range(n_perturbed):
+        p = perturbed_seeds({"seed_par": cand["par"], "hits_flat": ev["hits_flat"]}, 2000 + i)
+        var.append(run(dict(ev, hits_flat=p["hits_flat"]), dict(cand, par=p["seed_par"]), i % 2 == 0))
+    spread = track_spread((c32[0], c32[1], c32[3]), (c64[0], c64[1], c64[3]), var)
+    alt = []
+    for shift in TIE_SHIFTS:
+        a = oracle.backward_fit(ev, descs, cand, tie_shift=shift, **kw)
+        alt.append((a["par"], a["err"], a["chi2"]))
+    return c32, c64, {"spread": spread, "tie": r["tie"], "alt": alt}

@@ -121,8 +121,31 @@ FIND_MUTANTS = {
                           "if (z < d.zmax && z > d.zmin) return WSR_Inside;"),
 }
 
-ORACLE_FILES = ("oracle.cc", "oracle_find.cc", "oracle_ce.cc", "oracle_align.cc", "oracle.h", "oracle_core.h",
-                "oracle_find_core.h")
+# Backward fit of found candidates (oracle_bkfit.cc): the input scaling (MkFinder.cc:1368-1399), the node walk,
+# fake hits and block counts (MkFinder.cc:1573-1742), the plan limits (SteeringParams.h:138-151), the charge
+# flip, the PCA step and the output score (MkFinder.cc:1425-1446, HitStructures.h:574-585,873-889).
+BKFIT_MUTANTS = {
+    "bk_scale": ("* T(100.0f);   // Scale(100.0f)", "* T(10.0f);   // Scale(100.0f)"),
+    "bk_fake_err": ("const T fake[6] = {666, 0, 666, 0, 0, 666};", "const T fake[6] = {666, 0, 666, 0, 0, 66.6};"),
+    "bk_fake_material": ("k.nomat[ln] = p->fake_material ? 0 : 1;", "k.nomat[ln] = 0;"),
+    "bk_skip_empty_layers": ("    if (here_count == 0) continue;\n", ""),
+    "bk_skip_only_minus_one": ("while (c >= 0 && H[c].index < 0) c = H[c].prev;",
+                               "while (c >= 0 && H[c].index == -1) c = H[c].prev;"),
+    "bk_forward_order": ("for (int l = L - 1; l >= p->last_layer; --l) {",
+                         "for (int l = p->last_layer; l < L; ++l) {"),
+    "bk_last_layer_excluded": ("for (int l = L - 1; l >= p->last_layer; --l) {",
+                               "for (int l = L - 1; l > p->last_layer; --l) {"),
+    "bk_pca_flags": ("k.errP, k.parP, cnt, p->pca_pflags, nullptr, ctx);", "k.errP, k.parP, cnt, p->pflags, nullptr, ctx);"),
+    "bk_endcap_as_barrel": ("k.ec[ln] = valid[i] ? !ev->layers[base[i] + l].is_barrel : 0;", "k.ec[ln] = 0;"),
+    "bk_score_inside": ("            ninside += ntail;\n", ""),
+    "bk_score_tail_any": ("          } else if (h == -1) {", "          } else {"),
+    "bk_score_lowpt": ("    penalty *= 1.5f;", "    penalty *= 1.7f;"),
+    "bk_layer_offset": ("ev->hits + (size_t)(ev->hit_offset[base[i] + l] + H[c].index) * 9;",
+                        "ev->hits + (size_t)(ev->hit_offset[l] + H[c].index) * 9;"),
+}
+
+ORACLE_FILES = ("oracle.cc", "oracle_find.cc", "oracle_ce.cc", "oracle_align.cc", "oracle_bkfit.cc", "oracle.h",
+                "oracle_core.h", "oracle_find_core.h")
 
 
 def _build(name, old, new, workdir, target="oracle_core.h"):
@@ -212,3 +235,46 @@ def test_find_mutant_is_caught(find_mutant_libs, name):
             except Exception as e:  # AssertionError, or numpy errors on the mutant's output
                 return
     pytest.fail("mutant %s passes every finding pin" % name)
+
+
+@pytest.fixture(scope="module")
+def bkfit_mutant_libs():
+    work = tempfile.mkdtemp(prefix="orc_bmut_")
+    try:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+            futs = {k: ex.submit(_build, k, a, b, work, "oracle_bkfit.cc") for k, (a, b) in BKFIT_MUTANTS.items()}
+            yield {k: f.result() for k, f in futs.items()}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def _bkfit_pins():
+    import test_oracle_bkfit as bk
+    barrel = bk._barrel()
+    mixed = []
+
+    def with_mixed(f):
+        if not mixed:
+            mixed.append(bk._mixed())
+        f(mixed[0])
+    return [
+        ("score_known_answers", lambda: [bk.test_score_known_answers(barrel, c) for c in range(len(bk.SCORE_CASES))]),
+        ("blocks_1", lambda: bk.test_blocks_equal_fit_composition(barrel, 1)),
+        ("blocks_16", lambda: bk.test_blocks_equal_fit_composition(barrel, 16)),
+        ("pca_flags", lambda: bk.test_without_pca_and_other_pca_flags(barrel)),
+        ("last_layer", lambda: bk.test_last_layer_stops_the_plan(barrel)),
+        ("material", lambda: bk.test_material_and_fake_hit_material(barrel)),
+        ("mixed", lambda: with_mixed(bk.test_mixed_layer_kinds_with_layer_base)),
+    ]
+
+
+@pytest.mark.parametrize("name", sorted(BKFIT_MUTANTS))
+def test_bkfit_mutant_is_caught(bkfit_mutant_libs, name):
+    so = bkfit_mutant_libs[name]
+    with oracle.use_lib(so):
+        for pin_name, pin in _bkfit_pins():
+            try:
+                pin()
+            except Exception:  # AssertionError, or numpy errors on the mutant's output
+                return
+    pytest.fail("mutant %s passes every backward-fit pin" % name)
