@@ -408,6 +408,15 @@ typedef struct mkf_ce_batch {
  * holds 2^28 hits or more (hit records carry 28-bit positions in the layer). */
 int mkf_find_tracks_clone_engine(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_ce_batch* batch, void* stream);
 
+/* MkBuilder::FindTracksStandard (MkBuilder.cc:1643-1845) with MkFinder::FindCandidates (MkFinder.cc:902-1109),
+ * same batch and outputs as mkf_find_tracks_clone_engine: per layer every (candidate, accepted hit) pair
+ * becomes a new candidate -- the candidate's state at the layer propagated to the hit with intra_pflags and
+ * updated with it, scored by getScoreCand with its updated pT and tail holes -- and every candidate inside
+ * the layer also becomes one with the invalid hit and its state at the layer; these and the held-back
+ * candidates are sorted by score, the best max_cands without a -2 hit are kept, and the first -2 one in that
+ * order competes for the best short candidate.  Hit charge and overlap hits as in the clone engine. */
+int mkf_find_tracks_standard(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_ce_batch* batch, void* stream);
+
 /* ---- Backward fit of found candidates -------------------------------------- */
 
 #define MKF_BKFIT_PCA           1 /* Config::includePCA: after the last layer, propagate to the PCA with pca_pflags

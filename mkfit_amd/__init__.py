@@ -96,7 +96,7 @@ ABI_SYMBOLS = [
     "mkf_event_of_hits_create", "mkf_event_of_hits_destroy", "mkf_event_of_hits_load", "mkf_event_of_hits_layer",
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
-    "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc", "mkf_backward_fit",
+    "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc", "mkf_backward_fit", "mkf_find_tracks_standard",
 ]
 CE_MAX_CANDS = 8
 
@@ -774,6 +774,10 @@ class Context:
     def find_tracks_clone_engine(self, eoh, batch, stream=None):
         _check(lib().mkf_find_tracks_clone_engine(self.h, eoh.h, ctypes.byref(batch.c_struct()),
                                                   self._stream(stream)))
+
+    def find_tracks_standard(self, eoh, batch, stream=None):
+        """mkf_find_tracks_standard: standard combinatorial finding of a CEBatch (same inputs and outputs)."""
+        _check(lib().mkf_find_tracks_standard(self.h, eoh.h, ctypes.byref(batch.c_struct()), self._stream(stream)))
 
     def backward_fit(self, eoh, batch, stream=None):
         """mkf_backward_fit of a BkFitBatch over the layers of `eoh`."""

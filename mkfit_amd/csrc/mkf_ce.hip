@@ -19,6 +19,14 @@
 // reference's CombCandidate::m_hots: cloning a candidate copies one index.
 // After the last layer MergeCandsAndBestShortOne orders the candidates, and
 // each lane walks its node chain to write the per-layer hit indices.
+//
+// The same kernel with STD runs standard combinatorial finding
+// (MkBuilder::FindTracksStandard, MkFinder::FindCandidates, MkFinder.cc:902-1109):
+// a new candidate carries its parent's state at the layer -- updated with its
+// hit after propagating to it with the intra-layer flags -- and is scored with
+// its own updated pT; held-back, new and invalid-hit candidates are ranked
+// together, the best max_cands without a -2 hit are kept and the first -2 one
+// in that order competes for the best short candidate (MkBuilder.cc:1760-1828).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -81,6 +89,18 @@ __device__ __forceinline__ void put(float* X, const Cand& c) {
   for (int e = 0; e < 6; ++e) X[e] = c.par[e];
 #pragma unroll
   for (int e = 0; e < 21; ++e) X[6 + e] = c.err[e];
+  X[27] = __int_as_float(c.chg);
+  X[28] = c.chi2;
+  X[29] = c.score;
+  X[30] = __int_as_float(c.nfound);
+  X[31] = __int_as_float(c.nmissing);
+  X[32] = __int_as_float(c.ninside);
+  X[33] = __int_as_float(c.ntail);
+  X[34] = __int_as_float(c.last);
+  X[35] = __int_as_float(c.last_hit);
+}
+// The words of `put` after the state: charge, chi2, score, counters, nodes.
+__device__ __forceinline__ void put_counts(float* X, const Cand& c) {
   X[27] = __int_as_float(c.chg);
   X[28] = c.chi2;
   X[29] = c.score;
@@ -156,8 +176,8 @@ __device__ __forceinline__ void propagate(bool barrel, float (&p)[6], float (&e)
 
 // KS candidate slots per seed (max_cands <= KS): KS = 6 packs 10 seeds into the
 // wavefront's 64 lanes (4 idle), KS = 8 packs 8.  Lanes past the last whole
-// seed group (g == kG) take no part.
-template <int KS>
+// seed group (g == kG) take no part.  STD: standard finding (see the top).
+template <int KS, bool STD>
 __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
   constexpr int kS = KS;
   constexpr int kG = kLanes / KS;             // seeds per workgroup
@@ -169,6 +189,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
                                             // order; from (a3) on its kept records, first ecnt
   __shared__ float Gc[kLanes][kHitIdxMax];  // their chi2 if accepted, else -1; then the kept records'
   __shared__ float Es[kLanes][kS];          // the kept records' scores
+  __shared__ float Gp[STD ? kLanes : 1][kHitIdxMax];   // STD: pT of each accepted hit's updated state
   __shared__ unsigned char ecnt[kLanes];    // kept records
   __shared__ signed char fcode[kLanes];     // invalid-hit record's hit code, 0 = none
   __shared__ unsigned char xkind[kLanes];   // 1 = held back unchanged, 2 = stopped replica
@@ -324,7 +345,16 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         propagate(ob, q, qe, __float_as_int(xs[27]), ob ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a);
       const float c2 = fabsf(ob ? kalman_barrel<true, false>(q, qe, mp, me)
                                 : kalman_endcap<true, false>(q, qe, mp, me));
-      Gc[o][k] = (c2 < xs[kXCut] && strip_q_compatible(ob, pe, q, mp, me)) ? c2 : -1.f;
+      const bool acc = c2 < xs[kXCut] && strip_q_compatible(ob, pe, q, mp, me);
+      Gc[o][k] = acc ? c2 : -1.f;
+      if (STD && acc) {
+        // getScoreCand of the new candidate takes the pT of its updated state (m_update_param_foo)
+        if (ob)
+          kalman_barrel<false, true>(q, qe, mp, me);
+        else
+          kalman_endcap<false, true>(q, qe, mp, me);
+        Gp[o][k] = fabsf(1.0f / q[3]);
+      }
     }
     __syncthreads();
     // (a3) records of the accepted hits, in scan order
@@ -334,7 +364,8 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         const float c2 = Gc[lane][hc];
         if (!(c2 >= 0.f)) continue;
         const float tot = st.chi2 + c2;
-        const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot, pt_l, true);
+        const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot,
+                                    STD ? Gp[lane][hc] : pt_l, true);
         const int rec = hc << kPosBits | Gh[lane][hc];
         // Keep only this lane's best K records, in hit order.  The seed's
         // top K (by score, earlier generation first on ties) never takes a
@@ -371,7 +402,12 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       }
       ecnt[lane] = ne;
     }
-    put(&X[lane * kW], st);
+    // STD: a searching lane's new candidates start from its state at the layer, which (a1) left in words
+    // 0-26; the others (held back, stopped, idle) keep their own.
+    if (STD && search)
+      put_counts(&X[lane * kW], st);
+    else
+      put(&X[lane * kW], st);
     __syncthreads();
     // ---- (b) CandCloner::ProcessSeedRange, one lane per seed
     if (c == 0 && s_merge[g]) {
@@ -403,6 +439,25 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
           }
         if (nt < K) ++nt;
       };
+      // STD: the first entry with a -2 hit in ranked order (highest score, earliest generated among equals)
+      // and its generation rank: held back 0..kS-1, records kS + hit_cnt * kS + slot, invalid-hit entries
+      // after every record.
+      float bsc = kScoreWorst;
+      int binfo = -1, bgen = 0;
+      auto consider_short = [&](float sc, int info, int gen) {
+        if (binfo < 0 || sc > bsc) {
+          bsc = sc;
+          binfo = info;
+          bgen = gen;
+        }
+      };
+      if (STD) {              // find_tracks_handle_missed_layers runs before FindCandidates
+        for (int s = 0; s < kS; ++s) {
+          const int l = g * kS + s;
+          if (xkind[l] == 1) insert(X[l * kW + 29], 1 | s << 2);
+          else if (xkind[l] == 2) consider_short(X[l * kW + 29], 1 | s << 2, s);
+        }
+      }
       // Generation order: hit_cnt, then lane.  Each slot's records are in
       // increasing hit_cnt, so the records are merged by always taking the
       // slot whose next record has the lowest hit_cnt (the lowest slot among
@@ -436,8 +491,57 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
       }
       for (int s = 0; s < kS; ++s) {
         const int l = g * kS + s;
-        if (fcode[l]) insert(X[l * kW + kXScore], 3 | s << 2 | -fcode[l] << 5);
+        if (STD && fcode[l] == -2)
+          consider_short(X[l * kW + kXScore], 3 | s << 2 | 2 << 5, kS + kHitIdxMax * kS + s);
+        else if (fcode[l])
+          insert(X[l * kW + kXScore], 3 | s << 2 | -fcode[l] << 5);
       }
+      if (STD) {
+        // MkBuilder.cc:1776-1828: the kept entries in ranked order become the next candidates; the first -2
+        // entry is met if it ranks before the K-th of them
+        int np = 0;
+        float sK = 0.f;
+        int iK = 0;
+        for (int ih = 0; ih < nt; ++ih) {
+          float sc = tsc[0];
+          int info = ti[0];
+#pragma unroll
+          for (int q = 1; q < kS; ++q)
+            if (q == ih) {
+              sc = tsc[q];
+              info = ti[q];
+            }
+          D[g][np++] = (info & 3) == 2 ? info : (info & 31);
+          sK = sc;
+          iK = info;
+        }
+        if (binfo >= 0) {
+          bool met = nt < K;
+          if (!met) {
+            const int kind = iK & 3, slot = (iK >> 2) & 7;
+            const int gen = kind == 1 ? slot
+                          : kind == 2 ? kS + (int)((unsigned)Gh[g * kS + slot][iK >> 5] >> kPosBits) * kS + slot
+                                      : kS + kHitIdxMax * kS + slot;
+            met = bsc > sK || (bsc == sK && bgen < gen);
+          }
+          if (met && bsc > s_bsc[g]) {                   // best short candidate
+            Cand b;
+            get(&X[(g * kS + ((binfo >> 2) & 7)) * kW], b);
+            if ((binfo & 3) == 3) {                      // an invalid-hit entry; a stopped replica has its -2
+              const int k = append_node(a, seed, &s_nnodes[g], -2, layer, b.last);
+              if (k >= 0) b.last = k;
+              add_hit_counts(b, -2, 0.f);
+            }
+            b.score = bsc;
+            put(a.bs + (size_t)seed * kBsW, b);
+            s_bsc[g] = bsc;
+            s_bs[g] = 1;
+          }
+        }
+        if (nent == 0 && binfo < 0)                      // no entry at all: the candidates stay
+          for (int i = 0; i < s_ncand[g]; ++i) D[g][np++] = 1 | i << 2;
+        s_nnew[g] = np;
+      } else {
       int np = 0;
       unsigned xm = 0;                                // held-back slots, lowest first
       for (int s = 0; s < kS; ++s)
@@ -487,6 +591,7 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
         }
       }
       s_nnew[g] = np;
+      }
     }
     __syncthreads();
     // ---- (c) new candidates; UpdateWithLastHit for those that took a hit
@@ -507,7 +612,8 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
             float mp[3], me[6];
             load_hit9(a.shits + (size_t)val * 9, mp, me);
             if (a.prop_to_hit)
-              propagate(barrel, st.par, st.err, st.chg, barrel ? hypot_ref(mp[0], mp[1]) : mp[2], a.inter_pflags, a);
+              propagate(barrel, st.par, st.err, st.chg, barrel ? hypot_ref(mp[0], mp[1]) : mp[2],
+                        STD ? a.intra_pflags : a.inter_pflags, a);
             if (barrel)
               kalman_barrel<false, true>(st.par, st.err, mp, me);
             else
@@ -602,8 +708,9 @@ __global__ void __launch_bounds__(kLanes) k_find_clone_engine(CeArgs a) {
 
 }  // namespace
 
-extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch* b,
-                                            void* stream) {
+namespace {
+
+int find_combinatorial(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch* b, void* stream, bool standard) {
   CHECK(set_device(c));
   if (!h || !b) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
   if (h->ctx != c) return fail(MKF_ERR_INVALID_ARG, "event of hits belongs to another context");
@@ -675,13 +782,29 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
   a.bs = reinterpret_cast<float*>(a.nodes + (size_t)node_cap * N);
   a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
   a.ranks = h->d_ranks;
+  const int grid6 = (int)((N + kLanes / 6 - 1) / (kLanes / 6)), grid8 = (int)((N + kLanes / 8 - 1) / (kLanes / 8));
   if (b->max_cands <= 6) {
-    const int grid = (int)((N + kLanes / 6 - 1) / (kLanes / 6));
-    hipLaunchKernelGGL(k_find_clone_engine<6>, dim3(grid), dim3(kLanes), 0, st, a);
+    if (standard)
+      hipLaunchKernelGGL((k_find_clone_engine<6, true>), dim3(grid6), dim3(kLanes), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_find_clone_engine<6, false>), dim3(grid6), dim3(kLanes), 0, st, a);
   } else {
-    const int grid = (int)((N + kLanes / 8 - 1) / (kLanes / 8));
-    hipLaunchKernelGGL(k_find_clone_engine<8>, dim3(grid), dim3(kLanes), 0, st, a);
+    if (standard)
+      hipLaunchKernelGGL((k_find_clone_engine<8, true>), dim3(grid8), dim3(kLanes), 0, st, a);
+    else
+      hipLaunchKernelGGL((k_find_clone_engine<8, false>), dim3(grid8), dim3(kLanes), 0, st, a);
   }
   HIP_TRY(hipGetLastError());
   return scratch_release(h->ce_done, st);
+}
+
+}  // namespace
+
+extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch* b,
+                                            void* stream) {
+  return find_combinatorial(c, h, b, stream, false);
+}
+
+extern "C" int mkf_find_tracks_standard(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch* b, void* stream) {
+  return find_combinatorial(c, h, b, stream, true);
 }

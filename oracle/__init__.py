@@ -309,10 +309,12 @@ class _CETracks(ctypes.Structure):
 
 
 def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.8, max_cands=6, max_holes=4,
-                      max_consec_holes=1, prop_to_hit=True, intra_pflags=0, n_threads=1, margins=False):
+                      max_consec_holes=1, prop_to_hit=True, intra_pflags=0, n_threads=1, margins=False,
+                      standard=False):
     """Clone-engine finding over a toy event dict; per seed up to max_cands candidates, best first.  margins=True
     adds the checker's decision margins [4][n_layers][n] (window edge cm, bin-edge fraction, chi2 vs cut,
-    score gap) and merge_gap [n]."""
+    score gap) and merge_gap [n].  standard=True runs the standard combinatorial finding
+    (MkBuilder::FindTracksStandard) with the same inputs and outputs."""
     L = lib()
     layers = layer_array(descs)
     n = ev["n_seeds"]
@@ -339,11 +341,17 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
     tr = _CETracks(n, _p(sp), _p(se), _p(sc), _p(sh), *[_p(out.get(k)) for k in (
         "n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx", "margins", "merge_gap")])
     ctx = OrcCtx(float(bfield), None, None)
-    rc = L.orc_find_clone_engine(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr),
-                                 ctypes.byref(tr), int(n_threads))
+    f = L.orc_find_standard if standard else L.orc_find_clone_engine
+    rc = f(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr), ctypes.byref(tr), int(n_threads))
     if rc:
-        raise RuntimeError("orc_find_clone_engine failed (%d)" % rc)
+        raise RuntimeError("oracle %s finding failed (%d)" % ("standard" if standard else "clone-engine", rc))
     return out
+
+
+def find_standard(ev, descs, first_layer, **kw):
+    """Standard combinatorial finding (MkBuilder::FindTracksStandard, MkFinder::FindCandidates): arguments and
+    outputs as find_clone_engine."""
+    return find_clone_engine(ev, descs, first_layer, standard=True, **kw)
 
 
 class _FinderLanes(ctypes.Structure):

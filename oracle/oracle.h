@@ -25,8 +25,8 @@ typedef struct orc_ctx {
 /* Fit flags. */
 #define ORC_FIT_CHI2        1 /* accumulate chi2 per hit (MkFinder::BkFitFitTracks) */
 #define ORC_FIT_CHARGE_FLIP 2 /* flip charge when 1/pT turns negative after update */
-#define ORC_FIT_FP64_FALLBACK 4
-#define ORC_FIT_PROP_TO_PCA 8 /* MkBase::PropagateTracksToPCAZ after the last hit (BkFitPropTracksToPCA) */ /* recompute an ill-conditioned updated covariance in FP64 (Joseph form) */
+#define ORC_FIT_FP64_FALLBACK 4 /* recompute an ill-conditioned updated covariance in FP64 (Joseph form) */
+#define ORC_FIT_PROP_TO_PCA 8 /* MkBase::PropagateTracksToPCAZ after the last hit (BkFitPropTracksToPCA) */
 
 /* Hit-slot flags. */
 #define ORC_HIT_ENDCAP  1 /* propagate to the hit z and use the endcap update */
@@ -224,6 +224,10 @@ typedef struct orc_ce_tracks {
 
 int orc_find_clone_engine(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, orc_ce_tracks* t,
                           int n_threads);
+/* Standard combinatorial finding (MkBuilder::FindTracksStandard, MkFinder::FindCandidates) with the same inputs
+ * and outputs; p->intra_pflags also serves the update of a candidate with its new hit. */
+int orc_find_standard(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, orc_ce_tracks* t,
+                      int n_threads);
 
 /* FindCandidatesCloneEngine on `layer` (reads prop_*, chg, chi2 and the counters of `lanes`; the chi2 cut
  * from ev, the rest from p): n_entries [n], entries [n][17] per lane in XHitArr order, invalid record last. */

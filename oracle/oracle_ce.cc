@@ -185,19 +185,21 @@ void prop_and_chi2(const CEEvent& E, bool barrel, const float* psErr, const floa
   }
 }
 
-// kalmanPropagateAndUpdate (KalmanUtilsMPlex.cc:471-504; endcap :697-730), with the charge flip.
+// kalmanPropagateAndUpdate (KalmanUtilsMPlex.cc:471-504; endcap :697-730), with the charge flip; the
+// propagation to the hit uses `pflags` (UpdateWithLastHit: the inter-layer flags, FindCandidates: the
+// intra-layer ones).
 void prop_and_update(const CEEvent& E, bool barrel, const float* psErr, const float* psPar, int* chg,
-                     const float* msErr, const float* msPar, int n_proc, float* outErr, float* outPar) {
+                     const float* msErr, const float* msPar, int n_proc, float* outErr, float* outPar, int pflags) {
   alignas(64) float c2[NN];
   if (E.p->prop_to_hit) {
     alignas(64) float propErr[21 * NN], propPar[6 * NN], ms[NN];
     if (barrel) {
       for (int l = 0; l < NN; ++l) ms[l] = std::hypot(msPar[0 * NN + l], msPar[1 * NN + l]);
-      propagate_helix_to_r(psErr, psPar, chg, ms, propErr, propPar, n_proc, E.ev->pflags, nullptr, nullptr, E.ctx);
+      propagate_helix_to_r(psErr, psPar, chg, ms, propErr, propPar, n_proc, pflags, nullptr, nullptr, E.ctx);
       kalman_operation(KFO_Update_Params, propErr, propPar, msErr, msPar, outErr, outPar, c2);
     } else {
       for (int l = 0; l < NN; ++l) ms[l] = msPar[2 * NN + l];
-      propagate_helix_to_z(psErr, psPar, chg, ms, propErr, propPar, n_proc, E.ev->pflags, nullptr, E.ctx);
+      propagate_helix_to_z(psErr, psPar, chg, ms, propErr, propPar, n_proc, pflags, nullptr, E.ctx);
       kalman_operation_endcap(KFO_Update_Params, propErr, propPar, msErr, msPar, outErr, outPar, c2);
     }
   } else {
@@ -292,6 +294,84 @@ void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const floa
   }
 }
 
+// MkFinder::FindCandidates (MkFinder.cc:902-1109) for one block of candidates idx of the seed, at the layer
+// (state parP/errP): appends to tmp, per hit_cnt in lane order, a new candidate for every accepted hit -- the
+// state at the layer propagated to the hit with the intra-layer flags and updated with it
+// (m_update_param_foo), its charge flipped with a negative 1/pT -- then per lane the candidate with the
+// invalid hit and the state at the layer (copy_out from iP); each scored by getScoreCand with
+// penalizeTailMissHits and inFindCandidates.  XHitArr entries outside [0, n_hits) are skipped.
+void find_candidates_std(const CEEvent& E, Comb& cc, int layer, const orc_layer_desc& d, const float* hits,
+                         int n_hits, const Block& B, const std::vector<int>& idx, const int* xs, const int* xa,
+                         const int* wsr, int n_proc, std::vector<Cand>& tmp, float* c2m = nullptr) {
+  if (c2m)
+    for (int i = 0; i < NN; ++i) c2m[i] = 1e30f;
+  const orc_find_event& ev = *E.ev;
+  const orc_ce_params& P = *E.p;
+  const bool barrel = d.is_barrel != 0;
+  auto score_new = [](const Cand& c) {                  // getScoreCand(c, true, true)
+    return score_calc(c.nfound, c.ntail, c.noverlap, c.ninside, c.chi2 < 0 ? 0.f : c.chi2, c.pT(), true);
+  };
+  int maxSize = 0;
+  for (int i = 0; i < n_proc; ++i)
+    if (xs[i] > 0) maxSize = std::max(maxSize, std::min(xs[i], kHitIdxMax));
+  alignas(64) float msPar[3 * NN], msErr[6 * NN];
+  for (int l = 0; l < 3 * NN; ++l) msPar[l] = 0.f;
+  for (int l = 0; l < 6 * NN; ++l) msErr[l] = 1.f;
+  for (int hc = 0; hc < maxSize; ++hc) {
+    bool ok[NN] = {};
+    for (int i = 0; i < n_proc; ++i)
+      if (hc < xs[i] && xa[hc * NN + i] >= 0 && xa[hc * NN + i] < n_hits) {
+        ok[i] = true;
+        const float* h = hits + (size_t)xa[hc * NN + i] * 9;
+        for (int e = 0; e < 3; ++e) msPar[e * NN + i] = h[e];
+        for (int e = 0; e < 6; ++e) msErr[e * NN + i] = h[3 + e];
+      }
+    alignas(64) float chi2[NN], propPar[6 * NN];
+    prop_and_chi2(E, barrel, B.errP, B.parP, B.chg, msErr, msPar, n_proc, chi2, propPar);
+    bool pass[NN] = {}, one_passes = false;
+    for (int i = 0; i < n_proc; ++i) {
+      if (!ok[i]) continue;
+      const float max_c2 = dyn_chi2_cut(d, ev.chi2_cut, B.parP[3 * NN + i], B.parP[5 * NN + i]);
+      const float c2 = std::abs(chi2[i]);
+      if (c2m) c2m[i] = std::min(c2m[i], std::abs(c2 - max_c2) / std::max(max_c2, 1.f));
+      if (c2 < max_c2 && strip_q_compatible(barrel, B.errP, propPar, msErr, msPar, i)) {
+        pass[i] = true;
+        one_passes = true;
+      }
+    }
+    if (!one_passes) continue;
+    alignas(64) float oErr[21 * NN], oPar[6 * NN];
+    int tmpChg[NN];
+    std::memcpy(tmpChg, B.chg, sizeof(tmpChg));
+    prop_and_update(E, barrel, B.errP, B.parP, tmpChg, msErr, msPar, n_proc, oErr, oPar, P.intra_pflags);
+    for (int i = 0; i < n_proc; ++i) {
+      if (!pass[i]) continue;
+      Cand nc = cc.cands[idx[i]];
+      for (int k = 0; k < 6; ++k) nc.par[k] = oPar[k * NN + i];
+      for (int k = 0; k < 21; ++k) nc.err[k] = oErr[k * NN + i];
+      nc.chg = tmpChg[i];
+      cc.add_hit(nc, xa[hc * NN + i], layer, std::abs(chi2[i]));
+      nc.score = score_new(nc);
+      nc.valid = true;
+      tmp.push_back(nc);
+    }
+  }
+  for (int i = 0; i < n_proc; ++i) {                    // the candidate with the invalid hit
+    if (wsr[i] == WSR_Outside) continue;
+    const Cand& c = cc.cands[idx[i]];
+    int fake = (c.ninside + c.ntail < P.max_holes && c.ntail < P.max_consec_holes) ? -1 : -2;
+    if (wsr[i] == WSR_Edge) fake = -3;
+    Cand nc = c;
+    for (int k = 0; k < 6; ++k) nc.par[k] = B.parP[k * NN + i];
+    for (int k = 0; k < 21; ++k) nc.err[k] = B.errP[k * NN + i];
+    nc.chg = B.chg[i];
+    cc.add_hit(nc, fake, layer, 0.f);
+    nc.score = score_new(nc);
+    nc.valid = true;
+    tmp.push_back(nc);
+  }
+}
+
 // Gap of two scores relative to the larger of their chi2 sums (for the checker's margins): scores differ
 // between FP32 implementations by their chi2 sums' rounding.
 inline float score_gap(float a, float ca, float b, float cb) {
@@ -302,7 +382,11 @@ inline float score_gap(float a, float ca, float b, float cb) {
 // seed's decision margins per layer: window edge (cm), bin edge (bin fraction), chi2 against the cut
 // (relative), and the smallest relative gap between scores that the candidate selection compares; `mgap`
 // gets that of the final merge.
-void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, float* mgap = nullptr) {
+// standard: MkBuilder::FindTracksStandard (MkBuilder.cc:1643-1845) instead of the clone engine: held-back and
+// new candidates go to the seed's tmp_cands, which is sorted by score; the best max_cands without a -2 last
+// hit become the next candidates and the first -2 one in that order competes for the best short candidate.
+void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, float* mgap = nullptr,
+               bool standard = false) {
   const orc_find_event& ev = *E.ev;
   const orc_ce_params& P = *E.p;
   const int max_cands = P.max_cands;
@@ -332,7 +416,8 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
       continue;
     }
     std::vector<IdxChi2> entries;
-    std::vector<Cand> extras;
+    std::vector<Cand> extras, tmp;
+    std::vector<Cand>& held = standard ? tmp : extras;   // where find_tracks_handle_missed_layers puts them
     for (size_t b0 = 0; b0 < active.size(); b0 += NN) {
       std::vector<int> idx(active.begin() + b0, active.begin() + std::min(active.size(), b0 + NN));
       const int n_proc = (int)idx.size();
@@ -355,10 +440,10 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
         if (barrel && cand_r < d.rin) {
           xs[i] = 0;
           wsr[i] = WSR_Outside;
-          extras.push_back(c);
-          cc.add_hit(extras.back(), -2, layer, 0);      // barrel region: stopped replica
+          held.push_back(c);
+          cc.add_hit(held.back(), -2, layer, 0);        // barrel region: stopped replica
         } else if (wsr[i] == WSR_Outside) {
-          extras.push_back(c);
+          held.push_back(c);
         }
       }
       // FindCandidatesCloneEngine
@@ -370,8 +455,12 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
         trk[i] = idx[i];
       }
       float c2m[NN];
-      find_candidates_block(E, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B.parP, B.errP, B.chg, lc,
-                            trk, xs, xa, wsr, n_proc, entries, mg ? c2m : nullptr);
+      if (standard)
+        find_candidates_std(E, cc, layer, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B, idx, xs, xa,
+                            wsr, n_proc, tmp, mg ? c2m : nullptr);
+      else
+        find_candidates_block(E, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B.parP, B.errP, B.chg,
+                              lc, trk, xs, xa, wsr, n_proc, entries, mg ? c2m : nullptr);
       if (mg) {
         const int L = ev.n_layers;
         for (int i = 0; i < n_proc; ++i) {
@@ -381,6 +470,37 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
           mg[2 * L + layer] = std::min(mg[2 * L + layer], c2m[i]);
         }
       }
+    }
+    if (standard) {
+      // sort the seed's tmp_cands (std::sort in the reference: here stable, ties keep generation order), then
+      // place the best max_cands without a -2 last hit; the first -2 one met competes for the best short one
+      std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
+      if (tmp.empty()) continue;
+      std::vector<Cand> cv;
+      int n_placed = 0;
+      bool first_short = true;
+      size_t ii = 0;
+      for (; ii < tmp.size() && n_placed < max_cands; ++ii) {
+        Cand& tc = tmp[ii];
+        if (cc.last_idx(tc) != -2) {
+          cv.push_back(tc);
+          ++n_placed;
+        } else if (first_short) {
+          first_short = false;
+          if (mg && cc.best_short.valid)
+            mg[3 * ev.n_layers + layer] = std::min(mg[3 * ev.n_layers + layer],
+                                                   score_gap(tc.score, tc.chi2, cc.best_short.score,
+                                                             cc.best_short.chi2));
+          if (tc.score > cc.best_short.score) cc.best_short = tc;
+        }
+      }
+      if (mg)      // the order up to one past the last entry examined
+        for (size_t i = 0; i + 1 < tmp.size() && i < ii; ++i)
+          mg[3 * ev.n_layers + layer] = std::min(mg[3 * ev.n_layers + layer],
+                                                 score_gap(tmp[i].score, tmp[i].chi2, tmp[i + 1].score,
+                                                           tmp[i + 1].chi2));
+      cc.cands = cv;
+      continue;
     }
     // CandCloner::ProcessSeedRange
     std::vector<Cand> cv;
@@ -441,7 +561,7 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
         for (int e = 0; e < 3; ++e) msPar[e * NN + l] = h[e];
         for (int e = 0; e < 6; ++e) msErr[e * NN + l] = h[3 + e];
       }
-      prop_and_update(E, barrel, B.err, B.par, B.chg, msErr, msPar, n_proc, oErr, oPar);
+      prop_and_update(E, barrel, B.err, B.par, B.chg, msErr, msPar, n_proc, oErr, oPar, ev.pflags);
       for (int i = 0; i < n_proc; ++i) {
         Cand& c = cc.cands[idx[i]];
         for (int k = 0; k < 6; ++k) c.par[k] = oPar[k * NN + i];
@@ -477,8 +597,10 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
 
 }  // namespace
 
-extern "C" int orc_find_clone_engine(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p,
-                                     orc_ce_tracks* tr, int n_threads) {
+namespace {
+
+int find_batch(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p, orc_ce_tracks* tr, int n_threads,
+               bool standard) {
   if (!ev || !p || !tr || ev->n_layers <= 0 || ev->first_layer < 0 || ev->first_layer > ev->n_layers ||
       p->max_cands < 1 || p->max_cands > 16)
     return 1;
@@ -512,7 +634,7 @@ extern "C" int orc_find_clone_engine(const orc_ctx* c, const orc_find_event* ev,
       cc.best_short.score = kScoreWorst;
       std::vector<float> mg(tr->margins ? 4 * (size_t)L : 0, 1e30f);
       float mgap = 1e30f;
-      find_seed(E, cc, ev->first_layer, tr->margins ? mg.data() : nullptr, tr->margins ? &mgap : nullptr);
+      find_seed(E, cc, ev->first_layer, tr->margins ? mg.data() : nullptr, tr->margins ? &mgap : nullptr, standard);
       if (tr->margins) {
         for (int k = 0; k < 4; ++k)
           for (int l = 0; l < L; ++l) tr->margins[((size_t)k * L + l) * n + s] = mg[(size_t)k * L + l];
@@ -549,6 +671,18 @@ extern "C" int orc_find_clone_engine(const orc_ctx* c, const orc_find_event* ev,
   work();
   for (auto& th : pool) th.join();
   return 0;
+}
+
+}  // namespace
+
+extern "C" int orc_find_clone_engine(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p,
+                                     orc_ce_tracks* tr, int n_threads) {
+  return find_batch(c, ev, p, tr, n_threads, false);
+}
+
+extern "C" int orc_find_standard(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p,
+                                 orc_ce_tracks* tr, int n_threads) {
+  return find_batch(c, ev, p, tr, n_threads, true);
 }
 
 namespace {
@@ -646,7 +780,7 @@ extern "C" int orc_update_with_last_hit(const orc_ctx* c, const orc_find_event* 
     }
     int chg_u[NN];
     std::memcpy(chg_u, chg, sizeof(chg));
-    prop_and_update(E, barrel, err, par, chg_u, msErr, msPar, cnt, oErr, oPar);
+    prop_and_update(E, barrel, err, par, chg_u, msErr, msPar, cnt, oErr, oPar, ev->pflags);
     for (int l = 0; l < cnt; ++l) {
       const size_t t = (size_t)t0 + l;
       for (int k = 0; k < 6; ++k) ln->par[k * (size_t)n + t] = ok[l] ? oPar[k * NN + l] : par[k * NN + l];

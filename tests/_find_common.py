@@ -131,12 +131,13 @@ def explain_best_hit(gpu_hot, cpu_hot, margins, first_layer):
     return len(diff), counts, bad
 
 
-def explain_clone_engine(gpu, cpu, first_layer):
+def explain_clone_engine(gpu, cpu, first_layer, alt_lists=()):
     """As explain_best_hit for the clone engine: gpu / cpu dicts with n_cands and hot_idx [K][L][n], cpu with the
     oracle's margins [4][L][n] and merge_gap [n].  A seed differs when its candidate lists (in order) differ.
-    Same set in another order: explained by a tie of the final merge.  Otherwise the first layer where the
-    candidate sets differ (a hit-list prefix of one side missing on the other) must hold a decision within
-    rounding (window, bin, chi2, score)."""
+    Explained when the lists equal those of the oracle with its helix-step threshold ties taken the other way
+    (alt_lists: (n_cands, hot_idx) of such runs).  Same set in another order: explained by a tie of the final
+    merge.  Otherwise the first layer where the candidate sets differ (a hit-list prefix of one side missing
+    on the other) must hold a decision within rounding (window, bin, chi2, score)."""
     n = cpu["n_cands"].shape[0]
     m = cpu["margins"]
     counts, bad, n_diff = {}, [], 0
@@ -147,6 +148,9 @@ def explain_clone_engine(gpu, cpu, first_layer):
         if rg == rc:
             continue
         n_diff += 1
+        if any(int(an[t]) == ng and all(tuple(ah[i, :, t]) == rg[i] for i in range(ng)) for an, ah in alt_lists):
+            counts["tie"] = counts.get("tie", 0) + 1
+            continue
         if sorted(rg) == sorted(rc):
             # the order comes from the final merge's sort, whose equal scores keep the order of the last
             # layer's selection: a tie in either
@@ -267,14 +271,15 @@ def clone_engine_reference(ev, descs, first_layer, n_variants=16, **kw):
         for k in range(K):
             g = group_error((v["par"][k], v["err"][k], v["chi2"][k]), (cpu["par"][k], cpu["err"][k], cpu["chi2"][k]))
             spread[k] = np.maximum(spread[k], np.where((same & (cpu["n_cands"] > k))[None, :], g, 0.0))
-    alt = []
+    alt, lists = [], []
     for a in _alternatives(run, ev):
         # per candidate: the alternative's candidate in the same slot with the same hit list
         row = np.all(a["hot_idx"] == cpu["hot_idx"], axis=1) & (a["n_cands"] > np.arange(K)[:, None])
         alt.append({"par": np.where(row[:, None, :], a["par"], cpu["par"]),
                     "err": np.where(row[:, None, :], a["err"], cpu["err"]),
                     "chi2": np.where(row, a["chi2"], cpu["chi2"])})
-    return cpu, {"spread": spread, "alt": alt}
+        lists.append((a["n_cands"], a["hot_idx"]))
+    return cpu, {"spread": spread, "alt": alt, "alt_lists": lists}
 
 
 def assert_ce_parity(name, gpu, cpu, spread, first_layer):
@@ -283,7 +288,7 @@ def assert_ce_parity(name, gpu, cpu, spread, first_layer):
     states within tolerance, or within four times their rounding spread where it reaches a third of it (at
     most 0.02 % beyond, by at most 2x: oracle/parity.py).  Returns (same mask, summary line)."""
     from oracle.parity import describe, flagged_parity
-    n_diff, counts, bad = explain_clone_engine(gpu, cpu, first_layer)
+    n_diff, counts, bad = explain_clone_engine(gpu, cpu, first_layer, spread.get("alt_lists", ()))
     same = (gpu["n_cands"] == cpu["n_cands"]) & np.all(gpu["hot_idx"] == cpu["hot_idx"], axis=(0, 1))
     K = cpu["par"].shape[0]
     cols = [(k, np.nonzero(same & (cpu["n_cands"] > k))[0]) for k in range(K)]

@@ -121,6 +121,22 @@ FIND_MUTANTS = {
                           "if (z < d.zmax && z > d.zmin) return WSR_Inside;"),
 }
 
+# Standard combinatorial finding (oracle_ce.cc, standard mode): new candidates' states (MkFinder.cc:1015-1060,
+# 1082-1106), the propagation flags of their update, their scores, and the per-seed selection
+# (MkBuilder.cc:1760-1828).
+STD_MUTANTS = {
+    "std_fake_parent_state": ("nc.par[k] = B.parP[k * NN + i];", "(void)0;"),
+    "std_hit_not_updated": ("nc.par[k] = oPar[k * NN + i];", "nc.par[k] = B.parP[k * NN + i];"),
+    "std_update_inter_flags": ("n_proc, oErr, oPar, P.intra_pflags);", "n_proc, oErr, oPar, ev.pflags);"),
+    "std_no_tail_penalty": ("return score_calc(c.nfound, c.ntail, c.noverlap,", "return score_calc(c.nfound, 0, c.noverlap,"),
+    "std_shorts_placed": ("if (cc.last_idx(tc) != -2) {", "if (true) {"),
+    "std_best_short_always": ("if (tc.score > cc.best_short.score) cc.best_short = tc;", "cc.best_short = tc;"),
+    "std_held_dropped": ("std::vector<Cand>& held = standard ? tmp : extras;", "std::vector<Cand>& held = extras;"),
+    "std_fake_consec_le": ("c.ntail < P.max_consec_holes) ? -1 : -2;", "c.ntail <= P.max_consec_holes) ? -1 : -2;"),
+    "std_sort_ascending": ("std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });",
+                           "std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score < b.score; });"),
+}
+
 # Backward fit of found candidates (oracle_bkfit.cc): the input scaling (MkFinder.cc:1368-1399), the node walk,
 # fake hits and block counts (MkFinder.cc:1573-1742), the plan limits (SteeringParams.h:138-151), the charge
 # flip, the PCA step and the output score (MkFinder.cc:1425-1446, HitStructures.h:574-585,873-889).
@@ -278,3 +294,37 @@ def test_bkfit_mutant_is_caught(bkfit_mutant_libs, name):
             except Exception:  # AssertionError, or numpy errors on the mutant's output
                 return
     pytest.fail("mutant %s passes every backward-fit pin" % name)
+
+
+@pytest.fixture(scope="module")
+def std_mutant_libs():
+    work = tempfile.mkdtemp(prefix="orc_smut_")
+    try:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+            futs = {k: ex.submit(_build, k, a, b, work, "oracle_ce.cc") for k, (a, b) in STD_MUTANTS.items()}
+            yield {k: f.result() for k, f in futs.items()}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def _std_pins():
+    import test_oracle_std_pins as sp
+    return [
+        ("selection_k1", lambda: sp.test_standard_selection(1, None, 0)),
+        ("selection_k3_intra", lambda: sp.test_standard_selection(3, None, 1)),
+        ("all_paths_4_1", lambda: sp.test_standard_equals_all_paths((4, 1, None))),
+        ("all_paths_1_1", lambda: sp.test_standard_equals_all_paths((1, 1, None))),
+        ("all_paths_2_2_short", lambda: sp.test_standard_equals_all_paths((2, 2, 20.0))),
+    ]
+
+
+@pytest.mark.parametrize("name", sorted(STD_MUTANTS))
+def test_std_mutant_is_caught(std_mutant_libs, name):
+    so = std_mutant_libs[name]
+    with oracle.use_lib(so):
+        for pin_name, pin in _std_pins():
+            try:
+                pin()
+            except Exception:  # AssertionError, or numpy errors on the mutant's output
+                return
+    pytest.fail("mutant %s passes every standard-finding pin" % name)
