@@ -17,9 +17,15 @@ The batch is packed in fit order (hit count, then endcap/barrel slot pattern;
 mkfit_amd.fit_order), as a caller's packer would build it; both forms fit the
 same packed batch.  --unordered packs it in generation order instead.
 `value` is the batched throughput; the plain one and their ratio ride along.
-The plain-fit-at-scale workload (1,000,000 tracks, 25 barrel layers, one hit
-per layer) rides along in the same JSON line under "fit_at_scale" (skip it
-with --no-secondary; --workload fit makes it the headline instead).
+The other workloads ride along in the same JSON line, each with its own
+roofline and CPU baseline (skip them with --no-secondary):
+  fit_at_scale   plain-fit-at-scale, 1,000,000 tracks, 25 barrel layers, one hit
+                 per layer (--workload fit makes it the headline instead);
+  find_ce / find_std / find_bh
+                 clone-engine, standard combinatorial and best-hit finding of a
+                 batch of 50 events (17 barrel layers, U(5k, 15k) seeds each,
+                 noise 2 hits / cm^2), LayerOfHits build + one finder launch;
+  backward_fit   the backward fit of every seed's best clone-engine candidate.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -64,15 +70,17 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle checker")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--no-secondary", action="store_true", help="skip the plain-fit-at-scale line")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="headline only: skip the fit-at-scale, finding and backward-fit sub-results")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--shape", type=str, default=None,
                     help="block,blocks_per_cu,variant[,sort_tile] for the batched kernel (skips the autotuner)")
-    ap.add_argument("--workload", default="fit-uneven", choices=["fit-uneven", "fit", "find-ce", "find-bh", "adjoint"],
+    ap.add_argument("--workload", default="fit-uneven",
+                    choices=["fit-uneven", "fit", "find-ce", "find-bh", "find-std", "adjoint"],
                     help="fit-uneven (default, the headline): 2M tracks, barrel + 2x9 disks, 8-30 hits, 5%% "
                          "missing, FP64 fallback on; fit: plain-fit-at-scale, 1M tracks, 25 barrel layers; "
-                         "find-ce / find-bh: clone-engine / best-hit finding per event; adjoint: gradient of the "
-                         "fit's chi2")
+                         "find-ce / find-std / find-bh: clone-engine / standard / best-hit finding of --events "
+                         "events; adjoint: gradient of the fit's chi2")
     ap.add_argument("--events", type=int, default=50,
                     help="events per step of the find workloads (all in one event of hits and one launch)")
     ap.add_argument("--unordered", action="store_true",
@@ -119,8 +127,11 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    if args.workload in ("find-ce", "find-bh"):
-        return bench_find(args, mk, torch, dev)
+    if args.workload in ("find-ce", "find-bh", "find-std"):
+        res = bench_find(args, mk, torch, dev, args.workload[5:])
+        res.pop("_out")
+        print(json.dumps(res), flush=True)
+        return
     if args.workload == "adjoint":
         return bench_adjoint(args, mk, torch, dev)
     uneven = args.workload == "fit-uneven"
@@ -133,6 +144,13 @@ def main():
         sec = bench_fit(a2, mk, torch, dev, False)
         res["fit_at_scale"] = {k: sec[k] for k in ("value", "unit", "ms_per_step", "config", "plain",
                                                     "batched_over_plain", "parity", "roofline", "cpu_baseline")}
+        keys = ("metric", "value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline")
+        finds = {}
+        for kind in ("ce", "std", "bh"):
+            r = bench_find(args, mk, torch, dev, kind)
+            finds[kind] = r.pop("_out")
+            res["find_" + kind] = {k: r[k] for k in keys}
+        res["backward_fit"] = bench_bkfit(args, mk, torch, dev, finds["ce"])
     print(json.dumps(res), flush=True)
 
 
@@ -338,67 +356,169 @@ def _timed(torch, stream, fn, steps, warmup):
     return time.perf_counter() - t_start, [e0.elapsed_time(e1) for e0, e1 in evs]
 
 
-def bench_find(args, mk, torch, dev):
-    """Track finding over a batch of events: 17-layer barrel (r = 4..100 cm), U(5k, 15k) seeds per event,
-    noise hits at 2 per cm^2 on every layer, 3-hit seeds; clone engine with up to 6 candidates (chi2 cut 30)
-    or best-hit (chi2 cut 15).  All events share one event of hits (event e owns layers e*17 ..) and one
-    finder launch; a step is the whole batch: LayerOfHits build + finding, inputs in HBM."""
-    ce = args.workload == "find-ce"
-    geo = mk.geometry(1)
-    descs = mk.barrel_layer_descs(geo)
+# Algorithmic FP32 operations of the finding steps (oracle/flopcount.cc, barrel, no material): a propagation
+# to a layer or to a hit, the chi2 of a hit, and the Kalman update of a state with its hit.
+FIND_FLOP = {"propagate": 1031, "chi2": 65, "update": 282}
+
+
+def find_events(args, mk):
+    """The finding workload, generated once per run: --events toy events of the 17-layer barrel, U(5k, 15k)
+    seeds each, noise 2 hits / cm^2, in one shared event of hits (event e owns layers e*17 ..)."""
+    if getattr(args, "_find_events", None) is None:
+        geo = mk.geometry(1)
+        descs = mk.barrel_layer_descs(geo)
+        rng = np.random.default_rng(args.seed)
+        events = []
+        t0 = time.time()
+        for e in range(args.events):
+            p = mk.default_toy_params(seed=args.seed + e, simulate_material=0)
+            events.append(mk.generate_find_event(geo, p, int(rng.integers(5000, 15001)), noise_per_cm2=2.0))
+        cat = mk.concat_find_events(events, len(descs))
+        log("generated %d finding events in %.1f s; %d seeds, %d hits" % (len(events), time.time() - t0,
+                                                                          cat["n_seeds"],
+                                                                          int(cat["hit_offset"][-1])))
+        args._find_events = (events, cat, descs)
+    return args._find_events
+
+
+def bench_find(args, mk, torch, dev, kind):
+    """Track finding over a batch of events (find_events): clone engine (kind "ce") or standard combinatorial
+    finding ("std") with up to 6 candidates and chi2 cut 30, propagating to each hit; or best-hit ("bh", chi2
+    cut 15).  A step is the whole batch: the LayerOfHits build of every layer plus one finder launch over all
+    seeds, inputs in HBM.  The roofline is the finder kernel's: its launch timed alone with HIP events on the
+    stream, against the algorithmic FP32 operations of the work it does (propagations to the layers, window
+    hits evaluated, updates), counted per seed by the oracle on the batch's first event (in the CPU baseline
+    leg) and scaled to the batch's seeds.  Returns the result dict (the last GPU output in "_out")."""
+    events, cat, descs = find_events(args, mk)
     L = len(descs)
-    rng = np.random.default_rng(args.seed)
-    events = []
-    t0 = time.time()
-    for e in range(args.events):
-        p = mk.default_toy_params(seed=args.seed + e, simulate_material=0)
-        events.append(mk.generate_find_event(geo, p, int(rng.integers(5000, 15001)), noise_per_cm2=2.0))
-    cat = mk.concat_find_events(events, L)
-    log("generated %d events in %.1f s; %d seeds, %d hits" % (len(events), time.time() - t0, cat["n_seeds"],
-                                                                int(cat["hit_offset"][-1])))
     ctx = mk.Context(bfield=3.8)
     eoh = mk.EventOfHits(ctx, descs * len(events))
     hits = torch.from_numpy(cat["hits_flat"]).to(dev)
-    batch = (mk.CEBatch(cat, L, 3, device=dev, chi2_cut=30.0, max_cands=6, layer_base=cat["layer_base"]) if ce else
-             mk.FindBatch(cat, L, 3, device=dev, chi2_cut=15.0, layer_base=cat["layer_base"]))
+    comb = kind in ("ce", "std")
+    batch = (mk.CEBatch(cat, L, 3, device=dev, chi2_cut=30.0, max_cands=6, layer_base=cat["layer_base"]) if comb
+             else mk.FindBatch(cat, L, 3, device=dev, chi2_cut=15.0, layer_base=cat["layer_base"]))
+    finder = {"ce": ctx.find_tracks_clone_engine, "std": ctx.find_tracks_standard,
+              "bh": ctx.find_tracks_best_hit}[kind]
     stream = torch.cuda.current_stream()
+    kern = []
 
     def step(i):
         eoh.load(cat["hit_offset"], hits)
-        if ce:
-            ctx.find_tracks_clone_engine(eoh, batch)
-        else:
+        if not comb:
             batch.reset()
-            ctx.find_tracks_best_hit(eoh, batch)
+        k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        k0.record(stream)
+        finder(eoh, batch)
+        k1.record(stream)
+        kern.append((k0, k1))
 
     wall, per = _timed(torch, stream, step, args.steps, args.warmup)
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in kern[args.warmup:]]))
     seeds = cat["n_seeds"] * args.steps
     ms = wall * 1e3 / args.steps
+    prop_to_hit = comb
+    unit_flop = {"propagate": FIND_FLOP["propagate"],
+                 "evaluate": FIND_FLOP["chi2"] + (FIND_FLOP["propagate"] if prop_to_hit else 0),
+                 "update": FIND_FLOP["update"] + (FIND_FLOP["propagate"] if prop_to_hit else 0)}
+    name = {"ce": "clone-engine", "std": "standard combinatorial", "bh": "best-hit"}[kind]
+    kname = {"ce": "k_find_clone_engine<6, false>", "std": "k_find_clone_engine<6, true>",
+             "bh": "k_find_best_hit"}[kind]
     cpu = None
+    roof = {"bound": "valu", "achieved": None, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": None,
+            "traffic": None, "kernel": kname, "kernel_ms": kern_ms, "flop_per_unit": unit_flop}
     if not args.no_cpu:
         import oracle
         ncpu = min(16, os.cpu_count() or 1)
         ev = events[0]
         t = time.perf_counter()
-        if ce:
-            oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=ncpu)
+        if comb:
+            o = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=ncpu,
+                                         standard=kind == "std", work=True)
         else:
-            oracle.find_best_hit(ev, descs, 3, chi2_cut=15.0, n_threads=ncpu)
+            o = oracle.find_best_hit(ev, descs, 3, chi2_cut=15.0, n_threads=ncpu, work=True)
         dt = time.perf_counter() - t
         cpu = {"value": ev["n_seeds"] / dt, "unit": "seeds/s", "cores": ncpu, "kind": "port",
                "sample": "one event (%d seeds), oracle %s restatement incl. its LayerOfHits build, %d threads" %
-                         (ev["n_seeds"], "clone-engine" if ce else "best-hit", ncpu)}
-    res = {"metric": "seeds processed/sec, %s track finding on 1 MI355X" % ("clone-engine" if ce else "best-hit"),
+                         (ev["n_seeds"], name, ncpu)}
+        w = o["work"].astype(np.float64).sum(axis=1) / ev["n_seeds"]
+        flop_seed = (w[0] * unit_flop["propagate"] + w[1] * unit_flop["evaluate"] + w[2] * unit_flop["update"])
+        achieved = flop_seed * cat["n_seeds"] / (kern_ms * 1e-3) / 1e12
+        roof.update({"achieved": achieved, "frac": achieved / PEAK_FP32_TFLOPS, "flop_per_seed": flop_seed,
+                     "work_per_seed": {"propagations": w[0], "hits_evaluated": w[1], "updates": w[2]},
+                     "work_source": "oracle work counts on the batch's first event (%d seeds), per seed, times "
+                                    "the batch's %d seeds" % (ev["n_seeds"], cat["n_seeds"])})
+    res = {"metric": "seeds processed/sec, %s track finding on 1 MI355X" % name,
            "value": seeds / wall, "unit": "seeds/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic: toy MC finding events (signal + uniform noise hits), seeds %d.." % args.seed,
-           "config": {"workload": "find, %d events per step in one launch: 17 barrel layers, U(5k,15k) seeds per "
-                                  "event, noise 2/cm^2, %s" % (len(events), "up to 6 candidates, chi2 cut 30" if ce
-                                                               else "best hit, chi2 cut 15"),
+           "config": {"workload": "find (%s), %d events per step in one launch: 17 barrel layers, U(5k,15k) seeds "
+                                  "per event, noise 2/cm^2, %s" % (name, len(events),
+                                                                   "up to 6 candidates, chi2 cut 30" if comb
+                                                                   else "best hit, chi2 cut 15"),
                       "events": len(events), "seeds": cat["n_seeds"], "hits": int(cat["hit_offset"][-1])},
-           "ms_per_event": ms / len(events), "cpu_baseline": cpu}
-    print(json.dumps(res), flush=True)
+           "ms_per_event": ms / len(events), "roofline": roof, "cpu_baseline": cpu}
+    log("find %s: %.3g seeds/s, %.2f ms per step, finder kernel %.2f ms%s" % (
+        kind, res["value"], ms, kern_ms, "" if roof["frac"] is None else ", %.3f of FP32 peak" % roof["frac"]))
+    res["_out"] = batch.numpy()
     ctx.close()
+    return res
+
+
+def bench_bkfit(args, mk, torch, dev, ce_out):
+    """Backward fit of found candidates (mkf_backward_fit): the best clone-engine candidate of every seed of
+    the finding batch (find_events), covariance x100, its hits in reverse layer order in blocks of 16
+    candidates, then to the PCA.  A step is one backward fit of all candidates, inputs in HBM.  Roofline: the
+    kernel (one launch) against 1322 FP32 operations per visited layer (propagation + chi2 + update) plus 413
+    for the PCA step (oracle/flopcount.cc)."""
+    events, cat, descs = find_events(args, mk)
+    L = len(descs)
+    keep = ce_out["n_cands"] > 0
+    cand = {"par": ce_out["par"][0][:, keep], "err": ce_out["err"][0][:, keep], "chg": ce_out["chg"][0][keep],
+            "hot_idx": ce_out["hot_idx"][0][:, keep], "score": ce_out["score"][0][keep]}
+    base = cat["layer_base"][keep]
+    n = int(keep.sum())
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs * len(events))
+    eoh.load(cat["hit_offset"], torch.from_numpy(cat["hits_flat"]).to(dev))
+    b = mk.BkFitBatch(cand, device=dev, group=16, pflags=1, pca_pflags=1, layer_base=base)
+    stream = torch.cuda.current_stream()
+    wall, per = _timed(torch, stream, lambda i: ctx.backward_fit(eoh, b), args.steps, args.warmup)
+    kern_ms = float(np.mean(per))
+    # layers visited per candidate: those where some candidate of its block of 16 has a hit
+    real = np.asarray(cand["hot_idx"]) >= 0
+    G = 16
+    pad = (-n) % G
+    blk = np.concatenate([real, np.zeros((L, pad), bool)], axis=1).reshape(L, -1, G).any(axis=2)
+    visited = int(blk.sum()) * G - pad * int(blk[:, -1].sum())      # the last block holds G - pad candidates
+    flop = visited * (FIND_FLOP["propagate"] + 291) + n * 413
+    achieved = flop / (kern_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        ncpu = min(16, os.cpu_count() or 1)
+        ev = events[0]
+        sel = np.nonzero(keep[:ev["n_seeds"]])[0]
+        c0 = {k: np.ascontiguousarray(v[..., :len(sel)]) for k, v in cand.items()}
+        t = time.perf_counter()
+        oracle.backward_fit(ev, descs, c0, group=16, pflags=1, pca_pflags=1, bfield=3.8, n_threads=ncpu)
+        dt = time.perf_counter() - t
+        cpu = {"value": len(sel) / dt, "unit": "candidates/s", "cores": ncpu, "kind": "port",
+               "sample": "the first event's %d candidates, oracle backward-fit restatement, %d threads" %
+                         (len(sel), ncpu)}
+    res = {"metric": "candidates backward-fitted/sec on 1 MI355X", "value": n * args.steps / wall,
+           "unit": "candidates/s", "ms_per_step": wall * 1e3 / args.steps, "dtype": "f32",
+           "config": {"workload": "backward fit: the best clone-engine candidate of each of %d seeds (%d events), "
+                                  "17 barrel layers, blocks of 16, to the PCA" % (n, len(events)),
+                      "candidates": n, "visited_layers": visited},
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_backward_fit",
+                        "kernel_ms": kern_ms, "flop_per_visited_layer": FIND_FLOP["propagate"] + 291,
+                        "flop_pca": 413},
+           "cpu_baseline": cpu}
+    log("backward fit: %.3g candidates/s, kernel %.3f ms, %.3f of FP32 peak" % (res["value"], kern_ms,
+                                                                              achieved / PEAK_FP32_TFLOPS))
+    ctx.close()
+    return res
 
 
 def bench_adjoint(args, mk, torch, dev):

@@ -428,6 +428,10 @@ class EventOfHits:
         h = _c_void_p()
         _check(lib().mkf_event_of_hits_create(ctx.h, self.n_layers, self._descs, ctypes.byref(h)))
         self.h = h
+        import weakref
+        if not hasattr(ctx, "_events"):
+            ctx._events = []
+        ctx._events.append(weakref.ref(self))
 
     def close(self):
         if getattr(self, "h", None):
@@ -725,6 +729,11 @@ class Context:
         self.device = int(device)
 
     def close(self):
+        """Destroys the context, after every event of hits made on it that is still open."""
+        for ref in getattr(self, "_events", []):
+            e = ref()
+            if e is not None:
+                e.close()
         if getattr(self, "h", None):
             lib().mkf_context_destroy(self.h)
             self.h = None

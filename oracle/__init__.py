@@ -228,7 +228,7 @@ class _FindEvent(ctypes.Structure):
 class _FindTracks(ctypes.Structure):
     _fields_ = [("n_tracks", ctypes.c_int), ("par", ctypes.c_void_p), ("err", ctypes.c_void_p),
                 ("chg", ctypes.c_void_p), ("chi2", ctypes.c_void_p), ("hot_idx", ctypes.c_void_p),
-                ("n_found", ctypes.c_void_p), ("margins", ctypes.c_void_p)]
+                ("n_found", ctypes.c_void_p), ("margins", ctypes.c_void_p), ("work", ctypes.c_void_p)]
 
 
 def layer_array(descs):
@@ -265,9 +265,10 @@ def select_hit_indices(desc, hits, par, err):
 
 
 def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n_threads=1, prop_to_hit=False,
-                  intra_pflags=0, margins=False):
+                  intra_pflags=0, margins=False, work=False):
     """Best-hit finding over a toy event dict (see mkfit_amd.generate_find_event).  margins=True adds the
-    checker's decision margins [3][n_layers][n] (window edge cm, bin-edge fraction, relative chi2 gap)."""
+    checker's decision margins [3][n_layers][n] (window edge cm, bin-edge fraction, relative chi2 gap);
+    work=True the work units [3][n] (propagations to a layer, window hits evaluated, updates)."""
     L = lib()
     layers = layer_array(descs)
     n = ev["n_seeds"]
@@ -286,7 +287,8 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags,
                     int(bool(prop_to_hit)), int(intra_pflags))
     mg = np.full((3, nl, n), 1e30, np.float32) if margins else None
-    ft = _FindTracks(n, _p(par), _p(err), _p(chg), _p(chi2), _p(hot), _p(nfound), _p(mg))
+    wk = np.zeros((3, n), np.int32) if work else None
+    ft = _FindTracks(n, _p(par), _p(err), _p(chg), _p(chi2), _p(hot), _p(nfound), _p(mg), _p(wk))
     ctx = OrcCtx(float(bfield), None, None)
     rc = L.orc_find_best_hit(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(ft), int(n_threads))
     if rc:
@@ -294,6 +296,8 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     out = dict(par=par, err=err, chg=chg, chi2=chi2, hot_idx=hot, n_found=nfound)
     if margins:
         out["margins"] = mg
+    if work:
+        out["work"] = wk
     return out
 
 
@@ -305,16 +309,17 @@ class _CEParams(ctypes.Structure):
 class _CETracks(ctypes.Structure):
     _fields_ = [("n_seeds", ctypes.c_int)] + [(k, ctypes.c_void_p) for k in (
         "seed_par", "seed_err", "seed_chg", "seed_hits", "n_cands", "par", "err", "chg", "chi2", "score", "n_found",
-        "hot_idx", "margins", "merge_gap")]
+        "hot_idx", "margins", "merge_gap", "work")]
 
 
 def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.8, max_cands=6, max_holes=4,
                       max_consec_holes=1, prop_to_hit=True, intra_pflags=0, n_threads=1, margins=False,
-                      standard=False):
+                      standard=False, work=False):
     """Clone-engine finding over a toy event dict; per seed up to max_cands candidates, best first.  margins=True
     adds the checker's decision margins [4][n_layers][n] (window edge cm, bin-edge fraction, chi2 vs cut,
     score gap) and merge_gap [n].  standard=True runs the standard combinatorial finding
-    (MkBuilder::FindTracksStandard) with the same inputs and outputs."""
+    (MkBuilder::FindTracksStandard) with the same inputs and outputs.  work=True adds the work units [3][n]
+    (candidates propagated to a layer, window hits evaluated, candidates updated with their hit)."""
     L = lib()
     layers = layer_array(descs)
     n = ev["n_seeds"]
@@ -338,8 +343,10 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
     if margins:
         out["margins"] = np.full((4, nl, n), 1e30, np.float32)
         out["merge_gap"] = np.full(n, 1e30, np.float32)
+    if work:
+        out["work"] = np.zeros((3, n), np.int32)
     tr = _CETracks(n, _p(sp), _p(se), _p(sc), _p(sh), *[_p(out.get(k)) for k in (
-        "n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx", "margins", "merge_gap")])
+        "n_cands", "par", "err", "chg", "chi2", "score", "n_found", "hot_idx", "margins", "merge_gap", "work")])
     ctx = OrcCtx(float(bfield), None, None)
     f = L.orc_find_standard if standard else L.orc_find_clone_engine
     rc = f(ctypes.byref(ctx), ctypes.byref(fe), ctypes.byref(pr), ctypes.byref(tr), int(n_threads))

@@ -22,6 +22,7 @@ struct CF {
   CF() : v(0) {}
   CF(double x) : v(x) {}
   explicit operator double() const { return v; }
+  explicit operator float() const { return (float)v; }
   CF& operator+=(CF o) { ++g.add; v += o.v; return *this; }
   CF& operator-=(CF o) { ++g.add; v -= o.v; return *this; }
   CF& operator*=(CF o) { ++g.mul; v *= o.v; return *this; }
@@ -88,6 +89,20 @@ int main() {
     tot.mul = prop.mul + kal.mul; tot.div = prop.div + kal.div; tot.sqrt = prop.sqrt + kal.sqrt;
     tot.trans = prop.trans + kal.trans;
     report(mat ? "barrel_step_material" : "barrel_step", tot);
+  }
+  {
+    // Finding: the chi2 of a hit alone (kalmanPropagateAndComputeChi2's update-less operation) and the update
+    // alone (UpdateWithLastHit / FindCandidates' m_update_param_foo).
+    g = Cnt();
+    kalman_operation(KFO_Calculate_Chi2, outErr, outPar, msErr, msPar, uErr, uPar, chi2);
+    Cnt c2 = g;
+    c2.add += 1; c2.mul += 2; c2.div += 1;
+    report("barrel_kalman_chi2", c2);
+    g = Cnt();
+    kalman_operation(KFO_Update_Params, outErr, outPar, msErr, msPar, uErr, uPar, chi2);
+    Cnt up = g;
+    up.add += 1; up.mul += 2; up.div += 1;
+    report("barrel_kalman_update", up);
   }
   {
     g = Cnt();

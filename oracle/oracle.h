@@ -152,6 +152,9 @@ typedef struct orc_find_tracks {
                      window edge to a hit of the visited bins (cm), nearest bin edge to a window limit or a
                      visited hit's phi (fraction of a bin), smallest relative chi2 gap (best vs other, any vs
                      cut); 1e30 where the layer was not searched */
+  int* work;      /* optional out [3][n]: per track the algorithmic work units of the search -- propagations to a
+                     layer, window hits evaluated (chi2, after a propagation to the hit with prop_to_hit), Kalman
+                     updates -- counted over the layers where the track is live */
 } orc_find_tracks;
 
 int orc_loh_geometry(const orc_layer_desc* d, int* nq, float* qmin, float* qmax, float* fq);
@@ -220,6 +223,9 @@ typedef struct orc_ce_tracks {
                             gap of the scores the selection compares relative to their chi2 sums; 1e30
                             where none */
   float* merge_gap;      /* [n] with margins: the same for the final merge */
+  int* work;             /* optional out [3][n]: per seed the algorithmic work units -- candidates propagated to a
+                            layer, window hits evaluated (propagation to the hit and chi2), candidates updated
+                            with their new hit */
 } orc_ce_tracks;
 
 int orc_find_clone_engine(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, orc_ce_tracks* t,

@@ -385,8 +385,9 @@ inline float score_gap(float a, float ca, float b, float cb) {
 // standard: MkBuilder::FindTracksStandard (MkBuilder.cc:1643-1845) instead of the clone engine: held-back and
 // new candidates go to the seed's tmp_cands, which is sorted by score; the best max_cands without a -2 last
 // hit become the next candidates and the first -2 one in that order competes for the best short candidate.
+// wk (checker only, 3 ints): candidates propagated to a layer, window hits evaluated, candidates updated.
 void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, float* mgap = nullptr,
-               bool standard = false) {
+               bool standard = false, int* wk = nullptr) {
   const orc_find_event& ev = *E.ev;
   const orc_ce_params& P = *E.p;
   const int max_cands = P.max_cands;
@@ -433,6 +434,7 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
       int xs[NN], xa[kHitIdxMax * NN], wsr[NN];
       SelMargins sm[NN];
       select_hit_indices(d, E.loh[layer], B.parP, B.errP, n_proc, xs, xa, wsr, mg ? sm : nullptr);
+      if (wk) wk[0] += n_proc;
       // find_tracks_handle_missed_layers
       for (int i = 0; i < n_proc; ++i) {
         const Cand& c = cc.cands[idx[i]];
@@ -454,6 +456,10 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
         lc[i] = {c.chi2, c.nfound, c.noverlap, c.ninside, c.ntail};
         trk[i] = idx[i];
       }
+      if (wk)
+        for (int i = 0; i < n_proc; ++i)
+          for (int hc = 0; hc < std::min(xs[i], kHitIdxMax); ++hc)
+            wk[1] += xa[hc * NN + i] >= 0 && xa[hc * NN + i] < ev.hit_offset[layer + 1] - ev.hit_offset[layer];
       float c2m[NN];
       if (standard)
         find_candidates_std(E, cc, layer, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B, idx, xs, xa,
@@ -472,6 +478,8 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
       }
     }
     if (standard) {
+      if (wk)
+        for (const Cand& t : tmp) wk[2] += cc.last_idx(t) >= 0 && cc.hots[t.last].layer == layer;
       // sort the seed's tmp_cands (std::sort in the reference: here stable, ties keep generation order), then
       // place the best max_cands without a -2 last hit; the first -2 one met competes for the best short one
       std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
@@ -549,6 +557,7 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
       cc.cands = extras;                                // state is Finding here
     }
     // UpdateWithLastHit over the candidates that took a hit on this layer
+    if (wk) wk[2] += (int)to_update.size();
     for (size_t b0 = 0; b0 < to_update.size(); b0 += NN) {
       std::vector<int> idx(to_update.begin() + b0, to_update.begin() + std::min(to_update.size(), b0 + NN));
       const int n_proc = (int)idx.size();
@@ -634,7 +643,11 @@ int find_batch(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* 
       cc.best_short.score = kScoreWorst;
       std::vector<float> mg(tr->margins ? 4 * (size_t)L : 0, 1e30f);
       float mgap = 1e30f;
-      find_seed(E, cc, ev->first_layer, tr->margins ? mg.data() : nullptr, tr->margins ? &mgap : nullptr, standard);
+      int wk[3] = {0, 0, 0};
+      find_seed(E, cc, ev->first_layer, tr->margins ? mg.data() : nullptr, tr->margins ? &mgap : nullptr, standard,
+                wk);
+      if (tr->work)
+        for (int k = 0; k < 3; ++k) tr->work[(size_t)k * n + s] = wk[k];
       if (tr->margins) {
         for (int k = 0; k < 4; ++k)
           for (int l = 0; l < L; ++l) tr->margins[((size_t)k * L + l) * n + s] = mg[(size_t)k * L + l];

@@ -305,7 +305,10 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
       }
       alignas(64) float parP[6 * NN], errP[21 * NN], rad[NN];
       int xs[NN], xa[kHitIdxMax * NN], wsr[NN], fail[NN];
+      int wk[3][NN] = {};                                         // checker: work units per lane
       for (int layer = ev->first_layer; layer < ev->n_layers; ++layer) {
+        const int n_layer_hits = ev->hit_offset[layer + 1] - ev->hit_offset[layer];
+        for (int i = 0; i < cnt; ++i) wk[0][i] += !B.stopped[i];
         const orc_layer_desc& d = ev->layers[layer];
         for (int l = 0; l < NN; ++l) rad[l] = d.propagate_to;
         if (d.is_barrel)
@@ -335,6 +338,12 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
             }
           }
         }
+        for (int i = 0; i < cnt; ++i) {
+          if (B.stopped[i] || wsr[i] == WSR_Outside) continue;
+          for (int hc = 0; hc < std::min(xs[i], kHitIdxMax); ++hc)
+            wk[1][i] += xa[hc * NN + i] >= 0 && xa[hc * NN + i] < n_layer_hits;
+          ++wk[2][i];
+        }
         float c2m[NN];
         add_best_hit(E, layer, ev->chi2_cut, B, parP, errP, xs, xa, wsr, cnt, tr->margins ? c2m : nullptr);
         if (tr->margins)
@@ -363,6 +372,8 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
         tr->chg[t] = S.chg[i];
         tr->chi2[t] = S.chi2[i];
         tr->n_found[t] = S.nfound[i];
+        if (tr->work)
+          for (int k = 0; k < 3; ++k) tr->work[(size_t)k * n + t] = wk[k][i];
         for (int k = 0; k < ev->n_layers; ++k) tr->hot_idx[(size_t)k * n + t] = -9;
         for (int k = 0; k < S.nhits[i] && k < kMaxTrkHits; ++k) {
           const int ly = S.hot_lyr[k][i];

@@ -77,7 +77,8 @@ FIND_MUTANTS = {
                        "lc[i].ntail <= P.max_consec_holes) ? -1 : -2;"),
     "fake_holes_inside_only": ("oracle_ce.cc", "(lc[i].ninside + lc[i].ntail < P.max_holes",
                                "(lc[i].ninside < P.max_holes"),
-    "fake_edge_code": ("oracle_ce.cc", "if (wsr[i] == WSR_Edge) fake = -3;", "if (wsr[i] == WSR_Edge) fake = -1;"),
+    "fake_edge_code": ("oracle_ce.cc", "if (wsr[i] == WSR_Edge) fake = -3;\n    IdxChi2 e;",
+                       "if (wsr[i] == WSR_Edge) fake = -1;\n    IdxChi2 e;"),
     "fake_tailholes": ("oracle_ce.cc", "e.ntailholes = fake == -1 ? lc[i].ntail + 1 : lc[i].ntail;",
                        "e.ntailholes = lc[i].ntail + 1;"),
     "hit_record_holes": ("oracle_ce.cc", "e.nholes = lc[i].ninside + lc[i].ntail;", "e.nholes = lc[i].ninside;"),
@@ -94,7 +95,7 @@ FIND_MUTANTS = {
     "merge_no_rescore": ("oracle_ce.cc", "    for (auto& c : cc.cands) c.score = score_cand(c);\n", ""),
     "apex_pt": ("oracle_ce.cc", "if (c.pT() < 1.2) {", "if (c.pT() < 0.6) {"),
     "apex_no_dphi_test": ("oracle_ce.cc", "625.f && dphi > 1.371f && dphi < 4.512f", "625.f"),
-    "barrel_miss_not_stopped": ("oracle_ce.cc", "cc.add_hit(extras.back(), -2, layer, 0);", ""),
+    "barrel_miss_not_stopped": ("oracle_ce.cc", "cc.add_hit(held.back(), -2, layer, 0);", ""),
     "update_no_charge_flip": ("oracle_ce.cc", "      chg[n] = -chg[n];\n", ""),
     "bh_chi2_le": ("oracle_find.cc", "if (c2 < minChi2[it]) {", "if (c2 <= minChi2[it]) {"),
     "bh_fake_inside_only": ("oracle_find.cc", "else if (B.ninside[it] + B.ntail[it])", "else if (B.ninside[it])"),
@@ -133,6 +134,8 @@ STD_MUTANTS = {
     "std_best_short_always": ("if (tc.score > cc.best_short.score) cc.best_short = tc;", "cc.best_short = tc;"),
     "std_held_dropped": ("std::vector<Cand>& held = standard ? tmp : extras;", "std::vector<Cand>& held = extras;"),
     "std_fake_consec_le": ("c.ntail < P.max_consec_holes) ? -1 : -2;", "c.ntail <= P.max_consec_holes) ? -1 : -2;"),
+    "std_fake_edge_code": ("if (wsr[i] == WSR_Edge) fake = -3;\n    Cand nc = c;",
+                           "if (wsr[i] == WSR_Edge) fake = -1;\n    Cand nc = c;"),
     "std_sort_ascending": ("std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });",
                            "std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score < b.score; });"),
 }
