@@ -418,6 +418,7 @@ int mkf_event_of_hits_create(mkf_context* c, int n_layers, const mkf_layer_desc*
   }
   auto* h = new mkf_event_of_hits();
   h->ctx = c;
+  h->device = c->device;
   h->n_layers = n_layers;
   h->layers = L;
   h->total_bins = bin_off;
@@ -436,7 +437,7 @@ int mkf_event_of_hits_create(mkf_context* c, int n_layers, const mkf_layer_desc*
 
 int mkf_event_of_hits_destroy(mkf_event_of_hits* h) {
   if (!h) return MKF_OK;
-  (void)hipSetDevice(h->ctx->device);
+  (void)hipSetDevice(h->device);
   (void)hipDeviceSynchronize();
   free_hit_arrays(h);
   if (h->d_ce_nodes) (void)hipFree(h->d_ce_nodes);

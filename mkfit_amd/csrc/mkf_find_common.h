@@ -217,6 +217,7 @@ __device__ __forceinline__ bool strip_q_compatible(bool barrel, const float (&pe
 
 struct mkf_event_of_hits {
   mkf_context* ctx = nullptr;
+  int device = 0;                 // the context's device, kept so that destroy does not read the context
   int n_layers = 0;
   std::vector<mkf_find::LayerDev> layers;   // host copy; hit_off / n_hits as of the last load
   mkf_find::LayerDev* d_layers = nullptr;

@@ -79,6 +79,68 @@ def test_loh_build_matches_oracle(setup):
     assert flips <= max(5, 1e-4 * off[-1]), (flips, off[-1])
 
 
+def _crowded_event(descs, seed=5):
+    """Hits of four barrel layers piled into few (q, fine phi) bins, in shuffled order: 10,000 copies of one
+    hit (beyond the LDS sort), 3,000 of another (sorted in LDS), 150 positions x 40 copies (one thread's
+    share exceeded many times), and hits at phi = +pi exactly (fine bin 1024, sorting last in its q row but
+    filling coarse bin 0) and -pi, each layer with random hits among them."""
+    rng = np.random.default_rng(seed)
+
+    def hit(r, phi, z):
+        return [r * np.cos(phi), r * np.sin(phi), z, 1e-4, 0.0, 1e-4, 0.0, 0.0, 1e-2]
+
+    def rand(d, n):
+        return [hit(d["propagate_to"], rng.uniform(-np.pi, np.pi), rng.uniform(d["zmin"], d["zmax"]))
+                for _ in range(n)]
+    layers = []
+    d = descs[0]
+    layers.append([hit(d["propagate_to"], 0.3, 1.0)] * 10000 + rand(d, 500))
+    d = descs[1]
+    layers.append([hit(d["propagate_to"], -2.0, -3.0)] * 3000 + rand(d, 200))
+    d = descs[2]
+    pos = [(rng.uniform(-np.pi, np.pi), rng.uniform(d["zmin"], d["zmax"])) for _ in range(150)]
+    layers.append([hit(d["propagate_to"], p, z) for p, z in pos for _ in range(40)] + rand(d, 100))
+    d = descs[3]
+    r = d["propagate_to"]
+    plus = [[-r, 0.0, 2.0, 1e-4, 0.0, 1e-4, 0.0, 0.0, 1e-2]] * 50          # atan2(+0, -r) = +pi
+    minus = [[-r, -0.0, 2.0, 1e-4, 0.0, 1e-4, 0.0, 0.0, 1e-2]] * 50        # atan2(-0, -r) = -pi
+    near = [hit(r, -np.pi + 1e-3, 2.0)] * 30                                # coarse bin 0 of the same row
+    layers.append(plus + minus + near + rand(d, 100))
+    out = []
+    for h in layers:
+        h = np.array(h, np.float32)
+        out.append(h[rng.permutation(len(h))])
+    return out
+
+
+def test_loh_crowded_bins(gpu_device):
+    """The LayerOfHits build keeps the reference's stable order and bin ranges however the hits crowd."""
+    geo = mk.geometry(1)
+    descs = mk.barrel_layer_descs(geo)[:4]
+    layers = _crowded_event(descs)
+    off = np.concatenate([[0], np.cumsum([len(h) for h in layers])]).astype(np.int32)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(off, _t(np.concatenate(layers), gpu_device))
+    for l, hl in enumerate(layers):
+        r_c, b_c, p_c, q_c, nq_c = oracle.loh_build(descs[l], hl)
+        r_g, b_g, p_g, q_g, nq_g = eoh.layer_views(l)
+        phi_c = np.empty_like(p_c); phi_c[r_c] = p_c
+        phi_g = np.empty_like(p_g); phi_g[r_g] = p_g
+        k_c, _ = _keys(descs[l], phi_c, hl[:, 2])
+        k_g, _ = _keys(descs[l], phi_g, hl[:, 2])
+        assert np.array_equal(np.argsort(k_g, kind="stable"), r_g), l
+        if np.array_equal(k_c, k_g):
+            assert np.array_equal(r_g, r_c), l
+            assert np.array_equal(b_g, b_c), l
+        else:
+            assert np.all(_fine_boundary(phi_c[k_c != k_g])), l
+    # fine bin 1024: the +pi hits own coarse bin 0 of their q row, after the row's other hits
+    r_g, b_g, p_g, _, _ = eoh.layer_views(3)
+    assert np.sum(p_g == np.float32(np.pi)) == 50
+    ctx.close()
+
+
 def test_loh_reload_smaller_event(setup, gpu_device):
     ev, descs, ctx, eoh = setup
     geo, ev2, _ = _event(7, 300, 0.5)
