@@ -1,22 +1,33 @@
-// fit_shim_demo.cc -- drives the C++ interface (include/mkfit_amd.hpp) the way
-// a reference caller drives MkFitter: pack Track / Hit records, fit, unpack.
-// Generates a toy event with the toy MC library, fits it on the GPU and writes
-// par [6][n], err [21][n], chi2 [n] (float32) to the output file.
-//   usage: fit_shim_demo OUT N_TRACKS SEED [plain]
+// fit_shim_demo.cc -- a caller written against the reference's records and
+// MkFitter (include/mkfit_amd.hpp), shaped like runFittingTestPlex
+// (fittestMPlex.cc:46-105): an Event whose simTracks_ carry their HitOnTrack
+// lists into ev.layerHits_, fitted with
+//   (default)  InputTracksForFit + FitTracksWithInterSlurp + OutputFittedTracks
+//              over the whole range in one call
+//   plain      the same with one launch per hit slot
+//   blocks     InputTracksAndHits + FitTracks + OutputFittedTracks per block
+//              of 16 tracks, as the reference's Matriplex loop
+//   steered    FitTracksSteered with every slot marked barrel
+// Writes par [6][n], err [21][n], chi2 [n] (float32) to the output file.
+//   usage: fit_shim_demo OUT N_TRACKS SEED [plain|blocks|steered]
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "mkfit_amd.hpp"
 #include "mkfit_toymc.h"
 
+using namespace mkfit_amd;
+
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: %s OUT N_TRACKS SEED [plain]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s OUT N_TRACKS SEED [plain|blocks|steered]\n", argv[0]);
     return 2;
   }
   const int n = std::atoi(argv[2]);
-  const bool plain = argc > 4;
+  const char* mode = argc > 4 ? argv[4] : "";
   mkf_geometry geo;
   mkf_toymc_geometry_preset(1, &geo);
   mkf_toymc_params p;
@@ -33,37 +44,67 @@ int main(int argc, char** argv) {
   std::vector<float> rl(300 * 120), xi(300 * 120);
   mkf_toymc_material_grid(&geo, rl.data(), xi.data());
 
-  // Reference-shaped records: layer l's hits, each track's HitOnTrack list.
-  std::vector<std::vector<mkfit_amd::Hit>> layer_hits(S, std::vector<mkfit_amd::Hit>(n));
-  std::vector<mkfit_amd::Track> tracks(n);
+  // The event as the reference holds it: track t's hit on layer l is
+  // ev.layerHits_[l][t], named by its HitOnTrack list.
+  Event ev;
+  ev.layerHits_.assign(S, HitVec(n));
   for (int t = 0; t < n; ++t) {
-    for (int e = 0; e < 6; ++e) tracks[t].state.par[e] = sp[(size_t)e * n + t];
-    for (int e = 0; e < 21; ++e) tracks[t].state.err[e] = se[(size_t)e * n + t];
-    tracks[t].state.charge = chg[t];
+    SVector6 par;
+    SMatrixSym66 err;
+    for (int e = 0; e < 6; ++e) par[e] = sp[(size_t)e * n + t];
+    for (int e = 0; e < 21; ++e) err.Array()[e] = se[(size_t)e * n + t];
+    Track trk(TrackState(par, err, chg[t]), 0.f, t, 0, nullptr);
     for (int s = 0; s < nh[t]; ++s) {
-      mkfit_amd::Hit& h = layer_hits[s][t];
-      for (int e = 0; e < 3; ++e) h.pos[e] = hp[((size_t)s * 3 + e) * n + t];
-      for (int e = 0; e < 6; ++e) h.err[e] = he[((size_t)s * 6 + e) * n + t];
-      tracks[t].hots.push_back({t, s});
+      Hit& h = ev.layerHits_[s][t];
+      for (int e = 0; e < 3; ++e) h.pos_[e] = hp[((size_t)s * 3 + e) * n + t];
+      for (int e = 0; e < 6; ++e) h.err_.Array()[e] = he[((size_t)s * 6 + e) * n + t];
+      trk.addHitIdx(t, s, 0.f);
     }
+    ev.simTracks_.push_back(trk);
   }
+  Config::forward_fit_pflags = PropagationFlags(PF_apply_material);
+  TrackVec rectracks(n);
   try {
-    mkfit_amd::Context ctx(p.bfield, rl.data(), xi.data());
-    mkfit_amd::MkFitter fitter(ctx);
-    fitter.InputTracksForFit(tracks, layer_hits, std::vector<bool>(S, false), 0, n);
-    fitter.FitTracks(MKF_PF_APPLY_MATERIAL, MKF_FIT_CHI2, !plain);
-    fitter.OutputFittedTracks(tracks, 0, n);
+    Context ctx(p.bfield, rl.data(), xi.data());
+    MkFitter mkfp(ctx);
+    mkfp.SetFitFlags(MKF_FIT_CHI2);
+    if (!std::strcmp(mode, "blocks")) {
+      const int NN = 16;
+      mkfp.SetNhits(S);
+      for (int itrack = 0; itrack < n; itrack += NN) {
+        const int end = std::min(itrack + NN, n);
+        mkfp.InputTracksAndHits(ev.simTracks_, ev.layerHits_, itrack, end);
+        mkfp.FitTracks(end - itrack, &ev, Config::forward_fit_pflags);
+        mkfp.OutputFittedTracks(rectracks, itrack, end);
+      }
+    } else if (!std::strcmp(mode, "steered")) {
+      bool is_brl[64];
+      std::fill(is_brl, is_brl + 64, true);
+      mkfp.InputTracksForFit(ev.simTracks_, 0, n);
+      mkfp.FitTracksSteered(is_brl, n, &ev, Config::forward_fit_pflags);
+      mkfp.OutputFittedTracks(rectracks, 0, n);
+    } else {
+      mkfp.SetBatched(std::strcmp(mode, "plain") != 0);
+      mkfp.InputTracksForFit(ev.simTracks_, 0, n);
+      mkfp.FitTracksWithInterSlurp(ev.layerHits_, n);
+      mkfp.OutputFittedTracks(rectracks, 0, n);
+    }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "%s\n", e.what());
     return 1;
   }
+  for (int t = 0; t < n; ++t)
+    if (rectracks[t].label() != t) {
+      std::fprintf(stderr, "label of track %d not carried through\n", t);
+      return 5;
+    }
   std::FILE* f = std::fopen(argv[1], "wb");
   if (!f) return 4;
   std::vector<float> out(28 * (size_t)n);
   for (int t = 0; t < n; ++t) {
-    for (int e = 0; e < 6; ++e) out[(size_t)e * n + t] = tracks[t].state.par[e];
-    for (int e = 0; e < 21; ++e) out[(size_t)(6 + e) * n + t] = tracks[t].state.err[e];
-    out[(size_t)27 * n + t] = tracks[t].chi2;
+    for (int e = 0; e < 6; ++e) out[(size_t)e * n + t] = rectracks[t].parameters()[e];
+    for (int e = 0; e < 21; ++e) out[(size_t)(6 + e) * n + t] = rectracks[t].errors().Array()[e];
+    out[(size_t)27 * n + t] = rectracks[t].chi2();
   }
   std::fwrite(out.data(), sizeof(float), out.size(), f);
   std::fclose(f);
