@@ -255,7 +255,9 @@ int mkf_event_of_hits_load(mkf_event_of_hits* eoh, const int* hit_offset, const 
 
 /* Device views of a loaded layer (valid until the next load or destroy):
  * ranks [n] sorted position -> hit index within the layer (LayerOfHits
- * m_hit_ranks), bins [nq*256][2] first/end per (q bin, phi bin) (m_phi_bin_infos),
+ * m_hit_ranks), bins [nq*256][2] first/end per (q bin, phi bin) (m_phi_bin_infos;
+ * a bin without hits is empty, and in phi bins 1..255 holds {p, p} with p the
+ * position where its hits would start, so those bins tile their q row),
  * phis/qs [n] per sorted hit (m_hit_phis, m_hit_qs). */
 int mkf_event_of_hits_layer(mkf_event_of_hits* eoh, int layer, int* n_hits, int* nq, const int** ranks,
                             const int** bins, const float** phis, const float** qs);

@@ -165,9 +165,25 @@ k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* 
   const bool last = i == total - 1 || (keys_s[i + 1] & ~3u) != jq;
   const int qb = (int)((key >> kQShift) & (kMaxQBins - 1));
   const int pb = (int)((key & 0x3ffu) >> 2);
-  int2* b = bins + d.bin_off + qb * kNPhi + pb;
-  if (first) b->x = local;
-  if (last) b->y = local + 1;
+  int2* row = bins + d.bin_off + qb * kNPhi;
+  if (first) row[pb].x = local;
+  if (last) row[pb].y = local + 1;
+  if (pass == 1) return;
+  // Empty bins 1..255 of a row hold {p, p}, p the position where their hits
+  // would start, so bins b..b' (1 <= b <= b' <= 255) of a row span the sorted
+  // positions [row[b].x, row[b'].y) and a window's run of them is read with
+  // two loads.  Bin 0 is left out: the phi == +pi run may replace its range.
+  const uint32_t rowkey = key >> kQShift;
+  if (first && pb > 1 && (i == 0 || (keys_s[i - 1] >> kQShift) != rowkey))
+    for (int b = 1; b < pb; ++b) row[b] = make_int2(local, local);
+  if (last) {
+    int nb = kNPhi;
+    if (i + 1 < total) {
+      const uint32_t nk = keys_s[i + 1];
+      if ((nk >> kQShift) == rowkey && !(nk & 0x400u)) nb = (int)((nk & 0x3ffu) >> 2);
+    }
+    for (int b = max(pb + 1, 1); b < nb; ++b) row[b] = make_int2(local + 1, local + 1);
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -222,9 +238,10 @@ struct FindArgs {
 // state the reference outputs for it (OutputTrackAndHitIdx at stop time).
 // HitOnTrack bookkeeping as MkFinder::add_hit (MkFinder.h:312-354); with at
 // most MKF_MAX_FIND_LAYERS layers a track never exceeds Config::nMaxTrkHits.
-// Three waves per SIMD (168 VGPRs, a few spills to scratch) run this latency-bound
-// loop 7 % faster than the two that its unconstrained 212 VGPRs allow.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_find_best_hit(FindArgs a) {
+// Two waves per SIMD: 2.33 ms per 50 events, against 2.45 ms at three waves (168 VGPRs, 94 spilled)
+// and 2.70 ms at four.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) k_find_best_hit(FindArgs a) {
+  __shared__ int s_cand[kHitIdxMax][256];     // each thread's window hits of the layer, in scan order
   const int stride = gridDim.x * blockDim.x;
   const size_t n = (size_t)a.n;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.n; t += stride) {
@@ -262,7 +279,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
       if (w.wsr != MKF_WSR_OUTSIDE) {
         float min_c2 = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
         int best = -1;
-        scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) {
+        // The window's hits first (the scan keeps several loads in flight), then
+        // their chi2 in scan order, one copy of the evaluation.
+        int nc = 0;
+        scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) { s_cand[nc++][threadIdx.x] = gi; });
+        for (int hc = 0; hc < nc; ++hc) {
+          const int gi = s_cand[hc][threadIdx.x];
           float hp[3], he[6];
           load_hit9(a.shits + (size_t)gi * 9, hp, he);
           float q[6], qe[21];
@@ -282,7 +304,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
             min_c2 = c2;
             best = gi;
           }
-        });
+        }
         if (best >= 0) {
           load_hit9(a.shits + (size_t)best * 9, mp, me);
           chi2 += min_c2;

@@ -138,22 +138,56 @@ __device__ __forceinline__ void hit_window(const LayerDev& d, const float (&par)
 // Scan of the window's bins in the reference's order (q bins outer, phi bins
 // inner, sorted hits within a bin), at most kHitIdxMax accepted hits; calls
 // on_hit(global sorted position) for each (MkFinder.cc:433-684).
+//
+// Within a q row the hits of phi bins 1..255 are stored in bin order, and the
+// bin table gives empty bins their insertion point, so each run of the
+// window's bins that avoids bin 0 is one range [row[b].x, row[b'].y) read with
+// two independent loads; bin 0 (whose range the phi == +pi run may replace)
+// is read on its own.  The hits visited, and their order, are the bin-by-bin
+// scan's.
+constexpr int kScanUnroll = 8;
 template <class F>
 __device__ __forceinline__ int scan_window(const LayerDev& d, const Window& w, const int2* __restrict__ bins,
                                            const float* __restrict__ phis, const float* __restrict__ qs, F&& on_hit) {
   int cnt = 0;
-  for (int qi = w.qb1; qi < w.qb2 && cnt < kHitIdxMax; ++qi)
-    for (int pi = w.pb1; pi < w.pb2 && cnt < kHitIdxMax; ++pi) {
-      const int2 be = bins[d.bin_off + qi * kNPhi + (pi & 0xff)];
-      for (int hi = be.x; hi < be.y && cnt < kHitIdxMax; ++hi) {
-        const int gi = d.hit_off + hi;
-        const float ddq = fabsf(w.q - qs[gi]);
-        const float ddphi = cdist(fabsf(w.phi - phis[gi]));
-        if (ddq >= w.dq || ddphi >= w.dphi) continue;
-        on_hit(gi);
-        ++cnt;
+  for (int qi = w.qb1; qi < w.qb2 && cnt < kHitIdxMax; ++qi) {
+    const int2* __restrict__ row = bins + d.bin_off + qi * kNPhi;
+    for (int pi = w.pb1; pi < w.pb2 && cnt < kHitIdxMax;) {
+      const int b = pi & 0xff;
+      int lo, hi;
+      if (b == 0) {
+        const int2 be = row[0];
+        lo = be.x;
+        hi = be.y;
+        ++pi;
+      } else {
+        const int b2 = min(kNPhi - 1, b + (w.pb2 - 1 - pi));
+        lo = row[b].x;
+        hi = row[b2].y;
+        pi += b2 - b + 1;
+      }
+      // Eight hits' (q, phi) loads in flight at once, tested in order.
+      for (int h = lo; h < hi && cnt < kHitIdxMax; h += kScanUnroll) {
+        const int gi = d.hit_off + h;
+        float hq[kScanUnroll], hp[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+          const bool in = h + u < hi;
+          hq[u] = in ? qs[gi + u] : 0.f;
+          hp[u] = in ? phis[gi + u] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; ++u) {
+          if (h + u >= hi || cnt >= kHitIdxMax) break;
+          const float ddq = fabsf(w.q - hq[u]);
+          const float ddphi = cdist(fabsf(w.phi - hp[u]));
+          if (ddq >= w.dq || ddphi >= w.dphi) continue;
+          on_hit(gi + u);
+          ++cnt;
+        }
       }
     }
+  }
   return cnt;
 }
 

@@ -73,6 +73,19 @@ def _brute_select(desc, h, par, err, ranks, phis, qs):
 
 
 
+def assert_bins(b_g, b_c, tag=None):
+    """The device bin table against the oracle's (the reference's m_phi_bin_infos): every bin with hits has
+    the same [first, end); a bin without hits is empty on the device too, and there bins 1..255 of a row
+    hold their insertion point, so that those bins tile the row (bin b ends where bin b + 1 starts) and a
+    run of them is one contiguous range."""
+    g = b_g.reshape(-1, 256, 2)
+    c = b_c.reshape(-1, 256, 2)
+    full = c[..., 1] > c[..., 0]
+    assert np.array_equal(g[full], c[full]), tag
+    assert np.all(g[~full][:, 0] == g[~full][:, 1]), tag
+    assert np.all(g[:, 1:255, 1] == g[:, 2:256, 0]), tag
+
+
 def edge_explained(desc_win, par_t, h, diff, eps=2e-5):
     """True if every hit in `diff` lies within eps of the track's dq / dphi window edge."""
     dq, dphi = desc_win
