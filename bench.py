@@ -76,11 +76,12 @@ def parse():
     ap.add_argument("--shape", type=str, default=None,
                     help="block,blocks_per_cu,variant[,sort_tile] for the batched kernel (skips the autotuner)")
     ap.add_argument("--workload", default="fit-uneven",
-                    choices=["fit-uneven", "fit", "find-ce", "find-bh", "find-std", "adjoint"],
+                    choices=["fit-uneven", "fit", "find-ce", "find-bh", "find-std", "bkfit", "adjoint"],
                     help="fit-uneven (default, the headline): 2M tracks, barrel + 2x9 disks, 8-30 hits, 5%% "
                          "missing, FP64 fallback on; fit: plain-fit-at-scale, 1M tracks, 25 barrel layers; "
                          "find-ce / find-std / find-bh: clone-engine / standard / best-hit finding of --events "
-                         "events; adjoint: gradient of the fit's chi2")
+                         "events; bkfit: backward fit of the clone engine's best candidates of --events events "
+                         "(one untimed finding pass first); adjoint: gradient of the fit's chi2")
     ap.add_argument("--events", type=int, default=50,
                     help="events per step of the find workloads (all in one event of hits and one launch)")
     ap.add_argument("--unordered", action="store_true",
@@ -131,6 +132,12 @@ def main():
         res = bench_find(args, mk, torch, dev, args.workload[5:])
         res.pop("_out")
         print(json.dumps(res), flush=True)
+        return
+    if args.workload == "bkfit":
+        a1 = copy.copy(args)
+        a1.steps, a1.warmup, a1.no_cpu = 1, 0, True
+        ce = bench_find(a1, mk, torch, dev, "ce")["_out"]
+        print(json.dumps(bench_bkfit(args, mk, torch, dev, ce)), flush=True)
         return
     if args.workload == "adjoint":
         return bench_adjoint(args, mk, torch, dev)

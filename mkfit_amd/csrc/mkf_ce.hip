@@ -47,10 +47,11 @@ constexpr int kSMax = 8;              // candidate slots per seed, at most
 constexpr int kLanes = 64;            // one wavefront per workgroup
 constexpr int kW = 37;                // LDS words per exchanged candidate (36 used, odd stride)
 // Words of a lane's X row during (a), beside the propagated state in words 0-27:
-// chi2 cut, exclusive prefix of the window hit counts, layer hit offset and
-// barrel flag of the lane's layer.  Word 36, which put / get leave alone, holds
+// chi2 cut, exclusive prefix of the window hit counts, layer hit offset,
+// barrel flag of the lane's layer and, in words 32-35, sin / cos of the
+// propagated phi and theta.  Word 36, which put / get leave alone, holds
 // the score of the lane's invalid-hit record from (a) to the new candidates (c).
-constexpr int kXCut = 28, kXPre = 29, kXOff = 30, kXBar = 31, kXScore = 36;
+constexpr int kXCut = 28, kXPre = 29, kXOff = 30, kXBar = 31, kXTrig = 32, kXScore = 36;
 // A kept accepted-hit record is {hit_cnt << kPosBits | position in the layer}.
 constexpr int kPosBits = 28;
 constexpr int kPosMask = (1 << kPosBits) - 1;
@@ -167,11 +168,11 @@ __device__ __forceinline__ int append_node(const CeArgs& a, int s, int* cnt, int
 }
 
 __device__ __forceinline__ void propagate(bool barrel, float (&p)[6], float (&e)[21], int chg, float target,
-                                          int pflags, const CeArgs& a) {
+                                          int pflags, const CeArgs& a, const float* trig = nullptr) {
   if (barrel)
-    propagate_to_r(p, e, chg, target, pflags, false, a.bfield, a.mat);
+    propagate_to_r(p, e, chg, target, pflags, false, a.bfield, a.mat, trig);
   else
-    propagate_to_z(p, e, chg, target, pflags, false, a.bfield, a.mat);
+    propagate_to_z(p, e, chg, target, pflags, false, a.bfield, a.mat, trig);
 }
 
 // KS candidate slots per seed (max_cands <= KS): KS = 6 packs 10 seeds into the
@@ -309,6 +310,10 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(2))
         xs[kXCut] = dyn_chi2_cut(d, a.chi2_cut, pp[3], pp[5]);
         xs[kXOff] = __int_as_float(d.hit_off);
         xs[kXBar] = __int_as_float(barrel ? 1 : 0);
+        if (a.prop_to_hit) {                             // sin, cos of phi and theta for every pair's propagation
+          sincosf(pp[4], &xs[kXTrig], &xs[kXTrig + 1]);
+          sincosf(pp[5], &xs[kXTrig + 2], &xs[kXTrig + 3]);
+        }
         int fake = (st.ninside + st.ntail < a.max_holes && st.ntail < a.max_consec) ? -1 : -2;
         if (w.wsr == MKF_WSR_EDGE) fake = -3;
         const int nt = fake == -1 ? st.ntail + 1 : st.ntail;
@@ -345,7 +350,8 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(2))
 #pragma unroll
       for (int e = 0; e < 21; ++e) qe[e] = pe[e] = xs[6 + e];
       if (a.prop_to_hit)
-        propagate(ob, q, qe, __float_as_int(xs[27]), ob ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a);
+        propagate(ob, q, qe, __float_as_int(xs[27]), ob ? hypot_ref(mp[0], mp[1]) : mp[2], a.intra_pflags, a,
+                  &xs[kXTrig]);
       const float c2 = fabsf(ob ? kalman_barrel<true, false>(q, qe, mp, me)
                                 : kalman_endcap<true, false>(q, qe, mp, me));
       const bool acc = c2 < xs[kXCut] && strip_q_compatible(ob, pe, q, mp, me);

@@ -118,7 +118,9 @@ struct JacZ {
 // and dDdphi brackets, and the small differences in the Jacobian, keep the
 // reference's term order: their terms nearly cancel once the step has
 // converged, and the order fixes the rounding.
-__device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, float bfield, bool param_b, JacR& J) {
+// trig, when given: {sin, cos} of the input phi and theta, as sincosf gives them.
+__device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, float bfield, bool param_b, JacR& J,
+                                           const float* trig = nullptr) {
   const float xin = par[0], yin = par[1], zin = par[2];
   const float ipt = par[3], phiin = par[4], theta = par[5];
   float r0 = hipo(xin, yin);
@@ -130,7 +132,12 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
 
   float D = 0.f, cosa = 0.f, sina = 0.f, cosah = 0.f, sinah = 0.f, id = 0.f;
   float cosP, sinP;
-  sincosf(phiin, &sinP, &cosP);
+  if (trig) {
+    sinP = trig[0];
+    cosP = trig[1];
+  } else {
+    sincosf(phiin, &sinP, &cosP);
+  }
   float pxin = cosP * pt;
   float pyin = sinP * pt;
 
@@ -210,7 +217,12 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
   J.j14 = kp * (sinP * dadphi * cosa + cosP * dadphi * sina + sinP * cosa + cosP * sina - sinP);
 
   float cosT, sinT;
-  sincosf(theta, &sinT, &cosT);
+  if (trig) {
+    sinT = trig[2];
+    cosT = trig[3];
+  } else {
+    sincosf(theta, &sinT, &cosT);
+  }
   const float isinT = frcp(sinT);
   J.isin_t = isinT;
   J.cos_t = cosT;
@@ -233,15 +245,23 @@ __device__ __forceinline__ bool helix_at_r(float (&par)[6], int chg, float r, fl
 }
 
 // ---- helix to plane z (helixAtZ) --------------------------------------------
-__device__ __forceinline__ void helix_at_z(float (&par)[6], int chg, float zout, float bfield, bool param_b, JacZ& J) {
+__device__ __forceinline__ void helix_at_z(float (&par)[6], int chg, float zout, float bfield, bool param_b, JacZ& J,
+                                           const float* trig = nullptr) {
   const float zin = par[2], ipt = par[3], phiin = par[4], theta = par[5];
   const float bf = param_b ? bfield_from_zr(zin, hipo(par[0], par[1])) : bfield;
   const float k = fdiv(float(chg) * 100.f, -kSol * bf);
   const float kinv = frcp(k);
   const float pt = frcp(ipt);
   float cosP, sinP, cosT, sinT;
-  sincosf(phiin, &sinP, &cosP);
-  sincosf(theta, &sinT, &cosT);
+  if (trig) {
+    sinP = trig[0];
+    cosP = trig[1];
+    sinT = trig[2];
+    cosT = trig[3];
+  } else {
+    sincosf(phiin, &sinP, &cosP);
+    sincosf(theta, &sinT, &cosT);
+  }
   J.isin_t = frcp(sinT);
   J.cos_t = cosT;
   const float tanT = fdiv(sinT, cosT);
@@ -401,12 +421,13 @@ __device__ __forceinline__ void similarity_z(const JacZ& J, float (&C)[21]) {
 // ---- propagateHelixToRMPlex, one lane ---------------------------------------
 // On failure the state is left as it was (the reference resets output to input).
 __device__ __forceinline__ bool propagate_to_r(float (&par)[6], float (&err)[21], int chg, float r, int pflags,
-                                               bool no_mat, float bfield, const Material& mat) {
+                                               bool no_mat, float bfield, const Material& mat,
+                                               const float* trig = nullptr) {
   float out[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) out[i] = par[i];
   JacR J;
-  const bool fail = helix_at_r(out, chg, r, bfield, pflags & PF_PARAM_B, J);
+  const bool fail = helix_at_r(out, chg, r, bfield, pflags & PF_PARAM_B, J, trig);
   if (fail) return true;
   if (pflags & PF_MATERIAL) {
     float rl = 0.f, xi = 0.f;
@@ -424,10 +445,11 @@ __device__ __forceinline__ bool propagate_to_r(float (&par)[6], float (&err)[21]
 
 // ---- propagateHelixToZMPlex, one lane ---------------------------------------
 __device__ __forceinline__ void propagate_to_z(float (&par)[6], float (&err)[21], int chg, float zout, int pflags,
-                                               bool no_mat, float bfield, const Material& mat) {
+                                               bool no_mat, float bfield, const Material& mat,
+                                               const float* trig = nullptr) {
   const float zin = par[2];
   JacZ J;
-  helix_at_z(par, chg, zout, bfield, pflags & PF_PARAM_B, J);
+  helix_at_z(par, chg, zout, bfield, pflags & PF_PARAM_B, J, trig);
   if (pflags & PF_MATERIAL) {
     float rl = 0.f, xi = 0.f;
     if (!no_mat) lookup_material(mat, zout, fhypot(par[0], par[1]), rl, xi);

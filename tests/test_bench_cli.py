@@ -17,5 +17,5 @@ def test_workloads_listed():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 0
-    for w in ("fit-uneven", "find-ce", "find-bh", "adjoint"):
+    for w in ("fit-uneven", "find-ce", "find-std", "find-bh", "bkfit", "adjoint"):
         assert w in r.stdout
