@@ -1,9 +1,11 @@
 """An event read back from a binary event file drives the GPU finders to the same result as the
-event it was written from."""
+event it was written from, and that result is the oracle's on the read-back event (best hit and clone
+engine, seed by seed with the decision-margin explanations of tests/_find_common.py)."""
 import numpy as np
 import pytest
 
 import mkfit_amd as mk
+from _find_common import assert_ce_parity, assert_find_parity, best_hit_reference, clone_engine_reference
 from mkfit_amd import event_file as ef
 
 pytestmark = pytest.mark.gpu
@@ -45,3 +47,7 @@ def test_file_event_finds_the_same_tracks(gpu_device, tmp_path):
             assert np.array_equal(b0[k], b1[k]), k
         for k in c0:
             assert np.array_equal(c0[k], c1[k]), k
+        cpu, spread = best_hit_reference(bk, descs, 3, chi2_cut=15.0, bfield=3.8, n_threads=8)
+        assert_find_parity("file event, best hit", b1, cpu, spread, 3)
+        cpu, spread = clone_engine_reference(bk, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=8)
+        assert_ce_parity("file event, clone engine", c1, cpu, spread, 3)

@@ -87,6 +87,33 @@ def test_select_hit_indices_matches_brute_force():
     assert set(np.unique(wsr)) <= {0, 1, 2}
 
 
+def test_window_edge_counts_the_layer_thickness():
+    """SelectHitIndices' z test (MkFinder.cc:310, 336) widens the window's z reach by
+    the layer's half thickness seen along the track, 0.5 (rout - rin) / tan(theta), added in quadrature: on a
+    2 cm thick layer a track at 45 degrees whose 3-sigma z window ends 0.3 cm inside the layer end is on the
+    edge (that reach is 1.0 cm), while the same track crossing at 90 degrees (no reach) is inside, and one
+    1.5 cm beyond the end is outside only when the reach does not cover it."""
+    desc = _layer(rin=19.0, rout=21.0, zmin=-60.0, zmax=60.0)
+    rng = np.random.default_rng(6)
+    h = _random_hits(rng, 2000)
+    cases = [  # (z, theta, wsr)
+        (60.0 - 0.6, np.pi / 4, 1),     # dq 0.3: z + dq inside; with the 1.0 cm reach on the edge
+        (60.0 - 0.6, np.pi / 2, 0),     # no reach: inside
+        (-60.0 + 0.6, 3 * np.pi / 4, 1),
+        (60.0 + 0.9, np.pi / 4, 1),     # beyond the end by more than dq, within the reach: edge
+        (60.0 + 0.9, np.pi / 2, 2),     # without the reach: outside
+        (60.0 + 1.5, np.pi / 4, 2),     # beyond the reach sqrt(0.3^2 + 1^2): outside
+    ]
+    n = len(cases)
+    par = np.zeros((6, n), np.float32); err = np.zeros((21, n), np.float32)
+    for i, (z, th, _) in enumerate(cases):
+        par[:, i] = [20.0, 0.0, z, 0.5, 0.0, th]
+    err[0] = err[2] = 1e-4
+    err[5] = 0.1 ** 2                   # sigma_z 0.1 cm: dq = 0.3 cm
+    _, _, wsr = oracle.select_hit_indices(desc, h, par, err)
+    assert wsr.tolist() == [c[2] for c in cases], wsr.tolist()
+
+
 def _find_composed(O, ev, descs, first_layer, chi2_cut):
     """FindTracksBestHit over barrel layers from the oracle's primitives (MkBuilder.cc:1395-1478)."""
     n = ev["n_seeds"]
@@ -173,8 +200,11 @@ def test_find_loop_matches_composition(find_event):
     assert same.all(), np.nonzero(~same)[0][:10]
     assert np.array_equal(got["n_found"][same], ref["n_found"][same])
     assert np.array_equal(got["chg"][same], ref["chg"][same])
-    np.testing.assert_allclose(got["chi2"][same], ref["chi2"][same], rtol=1e-4, atol=1e-4)
-    np.testing.assert_allclose(got["par"][:, same], ref["par"][:, same], rtol=1e-4, atol=1e-4)
+    # the same pinned primitives in the same order: bit for bit, the fake-hit updates' covariance included
+    assert np.array_equal(got["chi2"][same], ref["chi2"][same])
+    assert np.array_equal(got["par"][:, same], ref["par"][:, same])
+    assert np.array_equal(got["err"][:, same], ref["err"][:, same])
+    assert (got["hot_idx"] == -1).sum() >= 10
 
 
 def test_find_loop_matches_composition_short_barrel(find_event):
@@ -188,7 +218,7 @@ def test_find_loop_matches_composition_short_barrel(find_event):
     assert np.array_equal(got["hot_idx"], ref["hot_idx"])
     assert (got["hot_idx"] == -3).sum() >= 1 and (got["hot_idx"][3:] == -9).sum() >= 500
     assert np.array_equal(got["n_found"], ref["n_found"])
-    np.testing.assert_allclose(got["par"], ref["par"], rtol=1e-4, atol=1e-4)
+    assert np.array_equal(got["par"], ref["par"]) and np.array_equal(got["err"], ref["err"])
 
 
 def test_find_invariants_and_threads(find_event):

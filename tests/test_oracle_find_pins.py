@@ -188,18 +188,21 @@ def _prop_r(O, paths, r, pflags=0):
 
 
 def _children(O, path, layer, d, hl, chi2_cut, max_holes, max_consec):
-    """What one live path becomes at a layer: (kind, path, in-find score) in generation order, kind one of
-    'hit', 'hole', 'short' (a -2 record), 'carry' (outside the layer: kept unchanged, no record),
-    'stopped' (barrel layer missed from inside: kept with a -2, no record)."""
+    """What one live path becomes at a layer: (kind, path, in-find score, position in the window's hit list)
+    in generation order, kind one of 'hit', 'hole', 'short' (a -2 record), 'carry' (outside the layer: kept
+    unchanged, no record), 'stopped' (barrel layer missed from inside: kept with a -2, no record); the
+    position is 16 for all but hits."""
     errP, parP, chg = _prop_r(O, [path], d["propagate_to"])
     xs, xa, wsr = oracle.select_hit_indices(d, hl, parP, errP)
     if np.hypot(parP[0, 0], parP[1, 0]) < F(d["rin"]):
-        return [("stopped", path.child(layer, -2, 0.0), None)]
+        c = path.child(layer, -2, 0.0)
+        c.score = path.score
+        return [("stopped", c, None, 16)]
     if wsr[0] == 2:
-        return [("carry", path, None)]
+        return [("carry", path, None, 16)]
     out = []
     pt = abs(F(1) / parP[3, 0])
-    for h in xa[:xs[0], 0]:
+    for hc, h in enumerate(xa[:xs[0], 0]):
         ms_par = hl[h, :3].reshape(3, 1).copy(); ms_err = hl[h, 3:].reshape(6, 1).copy()
         pe, pp, _ = O.propagate_to_r(errP, parP, chg, np.hypot(ms_par[0], ms_par[1]).astype(F))
         _, _, c2 = O.kalman_operation(1, pe, pp, ms_err, ms_par)
@@ -218,14 +221,14 @@ def _children(O, path, layer, d, hl, chi2_cut, max_holes, max_consec):
         c.par, c.err = p2[:, 0].copy(), e2[:, 0].copy()
         if c.par[3] < 0:
             c.chg, c.par[3] = -c.chg, -c.par[3]
-        out.append(("hit", c, score(path.nfound + 1, 0, path.ninside + path.ntail, c.chi2, pt, True)))
+        out.append(("hit", c, score(path.nfound + 1, 0, path.ninside + path.ntail, c.chi2, pt, True), hc))
     holes = path.ninside + path.ntail
     code = -1 if holes < max_holes and path.ntail < max_consec else -2
     if wsr[0] == 1:
         code = -3
     c = path.child(layer, code, 0.0)
     sc = score(path.nfound, path.ntail + 1 if code == -1 else path.ntail, path.ninside, path.chi2, pt, True)
-    out.append(("short" if code == -2 else "hole", c, sc))
+    out.append(("short" if code == -2 else "hole", c, sc, 16))
     return out
 
 
@@ -264,7 +267,7 @@ def enumerate_paths(O, ev, descs, s, first_layer, chi2_cut, max_holes, max_conse
             continue
         records, extras = [], []
         for p in live:
-            for kind, c, sc in _children(O, p, layer, d, hl, chi2_cut, max_holes, max_consec):
+            for kind, c, sc, _ in _children(O, p, layer, d, hl, chi2_cut, max_holes, max_consec):
                 if kind in ("carry", "stopped"):
                     extras.append(c)
                 else:
@@ -382,6 +385,185 @@ def test_clone_engine_k1_is_greedy(zmax):
         cands, _ = enumerate_paths(O, ev, descs, s, 3, 30.0, 4, 1, greedy=True)
         one = {k: (v[..., s:s + 1] if k != "n_cands" else v[s:s + 1]) for k, v in got.items()}
         _compare(one, cands, L, 1)
+
+
+def enumerate_ce_k(O, ev, descs, s, first_layer, chi2_cut, max_holes, max_consec, K):
+    """Seed s through the layers with CandCloner's K-limited selection (CandCloner.cc:37-191), built on the
+    same per-path steps as enumerate_paths: per layer the records of all live candidates in generation order
+    (accepted hits by window position, then candidate; then the invalid-hit records by candidate) ranked by
+    score (earlier generated first among equals); the best K of them are taken in rank order, a -2 one
+    competing for the best short candidate and every other one placed after the held-back candidates (in
+    candidate order) that score higher than it, the held-back ones filling what is left of the K places.
+    Returns (final candidates, number of held-back candidates placed ahead of a new one by their score)."""
+    off, hits = ev["hit_offset"], ev["hits_flat"]
+    seed = Path(ev["seed_par"][:, s], ev["seed_err"][:, s], ev["seed_chg"][s])
+    for layer in range(first_layer):
+        seed = seed.child(layer, int(ev["seed_hits"][layer, s]), 0.0)
+    seed.score = score(seed.nfound, 0, seed.ninside, seed.chi2, seed.pt(), False)
+    cands, best_short, ahead = [seed], None, 0
+    for layer in range(first_layer, len(descs)):
+        d = descs[layer]
+        hl = hits[off[layer]:off[layer + 1]]
+        live = []
+        for i, p in enumerate(cands):
+            if p.last == -2:
+                continue
+            if _apex_stop(p):
+                cands[i] = p.child(layer, -2, 0.0)
+                cands[i].score = p.score
+                continue
+            live.append(p)
+        if not live:
+            break                                   # CombCandidate::Finished: the candidates stay
+        made, fakes, extras = [], [], []
+        for lane, p in enumerate(live):
+            for kind, c, sc, hc in _children(O, p, layer, d, hl, chi2_cut, max_holes, max_consec):
+                if kind in ("carry", "stopped"):
+                    extras.append(c)
+                elif kind == "hit":
+                    made.append((hc, lane, kind, c, sc))
+                else:
+                    fakes.append((kind, c, sc))
+        records = [(k, c, sc) for _, _, k, c, sc in sorted(made, key=lambda x: (x[0], x[1]))] + fakes
+        if not records:
+            cands = extras
+            continue
+        ranked = sorted(records, key=lambda r: -r[2])[:K]            # stable
+        placed, xi = [], 0
+        for kind, c, sc in ranked:
+            c.score = sc
+            if kind == "short":
+                if best_short is None or sc > best_short.score:
+                    best_short = c
+                continue
+            while xi < len(extras) and extras[xi].score > sc and len(placed) < K:
+                placed.append(extras[xi])
+                xi += 1
+                ahead += 1
+            if len(placed) >= K:
+                break
+            placed.append(c)
+        placed += extras[xi:xi + max(0, K - len(placed))]
+        cands = placed
+    # MergeCandsAndBestShortOne with final scores, at most K candidates
+    for c in cands:
+        c.score = score(c.nfound, 0, c.ninside, c.chi2, c.pt(), False)
+    cands = sorted(cands, key=lambda c: -c.score)
+    if best_short is not None:
+        if not cands:
+            cands = [best_short]
+        else:
+            best_short.score = score(best_short.nfound, 0, best_short.ninside, best_short.chi2, best_short.pt(),
+                                     False)
+            if best_short.score > cands[-1].score:
+                at = next(i for i, c in enumerate(cands) if not c.score > best_short.score)
+                cands.insert(at, best_short)
+    return cands[:K], ahead
+
+
+@pytest.mark.parametrize("zmax", [30.0, None])
+def test_clone_engine_k_limited(zmax):
+    """Fewer places than candidates (K = 2): the K-limited selection with held-back candidates (tracks
+    leaving the shortened layers keep their place by score among the new candidates) and the best short
+    candidate."""
+    ev, descs = _tiny_events(63, 80, 1.5, n_layers=8, zmax=zmax)
+    _check_k_limited(ev, descs, 2, range(ev["n_seeds"]))
+
+
+def test_clone_engine_held_back_goes_ahead():
+    """Held-back candidates that outscore a new one take their place ahead of it (CandCloner.cc:103-109): on
+    layers cut to |z| <= 20 cm with noise 3 / cm^2 this happens to about one seed in fifteen of those that
+    pass within 0.5 cm of a layer end (the seeds checked here)."""
+    ev, descs = _tiny_events(63, 1000, 3.0, n_layers=7, zmax=20.0)
+    sp = ev["seed_par"]
+    r0, cot = np.hypot(sp[0], sp[1]), 1 / np.tan(sp[5])
+    near = np.zeros(ev["n_seeds"], bool)
+    for d in descs[3:]:
+        near |= np.abs(np.abs(sp[2] + (d["propagate_to"] - r0) * cot) - 20.0) < 0.5
+    ahead = _check_k_limited(ev, descs, 3, np.nonzero(near)[0])
+    assert ahead >= 3, ahead
+
+
+def _check_k_limited(ev, descs, K, seeds):
+    O = oracle.Oracle(bfield=3.8)
+    L = len(descs)
+    got = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=K, n_threads=4)
+    ahead = 0
+    for s in seeds:
+        cands, a = enumerate_ce_k(O, ev, descs, s, 3, 30.0, 4, 1, K)
+        ahead += a
+        one = {k: (v[..., s:s + 1] if k != "n_cands" else v[s:s + 1]) for k, v in got.items()}
+        _compare(one, cands, L, K)
+    return ahead
+
+
+def _short_layer_event(noise, n):
+    """Seven toy barrel layers, the fifth cut to |z| <= 8 cm and 6 cm thick (a wide window edge)."""
+    ev, descs = _tiny_events(66, n, noise, n_layers=7)
+    return ev, [dict(d, zmin=-8.0, zmax=8.0, rin=d["propagate_to"] - 3.0, rout=d["propagate_to"] + 3.0) if i == 4
+                else d for i, d in enumerate(descs)]
+
+
+def test_clone_engine_across_a_short_layer():
+    """One finding layer cut to |z| <= 8 cm (and 6 cm thick, so that its window edge is wide) between full
+    ones: paths take a window-edge record (-3) there and hits after it, which count it neither as a tail
+    hole nor as an inside hole (TrackCand::addHitIdx)."""
+    ev, descs = _short_layer_event(0.2, 300)
+    O = oracle.Oracle(bfield=3.8)
+    L, K = len(descs), 16
+    checked = edge_then_hit = 0
+    for s in range(ev["n_seeds"]):
+        cands, rec = enumerate_paths(O, ev, descs, s, 3, 30.0, 4, 1)
+        if rec >= K:
+            continue
+        got = oracle.find_clone_engine(_one_seed(ev, s), descs, 3, chi2_cut=30.0, max_cands=K)
+        _compare(got, cands, L, K)
+        checked += 1
+        edge_then_hit += any(c.codes.get(4) == -3 and c.codes.get(5, -1) >= 0 for c in cands)
+    assert checked >= 50 and edge_then_hit >= 5, (checked, edge_then_hit)
+
+
+def test_apex_stop_before_a_disk():
+    """The apex stop (MkBuilder.cc:1521-1533) applies whatever the next layer is: before an endcap disk,
+    where the propagation to z does not fail as the one to the next barrel radius would, a candidate of
+    pT 1 GeV at r = 30 cm whose momentum turns 1.45 rad from its position's azimuth still stops there."""
+    ev, descs = _tiny_events(64, 4, 0.0, n_layers=5)
+    disks = mk.disk_layer_descs(mk.geometry(4))
+    descs = descs + [dict(dd) for dd in disks[:3]]
+    L = len(descs)
+    one = _one_seed(ev, 0)
+    off = np.concatenate([ev["hit_offset"], np.full(L - 5, ev["hit_offset"][-1])]).astype(np.int32)
+    one = dict(one, hit_offset=off)
+    for z, theta in ((0.0, np.pi / 4), (0.0, 3 * np.pi / 4)):
+        sp = one["seed_par"].copy()
+        sp[:, 0] = [30.0, 0.0, z, 1.0, 1.45, theta]
+        o = dict(one, seed_par=sp.astype(F), seed_hits=np.zeros((5, 1), np.int32))
+        got = oracle.find_clone_engine(o, descs, 5, chi2_cut=30.0, max_cands=6)
+        assert got["n_cands"][0] == 1 and got["hot_idx"][0, 5, 0] == -2, got["hot_idx"][0, :, 0]
+        assert np.all(got["hot_idx"][0, 6:, 0] == -9)
+        np.testing.assert_array_equal(got["par"][0, :, 0], sp[:, 0].astype(F))
+
+
+def test_wrong_charge_seeds_flip_to_the_true_charge():
+    """Seeds given the wrong charge, 1/pT = 0.01 and a 1/pT uncertainty of 1 / GeV: the updates pull 1/pT
+    through zero, and UpdateWithLastHit's update moves the sign to the charge (KalmanUtilsMPlex.cc:497-504),
+    so the best candidates end with the true charge, 1/pT >= 0 everywhere, and as many hits as with the
+    right charge."""
+    geo = mk.geometry(1)
+    descs = mk.barrel_layer_descs(geo)
+    ev = mk.generate_find_event(geo, mk.default_toy_params(seed=8, simulate_material=0, pt_min=2.0, pt_max=10.0),
+                                300, noise_per_cm2=0.5)
+    sp = ev["seed_par"].copy(); se = ev["seed_err"].copy()
+    sp[3] = F(0.01)
+    se[9] = F(1.0)
+    wrong = dict(ev, seed_par=sp, seed_err=se, seed_chg=(-ev["seed_chg"]).astype(np.int32))
+    r = oracle.find_clone_engine(wrong, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=8)
+    K = r["par"].shape[0]
+    valid = np.arange(K)[:, None] < r["n_cands"][None, :]
+    assert np.all(r["par"][:, 3][valid] >= 0)
+    assert (r["chg"][0] == ev["seed_chg"]).mean() > 0.95
+    right = oracle.find_clone_engine(ev, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=8)
+    assert np.median(r["n_found"][0]) >= np.median(right["n_found"][0]) - 1
 
 
 def _duplicate_hits(ev, n_layers):

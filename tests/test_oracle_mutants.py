@@ -57,7 +57,9 @@ MUTANTS = {
 # candidate bookkeeping (CandCloner.cc:37-191, HitStructures.h:873-889), the best short candidate and the
 # final merge (HitStructures.cc:511-552), the stop rules (MkBuilder.cc:1521-1533, 1568-1630), best-hit
 # selection and fake codes (MkFinder.cc:691-832) and the window / bin arithmetic (MkFinder.cc:227-684,
-# HitStructures.cc:77-183).
+# HitStructures.cc:77-183).  Left out as equivalent, no input telling them apart: dropping the pop_back
+# before the best short candidate's insertion (HitStructures.cc:540-545), since the output keeps the
+# first max_cands either way.
 FIND_MUTANTS = {
     "score_valid_bonus": ("oracle_ce.cc", "kValidHitBonus = 4,", "kValidHitBonus = 5,"),
     "score_valid_slope": ("oracle_ce.cc", "kValidHitSlope = 0.2f", "kValidHitSlope = 0.25f"),
@@ -91,7 +93,6 @@ FIND_MUTANTS = {
                       "while (false && xi < extras.size() && extras[xi].score > tc.score"),
     "top_k_one_short": ("oracle_ce.cc", "const int num_hits = std::min((int)entries.size(), max_cands);",
                         "const int num_hits = std::min((int)entries.size(), std::max(1, max_cands - 1));"),
-    "merge_no_pop": ("oracle_ce.cc", "if ((int)cc.cands.size() >= max_cands) cc.cands.pop_back();", ""),
     "merge_no_rescore": ("oracle_ce.cc", "    for (auto& c : cc.cands) c.score = score_cand(c);\n", ""),
     "apex_pt": ("oracle_ce.cc", "if (c.pT() < 1.2) {", "if (c.pT() < 0.6) {"),
     "apex_no_dphi_test": ("oracle_ce.cc", "625.f && dphi > 1.371f && dphi < 4.512f", "625.f"),
@@ -185,42 +186,25 @@ def _build(name, old, new, workdir, target="oracle_core.h"):
     return so
 
 
-@pytest.fixture(scope="module")
-def mutant_libs():
-    work = tempfile.mkdtemp(prefix="orc_mut_")
-    try:
-        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-            futs = {k: ex.submit(_build, k, a, b, work) for k, (a, b) in MUTANTS.items()}
-            yield {k: f.result() for k, f in futs.items()}
-    finally:
-        shutil.rmtree(work, ignore_errors=True)
+def _first_caught(pin_list):
+    """Name of the first pin that fails (an assertion, or a numpy error on the mutant's output), else None."""
+    for pin_name, pin in pin_list:
+        try:
+            pin()
+        except Exception as e:
+            return "%s: %s" % (pin_name, type(e).__name__)
+    return None
 
 
-@pytest.mark.parametrize("name", sorted(MUTANTS))
-def test_mutant_is_caught(mutant_libs, name):
-    so = mutant_libs[name]
-
+def _fit_pins(so):
     def make(bfield, rl, xi):
         return oracle.Oracle(bfield, rl, xi, lib_path=so)
-
-    caught = []
-    for pin in pins.ALL_PINS:
-        try:
-            pin(make)
-        except Exception as e:  # AssertionError, or NaN-driven numpy errors
-            caught.append("%s: %s" % (pin.__name__, type(e).__name__))
-    assert caught, "mutant %s passes every pin" % name
+    return [(pin.__name__, (lambda pin=pin: pin(make))) for pin in pins.ALL_PINS]
 
 
-@pytest.fixture(scope="module")
-def find_mutant_libs():
-    work = tempfile.mkdtemp(prefix="orc_fmut_")
-    try:
-        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-            futs = {k: ex.submit(_build, k, a, b, work, f) for k, (f, a, b) in FIND_MUTANTS.items()}
-            yield {k: f.result() for k, f in futs.items()}
-    finally:
-        shutil.rmtree(work, ignore_errors=True)
+def _param_values(test):
+    """The values of a test's pytest.mark.parametrize (one argument: values; several: tuples)."""
+    return [m for m in test.pytestmark if m.name == "parametrize"][0].args[1]
 
 
 def _find_pins():
@@ -235,79 +219,43 @@ def _find_pins():
         ("loh_stable_sort", bh.test_loh_matches_stable_sort),
         ("select_brute_force", bh.test_select_hit_indices_matches_brute_force),
         ("duplicated_hits", fp.test_duplicated_hits_resolve_to_first_copy),
-        ("ce_greedy", fp.test_clone_engine_k1_is_greedy),
-        ("ce_all_paths_4_1", lambda: fp.test_clone_engine_equals_all_paths((4, 1))),
-        ("ce_all_paths_1_1", lambda: fp.test_clone_engine_equals_all_paths((1, 1))),
-        ("ce_all_paths_3_2", lambda: fp.test_clone_engine_equals_all_paths((3, 2))),
+        ("score_formula", fp.test_score_formula_known_answers),
+        ("apex_stop_rule", lambda: [fp.test_apex_stop_rule(c) for c in _param_values(fp.test_apex_stop_rule)]),
+        ("ce_greedy", lambda: [fp.test_clone_engine_k1_is_greedy(z) for z in (None, 30.0)]),
+        ("ce_all_paths_4_1", lambda: fp.test_clone_engine_equals_all_paths((4, 1, None))),
+        ("ce_all_paths_1_1", lambda: fp.test_clone_engine_equals_all_paths((1, 1, None))),
+        ("ce_all_paths_3_2", lambda: fp.test_clone_engine_equals_all_paths((3, 2, None))),
+        ("ce_all_paths_edges", lambda: fp.test_clone_engine_equals_all_paths((2, 2, 20.0))),
         ("bh_loop_composition", lambda: bh.test_find_loop_matches_composition(bh.make_find_event())),
+        ("bh_short_barrel", lambda: bh.test_find_loop_matches_composition_short_barrel(bh.make_find_event())),
+        ("window_edge", bh.test_window_edge_counts_the_layer_thickness),
+        ("ce_apex_before_disk", fp.test_apex_stop_before_a_disk),
+        ("ce_short_layer", fp.test_clone_engine_across_a_short_layer),
+        ("ce_k_limited", lambda: [fp.test_clone_engine_k_limited(z) for z in _param_values(fp.test_clone_engine_k_limited)]),
+        ("ce_held_back_ahead", fp.test_clone_engine_held_back_goes_ahead),
+        ("ce_charge_flip", fp.test_wrong_charge_seeds_flip_to_the_true_charge),
         ("ce_composition_k6", lambda: ce.test_ce_matches_composition(6)),
     ]
 
 
-@pytest.mark.parametrize("name", sorted(FIND_MUTANTS))
-def test_find_mutant_is_caught(find_mutant_libs, name):
-    so = find_mutant_libs[name]
-    with oracle.use_lib(so):
-        for pin_name, pin in _find_pins():
-            try:
-                pin()
-            except Exception as e:  # AssertionError, or numpy errors on the mutant's output
-                return
-    pytest.fail("mutant %s passes every finding pin" % name)
-
-
-@pytest.fixture(scope="module")
-def bkfit_mutant_libs():
-    work = tempfile.mkdtemp(prefix="orc_bmut_")
-    try:
-        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-            futs = {k: ex.submit(_build, k, a, b, work, "oracle_bkfit.cc") for k, (a, b) in BKFIT_MUTANTS.items()}
-            yield {k: f.result() for k, f in futs.items()}
-    finally:
-        shutil.rmtree(work, ignore_errors=True)
-
-
 def _bkfit_pins():
     import test_oracle_bkfit as bk
-    barrel = bk._barrel()
-    mixed = []
+    events = {}
 
-    def with_mixed(f):
-        if not mixed:
-            mixed.append(bk._mixed())
-        f(mixed[0])
+    def ev(name):                           # built on first use, once per process
+        if name not in events:
+            events[name] = bk._barrel() if name == "barrel" else bk._mixed()
+        return events[name]
     return [
-        ("score_known_answers", lambda: [bk.test_score_known_answers(barrel, c) for c in range(len(bk.SCORE_CASES))]),
-        ("blocks_1", lambda: bk.test_blocks_equal_fit_composition(barrel, 1)),
-        ("blocks_16", lambda: bk.test_blocks_equal_fit_composition(barrel, 16)),
-        ("pca_flags", lambda: bk.test_without_pca_and_other_pca_flags(barrel)),
-        ("last_layer", lambda: bk.test_last_layer_stops_the_plan(barrel)),
-        ("material", lambda: bk.test_material_and_fake_hit_material(barrel)),
-        ("mixed", lambda: with_mixed(bk.test_mixed_layer_kinds_with_layer_base)),
+        ("score_known_answers", lambda: [bk.test_score_known_answers(ev("barrel"), c)
+                                         for c in range(len(bk.SCORE_CASES))]),
+        ("blocks_1", lambda: bk.test_blocks_equal_fit_composition(ev("barrel"), 1)),
+        ("blocks_16", lambda: bk.test_blocks_equal_fit_composition(ev("barrel"), 16)),
+        ("pca_flags", lambda: bk.test_without_pca_and_other_pca_flags(ev("barrel"))),
+        ("last_layer", lambda: bk.test_last_layer_stops_the_plan(ev("barrel"))),
+        ("material", lambda: bk.test_material_and_fake_hit_material(ev("barrel"))),
+        ("mixed", lambda: bk.test_mixed_layer_kinds_with_layer_base(ev("mixed"))),
     ]
-
-
-@pytest.mark.parametrize("name", sorted(BKFIT_MUTANTS))
-def test_bkfit_mutant_is_caught(bkfit_mutant_libs, name):
-    so = bkfit_mutant_libs[name]
-    with oracle.use_lib(so):
-        for pin_name, pin in _bkfit_pins():
-            try:
-                pin()
-            except Exception:  # AssertionError, or numpy errors on the mutant's output
-                return
-    pytest.fail("mutant %s passes every backward-fit pin" % name)
-
-
-@pytest.fixture(scope="module")
-def std_mutant_libs():
-    work = tempfile.mkdtemp(prefix="orc_smut_")
-    try:
-        with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-            futs = {k: ex.submit(_build, k, a, b, work, "oracle_ce.cc") for k, (a, b) in STD_MUTANTS.items()}
-            yield {k: f.result() for k, f in futs.items()}
-    finally:
-        shutil.rmtree(work, ignore_errors=True)
 
 
 def _std_pins():
@@ -321,13 +269,83 @@ def _std_pins():
     ]
 
 
-@pytest.mark.parametrize("name", sorted(STD_MUTANTS))
-def test_std_mutant_is_caught(std_mutant_libs, name):
-    so = std_mutant_libs[name]
+def _pins_of(kind, so):
+    return _fit_pins(so) if kind == "fit" else {"find": _find_pins, "bkfit": _bkfit_pins, "std": _std_pins}[kind]()
+
+
+def _verdict(kind, so, only=None):
+    """Runs in a worker process: the first pin of the mutant's kind that catches it (only: just that pin),
+    or None."""
     with oracle.use_lib(so):
-        for pin_name, pin in _std_pins():
-            try:
-                pin()
-            except Exception:  # AssertionError, or numpy errors on the mutant's output
-                return
-    pytest.fail("mutant %s passes every standard-finding pin" % name)
+        pl = _pins_of(kind, so)
+        return _first_caught(pl if only is None else [pl[only]])
+
+
+
+
+# Every mutant: (kind, file, original text, mutated text).
+ALL_MUTANTS = {}
+ALL_MUTANTS.update({("fit", k): ("oracle_core.h", a, b) for k, (a, b) in MUTANTS.items()})
+ALL_MUTANTS.update({("find", k): (f, a, b) for k, (f, a, b) in FIND_MUTANTS.items()})
+ALL_MUTANTS.update({("bkfit", k): ("oracle_bkfit.cc", a, b) for k, (a, b) in BKFIT_MUTANTS.items()})
+ALL_MUTANTS.update({("std", k): ("oracle_ce.cc", a, b) for k, (a, b) in STD_MUTANTS.items()})
+# Controls: the unchanged oracle, built and run the same way, passes every pin of each kind, so a
+# caught mutant is caught by its change and not by the runner.
+for _kind, _file in (("fit", "oracle_core.h"), ("find", "oracle_find.cc"), ("bkfit", "oracle_bkfit.cc"),
+                     ("std", "oracle_ce.cc")):
+    _head = open(os.path.join(ORACLE_DIR, _file)).read()[:80]
+    ALL_MUTANTS[(_kind, "control")] = (_file, _head, _head)
+
+
+@pytest.fixture(scope="module")
+def verdicts():
+    """All mutants built on 8 threads, then run through their pins in 8 forked worker processes (the
+    pins are deterministic and share nothing between mutants)."""
+    import multiprocessing
+    work = tempfile.mkdtemp(prefix="orc_mut_")
+    n = min(8, os.cpu_count() or 1)
+    try:
+        with concurrent.futures.ThreadPoolExecutor(max_workers=n) as ex:
+            futs = {k: ex.submit(_build, "%s_%s" % k, a, b, work, f) for k, (f, a, b) in ALL_MUTANTS.items()}
+            libs = {k: f.result() for k, f in futs.items()}
+        with concurrent.futures.ProcessPoolExecutor(max_workers=n,
+                                                    mp_context=multiprocessing.get_context("fork")) as ex:
+            # controls run every pin, so each pin is a job of its own; mutants stop at their first catch
+            futs = {}
+            for k, so in libs.items():
+                if k[1] == "control":
+                    futs[k] = [ex.submit(_verdict, k[0], so, i) for i in range(len(_pins_of(k[0], so)))]
+                else:
+                    futs[k] = [ex.submit(_verdict, k[0], so)]
+            out = {}
+            for k, fl in futs.items():
+                v = [f.result() for f in fl]
+                out[k] = next((x for x in v if x), None)
+            yield out
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+@pytest.mark.parametrize("name", sorted(MUTANTS))
+def test_mutant_is_caught(verdicts, name):
+    assert verdicts[("fit", name)], "mutant %s passes every pin" % name
+
+
+@pytest.mark.parametrize("name", sorted(FIND_MUTANTS))
+def test_find_mutant_is_caught(verdicts, name):
+    assert verdicts[("find", name)], "mutant %s passes every finding pin" % name
+
+
+@pytest.mark.parametrize("name", sorted(BKFIT_MUTANTS))
+def test_bkfit_mutant_is_caught(verdicts, name):
+    assert verdicts[("bkfit", name)], "mutant %s passes every backward-fit pin" % name
+
+
+@pytest.mark.parametrize("name", sorted(STD_MUTANTS))
+def test_std_mutant_is_caught(verdicts, name):
+    assert verdicts[("std", name)], "mutant %s passes every standard-finding pin" % name
+
+
+@pytest.mark.parametrize("kind", ["fit", "find", "bkfit", "std"])
+def test_unchanged_oracle_passes_every_pin(verdicts, kind):
+    assert verdicts[(kind, "control")] is None, verdicts[(kind, "control")]
