@@ -419,6 +419,14 @@ def bench_find(args, mk, torch, dev, kind):
         k1.record(stream)
         kern.append((k0, k1))
 
+    shape = None
+    if comb and not args.no_autotune:
+        # the finders' autotuner: slots per seed in a wavefront and waves per SIMD, timed on this batch
+        eoh.load(cat["hit_offset"], hits)
+        t0 = time.time()
+        shape, tune_ms = ctx.autotune_find(eoh, batch, standard=kind == "std")
+        log("find %s: autotuned shape (slots %d, waves/SIMD %d): %.2f ms per launch (%.1f s)" %
+            (kind, shape[0], shape[1], tune_ms, time.time() - t0))
     wall, per = _timed(torch, stream, step, args.steps, args.warmup)
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in kern[args.warmup:]]))
     seeds = cat["n_seeds"] * args.steps
@@ -428,7 +436,8 @@ def bench_find(args, mk, torch, dev, kind):
                  "evaluate": FIND_FLOP["chi2"] + (FIND_FLOP["propagate"] if prop_to_hit else 0),
                  "update": FIND_FLOP["update"] + (FIND_FLOP["propagate"] if prop_to_hit else 0)}
     name = {"ce": "clone-engine", "std": "standard combinatorial", "bh": "best-hit"}[kind]
-    kname = {"ce": "k_find_clone_engine<6, false>", "std": "k_find_clone_engine<6, true>",
+    sl, wv = shape if shape else (6, 2)
+    kname = {"ce": "k_find_clone_engine<%d, false, %d>" % (sl, wv), "std": "k_find_clone_engine<%d, true, %d>" % (sl, wv),
              "bh": "k_find_best_hit"}[kind]
     cpu = None
     roof = {"bound": "valu", "achieved": None, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": None,
@@ -462,7 +471,8 @@ def bench_find(args, mk, torch, dev, kind):
                                   "per event, noise 2/cm^2, %s" % (name, len(events),
                                                                    "up to 6 candidates, chi2 cut 30" if comb
                                                                    else "best hit, chi2 cut 15"),
-                      "events": len(events), "seeds": cat["n_seeds"], "hits": int(cat["hit_offset"][-1])},
+                      "events": len(events), "seeds": cat["n_seeds"], "hits": int(cat["hit_offset"][-1]),
+                      "find_shape": None if shape is None else {"slots": shape[0], "waves_per_simd": shape[1]}},
            "ms_per_event": ms / len(events), "roofline": roof, "cpu_baseline": cpu}
     log("find %s: %.3g seeds/s, %.2f ms per step, finder kernel %.2f ms%s" % (
         kind, res["value"], ms, kern_ms, "" if roof["frac"] is None else ", %.3f of FP32 peak" % roof["frac"]))

@@ -419,6 +419,28 @@ int mkf_find_tracks_clone_engine(mkf_context* ctx, mkf_event_of_hits* eoh, const
  * order competes for the best short candidate.  Hit charge and overlap hits as in the clone engine. */
 int mkf_find_tracks_standard(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_ce_batch* batch, void* stream);
 
+/* Launch shape of the combinatorial finders (mkf_find_tracks_clone_engine,
+ * mkf_find_tracks_standard), per finder: no counterpart in the reference, whose
+ * Matriplex width and thread pool are fixed at build time; the shapes give the
+ * same results.  slots: candidate slots per seed in a 64-lane wavefront, 6 (ten
+ * seeds per wavefront) or 8 (eight), at least max_cands; 0 (the default): 6
+ * when max_cands <= 6, else 8.  waves_per_simd: 2 (the default) or 1, the
+ * occupancy the kernel's registers are allotted for. */
+typedef struct mkf_find_shape {
+  int slots;
+  int waves_per_simd;
+} mkf_find_shape;
+int mkf_set_find_shape(mkf_context* ctx, int standard, const mkf_find_shape* shape);
+int mkf_get_find_shape(mkf_context* ctx, int standard, mkf_find_shape* shape);
+/* The finders' autotuner (as mkf_autotune_batched is the fit's): times every
+ * shape valid for the batch's max_cands on `batch` (one warm-up launch and
+ * `reps` timed launches each, on `stream`, synchronising it), keeps the fastest
+ * for later calls of that finder on this context and returns it with its time
+ * per launch.  The batch's outputs are those of a finding run (the same for
+ * every shape). */
+int mkf_autotune_find(mkf_context* ctx, mkf_event_of_hits* eoh, const mkf_ce_batch* batch, int standard, int reps,
+                      mkf_find_shape* best, float* best_ms, void* stream);
+
 /* ---- Backward fit of found candidates -------------------------------------- */
 
 #define MKF_BKFIT_PCA           1 /* Config::includePCA: after the last layer, propagate to the PCA with pca_pflags

@@ -47,6 +47,10 @@ class MkfFitBatch(ctypes.Structure):
                 ("pca_pflags", ctypes.c_int)]
 
 
+class MkfFindShape(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("waves_per_simd", ctypes.c_int)]
+
+
 class MkfLaunchShape(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int), ("blocks_per_cu", ctypes.c_int), ("variant", ctypes.c_int),
                 ("sort_tile", ctypes.c_int)]
@@ -97,6 +101,7 @@ ABI_SYMBOLS = [
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
     "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc", "mkf_backward_fit", "mkf_find_tracks_standard",
+    "mkf_set_find_shape", "mkf_get_find_shape", "mkf_autotune_find",
 ]
 CE_MAX_CANDS = 8
 
@@ -204,6 +209,10 @@ def lib():
                                                _c_void_p, ctypes.c_int, ctypes.c_int, _c_void_p]
         L.mkf_autotune_batched.argtypes = [_c_void_p, ctypes.POINTER(MkfFitBatch), _c_void_p, ctypes.c_int,
                                            ctypes.POINTER(MkfLaunchShape), _c_float_p]
+        L.mkf_set_find_shape.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(MkfFindShape)]
+        L.mkf_get_find_shape.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(MkfFindShape)]
+        L.mkf_autotune_find.argtypes = [_c_void_p, _c_void_p, ctypes.POINTER(MkfCEBatch), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(MkfFindShape), _c_float_p, _c_void_p]
         _LIB = L
     return _LIB
 
@@ -827,6 +836,24 @@ class Context:
         s = MkfLaunchShape()
         _check(lib().mkf_get_batched_shape(self.h, ctypes.byref(s)))
         return (s.block, s.blocks_per_cu, s.variant, s.sort_tile)
+
+    def set_find_shape(self, standard, slots=0, waves_per_simd=2):
+        """Launch shape of the clone-engine (standard=False) or standard finder (mkf_set_find_shape)."""
+        _check(lib().mkf_set_find_shape(self.h, int(bool(standard)), ctypes.byref(MkfFindShape(slots, waves_per_simd))))
+
+    def get_find_shape(self, standard):
+        s = MkfFindShape()
+        _check(lib().mkf_get_find_shape(self.h, int(bool(standard)), ctypes.byref(s)))
+        return (s.slots, s.waves_per_simd)
+
+    def autotune_find(self, eoh, batch, standard=False, reps=2, stream=None):
+        """mkf_autotune_find: times every finder shape on `batch` (a CEBatch), keeps the fastest; returns
+        ((slots, waves_per_simd), ms per launch).  The batch holds a finding result afterwards."""
+        s = MkfFindShape()
+        ms = ctypes.c_float()
+        _check(lib().mkf_autotune_find(self.h, eoh.h, ctypes.byref(batch.c_struct()), int(bool(standard)), int(reps),
+                                       ctypes.byref(s), ctypes.byref(ms), self._stream(stream)))
+        return (s.slots, s.waves_per_simd), ms.value
 
     def autotune_batched(self, batch, reps=3):
         import torch

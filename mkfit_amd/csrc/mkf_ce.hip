@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "../../include/mkfit_amd.h"
 #include "mkf_find_common.h"
@@ -178,10 +179,11 @@ __device__ __forceinline__ void propagate(bool barrel, float (&p)[6], float (&e)
 // KS candidate slots per seed (max_cands <= KS): KS = 6 packs 10 seeds into the
 // wavefront's 64 lanes (4 idle), KS = 8 packs 8.  Lanes past the last whole
 // seed group (g == kG) take no part.  STD: standard finding (see the top).
-template <int KS, bool STD>
-// Two waves per SIMD: left to itself the compiler spends all 512 registers of a lane on one wave
-// (21.0 against 29.7 ms for the standard form, 15.3 against 28.1 ms for the clone engine, 50 events).
-__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(2))) k_find_clone_engine(CeArgs a) {
+// WAVES: waves per SIMD the registers are allotted for.  Left to itself the compiler spends all 512
+// registers of a lane on one wave; two measured 21.0 against 29.7 ms for the standard form and 15.3
+// against 28.1 ms for the clone engine (50 events).  mkf_autotune_find picks KS and WAVES by timing.
+template <int KS, bool STD, int WAVES>
+__global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(WAVES))) k_find_clone_engine(CeArgs a) {
   constexpr int kS = KS;
   constexpr int kG = kLanes / KS;             // seeds per workgroup
   constexpr int kGA = kG + 1;                 // seed rows in LDS: one more for the idle lanes
@@ -791,17 +793,19 @@ int find_combinatorial(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch*
   a.bs = reinterpret_cast<float*>(a.nodes + (size_t)node_cap * N);
   a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
   a.ranks = h->d_ranks;
-  const int grid6 = (int)((N + kLanes / 6 - 1) / (kLanes / 6)), grid8 = (int)((N + kLanes / 8 - 1) / (kLanes / 8));
-  if (b->max_cands <= 6) {
-    if (standard)
-      hipLaunchKernelGGL((k_find_clone_engine<6, true>), dim3(grid6), dim3(kLanes), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_find_clone_engine<6, false>), dim3(grid6), dim3(kLanes), 0, st, a);
-  } else {
-    if (standard)
-      hipLaunchKernelGGL((k_find_clone_engine<8, true>), dim3(grid8), dim3(kLanes), 0, st, a);
-    else
-      hipLaunchKernelGGL((k_find_clone_engine<8, false>), dim3(grid8), dim3(kLanes), 0, st, a);
+  const mkf_find_shape sh = c->find_shape[standard ? 1 : 0];
+  const int slots = sh.slots ? sh.slots : (b->max_cands <= 6 ? 6 : 8);
+  if (slots < b->max_cands) return fail(MKF_ERR_INVALID_ARG, "find shape: %d slots < max_cands %d", slots, b->max_cands);
+  const int grid = (int)((N + kLanes / slots - 1) / (kLanes / slots));
+  const int sel = (slots == 8 ? 4 : 0) | (standard ? 2 : 0) | (sh.waves_per_simd == 1 ? 1 : 0);
+  switch (sel) {
+#define MKF_CE_CASE(I, KS, STD, W)                                                                   \
+    case I:                                                                                          \
+      hipLaunchKernelGGL((k_find_clone_engine<KS, STD, W>), dim3(grid), dim3(kLanes), 0, st, a);     \
+      break;
+    MKF_CE_CASE(0, 6, false, 2) MKF_CE_CASE(1, 6, false, 1) MKF_CE_CASE(2, 6, true, 2) MKF_CE_CASE(3, 6, true, 1)
+    MKF_CE_CASE(4, 8, false, 2) MKF_CE_CASE(5, 8, false, 1) MKF_CE_CASE(6, 8, true, 2) MKF_CE_CASE(7, 8, true, 1)
+#undef MKF_CE_CASE
   }
   HIP_TRY(hipGetLastError());
   return scratch_release(h->ce_done, st);
@@ -816,4 +820,69 @@ extern "C" int mkf_find_tracks_clone_engine(mkf_context* c, mkf_event_of_hits* h
 
 extern "C" int mkf_find_tracks_standard(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch* b, void* stream) {
   return find_combinatorial(c, h, b, stream, true);
+}
+
+extern "C" int mkf_set_find_shape(mkf_context* c, int standard, const mkf_find_shape* shape) {
+  if (!c || !shape) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  if (shape->slots != 0 && shape->slots != 6 && shape->slots != 8)
+    return fail(MKF_ERR_INVALID_ARG, "find shape: slots must be 0, 6 or 8");
+  if (shape->waves_per_simd != 1 && shape->waves_per_simd != 2)
+    return fail(MKF_ERR_INVALID_ARG, "find shape: waves_per_simd must be 1 or 2");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->find_shape[standard ? 1 : 0] = *shape;
+  return MKF_OK;
+}
+
+extern "C" int mkf_get_find_shape(mkf_context* c, int standard, mkf_find_shape* shape) {
+  if (!c || !shape) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  *shape = c->find_shape[standard ? 1 : 0];
+  return MKF_OK;
+}
+
+// Every shape valid for the batch's max_cands, timed on the batch: one warm-up launch of each, then reps
+// timed launches; the fastest stays in the context.  Results do not depend on the shape.
+extern "C" int mkf_autotune_find(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch* b, int standard, int reps,
+                                 mkf_find_shape* best, float* best_ms, void* stream) {
+  if (!c || !b) return fail(MKF_ERR_INVALID_ARG, "NULL argument");
+  if (reps < 1 || reps > 100) return fail(MKF_ERR_INVALID_ARG, "reps must be in [1, 100]");
+  CHECK(set_device(c));
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<mkf_find_shape> cand;
+  for (int slots : {6, 8})
+    if (slots >= b->max_cands)
+      for (int w : {2, 1}) cand.push_back(mkf_find_shape{slots, w});
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const mkf_find_shape keep = c->find_shape[standard ? 1 : 0];
+  float best_t = 1e30f;
+  mkf_find_shape best_s = keep;
+  int rc = MKF_OK;
+  for (const mkf_find_shape& s : cand) {
+    c->find_shape[standard ? 1 : 0] = s;
+    rc = find_combinatorial(c, h, b, stream, standard != 0);                  // warm-up
+    if (rc != MKF_OK) break;
+    if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(MKF_ERR_HIP, "hipEventRecord"); break; }
+    for (int r = 0; r < reps && rc == MKF_OK; ++r) rc = find_combinatorial(c, h, b, stream, standard != 0);
+    if (rc != MKF_OK) break;
+    float ms = 0.f;
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(MKF_ERR_HIP, "timing the find shapes");
+      break;
+    }
+    ms /= reps;
+    if (ms < best_t) {
+      best_t = ms;
+      best_s = s;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->find_shape[standard ? 1 : 0] = rc == MKF_OK ? best_s : keep;
+  if (rc != MKF_OK) return rc;
+  if (best) *best = best_s;
+  if (best_ms) *best_ms = best_t;
+  return MKF_OK;
 }

@@ -39,6 +39,7 @@ struct mkf_context {
   float* d_rl = nullptr;
   float* d_xi = nullptr;
   mkf_launch_shape shape{256, 0, 0, 0};
+  mkf_find_shape find_shape[2] = {{0, 2}, {0, 2}};   // clone engine, standard finding
   hipStream_t capture_stream = nullptr;
   // Instantiated batched-fit graphs by batch (pointers, sizes, flags, shape),
   // at most kMaxGraphs; the least recently launched one is destroyed to make

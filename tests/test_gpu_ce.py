@@ -150,3 +150,50 @@ def test_ce_rejects_bad_arguments(gpu_device):
         b.hot_idx = short
         with pytest.raises(mk.MkfError, match="hot_idx"):
             ctx.find_tracks_clone_engine(eoh, b)
+
+
+def test_every_find_shape_gives_the_same_candidates(gpu_device):
+    """The finders' launch shapes (6 or 8 candidate slots per seed in a wavefront, registers for one or two
+    waves per SIMD) give the same candidates bit for bit, for the clone engine and standard finding; the
+    autotuner picks one of them and keeps it; a shape with fewer slots than max_cands is refused."""
+    import torch
+    ev, descs = _event(33, 3000, 2.0)
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+
+    def run(standard, K=6):
+        b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, chi2_cut=30.0, max_cands=K)
+        (ctx.find_tracks_standard if standard else ctx.find_tracks_clone_engine)(eoh, b)
+        torch.cuda.synchronize()
+        return b.numpy()
+
+    def same(a, b):
+        assert np.array_equal(a["n_cands"], b["n_cands"])
+        for k in ("hot_idx", "score", "n_found"):
+            assert np.array_equal(a[k], b[k]), k
+        K = a["par"].shape[0]
+        valid = np.arange(K)[:, None] < a["n_cands"][None, :]
+        for k in ("chg", "chi2"):
+            assert np.array_equal(a[k][valid], b[k][valid]), k
+        for k in ("par", "err"):
+            assert np.array_equal(np.moveaxis(a[k], 1, 2)[valid], np.moveaxis(b[k], 1, 2)[valid]), k
+
+    for standard in (False, True):
+        assert ctx.get_find_shape(standard) == (0, 2)
+        ref = run(standard)
+        for slots in (6, 8):
+            for waves in (2, 1):
+                ctx.set_find_shape(standard, slots, waves)
+                same(run(standard), ref)
+        b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, chi2_cut=30.0, max_cands=6)
+        shape, ms = ctx.autotune_find(eoh, b, standard=standard)
+        torch.cuda.synchronize()
+        assert shape in {(6, 2), (6, 1), (8, 2), (8, 1)} and ms > 0
+        assert ctx.get_find_shape(standard) == shape
+        same(b.numpy(), ref)
+        same(run(standard), ref)
+    ctx.set_find_shape(False, 6, 2)
+    with pytest.raises(mk.MkfError):
+        run(False, K=8)
+    ctx.close()
