@@ -253,6 +253,15 @@ int mkf_event_of_hits_destroy(mkf_event_of_hits* eoh);
  * may be reused once the stream has passed this call. */
 int mkf_event_of_hits_load(mkf_event_of_hits* eoh, const int* hit_offset, const float* hits, void* stream);
 
+/* The iteration's hit mask (MkJob::get_mask_for_layer, MkFinder.cc:476-482): mask
+ * is a device array with one byte per hit of the last load, in the order the
+ * hits were given to it; a nonzero byte marks a hit taken by an earlier
+ * iteration, which hit selection then passes over (it does not count toward the
+ * MKF_HIT_IDX_MAX hits of a window).  NULL clears the mask; every load clears it.
+ * The mask is copied on `stream`, so `mask` may be reused once the stream has
+ * passed this call. */
+int mkf_event_of_hits_set_hit_mask(mkf_event_of_hits* eoh, const unsigned char* mask, void* stream);
+
 /* Device views of a loaded layer (valid until the next load or destroy):
  * ranks [n] sorted position -> hit index within the layer (LayerOfHits
  * m_hit_ranks), bins [nq*256][2] first/end per (q bin, phi bin) (m_phi_bin_infos;

@@ -651,6 +651,25 @@ class EventOfHits {
     n_hits_.assign(off.begin(), off.end());
     c_.sync();
   }
+  // The iteration's hit mask (MkJob::get_mask_for_layer, MkFinder.cc:476-482):
+  // per layer, true for a hit an earlier iteration took, which hit selection
+  // then passes over; an empty vector clears the mask.  Set after SuckInHits
+  // (which clears it).
+  void SetIterationHitMasks(const std::vector<std::vector<bool>>& layer_masks) {
+    if (layer_masks.empty()) {
+      check(mkf_event_of_hits_set_hit_mask(h_, nullptr, c_.stream()));
+      return;
+    }
+    require((int)layer_masks.size() == n_layers_, "SetIterationHitMasks: one mask per layer");
+    std::vector<unsigned char> flat;
+    for (int l = 0; l < n_layers_; ++l) {
+      require((int)layer_masks[l].size() == n_hits(l), "SetIterationHitMasks: one flag per hit of the layer");
+      for (bool b : layer_masks[l]) flat.push_back(b ? 1 : 0);
+    }
+    mask_.upload(flat, c_.stream());
+    check(mkf_event_of_hits_set_hit_mask(h_, mask_.get(), c_.stream()));
+    c_.sync();
+  }
   mkf_event_of_hits* get() const { return h_; }
   int n_layers() const { return n_layers_; }
   int n_hits(int layer) const { return n_hits_.empty() ? 0 : n_hits_[layer + 1] - n_hits_[layer]; }
@@ -662,6 +681,7 @@ class EventOfHits {
   int n_layers_;
   mkf_event_of_hits* h_ = nullptr;
   DeviceArray<float> hits_;
+  DeviceArray<unsigned char> mask_;
   std::vector<int> n_hits_;
 };
 

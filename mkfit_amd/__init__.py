@@ -101,7 +101,7 @@ ABI_SYMBOLS = [
     "mkf_select_hit_indices", "mkf_find_tracks_best_hit", "mkf_find_tracks_clone_engine",
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
     "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc", "mkf_backward_fit", "mkf_find_tracks_standard",
-    "mkf_set_find_shape", "mkf_get_find_shape", "mkf_autotune_find",
+    "mkf_set_find_shape", "mkf_get_find_shape", "mkf_autotune_find", "mkf_event_of_hits_set_hit_mask",
 ]
 CE_MAX_CANDS = 8
 
@@ -194,6 +194,7 @@ def lib():
                                                ctypes.POINTER(_c_void_p)]
         L.mkf_event_of_hits_destroy.argtypes = [_c_void_p]
         L.mkf_event_of_hits_load.argtypes = [_c_void_p, _c_void_p, _c_void_p, _c_void_p]
+        L.mkf_event_of_hits_set_hit_mask.argtypes = [_c_void_p, _c_void_p, _c_void_p]
         L.mkf_event_of_hits_layer.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_int)] + [ctypes.POINTER(_c_void_p)] * 4
         L.mkf_select_hit_indices.argtypes = [_c_void_p, ctypes.c_int, ctypes.c_int] + [_c_void_p] * 6
@@ -461,6 +462,16 @@ class EventOfHits:
         self._hits = hits
         _check(lib().mkf_event_of_hits_load(self.h, off.ctypes.data_as(_c_void_p), _ptr(hits),
                                             self.ctx._stream(stream)))
+
+    def set_hit_mask(self, mask, stream=None):
+        """mkf_event_of_hits_set_hit_mask: mask, a device uint8 tensor with one byte per loaded hit in load
+        order (nonzero: taken by an earlier iteration, never selected), or None to clear it."""
+        if mask is not None:
+            import torch
+            if mask.dtype != torch.uint8 or not mask.is_contiguous():
+                raise ValueError("mask must be a contiguous uint8 device tensor")
+        self._mask = mask
+        _check(lib().mkf_event_of_hits_set_hit_mask(self.h, _ptr(mask), self.ctx._stream(stream)))
 
     def layer_views(self, layer):
         """Copies of (ranks, bins [nq*256, 2], phis, qs) of one loaded layer, as numpy arrays."""

@@ -154,6 +154,7 @@ struct CeArgs {
   const int2* __restrict__ bins;
   const float* __restrict__ phis;
   const float* __restrict__ qs;
+  const unsigned char* __restrict__ mask;
   const float* __restrict__ shits;
   const int* __restrict__ ranks;
 };
@@ -302,7 +303,7 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(WAV
       } else {
         search = true;
         pt_l = fabsf(1.0f / pp[3]);
-        nh = scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) { Gh[lane][nh++] = gi - d.hit_off; });
+        nh = scan_window(d, w, a.bins, a.phis, a.qs, a.mask, [&](int gi) { Gh[lane][nh++] = gi - d.hit_off; });
         float* xs = &X[lane * kW];                       // words 0-35 are free until the end of (a)
 #pragma unroll
         for (int e = 0; e < 6; ++e) xs[e] = pp[e];
@@ -792,6 +793,7 @@ int find_combinatorial(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch*
   a.node_cap = node_cap;
   a.bs = reinterpret_cast<float*>(a.nodes + (size_t)node_cap * N);
   a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
+  a.mask = h->mask();
   a.ranks = h->d_ranks;
   const mkf_find_shape sh = c->find_shape[standard ? 1 : 0];
   const int slots = sh.slots ? sh.slots : (b->max_cands <= 6 ? 6 : 8);

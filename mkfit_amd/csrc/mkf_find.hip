@@ -186,10 +186,19 @@ k_loh_fill(int total, int pass, const uint32_t* __restrict__ keys_s, const int* 
   }
 }
 
+// The iteration's hit mask in sorted order: mask_s[inv[j]] = mask[j] for every event hit j.
+__global__ void __launch_bounds__(256)
+k_mask_sorted(int total, const int* __restrict__ inv, const unsigned char* __restrict__ mask,
+              unsigned char* __restrict__ mask_s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < total) mask_s[inv[j]] = mask[j] ? 1 : 0;
+}
+
 __global__ void __launch_bounds__(256)
 k_select(LayerDev d, int n, const float* __restrict__ par, const float* __restrict__ err,
          const int2* __restrict__ bins, const float* __restrict__ phis, const float* __restrict__ qs,
-         const int* __restrict__ ranks, int* __restrict__ xsize, int* __restrict__ xarr, int* __restrict__ wsr) {
+         const unsigned char* __restrict__ mask, const int* __restrict__ ranks, int* __restrict__ xsize,
+         int* __restrict__ xarr, int* __restrict__ wsr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   float p[6], e[21];
@@ -203,7 +212,7 @@ k_select(LayerDev d, int n, const float* __restrict__ par, const float* __restri
   int cnt = -1;
   if (w.wsr != MKF_WSR_OUTSIDE) {
     int c = 0;
-    scan_window(d, w, bins, phis, qs, [&](int gi) { xarr[(size_t)(c++) * n + t] = ranks[gi]; });
+    scan_window(d, w, bins, phis, qs, mask, [&](int gi) { xarr[(size_t)(c++) * n + t] = ranks[gi]; });
     cnt = c;
   }
   xsize[t] = cnt;
@@ -229,6 +238,7 @@ struct FindArgs {
   const int2* __restrict__ bins;
   const float* __restrict__ phis;
   const float* __restrict__ qs;
+  const unsigned char* __restrict__ mask;
   const float* __restrict__ shits;
   const int* __restrict__ ranks;
 };
@@ -282,7 +292,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))
         // The window's hits first (the scan keeps several loads in flight), then
         // their chi2 in scan order, one copy of the evaluation.
         int nc = 0;
-        scan_window(d, w, a.bins, a.phis, a.qs, [&](int gi) { s_cand[nc++][threadIdx.x] = gi; });
+        scan_window(d, w, a.bins, a.phis, a.qs, a.mask, [&](int gi) { s_cand[nc++][threadIdx.x] = gi; });
         for (int hc = 0; hc < nc; ++hc) {
           const int gi = s_cand[hc][threadIdx.x];
           float hp[3], he[6];
@@ -463,6 +473,7 @@ int mkf_event_of_hits_destroy(mkf_event_of_hits* h) {
   (void)hipDeviceSynchronize();
   free_hit_arrays(h);
   if (h->d_ce_nodes) (void)hipFree(h->d_ce_nodes);
+  if (h->d_mask_s) (void)hipFree(h->d_mask_s);
   if (h->ce_done) (void)hipEventDestroy(h->ce_done);
   if (h->d_layers) (void)hipFree(h->d_layers);
   if (h->d_bins) (void)hipFree(h->d_bins);
@@ -484,6 +495,7 @@ int mkf_event_of_hits_load(mkf_event_of_hits* h, const int* hit_offset, const fl
   if (total) CHECK(check_dev(hits, total * 9 * 4, "hits"));
   hipStream_t st = (hipStream_t)stream;
   CHECK(grow(h, total, st));
+  h->has_mask = false;
   for (int l = 0; l < L; ++l) {
     h->layers[l].hit_off = hit_offset[l];
     h->layers[l].n_hits = hit_offset[l + 1] - hit_offset[l];
@@ -521,6 +533,35 @@ int mkf_event_of_hits_load(mkf_event_of_hits* h, const int* hit_offset, const fl
   return MKF_OK;
 }
 
+int mkf_event_of_hits_set_hit_mask(mkf_event_of_hits* h, const unsigned char* mask, void* stream) {
+  if (!h) return fail(MKF_ERR_INVALID_ARG, "event of hits is NULL");
+  CHECK(set_device(h->ctx));
+  if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
+  if (!mask) {
+    h->has_mask = false;
+    return MKF_OK;
+  }
+  const LayerDev& last = h->layers[h->n_layers - 1];
+  const size_t total = (size_t)last.hit_off + last.n_hits;
+  hipStream_t st = (hipStream_t)stream;
+  if (total) CHECK(check_dev(mask, total, "mask"));
+  if (total > h->mask_cap) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (h->d_mask_s) HIP_TRY(hipFree(h->d_mask_s));
+    h->d_mask_s = nullptr;
+    h->mask_cap = 0;
+    HIP_TRY(hipMalloc(&h->d_mask_s, total));
+    h->mask_cap = total;
+  }
+  if (total) {
+    hipLaunchKernelGGL(k_mask_sorted, dim3(blocks_for(total, 256)), dim3(256), 0, st, (int)total, h->d_inv, mask,
+                       h->d_mask_s);
+    HIP_TRY(hipGetLastError());
+  }
+  h->has_mask = true;
+  return MKF_OK;
+}
+
 int mkf_event_of_hits_layer(mkf_event_of_hits* h, int layer, int* n_hits, int* nq, const int** ranks,
                             const int** bins, const float** phis, const float** qs) {
   if (!h) return fail(MKF_ERR_INVALID_ARG, "event of hits is NULL");
@@ -551,7 +592,7 @@ int mkf_select_hit_indices(mkf_event_of_hits* h, int layer, int n, const float* 
   CHECK(check_dev(xarr, kHitIdxMax * N * 4, "xarr"));
   CHECK(check_dev(wsr, N * 4, "wsr"));
   hipLaunchKernelGGL(k_select, dim3(blocks_for(N, 256)), dim3(256), 0, (hipStream_t)stream, h->layers[layer], n, par,
-                     err, h->d_bins, h->d_phis, h->d_qs, h->d_ranks, xsize, xarr, wsr);
+                     err, h->d_bins, h->d_phis, h->d_qs, h->mask(), h->d_ranks, xsize, xarr, wsr);
   HIP_TRY(hipGetLastError());
   return MKF_OK;
 }
@@ -599,6 +640,7 @@ int mkf_find_tracks_best_hit(mkf_context* c, mkf_event_of_hits* h, const mkf_fin
   a.layers = h->d_layers;
   a.bins = h->d_bins;
   a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits; a.ranks = h->d_ranks;
+  a.mask = h->mask();
   hipLaunchKernelGGL(k_find_best_hit, dim3(blocks_for(N, 256)), dim3(256), 0, (hipStream_t)stream, a);
   HIP_TRY(hipGetLastError());
   return MKF_OK;

@@ -146,9 +146,12 @@ __device__ __forceinline__ void hit_window(const LayerDev& d, const float (&par)
 // is read on its own.  The hits visited, and their order, are the bin-by-bin
 // scan's.
 constexpr int kScanUnroll = 8;
+// mask: per sorted hit, nonzero for a hit taken by an earlier iteration (MkFinder.cc:476-482), or NULL;
+// such a hit is passed over and does not count toward the kHitIdxMax.
 template <class F>
 __device__ __forceinline__ int scan_window(const LayerDev& d, const Window& w, const int2* __restrict__ bins,
-                                           const float* __restrict__ phis, const float* __restrict__ qs, F&& on_hit) {
+                                           const float* __restrict__ phis, const float* __restrict__ qs,
+                                           const unsigned char* __restrict__ mask, F&& on_hit) {
   int cnt = 0;
   for (int qi = w.qb1; qi < w.qb2 && cnt < kHitIdxMax; ++qi) {
     const int2* __restrict__ row = bins + d.bin_off + qi * kNPhi;
@@ -182,6 +185,7 @@ __device__ __forceinline__ int scan_window(const LayerDev& d, const Window& w, c
           const float ddq = fabsf(w.q - hq[u]);
           const float ddphi = cdist(fabsf(w.phi - hp[u]));
           if (ddq >= w.dq || ddphi >= w.dphi) continue;
+          if (mask && mask[gi + u]) continue;
           on_hit(gi + u);
           ++cnt;
         }
@@ -277,4 +281,9 @@ struct mkf_event_of_hits {
   // each call's kernel, makes the next call's stream wait for it.
   std::mutex ce_mu;
   hipEvent_t ce_done = nullptr;
+  // The iteration's hit mask per sorted hit (mkf_event_of_hits_set_hit_mask); a load clears it.
+  unsigned char* d_mask_s = nullptr;
+  size_t mask_cap = 0;
+  bool has_mask = false;
+  const unsigned char* mask() const { return has_mask ? d_mask_s : nullptr; }
 };

@@ -222,7 +222,14 @@ class LayerDesc(ctypes.Structure):
 class _FindEvent(ctypes.Structure):
     _fields_ = [("n_layers", ctypes.c_int), ("layers", ctypes.c_void_p), ("hit_offset", ctypes.c_void_p),
                 ("hits", ctypes.c_void_p), ("first_layer", ctypes.c_int), ("chi2_cut", ctypes.c_float),
-                ("pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int), ("intra_pflags", ctypes.c_int)]
+                ("pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int), ("intra_pflags", ctypes.c_int),
+                ("hit_mask", ctypes.c_void_p)]
+
+
+def _hit_mask(ev):
+    """The event's iteration hit mask as a uint8 array (kept alive by the caller), or None."""
+    m = ev.get("hit_mask")
+    return None if m is None else np.ascontiguousarray(m, np.uint8)
 
 
 class _FindTracks(ctypes.Structure):
@@ -253,14 +260,17 @@ def loh_build(desc, hits):
     return ranks, bins, phis, qs, nq.value
 
 
-def select_hit_indices(desc, hits, par, err):
+def select_hit_indices(desc, hits, par, err, mask=None):
+    """SelectHitIndices on one layer; mask: optional per-hit flags of hits taken by an earlier iteration."""
     L = lib()
     d = layer_array([desc])
     h = np.ascontiguousarray(hits, np.float32)
     p = np.ascontiguousarray(par, np.float32); e = np.ascontiguousarray(err, np.float32)
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
     n = p.shape[1]
     xs = np.zeros(n, np.int32); xa = np.zeros((16, n), np.int32); w = np.zeros(n, np.int32)
-    L.orc_select_hit_indices(ctypes.byref(d[0]), h.shape[0], _p(h), n, _p(p), _p(e), _p(xs), _p(xa), _p(w))
+    L.orc_select_hit_indices_masked(ctypes.byref(d[0]), h.shape[0], _p(h), _p(m), n, _p(p), _p(e), _p(xs), _p(xa),
+                                    _p(w))
     return xs, xa, w
 
 
@@ -284,8 +294,9 @@ def find_best_hit(ev, descs, first_layer, chi2_cut=15.0, pflags=0, bfield=3.8, n
     nfound = np.zeros(n, np.int32)
     off = np.ascontiguousarray(ev["hit_offset"], np.int32)
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
+    hm = _hit_mask(ev)
     fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags,
-                    int(bool(prop_to_hit)), int(intra_pflags))
+                    int(bool(prop_to_hit)), int(intra_pflags), _p(hm))
     mg = np.full((3, nl, n), 1e30, np.float32) if margins else None
     wk = np.zeros((3, n), np.int32) if work else None
     ft = _FindTracks(n, _p(par), _p(err), _p(chg), _p(chi2), _p(hot), _p(nfound), _p(mg), _p(wk))
@@ -338,7 +349,9 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
                hot_idx=np.zeros((K, nl, n), np.int32))
     off = np.ascontiguousarray(ev["hit_offset"], np.int32)
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
-    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags, 0, 0)
+    hm = _hit_mask(ev)
+    fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags, 0, 0,
+                    _p(hm))
     pr = _CEParams(K, int(max_holes), int(max_consec_holes), int(bool(prop_to_hit)), int(intra_pflags))
     if margins:
         out["margins"] = np.full((4, nl, n), 1e30, np.float32)

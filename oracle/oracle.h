@@ -137,6 +137,8 @@ typedef struct orc_find_event {
   int pflags;                    /* finding_inter_layer_pflags */
   int prop_to_hit;               /* best hit: finding_requires_propagation_to_hit_pos (clone engine: see orc_ce_params) */
   int intra_pflags;              /* best hit: finding_intra_layer_pflags */
+  const unsigned char* hit_mask; /* [total] per hit in hit order, nonzero: taken by an earlier iteration, never
+                                    selected (MkFinder.cc:476-482); NULL: none */
 } orc_find_event;
 
 typedef struct orc_find_tracks {
@@ -162,6 +164,9 @@ int orc_loh_geometry(const orc_layer_desc* d, int* nq, float* qmin, float* qmax,
 int orc_loh_build(const orc_layer_desc* d, int n_hits, const float* hits, int* ranks, int* bins, float* phis,
                   float* qs);
 /* SelectHitIndices on states already at the layer: xsize [n], xarr [16][n] (-1 padded), wsr [n]. */
+/* mask: [n_hits] or NULL, as orc_find_event's hit_mask for the layer */
+int orc_select_hit_indices_masked(const orc_layer_desc* d, int n_hits, const float* hits, const unsigned char* mask,
+                                  int n, const float* par, const float* err, int* xsize, int* xarr, int* wsr);
 int orc_select_hit_indices(const orc_layer_desc* d, int n_hits, const float* hits, int n, const float* par,
                            const float* err, int* xsize, int* xarr, int* wsr);
 int orc_find_best_hit(const orc_ctx* ctx, const orc_find_event* ev, orc_find_tracks* tracks, int n_threads);

@@ -620,9 +620,11 @@ int find_batch(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* 
   E.ctx.rl_grid = c ? c->rl_grid : nullptr;
   E.ctx.xi_grid = c ? c->xi_grid : nullptr;
   E.loh.resize(ev->n_layers);
-  for (int l = 0; l < ev->n_layers; ++l)
+  for (int l = 0; l < ev->n_layers; ++l) {
     build_loh(ev->layers[l], ev->hit_offset[l + 1] - ev->hit_offset[l], ev->hits + (size_t)ev->hit_offset[l] * 9,
               E.loh[l]);
+    if (ev->hit_mask) E.loh[l].mask = ev->hit_mask + ev->hit_offset[l];
+  }
   const int n = tr->n_seeds, L = ev->n_layers, K = p->max_cands;
   std::atomic<int> next{0};
   auto work = [&]() {

@@ -234,9 +234,15 @@ int orc_loh_build(const orc_layer_desc* d, int n, const float* hits, int* ranks,
 
 int orc_select_hit_indices(const orc_layer_desc* d, int n_hits, const float* hits, int n, const float* par,
                            const float* err, int* xsize, int* xarr, int* wsr) {
+  return orc_select_hit_indices_masked(d, n_hits, hits, nullptr, n, par, err, xsize, xarr, wsr);
+}
+
+int orc_select_hit_indices_masked(const orc_layer_desc* d, int n_hits, const float* hits, const unsigned char* mask,
+                                  int n, const float* par, const float* err, int* xsize, int* xarr, int* wsr) {
   if (!d || n < 0) return 1;
   LoH L;
   build_loh(*d, n_hits, hits, L);
+  L.mask = mask;
   for (int t0 = 0; t0 < n; t0 += NN) {
     const int cnt = std::min(NN, n - t0);
     alignas(64) float p[6 * NN], e[21 * NN];
@@ -270,9 +276,11 @@ int orc_find_best_hit(const orc_ctx* c, const orc_find_event* ev, orc_find_track
   E.intra_pflags = ev->intra_pflags;
   E.ctx = ctx;
   E.loh.resize(ev->n_layers);
-  for (int l = 0; l < ev->n_layers; ++l)
+  for (int l = 0; l < ev->n_layers; ++l) {
     build_loh(ev->layers[l], ev->hit_offset[l + 1] - ev->hit_offset[l], ev->hits + (size_t)ev->hit_offset[l] * 9,
               E.loh[l]);
+    if (ev->hit_mask) E.loh[l].mask = ev->hit_mask + ev->hit_offset[l];
+  }
   const int n = tr->n_tracks;
   const int nblk = (n + NN - 1) / NN;
   std::atomic<int> next{0};

@@ -63,6 +63,7 @@ struct LoH {
   std::vector<int> first, end;  // [nq * 256]
   std::vector<int> ranks;       // sorted position -> original hit index
   std::vector<float> phis, qs;  // in sorted order
+  const unsigned char* mask = nullptr;   // per original hit: taken by an earlier iteration, or NULL
 };
 
 // SuckInHits (HitStructures.cc:77-183); the LSD radix sort's ranks are the
@@ -232,6 +233,7 @@ inline void select_hit_indices(const orc_layer_desc& d, const LoH& L, const floa
         const int pb = pi & kPhiMask;
         const size_t bi = (size_t)qi * kNPhi + pb;
         for (int hi = L.first[bi]; hi < L.end[bi]; ++hi) {
+          if (L.mask && L.mask[L.ranks[hi]]) continue;      // the iteration's hit mask (MkFinder.cc:476-482)
           if (xsize[it] >= kHitIdxMax) break;
           const float ddq = std::abs(q - L.qs[hi]);
           const float ddphi = cdist(std::abs(phi - L.phis[hi]));

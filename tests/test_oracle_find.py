@@ -87,6 +87,72 @@ def test_select_hit_indices_matches_brute_force():
     assert set(np.unique(wsr)) <= {0, 1, 2}
 
 
+def test_select_hit_indices_skips_masked_hits():
+    """The iteration's hit mask (MkFinder.cc:476-482): a masked hit is passed over before the window test and
+    does not count toward the 16-hit cap, so each list is the brute-force list without the masked hits, cut
+    at 16; an all-zero mask changes nothing."""
+    rng = np.random.default_rng(7)
+    desc = _layer()
+    h = _random_hits(rng, 40000)
+    n = 1500
+    par = np.zeros((6, n), np.float32); err = np.zeros((21, n), np.float32)
+    phi = rng.uniform(-np.pi, np.pi, n).astype(np.float32)
+    par[0] = 20 * np.cos(phi); par[1] = 20 * np.sin(phi); par[2] = rng.uniform(-59, 59, n)
+    par[3] = rng.uniform(0.1, 2, n); par[4] = phi; par[5] = rng.uniform(0.3, 2.8, n)
+    s = rng.uniform(1e-2, 0.3, n).astype(np.float32) ** 2
+    err[0] = s; err[2] = s; err[5] = rng.uniform(1e-2, 0.8, n) ** 2
+    mask = (rng.random(len(h)) < 0.4).astype(np.uint8)
+    xs, xa, wsr = oracle.select_hit_indices(desc, h, par, err, mask=mask)
+    xs0, xa0, _ = oracle.select_hit_indices(desc, h, par, err)
+    xz, xaz, _ = oracle.select_hit_indices(desc, h, par, err, mask=np.zeros(len(h), np.uint8))
+    assert np.array_equal(xz, xs0) and np.array_equal(xaz, xa0)
+    ranks, bins, phis, qs, nq = oracle.loh_build(desc, h)
+    ref, wins = _brute_select(desc, h, par, err, ranks, phis, qs)
+    capped = edge = 0
+    for t in range(n):
+        got = xa[:xs[t], t]
+        assert not mask[got].any()
+        exp = np.array([i for i in ref[t] if not mask[i]][:16], np.int64)
+        if not np.array_equal(got, exp):
+            assert edge_explained(wins[t], par[:, t], h, np.setxor1d(got, exp), eps=1e-5), (t, got, exp)
+            edge += 1
+        capped += len(ref[t]) > 16 and xs0[t] == 16
+    assert capped > 50 and edge <= 3
+
+
+def test_finders_never_take_masked_hits():
+    """Best-hit and clone-engine finding with a mask over half of every finding layer's hits: no masked hit
+    in any candidate, an all-zero mask gives the unmasked result bit for bit, and masking every hit of the
+    seeds' own tracks leaves those tracks without hits."""
+    geo, ev = make_find_event()
+    descs = mk.barrel_layer_descs(geo)
+    rng = np.random.default_rng(8)
+    total = ev["hit_offset"][-1]
+    mask = (rng.random(total) < 0.5).astype(np.uint8)
+    mask[:ev["hit_offset"][3]] = 0                   # seed layers
+    for run in (lambda e: oracle.find_best_hit(e, descs, 3, chi2_cut=15.0, n_threads=4),
+                lambda e: oracle.find_clone_engine(e, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=4)):
+        plain = run(ev)
+        zero = run(dict(ev, hit_mask=np.zeros(total, np.uint8)))
+        for k in plain:
+            assert np.array_equal(plain[k], zero[k]), k
+        got = run(dict(ev, hit_mask=mask))
+        hot = got["hot_idx"]
+        off = ev["hit_offset"]
+        for l in range(3, len(descs)):
+            h = hot[..., l, :] if hot.ndim == 3 else hot[l]
+            taken = h[h >= 0] + off[l]
+            assert not mask[taken].any(), l
+    own = np.isin(ev["hit_mc"], ev["seed_mc"]).astype(np.uint8)
+    own[:ev["hit_offset"][3]] = 0
+    got = oracle.find_best_hit(dict(ev, hit_mask=own), descs, 3, chi2_cut=15.0, n_threads=4)
+    hot = got["hot_idx"][3:]
+    off = ev["hit_offset"][3:-1]
+    ok = hot >= 0
+    truth = np.where(ok, ev["hit_mc"][off[:, None] + np.maximum(hot, 0)], -5)
+    assert not (truth == ev["seed_mc"][None, :]).any()
+
+
 def test_window_edge_counts_the_layer_thickness():
     """SelectHitIndices' z test (MkFinder.cc:310, 336) widens the window's z reach by
     the layer's half thickness seen along the track, 0.5 (rout - rin) / tan(theta), added in quadrature: on a
