@@ -262,6 +262,17 @@ int mkf_event_of_hits_load(mkf_event_of_hits* eoh, const int* hit_offset, const 
  * passed this call. */
 int mkf_event_of_hits_set_hit_mask(mkf_event_of_hits* eoh, const unsigned char* mask, void* stream);
 
+/* Dead modules (DeadRegion, Hit.h:333-337; LayerOfHits::SuckInDeads,
+ * HitStructures.cc:186-212): regions is a host array [n][5] of {layer, phi1,
+ * phi2, q1, q2}, each marking the (q bin, phi bin) rectangle of its layer (the
+ * phi range wrapping when phi1's bin lies after phi2's).  The combinatorial
+ * finders then give a candidate whose window's central q row crosses a marked
+ * bin, and that takes no hit on the layer, the invalid hit -7, which is no hole
+ * (MkFinder.cc:451-462, 1092-1096, 1263-1267); best-hit finding and the
+ * per-layer entry points do not model it, as the reference's AddBestHit does
+ * not.  Replaces earlier regions; n = 0 clears them; loads keep them. */
+int mkf_event_of_hits_set_dead_regions(mkf_event_of_hits* eoh, int n, const float* regions, void* stream);
+
 /* Device views of a loaded layer (valid until the next load or destroy):
  * ranks [n] sorted position -> hit index within the layer (LayerOfHits
  * m_hit_ranks), bins [nq*256][2] first/end per (q bin, phi bin) (m_phi_bin_infos;

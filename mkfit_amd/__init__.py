@@ -102,6 +102,7 @@ ABI_SYMBOLS = [
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
     "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc", "mkf_backward_fit", "mkf_find_tracks_standard",
     "mkf_set_find_shape", "mkf_get_find_shape", "mkf_autotune_find", "mkf_event_of_hits_set_hit_mask",
+    "mkf_event_of_hits_set_dead_regions",
 ]
 CE_MAX_CANDS = 8
 
@@ -195,6 +196,7 @@ def lib():
         L.mkf_event_of_hits_destroy.argtypes = [_c_void_p]
         L.mkf_event_of_hits_load.argtypes = [_c_void_p, _c_void_p, _c_void_p, _c_void_p]
         L.mkf_event_of_hits_set_hit_mask.argtypes = [_c_void_p, _c_void_p, _c_void_p]
+        L.mkf_event_of_hits_set_dead_regions.argtypes = [_c_void_p, ctypes.c_int, _c_void_p, _c_void_p]
         L.mkf_event_of_hits_layer.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_int)] + [ctypes.POINTER(_c_void_p)] * 4
         L.mkf_select_hit_indices.argtypes = [_c_void_p, ctypes.c_int, ctypes.c_int] + [_c_void_p] * 6
@@ -472,6 +474,13 @@ class EventOfHits:
                 raise ValueError("mask must be a contiguous uint8 device tensor")
         self._mask = mask
         _check(lib().mkf_event_of_hits_set_hit_mask(self.h, _ptr(mask), self.ctx._stream(stream)))
+
+    def set_dead_regions(self, regions, stream=None):
+        """mkf_event_of_hits_set_dead_regions: regions [n][5] (layer, phi1, phi2, q1, q2), or None to clear."""
+        r = np.zeros((0, 5), np.float32) if regions is None else np.ascontiguousarray(regions, np.float32)
+        r = r.reshape(-1, 5)
+        _check(lib().mkf_event_of_hits_set_dead_regions(self.h, len(r), r.ctypes.data_as(_c_void_p) if len(r)
+                                                        else None, self.ctx._stream(stream)))
 
     def layer_views(self, layer):
         """Copies of (ranks, bins [nq*256, 2], phis, qs) of one loaded layer, as numpy arrays."""

@@ -155,6 +155,7 @@ struct CeArgs {
   const float* __restrict__ phis;
   const float* __restrict__ qs;
   const unsigned char* __restrict__ mask;
+  const unsigned char* __restrict__ dead;     // dead-module bins per (layer, q bin, phi bin), or NULL
   const float* __restrict__ shits;
   const int* __restrict__ ranks;
 };
@@ -282,7 +283,7 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(WAV
     //   (a3) each lane turns its accepted hits into records, in scan order.
     int nh = 0;
     float pt_l = 0.f;
-    bool search = false;
+    bool search = false, gap = false;
     if (active) {
       float pp[6], pe[21];
 #pragma unroll
@@ -319,6 +320,10 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(WAV
         }
         int fake = (st.ninside + st.ntail < a.max_holes && st.ntail < a.max_consec) ? -1 : -2;
         if (w.wsr == MKF_WSR_EDGE) fake = -3;
+        else if (a.dead) {          // XWsrResult::m_in_gap: a dead bin of the central q row in the window (MkFinder.cc:451-462)
+          const unsigned char* __restrict__ row = a.dead + d.bin_off + q_bin_checked(d, w.q) * kNPhi;
+          for (int pi = w.pb1; pi < w.pb2 && !gap; ++pi) gap = row[pi & 0xff] != 0;
+        }
         const int nt = fake == -1 ? st.ntail + 1 : st.ntail;
         fcode[lane] = fake;
         X[lane * kW + kXScore] = score_calc(st.nfound, nt, 0, st.ninside, st.chi2 < 0 ? 0.f : st.chi2, pt_l, true);
@@ -371,10 +376,11 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(WAV
     __syncthreads();
     // (a3) records of the accepted hits, in scan order
     if (search) {
-      int ne = 0;
+      int ne = 0, nacc = 0;
       for (int hc = 0; hc < nh; ++hc) {
         const float c2 = Gc[lane][hc];
         if (!(c2 >= 0.f)) continue;
+        ++nacc;
         const float tot = st.chi2 + c2;
         const float sc = score_calc(st.nfound + 1, 0, 0, st.ninside + st.ntail, tot < 0 ? 0.f : tot,
                                     STD ? Gp[lane][hc] : pt_l, true);
@@ -413,6 +419,10 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(WAV
         }
       }
       ecnt[lane] = ne;
+      if (gap && nacc == 0) {      // dead module, no hit taken: -7, which is no hole (MkFinder.cc:1092-1096, 1263-1267)
+        fcode[lane] = -7;
+        X[lane * kW + kXScore] = score_calc(st.nfound, st.ntail, 0, st.ninside, st.chi2 < 0 ? 0.f : st.chi2, pt_l, true);
+      }
     }
     // STD: a searching lane's new candidates start from its state at the layer, which (a1) left in words
     // 0-26; the others (held back, stopped, idle) keep their own.
@@ -794,6 +804,7 @@ int find_combinatorial(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch*
   a.bs = reinterpret_cast<float*>(a.nodes + (size_t)node_cap * N);
   a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
   a.mask = h->mask();
+  a.dead = h->dead();
   a.ranks = h->d_ranks;
   const mkf_find_shape sh = c->find_shape[standard ? 1 : 0];
   const int slots = sh.slots ? sh.slots : (b->max_cands <= 6 ? 6 : 8);

@@ -474,6 +474,7 @@ int mkf_event_of_hits_destroy(mkf_event_of_hits* h) {
   free_hit_arrays(h);
   if (h->d_ce_nodes) (void)hipFree(h->d_ce_nodes);
   if (h->d_mask_s) (void)hipFree(h->d_mask_s);
+  if (h->d_dead) (void)hipFree(h->d_dead);
   if (h->ce_done) (void)hipEventDestroy(h->ce_done);
   if (h->d_layers) (void)hipFree(h->d_layers);
   if (h->d_bins) (void)hipFree(h->d_bins);
@@ -559,6 +560,46 @@ int mkf_event_of_hits_set_hit_mask(mkf_event_of_hits* h, const unsigned char* ma
     HIP_TRY(hipGetLastError());
   }
   h->has_mask = true;
+  return MKF_OK;
+}
+
+int mkf_event_of_hits_set_dead_regions(mkf_event_of_hits* h, int n, const float* regions, void* stream) {
+  if (!h) return fail(MKF_ERR_INVALID_ARG, "event of hits is NULL");
+  CHECK(set_device(h->ctx));
+  if (n < 0 || (n > 0 && !regions)) return fail(MKF_ERR_INVALID_ARG, "dead regions: bad count or NULL");
+  if (n == 0) {
+    h->has_dead = false;
+    return MKF_OK;
+  }
+  // LayerOfHits::SuckInDeads (HitStructures.cc:186-212) per region's layer; a phi bin of 256 (phi2 = +pi)
+  // stands for bin 0.
+  std::vector<unsigned char> dead((size_t)h->total_bins, 0);
+  for (int r = 0; r < n; ++r) {
+    const float* g = regions + (size_t)r * 5;
+    const int layer = (int)g[0];
+    if (layer < 0 || layer >= h->n_layers || !(g[0] == (float)layer))
+      return fail(MKF_ERR_INVALID_ARG, "dead region %d: layer %g is not a layer of the event of hits", r, g[0]);
+    const LayerDev& d = h->layers[layer];
+    const int q1 = q_bin_checked(d, g[3]), q2 = q_bin_checked(d, g[4]) + 1;
+    const int p1 = phi_bin(g[1]), p2 = phi_bin(g[2]) + 1;
+    if (p1 < -2 || p1 > kNPhi || p2 < -1 || p2 > kNPhi + 1)
+      return fail(MKF_ERR_INVALID_ARG, "dead region %d: phi outside [-pi, pi]", r);
+    unsigned char* row0 = dead.data() + d.bin_off;
+    for (int q = q1; q < q2; ++q) {
+      unsigned char* row = row0 + (size_t)q * kNPhi;
+      if (p1 > p2) {
+        for (int p = p1; p < kNPhi; ++p) row[p & 0xff] = 1;
+        for (int p = 0; p < p2; ++p) row[p & 0xff] = 1;
+      } else {
+        for (int p = p1; p < p2; ++p) row[p & 0xff] = 1;
+      }
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (!h->d_dead) HIP_TRY(hipMalloc(&h->d_dead, (size_t)h->total_bins));
+  HIP_TRY(hipMemcpyAsync(h->d_dead, dead.data(), (size_t)h->total_bins, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  h->has_dead = true;
   return MKF_OK;
 }
 

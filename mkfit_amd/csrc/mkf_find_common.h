@@ -50,17 +50,17 @@ inline void setup_bins(float qmin, float qmax, float dq, int& nq, float& qmin_ex
   fq = nq / (qmax - qmin);
 }
 
-__device__ __forceinline__ int q_bin_checked(const LayerDev& d, float q) {   // HitStructures.h:237-245
+__host__ __device__ inline int q_bin_checked(const LayerDev& d, float q) {   // HitStructures.h:237-245
   const int b = (int)((q - d.qmin) * d.fq);
-  return min(max(b, 0), d.nq - 1);
+  return b < 0 ? 0 : (b > d.nq - 1 ? d.nq - 1 : b);
 }
 // atan2 of float coordinates, correctly rounded to float (as the host's atan2f of the reference): the
 // double-precision result rounded once.  Phi decides fine-bin keys and window bins, where the 1-ulp
 // differences of a faster float atan2 change which bin a hit or a window edge falls in.
 __device__ __forceinline__ float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
 
-__device__ __forceinline__ int phi_bin_fine(float phi) { return (int)floorf(kFPhiFine * (phi + kPI)); }
-__device__ __forceinline__ int phi_bin(float phi) { return phi_bin_fine(phi) >> 2; }
+__host__ __device__ inline int phi_bin_fine(float phi) { return (int)floorf(kFPhiFine * (phi + kPI)); }
+__host__ __device__ inline int phi_bin(float phi) { return phi_bin_fine(phi) >> 2; }
 __device__ __forceinline__ float cdist(float a) { return a > kPI ? kTwoPI - a : a; }   // Config.h:441
 
 // ---- SelectHitIndices window (MkFinder.cc:166-195, 227-430) -----------------
@@ -286,4 +286,8 @@ struct mkf_event_of_hits {
   size_t mask_cap = 0;
   bool has_mask = false;
   const unsigned char* mask() const { return has_mask ? d_mask_s : nullptr; }
+  // Dead-module bins, one byte per (layer, q bin, phi bin) (mkf_event_of_hits_set_dead_regions).
+  unsigned char* d_dead = nullptr;
+  bool has_dead = false;
+  const unsigned char* dead() const { return has_dead ? d_dead : nullptr; }
 };

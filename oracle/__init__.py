@@ -223,7 +223,13 @@ class _FindEvent(ctypes.Structure):
     _fields_ = [("n_layers", ctypes.c_int), ("layers", ctypes.c_void_p), ("hit_offset", ctypes.c_void_p),
                 ("hits", ctypes.c_void_p), ("first_layer", ctypes.c_int), ("chi2_cut", ctypes.c_float),
                 ("pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int), ("intra_pflags", ctypes.c_int),
-                ("hit_mask", ctypes.c_void_p)]
+                ("hit_mask", ctypes.c_void_p), ("n_dead", ctypes.c_int), ("dead_regions", ctypes.c_void_p)]
+
+
+def _dead_regions(ev):
+    """The event's dead-module regions [n][5] (layer, phi1, phi2, q1, q2) as float32, or None."""
+    d = ev.get("dead_regions")
+    return None if d is None or len(d) == 0 else np.ascontiguousarray(d, np.float32).reshape(-1, 5)
 
 
 def _hit_mask(ev):
@@ -349,9 +355,9 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
                hot_idx=np.zeros((K, nl, n), np.int32))
     off = np.ascontiguousarray(ev["hit_offset"], np.int32)
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
-    hm = _hit_mask(ev)
+    hm, dr = _hit_mask(ev), _dead_regions(ev)
     fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags, 0, 0,
-                    _p(hm))
+                    _p(hm), 0 if dr is None else len(dr), _p(dr))
     pr = _CEParams(K, int(max_holes), int(max_consec_holes), int(bool(prop_to_hit)), int(intra_pflags))
     if margins:
         out["margins"] = np.full((4, nl, n), 1e30, np.float32)

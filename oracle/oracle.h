@@ -139,6 +139,8 @@ typedef struct orc_find_event {
   int intra_pflags;              /* best hit: finding_intra_layer_pflags */
   const unsigned char* hit_mask; /* [total] per hit in hit order, nonzero: taken by an earlier iteration, never
                                     selected (MkFinder.cc:476-482); NULL: none */
+  int n_dead;                    /* dead-module regions (DeadRegion, Hit.h:333-337) marked in their layers' */
+  const float* dead_regions;     /* bins (LayerOfHits::SuckInDeads): [n_dead][5] layer, phi1, phi2, q1, q2 */
 } orc_find_event;
 
 typedef struct orc_find_tracks {

@@ -229,7 +229,8 @@ struct LaneCounts {
 void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const float* hits, int n_hits,
                            const float* parP, const float* errP, const int* chg, const LaneCounts* lc,
                            const int* trk, const int* xs, const int* xa, const int* wsr, int n_proc,
-                           std::vector<IdxChi2>& entries, float* c2m = nullptr) {
+                           std::vector<IdxChi2>& entries, float* c2m = nullptr, const int* in_gap = nullptr) {
+  int added[NN] = {};                                   // nHitsAdded
   if (c2m)
     for (int i = 0; i < NN; ++i) c2m[i] = 1e30f;
   const orc_find_event& ev = *E.ev;
@@ -272,12 +273,14 @@ void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const floa
       e.score = score_struct(e);
       e.xpos = hc;
       entries.push_back(e);
+      ++added[i];
     }
   }
   for (int i = 0; i < n_proc; ++i) {                    // the invalid-hit record
     if (wsr[i] == WSR_Outside) continue;
     int fake = (lc[i].ninside + lc[i].ntail < P.max_holes && lc[i].ntail < P.max_consec_holes) ? -1 : -2;
     if (wsr[i] == WSR_Edge) fake = -3;
+    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;   // dead module (MkFinder.cc:1263-1267)
     IdxChi2 e;
     e.trkIdx = trk[i];
     e.hitIdx = fake;
@@ -302,7 +305,9 @@ void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const floa
 // penalizeTailMissHits and inFindCandidates.  XHitArr entries outside [0, n_hits) are skipped.
 void find_candidates_std(const CEEvent& E, Comb& cc, int layer, const orc_layer_desc& d, const float* hits,
                          int n_hits, const Block& B, const std::vector<int>& idx, const int* xs, const int* xa,
-                         const int* wsr, int n_proc, std::vector<Cand>& tmp, float* c2m = nullptr) {
+                         const int* wsr, int n_proc, std::vector<Cand>& tmp, float* c2m = nullptr,
+                         const int* in_gap = nullptr) {
+  int added[NN] = {};                                   // nHitsAdded
   if (c2m)
     for (int i = 0; i < NN; ++i) c2m[i] = 1e30f;
   const orc_find_event& ev = *E.ev;
@@ -346,6 +351,7 @@ void find_candidates_std(const CEEvent& E, Comb& cc, int layer, const orc_layer_
     prop_and_update(E, barrel, B.errP, B.parP, tmpChg, msErr, msPar, n_proc, oErr, oPar, P.intra_pflags);
     for (int i = 0; i < n_proc; ++i) {
       if (!pass[i]) continue;
+      ++added[i];
       Cand nc = cc.cands[idx[i]];
       for (int k = 0; k < 6; ++k) nc.par[k] = oPar[k * NN + i];
       for (int k = 0; k < 21; ++k) nc.err[k] = oErr[k * NN + i];
@@ -361,6 +367,7 @@ void find_candidates_std(const CEEvent& E, Comb& cc, int layer, const orc_layer_
     const Cand& c = cc.cands[idx[i]];
     int fake = (c.ninside + c.ntail < P.max_holes && c.ntail < P.max_consec_holes) ? -1 : -2;
     if (wsr[i] == WSR_Edge) fake = -3;
+    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;   // dead module (MkFinder.cc:1092-1096)
     Cand nc = c;
     for (int k = 0; k < 6; ++k) nc.par[k] = B.parP[k * NN + i];
     for (int k = 0; k < 21; ++k) nc.err[k] = B.errP[k * NN + i];
@@ -431,9 +438,9 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
         propagate_helix_to_r(B.err, B.par, B.chg, rad, B.errP, B.parP, n_proc, ev.pflags, nullptr, fail, E.ctx);
       else
         propagate_helix_to_z(B.err, B.par, B.chg, rad, B.errP, B.parP, n_proc, ev.pflags, nullptr, E.ctx);
-      int xs[NN], xa[kHitIdxMax * NN], wsr[NN];
+      int xs[NN], xa[kHitIdxMax * NN], wsr[NN], gap[NN];
       SelMargins sm[NN];
-      select_hit_indices(d, E.loh[layer], B.parP, B.errP, n_proc, xs, xa, wsr, mg ? sm : nullptr);
+      select_hit_indices(d, E.loh[layer], B.parP, B.errP, n_proc, xs, xa, wsr, mg ? sm : nullptr, gap);
       if (wk) wk[0] += n_proc;
       // find_tracks_handle_missed_layers
       for (int i = 0; i < n_proc; ++i) {
@@ -463,10 +470,10 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
       float c2m[NN];
       if (standard)
         find_candidates_std(E, cc, layer, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B, idx, xs, xa,
-                            wsr, n_proc, tmp, mg ? c2m : nullptr);
+                            wsr, n_proc, tmp, mg ? c2m : nullptr, gap);
       else
         find_candidates_block(E, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B.parP, B.errP, B.chg,
-                              lc, trk, xs, xa, wsr, n_proc, entries, mg ? c2m : nullptr);
+                              lc, trk, xs, xa, wsr, n_proc, entries, mg ? c2m : nullptr, gap);
       if (mg) {
         const int L = ev.n_layers;
         for (int i = 0; i < n_proc; ++i) {
@@ -624,6 +631,7 @@ int find_batch(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* 
     build_loh(ev->layers[l], ev->hit_offset[l + 1] - ev->hit_offset[l], ev->hits + (size_t)ev->hit_offset[l] * 9,
               E.loh[l]);
     if (ev->hit_mask) E.loh[l].mask = ev->hit_mask + ev->hit_offset[l];
+    if (ev->n_dead > 0) suck_in_deads(E.loh[l], l, ev->n_dead, ev->dead_regions);
   }
   const int n = tr->n_seeds, L = ev->n_layers, K = p->max_cands;
   std::atomic<int> next{0};

@@ -64,7 +64,29 @@ struct LoH {
   std::vector<int> ranks;       // sorted position -> original hit index
   std::vector<float> phis, qs;  // in sorted order
   const unsigned char* mask = nullptr;   // per original hit: taken by an earlier iteration, or NULL
+  std::vector<unsigned char> dead;       // [nq * 256] dead-module bins; empty: none
 };
+
+// LayerOfHits::SuckInDeads (HitStructures.cc:186-212) of the regions [n][5] {layer, phi1, phi2, q1, q2} that
+// name `layer`: their (q bin, phi bin) rectangles, the phi range wrapping when it crosses -pi.  A phi bin of
+// 256 (phi2 = +pi) stands for bin 0.
+inline void suck_in_deads(LoH& L, int layer, int n, const float* regions) {
+  for (int r = 0; r < n; ++r) {
+    const float* g = regions + (size_t)r * 5;
+    if ((int)g[0] != layer) continue;
+    if (L.dead.empty()) L.dead.assign((size_t)L.b.nq * kNPhi, 0);
+    const int q1 = q_bin_checked(L.b, g[3]), q2 = q_bin_checked(L.b, g[4]) + 1;
+    const int p1 = phi_bin(g[1]), p2 = phi_bin(g[2]) + 1;
+    for (int q = q1; q < q2; ++q) {
+      if (p1 > p2) {
+        for (int p = p1; p < kNPhi; ++p) L.dead[(size_t)q * kNPhi + p] = 1;
+        for (int p = 0; p < p2; ++p) L.dead[(size_t)q * kNPhi + (p & kPhiMask)] = 1;
+      } else {
+        for (int p = p1; p < p2; ++p) L.dead[(size_t)q * kNPhi + (p & kPhiMask)] = 1;
+      }
+    }
+  }
+}
 
 // SuckInHits (HitStructures.cc:77-183); the LSD radix sort's ranks are the
 // stable order of the keys.
@@ -152,8 +174,11 @@ inline float bin_edge_margin(float x) { return std::abs(x - std::round(x)); }
 
 // SelectHitIndices for NN lanes of propagated state (MkFinder.cc:227-684).  With `mg`, also the lanes'
 // decision margins.
+// in_gap, when given: per lane 1 if a dead bin of the window's central q row lies in its phi range
+// (XWsrResult::m_in_gap, MkFinder.cc:451-462).
 inline void select_hit_indices(const orc_layer_desc& d, const LoH& L, const float* par, const float* err, int n_proc,
-                        int* xsize, int* xarr /*[16][NN]*/, int* wsr, SelMargins* mg = nullptr) {
+                        int* xsize, int* xarr /*[16][NN]*/, int* wsr, SelMargins* mg = nullptr,
+                        int* in_gap = nullptr) {
   float dqv[NN], dphiv[NN], qv[NN], phiv[NN];
   int qb1v[NN], qb2v[NN], pb1v[NN], pb2v[NN];
   const float nSigmaPhi = 3, nSigmaZ = 3, nSigmaR = 3;
@@ -198,11 +223,17 @@ inline void select_hit_indices(const orc_layer_desc& d, const LoH& L, const floa
     pb2v[it] = phi_bin(phi + dphi) + 1;
   }
   for (int it = 0; it < n_proc; ++it) {
+    if (in_gap) in_gap[it] = 0;
     if (wsr[it] == WSR_Outside) {
       xsize[it] = -1;
       continue;
     }
     const float q = qv[it], phi = phiv[it], dphi = dphiv[it], dq = dqv[it];
+    if (in_gap && !L.dead.empty()) {
+      const int qc = q_bin_checked(L.b, q);
+      for (int pi = pb1v[it]; pi < pb2v[it] && !in_gap[it]; ++pi)
+        in_gap[it] = L.dead[(size_t)qc * kNPhi + (pi & kPhiMask)];
+    }
     if (mg) {
       const float r = std::sqrt(par[0 * NN + it] * par[0 * NN + it] + par[1 * NN + it] * par[1 * NN + it]);
       SelMargins& m = mg[it];

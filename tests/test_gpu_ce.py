@@ -197,3 +197,47 @@ def test_every_find_shape_gives_the_same_candidates(gpu_device):
     with pytest.raises(mk.MkfError):
         run(False, K=8)
     ctx.close()
+
+
+@pytest.mark.parametrize("standard", [False, True])
+def test_dead_regions_match_oracle(gpu_device, standard):
+    """Dead modules (mkf_event_of_hits_set_dead_regions): 60 regions of 0.15 rad x 16 cm on the finding layers,
+    some wrapping across phi = +-pi, with a fifth of the hits masked so that windows in them often hold no hit:
+    the candidates (with their -7 records) match the oracle run with the same regions, seed by seed; clearing
+    the regions gives the run without them back bit for bit."""
+    import torch
+    ev, descs = _event(34, 3000, 2.0)
+    rng = np.random.default_rng(34)
+    n = 60
+    phi1 = rng.uniform(-np.pi, np.pi - 0.15, n)
+    phi1[:6] = rng.uniform(np.pi - 0.1, np.pi, 6)                 # wrapping across +-pi
+    phi2 = phi1 + 0.15
+    phi2[:6] -= 2 * np.pi
+    zc = rng.uniform(-60, 60, n)
+    regions = np.stack([rng.integers(3, len(descs), n), phi1, phi2, zc - 8, zc + 8], 1).astype(np.float32)
+    total = int(ev["hit_offset"][-1])
+    mask = (rng.random(total) < 0.2).astype(np.uint8)
+    mask[:ev["hit_offset"][3]] = 0
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+    eoh.set_hit_mask(_t(mask, gpu_device))
+
+    def run():
+        b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, chi2_cut=30.0, max_cands=6)
+        (ctx.find_tracks_standard if standard else ctx.find_tracks_clone_engine)(eoh, b)
+        torch.cuda.synchronize()
+        return b.numpy()
+    without = run()
+    eoh.set_dead_regions(regions)
+    g = run()
+    assert (g["hot_idx"] == -7).sum() >= 10, (g["hot_idx"] == -7).sum()
+    evd = dict(ev, hit_mask=mask, dead_regions=regions)
+    cpu, spread = clone_engine_reference(evd, descs, 3, chi2_cut=30.0, max_cands=6, n_threads=8, standard=standard)
+    assert (cpu["hot_idx"] == -7).sum() >= 10
+    assert_ce_parity("dead regions %s" % ("standard" if standard else "clone engine"), g, cpu, spread, 3)
+    eoh.set_dead_regions(None)
+    again = run()
+    for k in ("n_cands", "hot_idx", "score"):
+        assert np.array_equal(again[k], without[k]), k
+    ctx.close()

@@ -633,3 +633,51 @@ def test_apex_stop_rule(case):
         np.testing.assert_array_equal(got["par"][0, :, 0], one["seed_par"][:, 0])
     else:
         assert np.all(first != -2), first
+
+
+def _gap_event(standard, masked=True):
+    """One noise-free seed through 7 barrel layers, with (masked) its own hits on layers 4 and 5 masked (an
+    earlier iteration took them), and the run of the combinatorial finder with optional dead regions."""
+    ev, descs = _tiny_events(67, 20, 0.0, n_layers=7)
+    s = 0
+    off = ev["hit_offset"]
+    own = [int(np.nonzero(ev["hit_mc"][off[l]:off[l + 1]] == ev["seed_mc"][s])[0][0]) for l in range(7)]
+    mask = np.zeros(off[-1], np.uint8)
+    if masked:
+        mask[off[4] + own[4]] = mask[off[5] + own[5]] = 1
+    one = dict(_one_seed(ev, s), hit_mask=mask)
+    h4 = ev["hits_flat"][off[4] + own[4]]
+    phi4, z4 = float(np.arctan2(h4[1], h4[0])), float(h4[2])
+    find = oracle.find_standard if standard else oracle.find_clone_engine
+
+    def run(dead):
+        return find(dict(one, dead_regions=np.array(dead, np.float32).reshape(-1, 5)), descs, 3, chi2_cut=30.0,
+                    max_cands=6, max_holes=4, max_consec_holes=1)
+    return run, phi4, z4, own, ev, descs
+
+
+@pytest.mark.parametrize("standard", [False, True])
+def test_dead_module_gap(standard):
+    """Dead modules (LayerOfHits::SuckInDeads, MkFinder.cc:451-462, 1092-1096 / 1263-1267): a candidate whose
+    window's central q row crosses a dead bin and that takes no hit there gets -7 instead of a hole, which
+    does not count toward the consecutive-hole limit.  The seed's own hits on layers 4 and 5 are masked,
+    max_consec_holes 1: without a dead region it takes -1 on layer 4 and -2 on layer 5 (stop); with one over
+    its path on layer 4 it takes -7 there and -1 on layer 5, and goes on to layer 6.  A dead region away
+    from the window changes no candidate; without the mask, where the seed's own hit on layer 4 is found in
+    the dead region, no candidate takes -7 (nHitsAdded > 0)."""
+    run, phi4, z4, own, ev, descs = _gap_event(standard)
+    best = lambda r: r["hot_idx"][0, :, 0]
+    plain = best(run([]))
+    assert plain[4] == -1 and plain[5] == -2 and plain[6] == -9, plain
+    gap = best(run([[4, phi4 - 0.05, phi4 + 0.05, z4 - 5.0, z4 + 5.0]]))
+    assert gap[4] == -7 and gap[5] == -1 and gap[6] != -9, gap
+    all_plain = run([])["hot_idx"]
+    away = run([[4, phi4 - 0.05, phi4 + 0.05, z4 + 20.0, z4 + 30.0]])["hot_idx"]
+    assert np.array_equal(away, all_plain)
+    # unmasked: the hit on layer 4 is found inside the dead region, so no candidate takes -7 there (the one
+    # continuing from the invalid hit keeps -1 and takes the hits after it)
+    run_u = _gap_event(standard, masked=False)[0]
+    unmasked = run_u([])["hot_idx"]
+    assert (unmasked[:, 4, 0] == -1).any(), unmasked[:, :, 0]
+    found = run_u([[4, phi4 - 0.05, phi4 + 0.05, z4 - 5.0, z4 + 5.0]])["hot_idx"]
+    assert np.array_equal(found, unmasked)

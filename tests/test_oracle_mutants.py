@@ -61,6 +61,11 @@ MUTANTS = {
 # before the best short candidate's insertion (HitStructures.cc:540-545), since the output keeps the
 # first max_cands either way.
 FIND_MUTANTS = {
+    # dead-module gaps (MkFinder.cc:451-462, 1263-1267)
+    "gap_ignores_hits": ("oracle_ce.cc",
+                         "else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;   // dead module (MkFinder.cc:1263-1267)",
+                         "else if (in_gap && in_gap[i]) fake = -7;"),
+    "gap_counts_as_hole": ("oracle_ce.cc", "if (idx == -1) ++c.ntail;", "if (idx == -1 || idx == -7) ++c.ntail;"),
     "score_valid_bonus": ("oracle_ce.cc", "kValidHitBonus = 4,", "kValidHitBonus = 5,"),
     "score_valid_slope": ("oracle_ce.cc", "kValidHitSlope = 0.2f", "kValidHitSlope = 0.25f"),
     "score_hole_penalty": ("oracle_ce.cc", "kMissingHitPenalty = 8,", "kMissingHitPenalty = 7,"),
@@ -234,6 +239,7 @@ def _find_pins():
         ("ce_k_limited", lambda: [fp.test_clone_engine_k_limited(z) for z in _param_values(fp.test_clone_engine_k_limited)]),
         ("ce_held_back_ahead", fp.test_clone_engine_held_back_goes_ahead),
         ("ce_charge_flip", fp.test_wrong_charge_seeds_flip_to_the_true_charge),
+        ("dead_module_gap", lambda: [fp.test_dead_module_gap(v) for v in (False, True)]),
         ("ce_composition_k6", lambda: ce.test_ce_matches_composition(6)),
     ]
 
