@@ -136,6 +136,13 @@ class Hit {
 };
 using HitVec = std::vector<Hit>;
 
+// Hit.h:333-338: a dead-module region of a layer, in phi and q (z on barrel layers, r on disks).
+struct DeadRegion {
+  float phi1, phi2, q1, q2;
+  DeadRegion(float a1, float a2, float b1, float b2) : phi1(a1), phi2(a2), q1(b1), q2(b2) {}
+};
+using DeadVec = std::vector<DeadRegion>;
+
 // Track.h:74-143: x, y, z, 1/pT, phi, theta; packed 6x6 covariance; charge.
 struct TrackState {
   TrackState() = default;
@@ -670,6 +677,18 @@ class EventOfHits {
     check(mkf_event_of_hits_set_hit_mask(h_, mask_.get(), c_.stream()));
     c_.sync();
   }
+  // EventOfHits::SuckInDeads (HitStructures.h:310-313): the layer's dead
+  // modules, replacing its earlier ones; kept across SuckInHits.
+  void SuckInDeads(int layer, const DeadVec& deadv) {
+    require(layer >= 0 && layer < n_layers_, "SuckInDeads: layer out of range");
+    deads_.resize(n_layers_);
+    deads_[layer] = deadv;
+    std::vector<float> flat;
+    for (int l = 0; l < n_layers_; ++l)
+      for (const DeadRegion& d : deads_[l]) flat.insert(flat.end(), {(float)l, d.phi1, d.phi2, d.q1, d.q2});
+    check(mkf_event_of_hits_set_dead_regions(h_, (int)(flat.size() / 5), flat.empty() ? nullptr : flat.data(),
+                                             c_.stream()));
+  }
   mkf_event_of_hits* get() const { return h_; }
   int n_layers() const { return n_layers_; }
   int n_hits(int layer) const { return n_hits_.empty() ? 0 : n_hits_[layer + 1] - n_hits_[layer]; }
@@ -683,6 +702,7 @@ class EventOfHits {
   DeviceArray<float> hits_;
   DeviceArray<unsigned char> mask_;
   std::vector<int> n_hits_;
+  std::vector<DeadVec> deads_;
 };
 
 using IdxChi2List = mkf_idx_chi2;

@@ -84,8 +84,8 @@ FIND_MUTANTS = {
                        "lc[i].ntail <= P.max_consec_holes) ? -1 : -2;"),
     "fake_holes_inside_only": ("oracle_ce.cc", "(lc[i].ninside + lc[i].ntail < P.max_holes",
                                "(lc[i].ninside < P.max_holes"),
-    "fake_edge_code": ("oracle_ce.cc", "if (wsr[i] == WSR_Edge) fake = -3;\n    IdxChi2 e;",
-                       "if (wsr[i] == WSR_Edge) fake = -1;\n    IdxChi2 e;"),
+    "fake_edge_code": ("oracle_ce.cc", "if (wsr[i] == WSR_Edge) fake = -3;\n    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;   // dead module (MkFinder.cc:1263-1267)",
+                       "if (wsr[i] == WSR_Edge) fake = -1;\n    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;"),
     "fake_tailholes": ("oracle_ce.cc", "e.ntailholes = fake == -1 ? lc[i].ntail + 1 : lc[i].ntail;",
                        "e.ntailholes = lc[i].ntail + 1;"),
     "hit_record_holes": ("oracle_ce.cc", "e.nholes = lc[i].ninside + lc[i].ntail;", "e.nholes = lc[i].ninside;"),
@@ -140,8 +140,8 @@ STD_MUTANTS = {
     "std_best_short_always": ("if (tc.score > cc.best_short.score) cc.best_short = tc;", "cc.best_short = tc;"),
     "std_held_dropped": ("std::vector<Cand>& held = standard ? tmp : extras;", "std::vector<Cand>& held = extras;"),
     "std_fake_consec_le": ("c.ntail < P.max_consec_holes) ? -1 : -2;", "c.ntail <= P.max_consec_holes) ? -1 : -2;"),
-    "std_fake_edge_code": ("if (wsr[i] == WSR_Edge) fake = -3;\n    Cand nc = c;",
-                           "if (wsr[i] == WSR_Edge) fake = -1;\n    Cand nc = c;"),
+    "std_fake_edge_code": ("if (wsr[i] == WSR_Edge) fake = -3;\n    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;   // dead module (MkFinder.cc:1092-1096)",
+                           "if (wsr[i] == WSR_Edge) fake = -1;\n    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;"),
     "std_sort_ascending": ("std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });",
                            "std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score < b.score; });"),
 }
