@@ -262,6 +262,19 @@ int mkf_event_of_hits_load(mkf_event_of_hits* eoh, const int* hit_offset, const 
  * passed this call. */
 int mkf_event_of_hits_set_hit_mask(mkf_event_of_hits* eoh, const unsigned char* mask, void* stream);
 
+/* Strip cluster charge (Hit::PackedData::charge_pcm, Hit.h:251-282): code is a
+ * device array of one byte per loaded hit, in load order, holding each hit's
+ * 8-bit charge code (0: below 1620 per cm; 0xff: overflow or pixel-like).  The
+ * combinatorial finders and mkf_find_candidates_clone_engine then take a hit
+ * only if its charge per cm times the track's path-length factor at the hit
+ * exceeds 1620 (passStripChargePCMfromTrack, MkFinder.cc:863-896, applied on
+ * every layer: the layer descriptors carry no pixel flag).  Best-hit finding
+ * does not apply it, as the reference's AddBestHit does not.  NULL clears the
+ * codes (every hit passes, as code 0xff); a load clears them.  The codes are
+ * copied on `stream`, so `code` may be reused once the stream has passed this
+ * call. */
+int mkf_event_of_hits_set_hit_charge(mkf_event_of_hits* eoh, const unsigned char* code, void* stream);
+
 /* Dead modules (DeadRegion, Hit.h:333-337; LayerOfHits::SuckInDeads,
  * HitStructures.cc:186-212): regions is a host array [n][5] of {layer, phi1,
  * phi2, q1, q2}, each marking the (q bin, phi bin) rectangle of its layer (the

@@ -102,7 +102,7 @@ ABI_SYMBOLS = [
     "mkf_add_best_hit", "mkf_find_candidates_clone_engine", "mkf_update_with_last_hit",
     "mkf_fit_adjoint", "mkf_fit_adjoint_noalloc", "mkf_backward_fit", "mkf_find_tracks_standard",
     "mkf_set_find_shape", "mkf_get_find_shape", "mkf_autotune_find", "mkf_event_of_hits_set_hit_mask",
-    "mkf_event_of_hits_set_dead_regions",
+    "mkf_event_of_hits_set_dead_regions", "mkf_event_of_hits_set_hit_charge",
 ]
 CE_MAX_CANDS = 8
 
@@ -196,6 +196,7 @@ def lib():
         L.mkf_event_of_hits_destroy.argtypes = [_c_void_p]
         L.mkf_event_of_hits_load.argtypes = [_c_void_p, _c_void_p, _c_void_p, _c_void_p]
         L.mkf_event_of_hits_set_hit_mask.argtypes = [_c_void_p, _c_void_p, _c_void_p]
+        L.mkf_event_of_hits_set_hit_charge.argtypes = [_c_void_p, _c_void_p, _c_void_p]
         L.mkf_event_of_hits_set_dead_regions.argtypes = [_c_void_p, ctypes.c_int, _c_void_p, _c_void_p]
         L.mkf_event_of_hits_layer.argtypes = [_c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_int)] + [ctypes.POINTER(_c_void_p)] * 4
@@ -474,6 +475,16 @@ class EventOfHits:
                 raise ValueError("mask must be a contiguous uint8 device tensor")
         self._mask = mask
         _check(lib().mkf_event_of_hits_set_hit_mask(self.h, _ptr(mask), self.ctx._stream(stream)))
+
+    def set_hit_charge(self, code, stream=None):
+        """mkf_event_of_hits_set_hit_charge: code, a device uint8 tensor with one byte per loaded hit in load
+        order, Hit::PackedData's charge code (0xff: pixel-like, passes), or None to clear it."""
+        if code is not None:
+            import torch
+            if code.dtype != torch.uint8 or not code.is_contiguous():
+                raise ValueError("charge codes must be a contiguous uint8 device tensor")
+        self._charge = code
+        _check(lib().mkf_event_of_hits_set_hit_charge(self.h, _ptr(code), self.ctx._stream(stream)))
 
     def set_dead_regions(self, regions, stream=None):
         """mkf_event_of_hits_set_dead_regions: regions [n][5] (layer, phi1, phi2, q1, q2), or None to clear."""

@@ -155,6 +155,7 @@ struct CeArgs {
   const float* __restrict__ phis;
   const float* __restrict__ qs;
   const unsigned char* __restrict__ mask;
+  const unsigned char* __restrict__ charge;   // per sorted hit: strip charge code, or NULL (every hit passes)
   const unsigned char* __restrict__ dead;     // dead-module bins per (layer, q bin, phi bin), or NULL
   const float* __restrict__ shits;
   const int* __restrict__ ranks;
@@ -362,7 +363,9 @@ __global__ void __launch_bounds__(kLanes) __attribute__((amdgpu_waves_per_eu(WAV
                   &xs[kXTrig]);
       const float c2 = fabsf(ob ? kalman_barrel<true, false>(q, qe, mp, me)
                                 : kalman_endcap<true, false>(q, qe, mp, me));
-      const bool acc = c2 < xs[kXCut] && strip_q_compatible(ob, pe, q, mp, me);
+      const int gi = __float_as_int(xs[kXOff]) + Gh[o][k];
+      const bool acc = c2 < xs[kXCut] && strip_q_compatible(ob, pe, q, mp, me) &&
+                       (!a.charge || pass_strip_charge(ob, a.charge[gi], q, me));
       Gc[o][k] = acc ? c2 : -1.f;
       if (STD && acc) {
         // getScoreCand of the new candidate takes the pT of its updated state (m_update_param_foo)
@@ -804,6 +807,7 @@ int find_combinatorial(mkf_context* c, mkf_event_of_hits* h, const mkf_ce_batch*
   a.bs = reinterpret_cast<float*>(a.nodes + (size_t)node_cap * N);
   a.layers = h->d_layers; a.bins = h->d_bins; a.phis = h->d_phis; a.qs = h->d_qs; a.shits = h->d_shits;
   a.mask = h->mask();
+  a.charge = h->charge();
   a.dead = h->dead();
   a.ranks = h->d_ranks;
   const mkf_find_shape sh = c->find_shape[standard ? 1 : 0];

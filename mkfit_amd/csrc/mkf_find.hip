@@ -194,6 +194,14 @@ k_mask_sorted(int total, const int* __restrict__ inv, const unsigned char* __res
   if (j < total) mask_s[inv[j]] = mask[j] ? 1 : 0;
 }
 
+// The hits' charge codes in sorted order: charge_s[inv[j]] = code[j] for every event hit j.
+__global__ void __launch_bounds__(256)
+k_charge_sorted(int total, const int* __restrict__ inv, const unsigned char* __restrict__ code,
+                unsigned char* __restrict__ charge_s) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < total) charge_s[inv[j]] = code[j];
+}
+
 __global__ void __launch_bounds__(256)
 k_select(LayerDev d, int n, const float* __restrict__ par, const float* __restrict__ err,
          const int2* __restrict__ bins, const float* __restrict__ phis, const float* __restrict__ qs,
@@ -474,6 +482,7 @@ int mkf_event_of_hits_destroy(mkf_event_of_hits* h) {
   free_hit_arrays(h);
   if (h->d_ce_nodes) (void)hipFree(h->d_ce_nodes);
   if (h->d_mask_s) (void)hipFree(h->d_mask_s);
+  if (h->d_charge_s) (void)hipFree(h->d_charge_s);
   if (h->d_dead) (void)hipFree(h->d_dead);
   if (h->ce_done) (void)hipEventDestroy(h->ce_done);
   if (h->d_layers) (void)hipFree(h->d_layers);
@@ -497,6 +506,7 @@ int mkf_event_of_hits_load(mkf_event_of_hits* h, const int* hit_offset, const fl
   hipStream_t st = (hipStream_t)stream;
   CHECK(grow(h, total, st));
   h->has_mask = false;
+  h->has_charge = false;
   for (int l = 0; l < L; ++l) {
     h->layers[l].hit_off = hit_offset[l];
     h->layers[l].n_hits = hit_offset[l + 1] - hit_offset[l];
@@ -560,6 +570,35 @@ int mkf_event_of_hits_set_hit_mask(mkf_event_of_hits* h, const unsigned char* ma
     HIP_TRY(hipGetLastError());
   }
   h->has_mask = true;
+  return MKF_OK;
+}
+
+int mkf_event_of_hits_set_hit_charge(mkf_event_of_hits* h, const unsigned char* code, void* stream) {
+  if (!h) return fail(MKF_ERR_INVALID_ARG, "event of hits is NULL");
+  CHECK(set_device(h->ctx));
+  if (!h->loaded) return fail(MKF_ERR_INVALID_ARG, "event of hits not loaded");
+  if (!code) {
+    h->has_charge = false;
+    return MKF_OK;
+  }
+  const LayerDev& last = h->layers[h->n_layers - 1];
+  const size_t total = (size_t)last.hit_off + last.n_hits;
+  hipStream_t st = (hipStream_t)stream;
+  if (total) CHECK(check_dev(code, total, "charge codes"));
+  if (total > h->charge_cap) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (h->d_charge_s) HIP_TRY(hipFree(h->d_charge_s));
+    h->d_charge_s = nullptr;
+    h->charge_cap = 0;
+    HIP_TRY(hipMalloc(&h->d_charge_s, total));
+    h->charge_cap = total;
+  }
+  if (total) {
+    hipLaunchKernelGGL(k_charge_sorted, dim3(blocks_for(total, 256)), dim3(256), 0, st, (int)total, h->d_inv, code,
+                       h->d_charge_s);
+    HIP_TRY(hipGetLastError());
+  }
+  h->has_charge = true;
   return MKF_OK;
 }
 

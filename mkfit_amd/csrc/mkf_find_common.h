@@ -251,6 +251,30 @@ __device__ __forceinline__ bool strip_q_compatible(bool barrel, const float (&pe
   return sqrtf(hitT2 * 3.f) + fmaxf(3.f * qErr, 0.5f) > resProj;
 }
 
+// passStripChargePCMfromTrack (MkFinder.cc:863-896) for one lane: the hit's charge per cm
+// (Hit::PackedData::get_charge_pcm, Hit.h:270-274, from its 8-bit code) times the track's
+// path-length factor at the hit (q: the state propagated to the hit) must exceed
+// Hit::minChargePerCM = 1620; the overflow code 0xff (Hit::maxChargePerCM) passes.
+__device__ __forceinline__ bool pass_strip_charge(bool barrel, unsigned code, const float (&q)[6],
+                                                  const float (&me)[6]) {
+  const unsigned pcm = code == 0 ? 0u : ((code - 1u) << 3) + 1620u;
+  if (pcm >= 1620u + (0xfeu << 3)) return true;
+  float qSF;
+  if (barrel) {
+    const float hitT2 = me[0] + me[2];
+    const float inv = 1.f / hitT2;
+    const float p0 = me[0] * inv, p1 = me[1] * inv, p2 = me[2] * inv;
+    const bool detXY_OK = fabsf(p0 * p2 - p1 * p1) < 0.1f;
+    float sinP, cosP;
+    sincosf(q[4], &sinP, &cosP);
+    const float sinT = fabsf(sinf(q[5]));
+    qSF = detXY_OK ? sinT * sqrtf(fabsf(1.f + cosP * cosP * p0 + sinP * sinP * p2 - 2.f * cosP * sinP * p1)) : 1.f;
+  } else {
+    qSF = fabsf(cosf(q[5]));
+  }
+  return (float)pcm * qSF > 1620.f;
+}
+
 }  // namespace mkf_find
 
 struct mkf_event_of_hits {
@@ -286,6 +310,11 @@ struct mkf_event_of_hits {
   size_t mask_cap = 0;
   bool has_mask = false;
   const unsigned char* mask() const { return has_mask ? d_mask_s : nullptr; }
+  // The hits' strip charge codes per sorted hit (mkf_event_of_hits_set_hit_charge); a load clears them.
+  unsigned char* d_charge_s = nullptr;
+  size_t charge_cap = 0;
+  bool has_charge = false;
+  const unsigned char* charge() const { return has_charge ? d_charge_s : nullptr; }
   // Dead-module bins, one byte per (layer, q bin, phi bin) (mkf_event_of_hits_set_dead_regions).
   unsigned char* d_dead = nullptr;
   bool has_dead = false;

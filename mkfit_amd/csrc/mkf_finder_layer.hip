@@ -45,6 +45,7 @@ struct LayerStepArgs {
   const int* __restrict__ hit_idx;
   const float* __restrict__ shits;
   const int* __restrict__ inv;
+  const unsigned char* __restrict__ charge;   // per sorted hit: strip charge code, or NULL (every hit passes)
   int* __restrict__ hit_code;
   int* __restrict__ n_entries;
   mkf_idx_chi2* __restrict__ entries;
@@ -181,7 +182,8 @@ __global__ void __launch_bounds__(256) k_find_candidates(LayerStepArgs a) {
     float mp[3], me[6], q[6];
     load_hit9(a.shits + (size_t)pos * 9, mp, me);
     const float c2 = hit_chi2(a, pp, pe, chg, mp, me, q);
-    if (c2 < max_c2 && strip_q_compatible(barrel, pe, q, mp, me)) {
+    if (c2 < max_c2 && strip_q_compatible(barrel, pe, q, mp, me) &&
+        (!a.charge || pass_strip_charge(barrel, a.charge[pos], q, me))) {
       mkf_idx_chi2 r;
       r.trk_idx = t;
       r.hit_idx = idx;
@@ -271,6 +273,7 @@ int prepare(mkf_context* c, mkf_event_of_hits* h, int layer, const mkf_finder_la
   a.n_inside = ln->n_inside_minus_one; a.n_tail = ln->n_tail_minus_one;
   a.shits = h->d_shits;
   a.inv = h->d_inv;
+  a.charge = h->charge();
   return MKF_OK;
 }
 

@@ -223,7 +223,8 @@ class _FindEvent(ctypes.Structure):
     _fields_ = [("n_layers", ctypes.c_int), ("layers", ctypes.c_void_p), ("hit_offset", ctypes.c_void_p),
                 ("hits", ctypes.c_void_p), ("first_layer", ctypes.c_int), ("chi2_cut", ctypes.c_float),
                 ("pflags", ctypes.c_int), ("prop_to_hit", ctypes.c_int), ("intra_pflags", ctypes.c_int),
-                ("hit_mask", ctypes.c_void_p), ("n_dead", ctypes.c_int), ("dead_regions", ctypes.c_void_p)]
+                ("hit_mask", ctypes.c_void_p), ("n_dead", ctypes.c_int), ("dead_regions", ctypes.c_void_p),
+                ("hit_charge", ctypes.c_void_p)]
 
 
 def _dead_regions(ev):
@@ -236,6 +237,25 @@ def _hit_mask(ev):
     """The event's iteration hit mask as a uint8 array (kept alive by the caller), or None."""
     m = ev.get("hit_mask")
     return None if m is None else np.ascontiguousarray(m, np.uint8)
+
+
+def _hit_charge(ev):
+    """The event's per-hit strip charge codes (Hit::PackedData's 8-bit charge_pcm, Hit.h:251-274) as uint8,
+    or None: then the combinatorial finders' charge cut passes every hit."""
+    c = ev.get("hit_charge")
+    return None if c is None else np.ascontiguousarray(c, np.uint8)
+
+
+def charge_code(pcm):
+    """Hit::PackedData::set_charge_pcm (Hit.h:264-268): charge per cm -> 8-bit code."""
+    pcm = np.asarray(pcm, np.int64)
+    return np.where(pcm < 1620, 0, np.minimum(0xff, ((pcm - 1620) >> 3) + 1)).astype(np.uint8)
+
+
+def charge_pcm(code):
+    """Hit::PackedData::get_charge_pcm (Hit.h:270-274): 8-bit code -> charge per cm."""
+    code = np.asarray(code, np.int64)
+    return np.where(code == 0, 0, ((code - 1) << 3) + 1620).astype(np.uint32)
 
 
 class _FindTracks(ctypes.Structure):
@@ -355,9 +375,9 @@ def find_clone_engine(ev, descs, first_layer, chi2_cut=30.0, pflags=0, bfield=3.
                hot_idx=np.zeros((K, nl, n), np.int32))
     off = np.ascontiguousarray(ev["hit_offset"], np.int32)
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
-    hm, dr = _hit_mask(ev), _dead_regions(ev)
+    hm, dr, hq = _hit_mask(ev), _dead_regions(ev), _hit_charge(ev)
     fe = _FindEvent(nl, ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), first_layer, chi2_cut, pflags, 0, 0,
-                    _p(hm), 0 if dr is None else len(dr), _p(dr))
+                    _p(hm), 0 if dr is None else len(dr), _p(dr), _p(hq))
     pr = _CEParams(K, int(max_holes), int(max_consec_holes), int(bool(prop_to_hit)), int(intra_pflags))
     if margins:
         out["margins"] = np.full((4, nl, n), 1e30, np.float32)
@@ -409,9 +429,10 @@ def _layer_event(ev, descs, chi2_cut, pflags, prop_to_hit, intra_pflags):
     layers = layer_array(descs)
     off = np.ascontiguousarray(ev["hit_offset"], np.int32)
     hits = np.ascontiguousarray(ev["hits_flat"], np.float32)
+    hq = _hit_charge(ev)
     fe = _FindEvent(len(descs), ctypes.cast(layers, ctypes.c_void_p), _p(off), _p(hits), 0, chi2_cut, pflags,
-                    int(bool(prop_to_hit)), int(intra_pflags))
-    return fe, (layers, off, hits)
+                    int(bool(prop_to_hit)), int(intra_pflags), None, 0, None, _p(hq))
+    return fe, (layers, off, hits, hq)
 
 
 def add_best_hit(ev, descs, layer, lanes, xsize, xarr, wsr, chi2_cut=15.0, prop_to_hit=False, intra_pflags=0,
@@ -447,6 +468,17 @@ def find_candidates_ce(ev, descs, layer, lanes, xsize, xarr, wsr, chi2_cut=30.0,
     if rc:
         raise RuntimeError("orc_find_candidates_ce failed (%d)" % rc)
     return ne, ent
+
+
+def pass_strip_charge(barrel, code, par, herr):
+    """passStripChargePCMfromTrack (MkFinder.cc:867-896) for one state par[6] and hit covariance herr[6]
+    given the hit's 8-bit charge code: (passes, corrected charge per cm)."""
+    L = lib()
+    p = np.ascontiguousarray(par, np.float32)
+    e = np.ascontiguousarray(herr, np.float32)
+    q = ctypes.c_float()
+    ok = L.orc_pass_strip_charge(int(bool(barrel)), int(code), _p(p), _p(e), ctypes.byref(q))
+    return bool(ok), q.value
 
 
 def update_with_last_hit(ev, descs, layer, lanes, hit_idx, pflags=0, prop_to_hit=True, bfield=3.8):

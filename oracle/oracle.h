@@ -141,6 +141,9 @@ typedef struct orc_find_event {
                                     selected (MkFinder.cc:476-482); NULL: none */
   int n_dead;                    /* dead-module regions (DeadRegion, Hit.h:333-337) marked in their layers' */
   const float* dead_regions;     /* bins (LayerOfHits::SuckInDeads): [n_dead][5] layer, phi1, phi2, q1, q2 */
+  const unsigned char* hit_charge; /* [total] per hit in hit order, Hit::PackedData's 8-bit charge code (Hit.h:251-274):
+                                      the combinatorial finders' strip charge cut (MkFinder.cc:867-896) applies;
+                                      NULL: every hit passes it (as code 0xff, pixel-like) */
 } orc_find_event;
 
 typedef struct orc_find_tracks {
@@ -249,6 +252,9 @@ int orc_find_candidates_ce(const orc_ctx* ctx, const orc_find_event* ev, const o
                            int* n_entries, orc_idx_chi2* entries);
 /* UpdateWithLastHit on `layer`: update prop_* with hit hit_idx[t] (ev->pflags for the propagation to the hit
  * when p->prop_to_hit) into par / err / chg; lanes with hit_idx outside the layer copy prop_* unchanged. */
+/* Checker hook: passStripChargePCMfromTrack (MkFinder.cc:867-896) for one state par[6] and hit covariance herr[6]
+   given the hit's 8-bit charge code: 1 when the hit passes; *qcorr the corrected charge per cm. */
+int orc_pass_strip_charge(int barrel, unsigned code, const float* par, const float* herr, float* qcorr);
 int orc_update_with_last_hit(const orc_ctx* ctx, const orc_find_event* ev, const orc_ce_params* p, int layer,
                              const orc_finder_lanes* lanes, const int* hit_idx);
 

@@ -7,6 +7,7 @@
 //   MkBuilder::find_tracks_handle_missed_layers mkFit/MkBuilder.cc:1568-1630
 //   MkFinder::FindCandidatesCloneEngine         mkFit/MkFinder.cc:1114-1284
 //   isStripQCompatible                          mkFit/MkFinder.cc:837-862
+//   passStripChargePCMfromTrack                 mkFit/MkFinder.cc:863-896
 //   CandCloner::ProcessSeedRange                mkFit/CandCloner.cc:37-191
 //   MkFinder::UpdateWithLastHit                 mkFit/MkFinder.cc:1290-1308
 //   kalmanPropagateAnd{ComputeChi2,Update}      mkFit/KalmanUtilsMPlex.cc:471-543
@@ -15,13 +16,15 @@
 //   getScoreCalc / getScoreStruct / getScoreCand Track.h:648-697, HitStructures.h:589-601
 //
 // What the toy events do not exercise, and how it is handled here:
-//   * hit cluster charge: toy hits carry none; they are taken as the reference
-//     takes pixel-like clusters (charge code 0xff = Hit::maxChargePerCM), for
-//     which passStripChargePCMfromTrack returns true (MkFinder.cc:872-873);
+//   * hit cluster charge: toy hits carry none; without orc_find_event's
+//     hit_charge they are taken as the reference takes pixel-like clusters
+//     (charge code 0xff = Hit::maxChargePerCM), for which
+//     passStripChargePCMfromTrack returns true (MkFinder.cc:872-873); with it,
+//     the cut applies to every layer (the toy layers are all strip layers);
 //   * overlap hits: every toy hit of a layer has module id 0, and with a single
 //     module id HitMatchPair::find_overlap (HitStructures.h:372-394) never
 //     returns a match, so no overlap hit is ever added;
-//   * pixel layers, dead-module gaps (-7): the toy geometry has neither.
+//   * pixel layers: the toy geometry has none (dead-module gaps, -7, are modelled).
 // Both sorts of the reference are std::sort (not stable); this restatement uses
 // stable sorts, i.e. ties in score keep generation order.
 #include <algorithm>
@@ -136,6 +139,41 @@ bool strip_q_compatible(bool barrel, const float* pErr, const float* pPar, const
   return std::sqrt(hitT2 * 3.f) + std::max(3.f * qErr, 0.5f) > resProj;
 }
 
+// Hit::PackedData::get_charge_pcm (Hit.h:270-274): the 8-bit code's charge per cm.
+inline unsigned charge_pcm_of(unsigned char code) { return code == 0 ? 0u : ((code - 1u) << 3) + 1620u; }
+constexpr unsigned kMinChargePerCM = 1620, kMaxChargePerCM = 1620 + (0xfe << 3);   // Hit.h:281-282
+
+// passStripChargePCMfromTrack (MkFinder.cc:867-896), one lane: the hit's charge per cm times the
+// track's path-length factor at the hit (pPar: the state propagated to the hit) must exceed
+// Hit::minChargePerCM; the overflow code passes.  margin, when given: the relative gap of the
+// corrected charge to the cut, for the checker; qcorr, when given: the corrected charge (for the
+// overflow code, its charge per cm).
+bool pass_strip_charge(bool barrel, unsigned pcm, const float* pPar, const float* mErr, int it,
+                       float* margin = nullptr, float* qcorr = nullptr) {
+  if (qcorr) *qcorr = (float)pcm;
+  if (pcm >= kMaxChargePerCM) return true;
+  auto P = [&](int i) { return pPar[i * NN + it]; };
+  auto ME = [&](int i, int j) { return mErr[sym(i, j) * NN + it]; };
+  float qSF;
+  if (barrel) {
+    const float hitT2 = ME(0, 0) + ME(1, 1);
+    const float hitT2inv = 1.f / hitT2;
+    const float proj[3] = {ME(0, 0) * hitT2inv, ME(0, 1) * hitT2inv, ME(1, 1) * hitT2inv};
+    const bool detXY_OK = std::abs(proj[0] * proj[2] - proj[1] * proj[1]) < 0.1f;
+    const float cosP = std::cos(P(4)), sinP = std::sin(P(4));
+    const float sinT = std::abs(std::sin(P(5)));
+    qSF = detXY_OK ? sinT * std::sqrt(std::abs(1.f + cosP * cosP * proj[0] + sinP * sinP * proj[2] -
+                                               2.f * cosP * sinP * proj[1]))
+                   : 1.f;
+  } else {
+    qSF = std::abs(std::cos(P(5)));
+  }
+  const float qCorr = pcm * qSF;
+  if (qcorr) *qcorr = qCorr;
+  if (margin) *margin = std::min(*margin, std::abs(qCorr - (float)kMinChargePerCM) / (float)kMinChargePerCM);
+  return qCorr > kMinChargePerCM;
+}
+
 struct CEEvent {
   const orc_find_event* ev;
   const orc_ce_params* p;
@@ -229,7 +267,8 @@ struct LaneCounts {
 void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const float* hits, int n_hits,
                            const float* parP, const float* errP, const int* chg, const LaneCounts* lc,
                            const int* trk, const int* xs, const int* xa, const int* wsr, int n_proc,
-                           std::vector<IdxChi2>& entries, float* c2m = nullptr, const int* in_gap = nullptr) {
+                           std::vector<IdxChi2>& entries, float* c2m = nullptr, const int* in_gap = nullptr,
+                           const unsigned char* charge = nullptr) {
   int added[NN] = {};                                   // nHitsAdded
   if (c2m)
     for (int i = 0; i < NN; ++i) c2m[i] = 1e30f;
@@ -260,6 +299,9 @@ void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const floa
       if (c2m) c2m[i] = std::min(c2m[i], std::abs(c2 - max_c2) / std::max(max_c2, 1.f));
       if (!(c2 < max_c2)) continue;
       if (!strip_q_compatible(barrel, errP, propPar, msErr, msPar, i)) continue;
+      if (charge && !pass_strip_charge(barrel, charge_pcm_of(charge[xa[hc * NN + i]]), propPar, msErr, i,
+                                       c2m ? &c2m[i] : nullptr))
+        continue;
       IdxChi2 e;
       e.trkIdx = trk[i];
       e.hitIdx = xa[hc * NN + i];
@@ -306,7 +348,7 @@ void find_candidates_block(const CEEvent& E, const orc_layer_desc& d, const floa
 void find_candidates_std(const CEEvent& E, Comb& cc, int layer, const orc_layer_desc& d, const float* hits,
                          int n_hits, const Block& B, const std::vector<int>& idx, const int* xs, const int* xa,
                          const int* wsr, int n_proc, std::vector<Cand>& tmp, float* c2m = nullptr,
-                         const int* in_gap = nullptr) {
+                         const int* in_gap = nullptr, const unsigned char* charge = nullptr) {
   int added[NN] = {};                                   // nHitsAdded
   if (c2m)
     for (int i = 0; i < NN; ++i) c2m[i] = 1e30f;
@@ -339,7 +381,9 @@ void find_candidates_std(const CEEvent& E, Comb& cc, int layer, const orc_layer_
       const float max_c2 = dyn_chi2_cut(d, ev.chi2_cut, B.parP[3 * NN + i], B.parP[5 * NN + i]);
       const float c2 = std::abs(chi2[i]);
       if (c2m) c2m[i] = std::min(c2m[i], std::abs(c2 - max_c2) / std::max(max_c2, 1.f));
-      if (c2 < max_c2 && strip_q_compatible(barrel, B.errP, propPar, msErr, msPar, i)) {
+      if (c2 < max_c2 && strip_q_compatible(barrel, B.errP, propPar, msErr, msPar, i) &&
+          (!charge || pass_strip_charge(barrel, charge_pcm_of(charge[xa[hc * NN + i]]), propPar, msErr, i,
+                                        c2m ? &c2m[i] : nullptr))) {
         pass[i] = true;
         one_passes = true;
       }
@@ -470,10 +514,10 @@ void find_seed(const CEEvent& E, Comb& cc, int layer0, float* mg = nullptr, floa
       float c2m[NN];
       if (standard)
         find_candidates_std(E, cc, layer, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B, idx, xs, xa,
-                            wsr, n_proc, tmp, mg ? c2m : nullptr, gap);
+                            wsr, n_proc, tmp, mg ? c2m : nullptr, gap, E.loh[layer].charge);
       else
         find_candidates_block(E, d, hits, ev.hit_offset[layer + 1] - ev.hit_offset[layer], B.parP, B.errP, B.chg,
-                              lc, trk, xs, xa, wsr, n_proc, entries, mg ? c2m : nullptr, gap);
+                              lc, trk, xs, xa, wsr, n_proc, entries, mg ? c2m : nullptr, gap, E.loh[layer].charge);
       if (mg) {
         const int L = ev.n_layers;
         for (int i = 0; i < n_proc; ++i) {
@@ -631,6 +675,7 @@ int find_batch(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* 
     build_loh(ev->layers[l], ev->hit_offset[l + 1] - ev->hit_offset[l], ev->hits + (size_t)ev->hit_offset[l] * 9,
               E.loh[l]);
     if (ev->hit_mask) E.loh[l].mask = ev->hit_mask + ev->hit_offset[l];
+    if (ev->hit_charge) E.loh[l].charge = ev->hit_charge + ev->hit_offset[l];
     if (ev->n_dead > 0) suck_in_deads(E.loh[l], l, ev->n_dead, ev->dead_regions);
   }
   const int n = tr->n_seeds, L = ev->n_layers, K = p->max_cands;
@@ -756,7 +801,8 @@ extern "C" int orc_find_candidates_ce(const orc_ctx* c, const orc_find_event* ev
       trk[l] = t;
     }
     std::vector<IdxChi2> entries;
-    find_candidates_block(E, d, hits, n_hits, parP, errP, chg, lc, trk, xs, xa, w, cnt, entries);
+    find_candidates_block(E, d, hits, n_hits, parP, errP, chg, lc, trk, xs, xa, w, cnt, entries, nullptr, nullptr,
+                          ev->hit_charge ? ev->hit_charge + ev->hit_offset[layer] : nullptr);
     for (int l = 0; l < cnt; ++l) n_entries[t0 + l] = 0;
     for (const IdxChi2& e : entries) {
       const int t = e.trkIdx;
@@ -776,6 +822,18 @@ extern "C" int orc_find_candidates_ce(const orc_ctx* c, const orc_find_event* ev
     }
   }
   return 0;
+}
+
+// Checker hook: passStripChargePCMfromTrack for one state par[6] (propagated to the hit) and hit
+// covariance herr[6] (xx yx yy zx zy zz), given the hit's 8-bit charge code; returns 1 when the hit
+// passes, and the corrected charge per cm in *qcorr.
+extern "C" int orc_pass_strip_charge(int barrel, unsigned code, const float* par, const float* herr, float* qcorr) {
+  alignas(64) float p[6 * NN] = {}, e[6 * NN] = {};
+  for (int k = 0; k < 6; ++k) {
+    p[k * NN] = par[k];
+    e[k * NN] = herr[k];
+  }
+  return pass_strip_charge(barrel != 0, charge_pcm_of((unsigned char)code), p, e, 0, nullptr, qcorr) ? 1 : 0;
 }
 
 extern "C" int orc_update_with_last_hit(const orc_ctx* c, const orc_find_event* ev, const orc_ce_params* p,

@@ -64,6 +64,7 @@ struct LoH {
   std::vector<int> ranks;       // sorted position -> original hit index
   std::vector<float> phis, qs;  // in sorted order
   const unsigned char* mask = nullptr;   // per original hit: taken by an earlier iteration, or NULL
+  const unsigned char* charge = nullptr; // per original hit: the strip charge code, or NULL (cut passes)
   std::vector<unsigned char> dead;       // [nq * 256] dead-module bins; empty: none
 };
 

@@ -241,3 +241,40 @@ def test_dead_regions_match_oracle(gpu_device, standard):
     for k in ("n_cands", "hot_idx", "score"):
         assert np.array_equal(again[k], without[k]), k
     ctx.close()
+
+
+@pytest.mark.parametrize("standard", [False, True])
+def test_strip_charge_matches_oracle(gpu_device, standard):
+    """Strip cluster charge (mkf_event_of_hits_set_hit_charge): a fifth of the hits overflow (pass), a tenth
+    carry code 0 (never pass), the rest codes 2..79 (1628..2244 per cm, near the cut for the path-length factor
+    at the hit): the candidates match the oracle run with the same codes, seed by seed; clearing the codes
+    gives the run without them back bit for bit."""
+    import torch
+    ev, descs = _event(35, 3000, 2.0)
+    rng = np.random.default_rng(35)
+    total = int(ev["hit_offset"][-1])
+    code = rng.integers(2, 80, total).astype(np.uint8)
+    u = rng.random(total)
+    code[u < 0.2] = 0xff
+    code[(u >= 0.2) & (u < 0.3)] = 0
+    ctx = mk.Context(bfield=3.8)
+    eoh = mk.EventOfHits(ctx, descs)
+    eoh.load(ev["hit_offset"], _t(ev["hits_flat"], gpu_device))
+
+    def run():
+        b = mk.CEBatch(ev, len(descs), 3, device=gpu_device, chi2_cut=30.0, max_cands=6)
+        (ctx.find_tracks_standard if standard else ctx.find_tracks_clone_engine)(eoh, b)
+        torch.cuda.synchronize()
+        return b.numpy()
+    without = run()
+    eoh.set_hit_charge(_t(code, gpu_device))
+    g = run()
+    assert not np.array_equal(g["hot_idx"], without["hot_idx"])
+    cpu, spread = clone_engine_reference(dict(ev, hit_charge=code), descs, 3, chi2_cut=30.0, max_cands=6,
+                                         n_threads=8, standard=standard)
+    assert_ce_parity("strip charge %s" % ("standard" if standard else "clone engine"), g, cpu, spread, 3)
+    eoh.set_hit_charge(None)
+    again = run()
+    for k in ("n_cands", "hot_idx", "score"):
+        assert np.array_equal(again[k], without[k]), k
+    ctx.close()

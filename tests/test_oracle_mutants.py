@@ -90,6 +90,11 @@ FIND_MUTANTS = {
                        "e.ntailholes = lc[i].ntail + 1;"),
     "hit_record_holes": ("oracle_ce.cc", "e.nholes = lc[i].ninside + lc[i].ntail;", "e.nholes = lc[i].ninside;"),
     "ce_chi2_cut": ("oracle_ce.cc", "if (!(c2 < max_c2)) continue;", "if (!(c2 < 0.5f * max_c2)) continue;"),
+    "charge_cross_sign": ("oracle_ce.cc", "sinP * sinP * proj[2] -", "sinP * sinP * proj[2] +"),
+    "charge_decode": ("oracle_ce.cc", "((code - 1u) << 3) + 1620u", "(code << 3) + 1620u"),
+    "charge_endcap_sin": ("oracle_ce.cc", "qSF = std::abs(std::cos(P(5)));", "qSF = std::abs(std::sin(P(5)));"),
+    "charge_wrong_hit": ("oracle_ce.cc", "if (charge && !pass_strip_charge(barrel, charge_pcm_of(charge[xa[hc * NN + i]])",
+                         "if (charge && !pass_strip_charge(barrel, charge_pcm_of(charge[hc * NN + i])"),
     "best_short_compare": ("oracle_ce.cc", "if (h.score > cc.best_short.score) cc.best_short = tc;",
                            "if (h.score < cc.best_short.score) cc.best_short = tc;"),
     "short_kept_live": ("oracle_ce.cc", "if (h.score > cc.best_short.score) cc.best_short = tc;\n          continue;",
@@ -142,6 +147,7 @@ STD_MUTANTS = {
     "std_fake_consec_le": ("c.ntail < P.max_consec_holes) ? -1 : -2;", "c.ntail <= P.max_consec_holes) ? -1 : -2;"),
     "std_fake_edge_code": ("if (wsr[i] == WSR_Edge) fake = -3;\n    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;   // dead module (MkFinder.cc:1092-1096)",
                            "if (wsr[i] == WSR_Edge) fake = -1;\n    else if (in_gap && in_gap[i] && added[i] == 0) fake = -7;"),
+    "std_charge_ignored": ("(!charge || pass_strip_charge(", "(true || pass_strip_charge("),
     "std_sort_ascending": ("std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });",
                            "std::stable_sort(tmp.begin(), tmp.end(), [](const Cand& a, const Cand& b) { return a.score < b.score; });"),
 }
@@ -214,6 +220,7 @@ def _param_values(test):
 
 def _find_pins():
     import test_oracle_ce as ce
+    import test_oracle_charge as ch
     import test_oracle_find as bh
     import test_oracle_find_pins as fp
     return [
@@ -241,6 +248,9 @@ def _find_pins():
         ("ce_charge_flip", fp.test_wrong_charge_seeds_flip_to_the_true_charge),
         ("dead_module_gap", lambda: [fp.test_dead_module_gap(v) for v in (False, True)]),
         ("ce_composition_k6", lambda: ce.test_ce_matches_composition(6)),
+        ("strip_charge_cut", lambda: [ch.test_cut_matches_quadratic_form(b) for b in (True, False)]),
+        ("strip_charge_records", lambda: [ch.test_clone_engine_records_with_charge(ev, la)
+                                          for ev in [ch.make_event()] for la in (3, 9)]),
     ]
 
 
@@ -265,8 +275,10 @@ def _bkfit_pins():
 
 
 def _std_pins():
+    import test_oracle_charge as ch
     import test_oracle_std_pins as sp
     return [
+        ("strip_charge_std", lambda: ch.test_whole_event_overflow_and_zero_codes(ch.make_event(), True)),
         ("selection_k1", lambda: sp.test_standard_selection(1, None, 0)),
         ("selection_k3_intra", lambda: sp.test_standard_selection(3, None, 1)),
         ("all_paths_4_1", lambda: sp.test_standard_equals_all_paths((4, 1, None))),
