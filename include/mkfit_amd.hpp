@@ -128,6 +128,12 @@ class Hit {
   void setMCHitID(int id) { mcHitID_ = id; }
   unsigned packedData() const { return pdata_; }
   void setPackedData(unsigned d) { pdata_ = d; }
+  // Hit::PackedData (Hit.h:251-282): detid_in_layer bits 0-11, the charge code bits 12-19.
+  unsigned detIDinLayer() const { return pdata_ & 0xfffu; }
+  unsigned chargeCode() const { return (pdata_ >> 12) & 0xffu; }
+  unsigned chargePerCM() const { return chargeCode() == 0 ? 0u : ((chargeCode() - 1u) << 3) + 1620u; }
+  static unsigned minChargePerCM() { return 1620; }
+  static unsigned maxChargePerCM() { return 1620 + (0xfe << 3); }
 
   SVector3 pos_;
   SMatrixSym33 err_;
@@ -653,6 +659,9 @@ class EventOfHits {
         std::memcpy(d, layer_hits[l][i].posArray(), 3 * sizeof(float));
         std::memcpy(d + 3, layer_hits[l][i].errArray(), 6 * sizeof(float));
       }
+    codes_.clear();
+    for (int l = 0; l < n_layers_; ++l)
+      for (const Hit& h : layer_hits[l]) codes_.push_back((unsigned char)h.chargeCode());
     hits_.upload(flat, c_.stream());
     check(mkf_event_of_hits_load(h_, off.data(), hits_.get(), c_.stream()));
     n_hits_.assign(off.begin(), off.end());
@@ -675,6 +684,19 @@ class EventOfHits {
     }
     mask_.upload(flat, c_.stream());
     check(mkf_event_of_hits_set_hit_mask(h_, mask_.get(), c_.stream()));
+    c_.sync();
+  }
+  // The strip charge cut of FindCandidates[CloneEngine] (passStripChargePCMfromTrack,
+  // MkFinder.cc:863-896) over the charge codes the sucked-in hits carry in their
+  // packed data (setupAsStrip / setupAsPixel, Hit.h:285-299); off, every hit
+  // passes.  Set after SuckInHits (which turns it off).
+  void UseStripCharges(bool on) {
+    if (!on || codes_.empty()) {
+      check(mkf_event_of_hits_set_hit_charge(h_, nullptr, c_.stream()));
+      return;
+    }
+    charge_.upload(codes_, c_.stream());
+    check(mkf_event_of_hits_set_hit_charge(h_, charge_.get(), c_.stream()));
     c_.sync();
   }
   // EventOfHits::SuckInDeads (HitStructures.h:310-313): the layer's dead
@@ -700,7 +722,8 @@ class EventOfHits {
   int n_layers_;
   mkf_event_of_hits* h_ = nullptr;
   DeviceArray<float> hits_;
-  DeviceArray<unsigned char> mask_;
+  DeviceArray<unsigned char> mask_, charge_;
+  std::vector<unsigned char> codes_;
   std::vector<int> n_hits_;
   std::vector<DeadVec> deads_;
 };

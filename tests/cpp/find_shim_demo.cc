@@ -259,6 +259,35 @@ int main(int argc, char** argv) {
     if (Config::includePCA) finder.BkFitPropTracksToPCA(ns);
     finder.BkFitOutputTracks(best, 0, ns, Config::includePCA);
     Columns(best, L).write(w, "mf_bkfit", L);
+
+    // The clone engine again with strip charges in the hits' packed data
+    // (Hit.h:251-299): of every 10 hits in load order, 2 pixel-like (0xff),
+    // 1 below the cut (code 0), the rest codes 2..79 near it.
+    std::vector<HitVec> qhits = ev.layerHits_;
+    int g = 0;
+    for (HitVec& hv : qhits)
+      for (Hit& h : hv) {
+        const int r = g % 10;
+        const unsigned code = r < 2 ? 0xffu : r == 2 ? 0u : 2u + (unsigned)((long long)g * 7919 % 78);
+        h.setPackedData(code << 12);
+        ++g;
+      }
+    eoh.SuckInHits(qhits);
+    eoh.UseStripCharges(true);
+    builder.begin_event(eoh, ce_ip);
+    builder.find_tracks_load_seeds(ev.seedTracks_);
+    builder.FindTracksCloneEngine();
+    std::vector<TrackVec> qc;
+    builder.export_comb_cands(qc);
+    std::vector<int> qn(ns), qhot((size_t)K * L * ns, -9);
+    for (int t = 0; t < ns; ++t) {
+      qn[t] = (int)qc[t].size();
+      for (int k = 0; k < qn[t]; ++k)
+        for (int i = 0; i < qc[t][k].nTotalHits(); ++i)
+          qhot[((size_t)k * L + qc[t][k].getHitLyr(i)) * ns + t] = qc[t][k].getHitIdx(i);
+    }
+    w.put("ceq_n_cands", qn, {ns});
+    w.put("ceq_hot_idx", qhot, {K, L, ns});
   } catch (const std::exception& e) {
     std::fprintf(stderr, "%s\n", e.what());
     std::fclose(f);

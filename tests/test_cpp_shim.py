@@ -48,8 +48,13 @@ int main() {
   unsigned raw;
   static_assert(sizeof(h) == sizeof(raw), "");
   std::memcpy(&raw, &h, 4);
-  std::printf("%zu %zu %zu %zu %08x\n", sizeof(mkfit_amd::Track), sizeof(mkfit_amd::Hit),
-              sizeof(mkfit_amd::HitOnTrack), sizeof(mkfit_amd::TrackState), raw);
+  mkfit_amd::Hit s;
+  s.setPackedData(7u | (2u << 12) | (3u << 20));   // detid 7, charge code 2, span_rows 3
+  mkfit_amd::Hit p;
+  p.setPackedData(0xffu << 12);                      // setupAsPixel's charge code
+  std::printf("%zu %zu %zu %zu %08x %u %u %u %u\n", sizeof(mkfit_amd::Track), sizeof(mkfit_amd::Hit),
+              sizeof(mkfit_amd::HitOnTrack), sizeof(mkfit_amd::TrackState), raw, s.detIDinLayer(),
+              s.chargePerCM(), p.chargePerCM(), mkfit_amd::Hit::maxChargePerCM());
 }
 '''
     with tempfile.TemporaryDirectory() as d:
@@ -64,6 +69,8 @@ int main() {
     assert out[:4] == ["160", "44", "4", "112"]
     raw = int(out[4], 16)
     assert raw & 0xffffff == (-5) & 0xffffff and raw >> 24 == 7   # index low 24 bits, layer high 8
+    # Hit::PackedData (Hit.h:251-282): code 2 is 1628 per cm; the pixel code 0xff is maxChargePerCM
+    assert out[5:] == ["7", "1628", "3652", "3652"]
 
 
 def _read_arrays(path):
@@ -148,6 +155,24 @@ def test_shim_clone_engine_matches_oracle(gpu_device, find_run):
     g = {k[3:]: v for k, v in a.items() if k.startswith("ce_") and not k.startswith("ce_bkfit")}
     cpu, spread = clone_engine_reference(ev, descs, 3, chi2_cut=30.0, max_cands=5, prop_to_hit=True, n_threads=8)
     assert_ce_parity("shim clone engine", g, cpu, spread, 3)
+
+
+@pytest.mark.gpu
+def test_shim_strip_charges_match_oracle(gpu_device, find_run):
+    """EventOfHits::UseStripCharges over charge codes in the hits' packed data: the clone engine's hit lists
+    match the oracle run with the same codes, seed by seed (as far as the candidates' decision margins allow)."""
+    a, ev, descs = find_run
+    g = np.arange(int(ev["hit_offset"][-1]))
+    r = g % 10
+    code = np.where(r < 2, 0xff, np.where(r == 2, 0, 2 + (g * 7919) % 78)).astype(np.uint8)
+    cpu, spread = clone_engine_reference(dict(ev, hit_charge=code), descs, 3, chi2_cut=30.0, max_cands=5,
+                                         prop_to_hit=True, n_threads=8)
+    gq = {"n_cands": a["ceq_n_cands"], "hot_idx": a["ceq_hot_idx"]}
+    assert not np.array_equal(a["ceq_hot_idx"], a["ce_hot_idx"])
+    same = np.array([gq["n_cands"][t] == cpu["n_cands"][t] and
+                     np.array_equal(gq["hot_idx"][:cpu["n_cands"][t], :, t], cpu["hot_idx"][:cpu["n_cands"][t], :, t])
+                     for t in range(ev["n_seeds"])])
+    assert same.mean() > 0.999, (same.mean(), np.nonzero(~same)[0][:10])
 
 
 def _bkfit_check(name, a, prefix, ev, descs, cand, **kw):
